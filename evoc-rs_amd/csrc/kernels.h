@@ -1,0 +1,151 @@
+// Internal (C++) interface between the host orchestration and the HIP kernels.
+#pragma once
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "common.h"
+
+namespace evoc {
+
+// ------------------------------------------------------------------ device memory
+// Caching device allocator: freed blocks are kept by rounded size and reused, so a
+// second call of the pipeline on the same shapes performs no hipMalloc.
+class DevPool {
+ public:
+  void* get(size_t bytes) {
+    size_t r = round(bytes);
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = free_.find(r);
+    if (it != free_.end()) {
+      void* p = it->second;
+      free_.erase(it);
+      return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, r);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      release_locked();
+      EVOC_HIP(hipMalloc(&p, r));
+    }
+    return p;
+  }
+  void put(void* p, size_t bytes) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(mu_);
+    free_.insert({round(bytes), p});
+  }
+  void release() {
+    std::lock_guard<std::mutex> g(mu_);
+    release_locked();
+  }
+  ~DevPool() { release(); }
+
+ private:
+  static size_t round(size_t b) {
+    if (b < 256) return 256;
+    if (b < (1u << 20)) {
+      size_t r = 256;
+      while (r < b) r <<= 1;
+      return r;
+    }
+    const size_t g = 2u << 20;
+    return (b + g - 1) / g * g;
+  }
+  void release_locked() {
+    for (auto& kv : free_) (void)hipFree(kv.second);
+    free_.clear();
+  }
+  std::mutex mu_;
+  std::multimap<size_t, void*> free_;
+};
+
+template <typename T>
+struct Buf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevPool* pool = nullptr;
+  Buf() = default;
+  Buf(DevPool& pl, size_t count) { alloc(pl, count); }
+  void alloc(DevPool& pl, size_t count) {
+    reset();
+    pool = &pl;
+    n = count;
+    p = static_cast<T*>(pl.get(std::max<size_t>(count, 1) * sizeof(T)));
+  }
+  void reset() {
+    if (p && pool) pool->put(p, std::max<size_t>(n, 1) * sizeof(T));
+    p = nullptr;
+    n = 0;
+  }
+  ~Buf() { reset(); }
+  Buf(const Buf&) = delete;
+  Buf& operator=(const Buf&) = delete;
+  Buf(Buf&& o) noexcept : p(o.p), n(o.n), pool(o.pool) { o.p = nullptr; o.n = 0; }
+  Buf& operator=(Buf&& o) noexcept {
+    if (this != &o) {
+      reset();
+      p = o.p; n = o.n; pool = o.pool;
+      o.p = nullptr; o.n = 0;
+    }
+    return *this;
+  }
+};
+
+// ------------------------------------------------------------------ kNN
+enum KnnPrecision { KNN_AUTO = 0, KNN_SPLIT = 1, KNN_BF16 = 2 };
+struct KnnStats {
+  int split = 0;
+  int fallback_rows = 0;
+  int candidates = 0;
+  float mfma_ms = 0.0f;
+  double flops = 0.0;
+};
+void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k, int precision,
+                int32_t* d_idx, float* d_dist, KnnStats* stats, hipStream_t s, DevPool& pool);
+
+// ------------------------------------------------------------------ fuzzy set
+struct Graph {  // CSR, rows sorted by column when symmetrised
+  Buf<int64_t> indptr;
+  Buf<int32_t> indices;
+  Buf<float> weights;
+  int64_t n = 0, nnz = 0;
+};
+void fuzzy_device(const int32_t* d_idx, const float* d_dist, int64_t n, int k, float effective_k,
+                  bool symmetrise, float* d_sigma, float* d_rho, Graph& out, hipStream_t s,
+                  DevPool& pool);
+
+// ------------------------------------------------------------------ embedding
+struct EmbedParams {
+  int n_epochs = 50;
+  float noise_level = 0.5f;
+  float negative_sample_rate = 1.0f;
+  float initial_alpha = 0.1f;
+  float beta1 = 0.5f, beta2 = 0.9f, eps = 1e-7f;
+};
+// d_emb holds the initial layout on entry (or is filled from Philox when
+// random_init) and the final layout on exit.
+void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, bool random_init,
+                  uint64_t seed, hipStream_t s, DevPool& pool, float* kernel_ms = nullptr);
+
+// ------------------------------------------------------------------ core distances + MST
+void core_sq_device(const float* d_emb, int64_t n, int dim, int min_samples, float* d_core,
+                    hipStream_t s, DevPool& pool);
+// writes n-1 edges (u < v) sorted by (w, u, v); weights sqrt'd
+void mst_device(const float* d_emb, int64_t n, int dim, int min_samples, int32_t* h_u,
+                int32_t* h_v, float* h_w, hipStream_t s, DevPool& pool, int* rounds = nullptr);
+
+// ------------------------------------------------------------------ persistence (fp64)
+// persistence[k] = sum over barcodes b (in order, root skipped) with
+// bpos[b] <= k < dpos[b] of contrib[b]
+void persistence_bins_device(const int64_t* h_bpos, const int64_t* h_dpos, const double* h_contrib,
+                             int64_t nbc, int64_t nbins, double* h_out, hipStream_t s,
+                             DevPool& pool);
+
+// ------------------------------------------------------------------ scans
+void exclusive_scan_i64(const int64_t* d_in, int64_t* d_out, int64_t n, hipStream_t s,
+                        DevPool& pool);
+
+}  // namespace evoc

@@ -1,0 +1,547 @@
+// Exhaustive kNN self-join for gfx950 (the exhaustive arm of run_ann_search,
+// src/nearest_neighbours/nearest_neighbour_cpu.rs:313-317, with the j != i self
+// filter of src/nearest_neighbours/nearest_neighbour_gpu.rs:291-303).
+//
+//  1. knn_prep:   canonical f32 norms + bf16 copies (hi, and lo = bf16(x - hi) when
+//                 the input is not bf16-representable).
+//  2. knn_topc:   S = Q X^T on v_mfma_f32_32x32x16_bf16, 128x128 tiles, per-row
+//                 top-C candidate lists kept in registers (never materialises S).
+//  3. knn_rerank: exact f32 distances of the C candidates in the canonical order
+//                 (64 strided lane sums + halving tree), sort by (dist, idx), and a
+//                 per-row certificate that no non-candidate can enter the top k.
+//  4. knn_exact:  brute-force exact rows for the (rare) rows whose certificate fails.
+#include "kernels.h"
+
+namespace evoc {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 128;   // query rows per workgroup
+constexpr int BN = 128;   // database columns per tile
+constexpr int KC = 32;    // k-chunk staged through LDS
+constexpr int LDA = 40;   // padded LDS row stride (bf16 elements) of a staged chunk
+constexpr int SCS = 132;  // padded LDS row stride (floats) of the key tile
+constexpr float FINF = __builtin_huge_valf();
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t b = __float_as_uint(f);
+  if ((b & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+  uint32_t r = b + 0x7fffu + ((b >> 16) & 1u);
+  return (uint16_t)(r >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// canonical halving-tree reduction across the 64 lanes: result valid in lane 0
+__device__ __forceinline__ float tree64(float acc) {
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) acc = acc + __shfl_down(acc, s, 64);
+  return acc;
+}
+
+__device__ __forceinline__ float cos_dist(float dot, float na, float nb) {
+  float den = na * nb;
+  if (!(den > 0.0f)) return 1.0f;
+  float c = dot / den;
+  return 1.0f - c;
+}
+
+// ------------------------------------------------------------------ 1. prep
+__global__ __launch_bounds__(256) void knn_prep_kernel(
+    const float* __restrict__ X, int64_t n, int d, int64_t ld, int64_t npad, int dpad,
+    uint16_t* __restrict__ Xh, uint16_t* __restrict__ Xl, float* __restrict__ nrm,
+    float* __restrict__ sqn, float* __restrict__ colscale, int metric,
+    unsigned* __restrict__ flags /* [0]=inexact, [1]=max norm bits */) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + wave;
+  if (row >= npad) return;
+  float acc = 0.0f;
+  bool any_lo = false;
+  for (int t = lane; t < dpad; t += 64) {
+    float v = (row < n && t < d) ? X[row * ld + t] : 0.0f;
+    float p = v * v;
+    acc = acc + p;
+    uint16_t h = f2bf(v);
+    float r = v - bf2f(h);
+    uint16_t lo = f2bf(r);
+    Xh[row * dpad + t] = h;
+    if (Xl) Xl[row * dpad + t] = lo;
+    any_lo |= (lo & 0x7fffu) != 0;
+  }
+  acc = tree64(acc);
+  if (__any(any_lo) && lane == 0) atomicOr(&flags[0], 1u);
+  if (lane == 0) {
+    float nv = sqrtf(acc);
+    if (row < n) {
+      nrm[row] = nv;
+      sqn[row] = acc;
+      atomicMax(&flags[1], __float_as_uint(nv));
+    }
+    float cs;
+    if (metric == 1) cs = (row < n && nv > 0.0f) ? 1.0f / nv : 0.0f;
+    else cs = row < n ? acc : 0.0f;
+    colscale[row] = cs;
+  }
+}
+
+// ------------------------------------------------------------------ 2. MFMA top-C
+template <int C>
+__device__ __forceinline__ void list_insert(float (&lk)[C], int (&li)[C], float key, int idx) {
+  float nk = key;
+  int ni = idx;
+#pragma unroll
+  for (int q = 0; q < C; ++q) {
+    bool sw = (nk < lk[q]) || (nk == lk[q] && ni < li[q]);
+    float tk = sw ? lk[q] : nk;
+    int ti = sw ? li[q] : ni;
+    lk[q] = sw ? nk : lk[q];
+    li[q] = sw ? ni : li[q];
+    nk = tk;
+    ni = ti;
+  }
+}
+
+template <bool SPLIT, int C>  // C = list length per column half
+__global__ __launch_bounds__(256, 1) void knn_topc_kernel(
+    const uint16_t* __restrict__ Xh, const uint16_t* __restrict__ Xl, int64_t n, int64_t npad,
+    int dpad, const float* __restrict__ colscale, int metric, int32_t* __restrict__ cand_idx,
+    float* __restrict__ cand_key) {
+  constexpr int NOPS = SPLIT ? 4 : 2;  // A hi, B hi, (A lo, B lo)
+  constexpr int SE = BM * LDA;         // elements per staged operand
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  uint16_t* stage = reinterpret_cast<uint16_t*>(smem);
+  float* sc = reinterpret_cast<float*>(smem + 2 * NOPS * SE * sizeof(uint16_t));
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t row0 = (int64_t)blockIdx.x * BM;
+  const int nck = dpad / KC;
+  const int64_t ntile = npad / BN;
+  const int64_t total = ntile * nck;
+
+  float lk[C];
+  int li[C];
+#pragma unroll
+  for (int q = 0; q < C; ++q) {
+    lk[q] = FINF;
+    li[q] = -1;
+  }
+  f32x16 acc[2][2];
+  u32x4 pre[NOPS][2];
+  float cs[2] = {0.0f, 0.0f};
+
+#define EVOC_GLOAD(G)                                                                    \
+  do {                                                                                   \
+    const int64_t ct_ = (G) / nck;                                                       \
+    const int kc_ = (int)((G) - ct_ * nck);                                              \
+    _Pragma("unroll") for (int op = 0; op < NOPS; ++op) {                                \
+      const uint16_t* src = (op < 2) ? Xh : Xl;                                          \
+      const int64_t rbase = (op & 1) ? ct_ * BN : row0;                                  \
+      _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                    \
+        const int item = tid + h * 256;                                                  \
+        const int r = item >> 2, q = item & 3;                                           \
+        pre[op][h] =                                                                     \
+            *reinterpret_cast<const u32x4*>(src + (rbase + r) * dpad + kc_ * KC + q * 8); \
+      }                                                                                  \
+    }                                                                                    \
+  } while (0)
+#define EVOC_LSTORE(BUF)                                                                    \
+  do {                                                                                      \
+    _Pragma("unroll") for (int op = 0; op < NOPS; ++op)                                     \
+        _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                     \
+      const int item = tid + h * 256;                                                       \
+      const int r = item >> 2, q = item & 3;                                                \
+      *reinterpret_cast<u32x4*>(stage + ((BUF) * NOPS + op) * SE + r * LDA + q * 8) = pre[op][h]; \
+    }                                                                                       \
+  } while (0)
+
+  EVOC_GLOAD(0);
+  EVOC_LSTORE(0);
+  __syncthreads();
+  for (int64_t g = 0; g < total; ++g) {
+    const int buf = (int)(g & 1);
+    const int64_t ct = g / nck;
+    const int kc = (int)(g - ct * nck);
+    if (g + 1 < total) EVOC_GLOAD(g + 1);
+    if (kc == 0) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = (f32x16){};
+#pragma unroll
+      for (int tn = 0; tn < 2; ++tn) cs[tn] = colscale[ct * BN + wn * 64 + tn * 32 + (lane & 31)];
+    }
+    const uint16_t* sA = stage + (buf * NOPS + 0) * SE;
+    const uint16_t* sB = stage + (buf * NOPS + 1) * SE;
+#pragma unroll
+    for (int kk = 0; kk < KC / 16; ++kk) {
+      const int ko = kk * 16 + (lane >> 5) * 8;
+      bf16x8 a[2], b[2];
+#pragma unroll
+      for (int tm = 0; tm < 2; ++tm)
+        a[tm] = *reinterpret_cast<const bf16x8*>(sA + (wm * 64 + tm * 32 + (lane & 31)) * LDA + ko);
+#pragma unroll
+      for (int tn = 0; tn < 2; ++tn)
+        b[tn] = *reinterpret_cast<const bf16x8*>(sB + (wn * 64 + tn * 32 + (lane & 31)) * LDA + ko);
+#pragma unroll
+      for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn)
+          acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[tm], b[tn], acc[tm][tn], 0, 0, 0);
+      if constexpr (SPLIT) {
+        const uint16_t* sAl = stage + (buf * NOPS + 2) * SE;
+        const uint16_t* sBl = stage + (buf * NOPS + 3) * SE;
+        bf16x8 al[2], bl[2];
+#pragma unroll
+        for (int tm = 0; tm < 2; ++tm)
+          al[tm] = *reinterpret_cast<const bf16x8*>(sAl + (wm * 64 + tm * 32 + (lane & 31)) * LDA + ko);
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn)
+          bl[tn] = *reinterpret_cast<const bf16x8*>(sBl + (wn * 64 + tn * 32 + (lane & 31)) * LDA + ko);
+#pragma unroll
+        for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+          for (int tn = 0; tn < 2; ++tn) {
+            acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[tm], bl[tn], acc[tm][tn], 0, 0, 0);
+            acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[tm], b[tn], acc[tm][tn], 0, 0, 0);
+          }
+      }
+    }
+    if (g + 1 < total) EVOC_LSTORE(buf ^ 1);
+    __syncthreads();
+    if (kc == nck - 1) {
+      // keys of this 128x128 tile -> LDS (smaller key = closer)
+#pragma unroll
+      for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int rl = wm * 64 + tm * 32 + j * 8 + (lane >> 5) * 4 + i;
+              const int cl = wn * 64 + tn * 32 + (lane & 31);
+              const float s = acc[tm][tn][j * 4 + i];
+              float key = metric == 1 ? -(s * cs[tn]) : (cs[tn] - 2.0f * s);
+              const int64_t gc = ct * BN + cl;
+              if (gc >= n || gc == row0 + rl) key = FINF;
+              sc[rl * SCS + cl] = key;
+            }
+      __syncthreads();
+      const int r = tid & (BM - 1), hh = tid >> 7;
+      const float* srow = sc + r * SCS + hh * 64;
+      const int cbase = (int)(ct * BN) + hh * 64;
+      for (int q = 0; q < 64; q += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(srow + q);
+        if (v.x < lk[C - 1]) list_insert<C>(lk, li, v.x, cbase + q);
+        if (v.y < lk[C - 1]) list_insert<C>(lk, li, v.y, cbase + q + 1);
+        if (v.z < lk[C - 1]) list_insert<C>(lk, li, v.z, cbase + q + 2);
+        if (v.w < lk[C - 1]) list_insert<C>(lk, li, v.w, cbase + q + 3);
+      }
+    }
+  }
+  // each (row, column-half) list is written as is: candidates per row = 2*C
+  const int r = tid & (BM - 1), hh = tid >> 7;
+  const int64_t grow = row0 + r;
+  if (grow < n) {
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      cand_key[(grow * 2 + hh) * C + q] = lk[q];
+      cand_idx[(grow * 2 + hh) * C + q] = li[q];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ 3. re-rank
+constexpr int MAXT = 16;  // d <= 1024 handled with the query row held in registers
+
+__global__ __launch_bounds__(256) void knn_rerank_kernel(
+    const float* __restrict__ X, int64_t n, int d, int64_t ld, const float* __restrict__ nrm,
+    const float* __restrict__ sqn, const float* __restrict__ colscale,
+    const unsigned* __restrict__ flags, const int32_t* __restrict__ cand_idx,
+    const float* __restrict__ cand_key, int C, int k, int metric, float gamma,
+    int32_t* __restrict__ out_idx, float* __restrict__ out_dist, int* __restrict__ fail_count,
+    int32_t* __restrict__ fail_rows) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + wave;
+  if (row >= n) return;
+  const int nt = (d + 63) / 64;
+  float q[MAXT];
+#pragma unroll
+  for (int t = 0; t < MAXT; ++t) {
+    const int c = t * 64 + lane;
+    q[t] = (t < nt && c < d) ? X[row * ld + c] : 0.0f;
+  }
+  float my_d = FINF, my_sq = FINF;
+  int my_j = -1;
+  for (int c = 0; c < C; ++c) {
+    const int j = cand_idx[row * C + c];
+    if (j < 0) continue;
+    float acc = 0.0f;
+#pragma unroll
+    for (int t = 0; t < MAXT; ++t) {
+      const int cc = t * 64 + lane;
+      if (t < nt && cc < d) {
+        const float v = X[(int64_t)j * ld + cc];
+        if (metric == 1) {
+          const float p = q[t] * v;
+          acc = acc + p;
+        } else {
+          const float df = q[t] - v;
+          const float p = df * df;
+          acc = acc + p;
+        }
+      }
+    }
+    acc = tree64(acc);
+    acc = __shfl(acc, 0, 64);
+    const float dist = metric == 1 ? cos_dist(acc, nrm[row], nrm[j]) : sqrtf(acc);
+    if (lane == c) {
+      my_d = dist;
+      my_sq = acc;
+      my_j = j;
+    }
+  }
+  int rank = 0;
+  for (int c = 0; c < C; ++c) {
+    const float od = __shfl(my_d, c, 64);
+    const int oj = __shfl(my_j, c, 64);
+    if (oj >= 0 && (od < my_d || (od == my_d && oj < my_j))) ++rank;
+  }
+  const bool valid = my_j >= 0;
+  const int nvalid = __popcll(__ballot(valid));
+  if (valid && rank < k) {
+    out_idx[row * k + rank] = my_j;
+    out_dist[row * k + rank] = my_d;
+  }
+  for (int s = lane; s < k; s += 64)
+    if (s >= nvalid) {
+      out_idx[row * k + s] = -1;
+      out_dist[row * k + s] = FINF;
+    }
+  // certificate: every non-candidate's approximate key is >= the C-th kept key
+  bool pass;
+  if ((int64_t)nvalid >= n - 1) {
+    pass = true;
+  } else if (nvalid < k) {
+    pass = false;
+  } else {
+    const unsigned long long kb = __ballot(valid && rank == k - 1);
+    const int src = __ffsll((long long)kb) - 1;
+    const float dk = __shfl(my_d, src, 64);
+    const float dk2 = __shfl(my_sq, src, 64);
+    const int ch = C / 2;  // each column half kept its ch best keys
+    const float kc = fminf(cand_key[row * C + (ch - 1)], cand_key[row * C + (C - 1)]);
+    const float u = 5.9604645e-08f;  // 2^-24
+    if (metric == 1) {
+      const float eps = 1.5f * gamma + 64.0f * u;
+      const float lb = 1.0f + kc * colscale[row];
+      pass = (lb - eps) > dk;
+    } else {
+      const float nmax = __uint_as_float(flags[1]);
+      const float ni = nrm[row];
+      const float dt = (float)(d / 64 + 12);
+      const float E = 1.5f * (dt * u * (sqn[row] + nmax * nmax) + 2.0f * (gamma + 2.0f * u) * ni * nmax);
+      const float lb2 = sqn[row] + kc - E;
+      pass = lb2 > dk2 * (1.0f + (dt + 4.0f) * u) + 1e-30f;
+    }
+  }
+  if (!pass && lane == 0) {
+    const int pos = atomicAdd(fail_count, 1);
+    fail_rows[pos] = (int32_t)row;
+  }
+}
+
+// ------------------------------------------------------------------ 4. exact rows
+constexpr int KMAX = 32;
+__global__ __launch_bounds__(256) void knn_exact_kernel(
+    const float* __restrict__ X, int64_t n, int d, int64_t ld, const float* __restrict__ nrm,
+    int k, int metric, const int* __restrict__ fail_count, const int32_t* __restrict__ fail_rows,
+    int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+  extern __shared__ __attribute__((aligned(16))) char xsm[];
+  float* sk = reinterpret_cast<float*>(xsm);
+  int* si = reinterpret_cast<int*>(xsm + 256 * KMAX * sizeof(float));
+  const int tid = threadIdx.x;
+  const int nf = *fail_count;
+  for (int f = blockIdx.x; f < nf; f += gridDim.x) {
+    const int64_t row = fail_rows[f];
+    float lk[KMAX];
+    int li[KMAX];
+#pragma unroll
+    for (int q = 0; q < KMAX; ++q) {
+      lk[q] = FINF;
+      li[q] = -1;
+    }
+    const float* xr = X + row * ld;
+    for (int64_t j = tid; j < n; j += 256) {
+      if (j == row) continue;
+      const float* xj = X + j * ld;
+      float acc[64];
+#pragma unroll
+      for (int c = 0; c < 64; ++c) acc[c] = 0.0f;
+      for (int t0 = 0; t0 < d; t0 += 64) {
+#pragma unroll
+        for (int c = 0; c < 64; ++c)
+          if (t0 + c < d) {
+            const float a = xr[t0 + c], b = xj[t0 + c];
+            if (metric == 1) {
+              const float p = a * b;
+              acc[c] = acc[c] + p;
+            } else {
+              const float df = a - b;
+              const float p = df * df;
+              acc[c] = acc[c] + p;
+            }
+          }
+      }
+#pragma unroll
+      for (int s = 32; s >= 1; s >>= 1)
+#pragma unroll
+        for (int c = 0; c < s; ++c) acc[c] = acc[c] + acc[c + s];
+      const float dist = metric == 1 ? cos_dist(acc[0], nrm[row], nrm[j]) : sqrtf(acc[0]);
+      if (dist < lk[k - 1] || (dist == lk[k - 1] && (int)j < li[k - 1]))
+        list_insert<KMAX>(lk, li, dist, (int)j);
+    }
+#pragma unroll
+    for (int q = 0; q < KMAX; ++q) {
+      sk[tid * KMAX + q] = lk[q];
+      si[tid * KMAX + q] = li[q];
+    }
+    __syncthreads();
+    for (int stride = 1; stride < 256; stride <<= 1) {
+      if ((tid & (2 * stride - 1)) == 0) {
+        float* ak = sk + tid * KMAX;
+        int* ai = si + tid * KMAX;
+        const float* bk = sk + (tid + stride) * KMAX;
+        const int* bi = si + (tid + stride) * KMAX;
+        float mk[KMAX];
+        int mi[KMAX];
+        int pa = 0, pb = 0;
+        for (int q = 0; q < k; ++q) {
+          const bool ta = (ak[pa] < bk[pb]) || (ak[pa] == bk[pb] && (unsigned)ai[pa] < (unsigned)bi[pb]);
+          mk[q] = ta ? ak[pa] : bk[pb];
+          mi[q] = ta ? ai[pa] : bi[pb];
+          pa += ta;
+          pb += !ta;
+        }
+        for (int q = 0; q < k; ++q) {
+          ak[q] = mk[q];
+          ai[q] = mi[q];
+        }
+      }
+      __syncthreads();
+    }
+    for (int q = tid; q < k; q += 256) {
+      out_idx[row * k + q] = si[q];
+      out_dist[row * k + q] = sk[q];
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launcher
+void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k, int precision,
+                int32_t* d_idx, float* d_dist, KnnStats* stats, hipStream_t s, DevPool& pool) {
+  EVOC_CHECK(n >= 1 && d >= 1, "knn: empty input");
+  EVOC_CHECK(k >= 1 && k <= KMAX, "knn: k must be in [1, 32]");
+  EVOC_CHECK(d <= MAXT * 64, "knn: d must be <= 1024");
+  const int CH = (k <= 16) ? 16 : 32;  // per column half
+  const int C = 2 * CH;
+  const int64_t npad = ceil_div(n, BM) * BM;
+  const int dpad = (int)(ceil_div(d, KC) * KC);
+  Buf<uint16_t> Xh(pool, npad * dpad), Xl;
+  Buf<float> nrm(pool, npad), sqn(pool, npad), colscale(pool, npad);
+  Buf<unsigned> flags(pool, 4);
+  Buf<int32_t> cidx(pool, n * C), frows(pool, n);
+  Buf<float> ckey(pool, n * C);
+  Buf<int> fcount(pool, 1);
+  EVOC_HIP(hipMemsetAsync(flags.p, 0, 4 * sizeof(unsigned), s));
+  EVOC_HIP(hipMemsetAsync(fcount.p, 0, sizeof(int), s));
+  // decide precision: exact bf16 inputs need one MFMA per product, others the split form
+  bool split;
+  if (precision == KNN_BF16) {
+    split = false;
+  } else {
+    Xl.alloc(pool, npad * dpad);
+    split = true;
+  }
+  hipEvent_t e0, e1;
+  EVOC_HIP(hipEventCreate(&e0));
+  EVOC_HIP(hipEventCreate(&e1));
+  knn_prep_kernel<<<(unsigned)ceil_div(npad, 4), 256, 0, s>>>(dX, n, d, ld, npad, dpad, Xh.p,
+                                                               Xl.p, nrm.p, sqn.p, colscale.p,
+                                                               metric, flags.p);
+  EVOC_HIP(hipGetLastError());
+  unsigned hflags[2];
+  EVOC_HIP(hipMemcpyAsync(hflags, flags.p, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipStreamSynchronize(s));
+  if (split && precision == KNN_AUTO && hflags[0] == 0) split = false;  // input is bf16-exact
+  const float u = 5.9604645e-08f;
+  const float e_in = split ? 4.0f * 1.5258789e-05f : (hflags[0] ? 2.0f * 0.0078125f : 0.0f);
+  const float gamma = 2.0f * (float)dpad * u + e_in;
+  const int nops = split ? 4 : 2;
+  const size_t smem = 2 * nops * BM * LDA * sizeof(uint16_t) + BM * SCS * sizeof(float);
+  EVOC_HIP(hipEventRecord(e0, s));
+  const unsigned grid = (unsigned)(npad / BM);
+  static bool attr_set = false;
+  if (!attr_set) {
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<true, 16>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<true, 32>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<false, 16>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<false, 32>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_exact_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
+    attr_set = true;
+  }
+  if (split) {
+    if (CH == 16)
+      knn_topc_kernel<true, 16><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
+                                                        metric, cidx.p, ckey.p);
+    else
+      knn_topc_kernel<true, 32><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
+                                                        metric, cidx.p, ckey.p);
+  } else {
+    if (CH == 16)
+      knn_topc_kernel<false, 16><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad, colscale.p,
+                                                         metric, cidx.p, ckey.p);
+    else
+      knn_topc_kernel<false, 32><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad, colscale.p,
+                                                         metric, cidx.p, ckey.p);
+  }
+  EVOC_HIP(hipGetLastError());
+  EVOC_HIP(hipEventRecord(e1, s));
+  knn_rerank_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(
+      dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, cidx.p, ckey.p, C, k, metric, gamma, d_idx,
+      d_dist, fcount.p, frows.p);
+  EVOC_HIP(hipGetLastError());
+  int nfail = 0;
+  EVOC_HIP(hipMemcpyAsync(&nfail, fcount.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipStreamSynchronize(s));
+  if (nfail > 0) {
+    const unsigned g2 = (unsigned)std::min(nfail, 4096);
+    knn_exact_kernel<<<g2, 256, 256 * KMAX * 8, s>>>(dX, n, d, ld, nrm.p, k, metric, fcount.p, frows.p, d_idx,
+                                        d_dist);
+    EVOC_HIP(hipGetLastError());
+  }
+  float mfma_ms = 0.0f;
+  EVOC_HIP(hipEventElapsedTime(&mfma_ms, e0, e1));
+  EVOC_HIP(hipEventDestroy(e0));
+  EVOC_HIP(hipEventDestroy(e1));
+  if (stats) {
+    stats->split = split ? 1 : 0;
+    stats->fallback_rows = nfail;
+    stats->candidates = C;
+    stats->mfma_ms = mfma_ms;
+    stats->flops = 2.0 * (double)npad * (double)npad * (double)dpad * (split ? 3.0 : 1.0);
+  }
+}
+
+}  // namespace evoc

@@ -1,0 +1,155 @@
+/* evoc_hip — MI355X (gfx950) implementation of evoc-rs's clustering pipeline.
+ *
+ * C ABI: every function is extern "C", never throws, returns an int status
+ * (EVOC_OK = 0; otherwise evoc_hip_last_error() holds a thread-local message).
+ * Caller buffers are borrowed for the duration of a call and never retained.
+ * Host-pointer entry points accept ordinary host memory; evoc_hip_run_device takes
+ * the input matrix already resident in device memory. Calls are serialised inside
+ * the library (one device, one stream per device).
+ *
+ * The types mirror the reference's public interface:
+ *   evoc_params_t          <- EvocParams<T>            (src/lib.rs:52-99)
+ *   evoc_hip_nn_params_t   <- NearestNeighbourParamsEvoc.dist_metric
+ *                             (src/nearest_neighbours/nearest_neighbour_cpu.rs:43-181)
+ *                             plus the device knobs this implementation adds
+ *   evoc_result_t          <- EvocResult<T>            (src/lib.rs:106-161)
+ */
+#ifndef EVOC_HIP_H
+#define EVOC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  EVOC_OK = 0,
+  EVOC_ERR_INVALID_ARGUMENT = 1,
+  EVOC_ERR_HIP = 2,
+  EVOC_ERR_NO_DEVICE = 3,
+  EVOC_ERR_INTERNAL = 4
+};
+
+enum { EVOC_METRIC_EUCLIDEAN = 0, EVOC_METRIC_COSINE = 1 };
+/* kNN arithmetic: AUTO picks one bf16 MFMA per product when every input value is
+ * bf16-representable, else the three-product hi/lo split; both are followed by the
+ * exact f32 re-rank, so the returned neighbours are identical in every mode. */
+enum { EVOC_KNN_AUTO = 0, EVOC_KNN_SPLIT = 1, EVOC_KNN_BF16 = 2 };
+
+/* Mirrors EvocParams (src/lib.rs:52-80); defaults src/lib.rs:83-99. */
+typedef struct {
+  int32_t n_neighbours;            /* 15 */
+  float noise_level;               /* 0.5 */
+  int32_t n_epochs;                /* 50 */
+  int32_t embedding_dim;           /* 0 = None -> clamp(n_neighbours / 4, 4, 16) */
+  float neighbour_scale;           /* 1.0 */
+  int32_t symmetrise;              /* 1 */
+  int32_t min_samples;             /* 5 */
+  int32_t base_min_cluster_size;   /* 5 */
+  int32_t approx_n_clusters;       /* 0 = None */
+  double min_similarity_threshold; /* 0.2 */
+  int32_t max_layers;              /* 10 */
+} evoc_params_t;
+
+/* Nearest-neighbour and device settings (dist_metric of NearestNeighbourParamsEvoc,
+ * src/nearest_neighbours/nearest_neighbour_cpu.rs:45, plus this library's knobs). */
+typedef struct {
+  int32_t metric;          /* EVOC_METRIC_* */
+  int32_t knn_precision;   /* EVOC_KNN_* */
+  int32_t warm_start;      /* 0 = label-propagation init (plain), 1 = learned MLP init */
+  int32_t warm_epochs;     /* embedding epochs of the warm-started run; 0 = 30% of n_epochs */
+  int32_t mlp_subsample;   /* points in the MLP training subsample; 0 = 50000 */
+  int32_t device_id;       /* HIP device ordinal */
+} evoc_hip_nn_params_t;
+
+/* Mirrors EvocResult (src/lib.rs:106-118). Owned by the library; release with
+ * evoc_hip_result_free. Layers are sorted finest (most clusters) first. */
+typedef struct {
+  int64_t n;
+  int32_t k;
+  int32_t n_layers;
+  int32_t dim;
+  int32_t best_layer;              /* EvocResult::best_labels index (src/lib.rs:124-137) */
+  int64_t* cluster_layers;         /* n_layers * n, -1 = noise */
+  float* membership_strengths;     /* n_layers * n */
+  double* persistence_scores;      /* n_layers */
+  int32_t* nn_indices;             /* n * k, excluding self */
+  float* nn_distances;             /* n * k */
+  float* embedding;                /* n * dim, the node embedding clustered */
+  double stage_ms[8];              /* knn, fuzzy, init, embed, mst, layers, total, upload */
+  int64_t knn_fallback_rows;       /* rows re-searched exactly after a failed certificate */
+} evoc_result_t;
+
+void evoc_hip_default_params(evoc_params_t* p);
+void evoc_hip_default_nn_params(evoc_hip_nn_params_t* p);
+
+/* evoc() (src/lib.rs:199-352) on a host matrix with arbitrary strides (elements).
+ * pre_idx / pre_dist: PreComputedKnn (src/prelude.rs:85), n * pre_k each, or NULL. */
+int evoc_hip_run(const float* data, int64_t n, int64_t d, int64_t row_stride,
+                 int64_t col_stride, const int32_t* pre_idx, const float* pre_dist,
+                 int32_t pre_k, const evoc_params_t* params, const evoc_hip_nn_params_t* nn,
+                 uint64_t seed, int32_t verbose, evoc_result_t** out);
+/* Same, with data a row-major n x d device pointer (leading dimension ld). */
+int evoc_hip_run_device(const float* d_data, int64_t n, int64_t d, int64_t ld,
+                        const evoc_params_t* params, const evoc_hip_nn_params_t* nn,
+                        uint64_t seed, int32_t verbose, evoc_result_t** out);
+void evoc_hip_result_free(evoc_result_t* r);
+const char* evoc_hip_last_error(void);
+int evoc_hip_device_count(int32_t* count);
+/* Frees cached device memory held between calls. */
+int evoc_hip_release(void);
+
+/* ---- stage entry points (host pointers in and out), each one reference function ---- */
+
+/* Exhaustive arm of run_ann_search (src/nearest_neighbours/nearest_neighbour_cpu.rs:
+ * 225-364) with the j != i self filter (nearest_neighbour_gpu.rs:291-303): top-k per
+ * row by (distance, index). Euclidean distance = sqrt of squared L2; cosine = 1-cos. */
+int evoc_hip_knn(const float* x, int64_t n, int64_t d, int32_t metric, int32_t k,
+                 int32_t precision, int32_t* idx_out, float* dist_out, int64_t* fallback_rows,
+                 double* kernel_ms);
+
+/* build_fuzzy_simplicial_set (src/graph/fuzzy_graph.rs:275-295) as CSR. indices and
+ * weights need room for 2*n*k entries; *nnz receives the stored count. */
+int evoc_hip_fuzzy(const int32_t* knn_idx, const float* knn_dist, int64_t n, int32_t k,
+                   float effective_k, int32_t symmetrise, float* sigma, float* rho,
+                   int64_t* indptr, int32_t* indices, float* weights, int64_t* nnz);
+
+/* evoc_embedding (src/graph/embedding.rs:48-298) on a CSR graph. init may be NULL
+ * (random U(-0.25, 0.25) layout). */
+int evoc_hip_embed(const int64_t* indptr, const int32_t* indices, const float* weights,
+                   int64_t n, int32_t dim, int32_t n_epochs, float noise_level,
+                   float negative_sample_rate, float initial_alpha, float beta1, float beta2,
+                   float eps, const float* init, uint64_t seed, float* out);
+
+/* label_propagation_init (src/graph/label_prop.rs:537-564) on a column-sorted CSR.
+ * data (n x d, row-major) may be NULL. */
+int evoc_hip_label_prop_init(const int64_t* indptr, const int32_t* indices,
+                             const float* weights, int64_t n, int32_t dim, const float* data,
+                             int64_t d, uint64_t seed, float* out);
+
+/* Squared core distances (src/clustering/mst.rs:92-105). */
+int evoc_hip_core_sq(const float* emb, int64_t n, int32_t dim, int32_t min_samples,
+                     float* core_sq);
+
+/* build_mst (src/clustering/mst.rs:82-166): n-1 edges in (w, u, v) order, u < v,
+ * weights = sqrt of the squared mutual-reachability distance. */
+int evoc_hip_mst(const float* emb, int64_t n, int32_t dim, int32_t min_samples, int32_t* u,
+                 int32_t* v, float* w);
+
+/* build_cluster_layers (src/clustering/persistence.rs:387-463). labels/strengths
+ * need max_layers * n entries, persistence max_layers; *n_layers receives the count. */
+int evoc_hip_cluster_layers(const float* emb, int64_t n, int32_t dim, int32_t min_samples,
+                            int64_t base_min_cluster_size, double min_similarity_threshold,
+                            int64_t max_layers, int64_t* labels, float* strengths,
+                            double* persistence, int64_t* n_layers);
+
+/* search_for_n_clusters (src/lib.rs:377-447). */
+int evoc_hip_search_n_clusters(const float* emb, int64_t n, int32_t dim, int32_t min_samples,
+                               int64_t target_k, int64_t* labels, float* strengths);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EVOC_HIP_H */
