@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""End-to-end evoc() throughput on one MI355X.
+
+Workload (the precision form the headline metric is quoted on): 1,000,000 x 512
+bf16-valued unit vectors -- 200 centres uniform on the sphere, Zipf(1.1) cluster sizes,
+point = centre + N(0, 0.15^2 I) then L2-normalised, plus 5% uniform-on-sphere noise --
+cosine kNN with k = 15, embedding dim 8, default EvocParams. The matrix is generated
+on the device (seeded) and resident in HBM before the timed region; one "step" is one
+full evoc() pass (kNN -> fuzzy set -> initial layout -> embedding -> MST -> layers).
+
+Prints ONE JSON line: points/sec over the timed steps, the dominant kernel's roofline
+(the BF16-MFMA kNN tile kernel, timed with HIP events on its own stream), and a CPU
+baseline (the C++ oracle on the host cores, on a bounded sample, scaled to the job).
+
+    python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--n 1000000] [--warm-start]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(REPO, "evoc-rs_amd"), os.path.join(REPO, "tests"),
+                os.path.join(REPO, "oracle")]
+
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+HBM_PEAK_GBS = 8000.0
+
+
+def make_workload(n, d, n_centres=200, zipf=1.1, sd=0.15, noise_frac=0.05, seed=42):
+    """Seeded synthetic matrix generated directly in device memory (torch is only the
+    allocator here). Values are rounded to bf16-representable float32."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    c = torch.randn(n_centres, d, device="cuda", generator=g)
+    c = c / c.norm(dim=1, keepdim=True)
+    n_noise = int(round(noise_frac * n))
+    n_cl = n - n_noise
+    p = 1.0 / torch.arange(1, n_centres + 1, device="cuda", dtype=torch.float64) ** zipf
+    lab = torch.multinomial(p / p.sum(), n_cl, replacement=True, generator=g)
+    x = torch.empty(n, d, device="cuda", dtype=torch.float32)
+    chunk = 1 << 18
+    for s in range(0, n_cl, chunk):
+        e = min(n_cl, s + chunk)
+        pts = c[lab[s:e]] + sd * torch.randn(e - s, d, device="cuda", generator=g)
+        x[s:e] = pts / pts.norm(dim=1, keepdim=True)
+    if n_noise:
+        z = torch.randn(n_noise, d, device="cuda", generator=g)
+        x[n_cl:] = z / z.norm(dim=1, keepdim=True)
+    perm = torch.randperm(n, device="cuda", generator=g)
+    x = x[perm].to(torch.bfloat16).to(torch.float32).contiguous()
+    labels = torch.cat([lab, torch.full((n_noise,), -1, device="cuda", dtype=lab.dtype)])[perm]
+    torch.cuda.synchronize()
+    return x, labels
+
+
+def read_traffic():
+    """HBM bytes per launch of the kNN tile kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, KiB per dispatch, scaled by the copy-kernel
+    calibration factor recorded next to it), or None when no summary exists."""
+    path = os.path.join(REPO, "profiles", "knn_topc_pmc_1Mx512.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(x_host, k, dim, knn_rows=32, sub=20000, seed=42):
+    """The oracle (C++ restatement of evoc-rs on the host cores) on a bounded sample:
+    exact kNN of `knn_rows` query rows against all n rows (scaled by n / knn_rows),
+    plus the remaining stages on a `sub`-point subsample given its exact kNN (scaled
+    linearly by n / sub, which under-states the CPU's superlinear stages)."""
+    import numpy as np
+    import evoc_hip as H
+    import oracle as O
+    n = x_host.shape[0]
+    rng = np.random.default_rng(seed)
+    rows = np.sort(rng.choice(n, size=knn_rows, replace=False))
+    t0 = time.perf_counter()
+    O.knn_rows(x_host, rows, k, metric=1)
+    t_knn = (time.perf_counter() - t0) * n / knn_rows
+    si = np.sort(rng.choice(n, size=min(sub, n), replace=False))
+    xs = np.ascontiguousarray(x_host[si])
+    idx, dist = H.knn(xs, k, metric="cosine")
+    p = O.params(n_neighbours=k, embedding_dim=dim, metric=1)
+    t0 = time.perf_counter()
+    O.evoc(xs, p, seed=seed, pre_idx=idx, pre_dist=dist)
+    t_rest = (time.perf_counter() - t0) * n / len(si)
+    total = t_knn + t_rest
+    return {
+        "value": n / total,
+        "unit": "points/s",
+        "cores": int(O.lib().orc_num_threads()),
+        "kind": "port",
+        "sample": (f"exact cosine kNN of {knn_rows} query rows x {n} rows scaled by n/{knn_rows} "
+                   f"({t_knn:.0f} s est.); fuzzy+init+embed+cluster on a {len(si)}-point subsample "
+                   f"scaled by n/{len(si)} ({t_rest:.0f} s est.)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--d", type=int, default=512)
+    ap.add_argument("--warm-start", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--seed", type=int, default=42)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import evoc_hip as H
+    n, d, k, dim = args.n, args.d, 15, 8
+    print(f"# generating {n}x{d} workload on the device", file=sys.stderr, flush=True)
+    x, _ = make_workload(n, d, seed=args.seed)
+    params = H.EvocParams(n_neighbours=k, embedding_dim=dim)
+    nnp = H.NearestNeighbourParams(dist_metric="cosine", knn_precision="auto",
+                                   warm_start=args.warm_start)
+    ptr = x.data_ptr()
+    for w in range(args.warmup):
+        r = H.evoc_device(ptr, n, d, params=params, nn_params=nnp, seed=args.seed)
+        print(f"# warmup {w}: {r.stage_ms}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    results = []
+    for s in range(args.steps):
+        r = H.evoc_device(ptr, n, d, params=params, nn_params=nnp, seed=args.seed)
+        results.append(r)
+        print(f"# step {s}: {r.stage_ms}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = n * args.steps / elapsed
+    last = results[-1]
+    knn_ms = sum(r.knn_kernel_ms for r in results) / len(results)
+    flops = 2.0 * n * n * d
+    achieved = flops / (knn_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 4),
+                "traffic": read_traffic(), "kernel": "knn_topc_kernel",
+                "kernel_ms": round(knn_ms, 3)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        xh = x.cpu().numpy()
+        cpu = cpu_baseline(xh, k, dim)
+    line = {
+        "metric": "end-to-end evoc() points/sec at 1M x 512 (bf16-valued, cosine, k=15, dim 8)",
+        "value": round(value, 1),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (seeded on-device generator)",
+        "config": {"workload": f"{n}x{d} bf16-valued unit-norm mixture, 200 Zipf(1.1) clusters "
+                               f"+5% noise, cosine, k={k}, dim {dim}, "
+                               f"{'learned warm start' if args.warm_start else 'label-prop init'}",
+                   "n": n, "d": d, "k": k, "embedding_dim": dim,
+                   "stage_ms": {kk: round(v, 2) for kk, v in last.stage_ms.items()},
+                   "knn_fallback_rows": int(last.knn_fallback_rows),
+                   "n_layers": int(len(last.cluster_layers)),
+                   "n_clusters_best": int(last.n_clusters())},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,234 @@
+// EVoC node embedding (evoc_embedding, src/graph/embedding.rs:48-298) on gfx950.
+//
+// One thread per node and epoch, reading the pre-epoch layout (Jacobi update, as the
+// reference's parallel gradient pass does) and writing the next layout into a second
+// buffer. The per-edge schedule (next sample / next negative epoch) is stored once per
+// CSR slot: the two slots of an edge start equal and are advanced by the same f32
+// operations by their two owners, so they stay identical without a shared write and
+// without a separate schedule pass (embedding.rs:275-286). Negative samples are drawn
+// from Philox keyed by (seed, node, epoch, draw). Adam moments live in HBM next to the
+// layout. All arithmetic is f32 in the reference's operation order.
+#include "kernels.h"
+
+namespace evoc {
+namespace {
+
+constexpr int MAXD = 16;
+
+__global__ void max_weight_kernel(const float* __restrict__ w, int64_t nnz,
+                                  unsigned* __restrict__ out) {
+  unsigned m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz;
+       i += (int64_t)gridDim.x * blockDim.x)
+    m = max(m, __float_as_uint(fmaxf(w[i], 0.0f)));
+  // wave max then one atomic per wave
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+__global__ void schedule_init_kernel(const float* __restrict__ w, int64_t nnz,
+                                     const unsigned* __restrict__ maxw_bits, float neg_factor,
+                                     float* __restrict__ eps_s, float* __restrict__ next_s,
+                                     float* __restrict__ next_n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnz) return;
+  const float maxw = __uint_as_float(*maxw_bits);
+  const float norm = w[i] / maxw;
+  const float e = norm > 0.0f ? 1.0f / norm : 1e8f;
+  eps_s[i] = e;
+  next_s[i] = e;
+  next_n[i] = e * neg_factor;
+}
+
+__global__ void random_init_kernel(float* __restrict__ e, int64_t n, int dim, uint64_t seed) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * dim) return;
+  const int64_t i = q / dim;
+  const int c = (int)(q - i * dim);
+  e[q] = (float)(-0.25 + 0.5 * Philox::unit(seed, (uint32_t)i, (uint32_t)c, 0, TAG_EINI));
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void epoch_kernel(
+    int64_t n, int dim, const int64_t* __restrict__ indptr, const int32_t* __restrict__ other,
+    float* __restrict__ next_s, const float* __restrict__ eps_s, float* __restrict__ next_n,
+    float neg_factor, const float* __restrict__ e_old, float* __restrict__ e_new,
+    float* __restrict__ m, float* __restrict__ v, float epoch_t, uint32_t epoch, float lr_ad,
+    float epsc, float gamma, float noise, float one_m_b1, float one_m_b2, uint64_t seed) {
+  constexpr int DD = D > 0 ? D : MAXD;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int nd = D > 0 ? D : dim;
+  float ei[DD], g[DD];
+#pragma unroll
+  for (int c = 0; c < DD; ++c) {
+    ei[c] = c < nd ? e_old[i * nd + c] : 0.0f;
+    g[c] = 0.0f;
+  }
+  bool touched = false;
+  uint32_t draw = 0;
+  const int64_t s0 = indptr[i], s1 = indptr[i + 1];
+  for (int64_t s = s0; s < s1; ++s) {
+    const float ns = next_s[s];
+    if (ns > epoch_t) continue;
+    touched = true;
+    const int32_t j = other[s];
+    float eo[DD];
+#pragma unroll
+    for (int c = 0; c < DD; ++c) eo[c] = c < nd ? e_old[(int64_t)j * nd + c] : 0.0f;
+    float dsq = 0.0f;
+#pragma unroll
+    for (int c = 0; c < DD; ++c)
+      if (c < nd) {
+        const float df = ei[c] - eo[c];
+        const float p = df * df;
+        dsq = dsq + p;
+      }
+    if (dsq >= 1e-8f) {
+      const float dist = sqrtf(dsq);
+      const float num = -((2.0f * noise) * dist) - 2.0f;
+      const float den = ((2.0f * dsq) - (0.5f * dist)) + 1.0f;
+      const float coeff = num / den;
+#pragma unroll
+      for (int c = 0; c < DD; ++c)
+        if (c < nd) g[c] = g[c] + coeff * (ei[c] - eo[c]);
+    }
+    const float ps = eps_s[s];
+    const float pn = ps * neg_factor;
+    const float nn = next_n[s];
+    const float f = floorf((epoch_t - nn) / pn);
+    const uint64_t cnt = (f >= 0.0f && f < 1.8e19f) ? (uint64_t)f : 0;
+    if ((int64_t)j > i) {  // smaller endpoint draws the negatives
+      for (uint64_t q = 0; q < cnt; ++q) {
+        const uint32_t r = Philox::first(seed, draw++, epoch, (uint32_t)i, TAG_NEG);
+        const int64_t kk = Philox::range(r, (uint64_t)n);
+        if (kk == i) continue;
+        float ek[DD];
+#pragma unroll
+        for (int c = 0; c < DD; ++c) ek[c] = c < nd ? e_old[kk * nd + c] : 0.0f;
+        float dk = 0.0f;
+#pragma unroll
+        for (int c = 0; c < DD; ++c)
+          if (c < nd) {
+            const float df = ei[c] - ek[c];
+            const float p = df * df;
+            dk = dk + p;
+          }
+        if (dk > 0.01f) {
+          const float gc = (gamma * 4.0f) / ((1.0f + 0.25f * dk) * dk);
+#pragma unroll
+          for (int c = 0; c < DD; ++c)
+            if (c < nd) {
+              const float diff = ei[c] - ek[c];
+              g[c] = g[c] + fminf(fmaxf(gc * diff, -4.0f), 4.0f);
+            }
+        }
+      }
+    }
+    next_s[s] = ns + ps;
+    next_n[s] = nn + (float)cnt * pn;
+  }
+  if (touched) {
+#pragma unroll
+    for (int c = 0; c < DD; ++c)
+      if (c < nd) {
+        const int64_t q = i * nd + c;
+        const float gd = g[c];
+        float mq = m[q], vq = v[q];
+        mq = mq + one_m_b1 * (gd - mq);
+        vq = vq + one_m_b2 * (gd * gd - vq);
+        m[q] = mq;
+        v[q] = vq;
+        const float step = (lr_ad * mq) / (sqrtf(vq) + epsc);
+        e_new[q] = ei[c] + step;
+      }
+  } else {
+#pragma unroll
+    for (int c = 0; c < DD; ++c)
+      if (c < nd) e_new[i * nd + c] = ei[c];
+  }
+}
+
+template <int D>
+void launch_epoch(unsigned grid, hipStream_t s, int64_t n, int dim, const int64_t* indptr,
+                  const int32_t* other, float* ns, const float* es, float* nn, float nf,
+                  const float* eo, float* en, float* m, float* v, float et, uint32_t ep, float la,
+                  float epsc, float gamma, float noise, float b1, float b2, uint64_t seed) {
+  epoch_kernel<D><<<grid, 256, 0, s>>>(n, dim, indptr, other, ns, es, nn, nf, eo, en, m, v, et, ep,
+                                       la, epsc, gamma, noise, b1, b2, seed);
+}
+
+}  // namespace
+
+void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, bool random_init,
+                  uint64_t seed, hipStream_t s, DevPool& pool, float* kernel_ms) {
+  const int64_t n = g.n;
+  EVOC_CHECK(dim >= 1 && dim <= MAXD, "embedding: dim must be in [1, 16]");
+  if (n == 0) return;
+  const unsigned gn = (unsigned)ceil_div(n * dim, 256);
+  if (random_init) random_init_kernel<<<gn, 256, 0, s>>>(d_emb, n, dim, seed);
+  if (g.nnz == 0 || p.n_epochs <= 0) return;
+  const int64_t nnz = g.nnz;
+  Buf<unsigned> maxw(pool, 1);
+  Buf<float> eps_s(pool, nnz), next_s(pool, nnz), next_n(pool, nnz);
+  Buf<float> m(pool, n * dim), v(pool, n * dim), e2(pool, n * dim);
+  EVOC_HIP(hipMemsetAsync(maxw.p, 0, sizeof(unsigned), s));
+  EVOC_HIP(hipMemsetAsync(m.p, 0, n * dim * sizeof(float), s));
+  EVOC_HIP(hipMemsetAsync(v.p, 0, n * dim * sizeof(float), s));
+  max_weight_kernel<<<(unsigned)std::min<int64_t>(ceil_div(nnz, 256), 4096), 256, 0, s>>>(
+      g.weights.p, nnz, maxw.p);
+  const float neg_factor = 1.5f / p.negative_sample_rate;
+  schedule_init_kernel<<<(unsigned)ceil_div(nnz, 256), 256, 0, s>>>(
+      g.weights.p, nnz, maxw.p, neg_factor, eps_s.p, next_s.p, next_n.p);
+  unsigned hmax = 0;
+  EVOC_HIP(hipMemcpyAsync(&hmax, maxw.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipStreamSynchronize(s));
+  if (hmax == 0) return;  // every weight is zero: epochs_per_sample = 1/(0/0) -> reference NaN path skipped
+  const float one_m_b1 = 1.0f - p.beta1, one_m_b2 = 1.0f - p.beta2;
+  const float denom = (float)std::max(p.n_epochs - 1, 1);
+  const float n_epochs_f = (float)p.n_epochs;
+  const unsigned grid = (unsigned)ceil_div(n, 256);
+  hipEvent_t e0, e1;
+  EVOC_HIP(hipEventCreate(&e0));
+  EVOC_HIP(hipEventCreate(&e1));
+  EVOC_HIP(hipEventRecord(e0, s));
+  float* cur = d_emb;
+  float* nxt = e2.p;
+  for (int epoch = 0; epoch < p.n_epochs; ++epoch) {
+    const float lr = p.initial_alpha * (1.0f - (float)epoch / n_epochs_f);
+    const float epoch_t = (float)epoch;
+    const float t = (float)(epoch + 1);
+    const float b1t = std::pow(p.beta1, t), b2t = std::pow(p.beta2, t);
+    const float sqrt_b2 = std::sqrt(1.0f - b2t);
+    const float ad_scale = sqrt_b2 / (1.0f - b1t);
+    const float epsc = sqrt_b2 * p.eps;
+    const float gamma = 0.5f + (float)epoch / denom;
+    const float lr_ad = lr * ad_scale;
+#define EVOC_EPOCH(D)                                                                         \
+  launch_epoch<D>(grid, s, n, dim, g.indptr.p, g.indices.p, next_s.p, eps_s.p, next_n.p,    \
+                  neg_factor, cur, nxt, m.p, v.p, epoch_t, (uint32_t)epoch, lr_ad, epsc, gamma, \
+                  p.noise_level, one_m_b1, one_m_b2, seed)
+    switch (dim) {
+      case 2: EVOC_EPOCH(2); break;
+      case 4: EVOC_EPOCH(4); break;
+      case 8: EVOC_EPOCH(8); break;
+      case 16: EVOC_EPOCH(16); break;
+      default: EVOC_EPOCH(0); break;
+    }
+#undef EVOC_EPOCH
+    std::swap(cur, nxt);
+  }
+  EVOC_HIP(hipGetLastError());
+  EVOC_HIP(hipEventRecord(e1, s));
+  if (cur != d_emb)
+    EVOC_HIP(hipMemcpyAsync(d_emb, cur, n * dim * sizeof(float), hipMemcpyDeviceToDevice, s));
+  EVOC_HIP(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  EVOC_HIP(hipEventElapsedTime(&ms, e0, e1));
+  if (kernel_ms) *kernel_ms = ms;
+  EVOC_HIP(hipEventDestroy(e0));
+  EVOC_HIP(hipEventDestroy(e1));
+}
+
+}  // namespace evoc

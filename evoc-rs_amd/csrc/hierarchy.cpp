@@ -1,0 +1,377 @@
+// Cluster hierarchy on the host: single linkage, condensed tree, leaves, labels,
+// membership strengths, min-cluster-size barcodes, layer selection and the
+// approx_n_clusters search. Each step is O(n) (or O(n log n)); the persistence bin
+// sums run in fp64 on the GPU (persistence.hip).
+//
+//  linkage      src/clustering/linkage.rs:323-367   (union-find relabel, no parent walk)
+//  condense     src/clustering/condensed_tree.rs:127-248 (explicit-stack right-first DFS)
+//  leaves       src/clustering/condensed_tree.rs:264-294 (flag arrays)
+//  labels       src/clustering/condensed_tree.rs:311-369 (nearest selected ancestor, the
+//               representative the reference's union-find ends with)
+//  strengths    src/clustering/condensed_tree.rs:384-484
+//  barcodes     src/clustering/persistence.rs:42-118 (fp64)
+//  persistence  src/clustering/persistence.rs:135-173 (fp64, barcode order per bin)
+//  layers       src/clustering/persistence.rs:188-463
+//  search       src/lib.rs:377-447
+#include <numeric>
+#include <unordered_map>
+
+#include "host_stages.h"
+
+namespace evoc {
+
+Linkage linkage_from_sorted(const int32_t* u, const int32_t* v, const float* w, int64_t ne,
+                            int64_t n) {
+  Linkage L;
+  L.n = n;
+  L.left.resize(ne);
+  L.right.resize(ne);
+  L.size.resize(ne);
+  L.dist.resize(ne);
+  std::vector<int64_t> par(n), lab(n), sz(n + ne, 1), rk(n, 0);
+  std::iota(par.begin(), par.end(), 0);
+  std::iota(lab.begin(), lab.end(), 0);
+  auto find = [&](int64_t x) {
+    while (par[x] != x) {
+      par[x] = par[par[x]];
+      x = par[x];
+    }
+    return x;
+  };
+  for (int64_t i = 0; i < ne; ++i) {
+    int64_t ru = find(u[i]), rv = find(v[i]);
+    int64_t l = lab[ru], r = lab[rv];
+    const int64_t ns = sz[l] + sz[r];
+    if (l < r) std::swap(l, r);
+    L.left[i] = l;
+    L.right[i] = r;
+    L.dist[i] = w[i];
+    L.size[i] = ns;
+    sz[n + i] = ns;
+    if (rk[ru] < rk[rv]) std::swap(ru, rv);
+    par[rv] = ru;
+    if (rk[ru] == rk[rv]) ++rk[ru];
+    lab[ru] = n + i;
+  }
+  return L;
+}
+
+CTree condense(const Linkage& L, int64_t mcs) {
+  CTree t;
+  const int64_t n = L.n, nr = (int64_t)L.left.size();
+  t.n = n;
+  if (nr == 0) return t;
+  const int64_t root = 2 * nr;
+  std::vector<int64_t> relabel(root + 1, 0);
+  relabel[root] = n;
+  int64_t next = n + 1;
+  auto emit = [&](int64_t p, int64_t c, float lam, int64_t s) {
+    t.parent.push_back(p);
+    t.child.push_back(c);
+    t.lambda.push_back(lam);
+    t.size.push_back(s);
+  };
+  std::vector<int64_t> st, est;
+  auto eliminate = [&](int64_t x, int64_t pl, float lam) {
+    est.assign(1, x);
+    while (!est.empty()) {
+      const int64_t y = est.back();
+      est.pop_back();
+      if (y < n) {
+        emit(pl, y, lam, 1);
+      } else {
+        est.push_back(L.left[y - n]);
+        est.push_back(L.right[y - n]);
+      }
+    }
+  };
+  st.push_back(root);
+  while (!st.empty()) {
+    const int64_t node = st.back();
+    st.pop_back();
+    if (node < n) continue;
+    const int64_t pl = relabel[node];
+    const int64_t l = L.left[node - n], r = L.right[node - n];
+    const float d = L.dist[node - n];
+    const float lam = d > 0.0f ? 1.0f / d : INFINITY;
+    const int64_t ls = l >= n ? L.size[l - n] : 1, rs = r >= n ? L.size[r - n] : 1;
+    const bool lb = ls >= mcs, rb = rs >= mcs;
+    if (lb && rb) {
+      relabel[l] = next;
+      emit(pl, next++, lam, ls);
+      relabel[r] = next;
+      emit(pl, next++, lam, rs);
+      st.push_back(l);
+      st.push_back(r);
+    } else if (lb) {
+      relabel[l] = pl;
+      eliminate(r, pl, lam);
+      st.push_back(l);
+    } else if (rb) {
+      relabel[r] = pl;
+      eliminate(l, pl, lam);
+      st.push_back(r);
+    } else {
+      eliminate(l, pl, lam);
+      eliminate(r, pl, lam);
+    }
+  }
+  return t;
+}
+
+std::vector<int64_t> leaves(const CTree& t) {
+  std::vector<int64_t> out;
+  if (t.parent.empty()) return out;
+  int64_t max_id = 0;
+  for (size_t i = 0; i < t.parent.size(); ++i) max_id = std::max({max_id, t.parent[i], t.child[i]});
+  std::vector<char> is_parent(max_id + 1, 0), appears(max_id + 1, 0);
+  for (size_t i = 0; i < t.parent.size(); ++i) {
+    if (t.size[i] > 1) is_parent[t.parent[i]] = 1;
+    appears[t.parent[i]] = 1;
+  }
+  for (int64_t c = t.n; c <= max_id; ++c)
+    if (!is_parent[c] && appears[c]) out.push_back(c);
+  return out;
+}
+
+void labels_and_strengths(const CTree& t, const std::vector<int64_t>& clusters, int64_t* labels,
+                          float* strengths) {
+  const int64_t n = t.n;
+  std::fill(labels, labels + n, -1);
+  if (strengths) std::fill(strengths, strengths + n, 0.0f);
+  if (clusters.empty() || t.parent.empty()) return;
+  int64_t max_id = 0, root = INT64_MAX;
+  for (size_t i = 0; i < t.parent.size(); ++i) {
+    max_id = std::max({max_id, t.parent[i], t.child[i]});
+    root = std::min(root, t.parent[i]);
+  }
+  std::vector<int64_t> sorted(clusters);
+  std::sort(sorted.begin(), sorted.end());
+  const int64_t span = max_id + 1;
+  std::vector<int64_t> rank(span, -1);
+  for (size_t q = 0; q < sorted.size(); ++q)
+    if (sorted[q] >= 0 && sorted[q] < span) rank[sorted[q]] = (int64_t)q;
+  // nearest selected ancestor of every cluster id (parents precede children)
+  std::vector<int64_t> cpar(span, -1);
+  for (size_t i = 0; i < t.parent.size(); ++i)
+    if (t.child[i] >= n) cpar[t.child[i]] = t.parent[i];
+  std::vector<int64_t> sel(span, -1);
+  for (int64_t c = n; c < span; ++c) {
+    if (rank[c] >= 0) sel[c] = c;
+    else if (cpar[c] >= n) sel[c] = sel[cpar[c]];
+  }
+  std::vector<int64_t> ppar(n, -1);
+  for (size_t i = 0; i < t.parent.size(); ++i)
+    if (t.child[i] < n) ppar[t.child[i]] = t.parent[i];
+  for (int64_t p = 0; p < n; ++p) {
+    if (ppar[p] < 0) continue;
+    const int64_t s = sel[ppar[p]];
+    labels[p] = (s > root) ? rank[s] : -1;
+  }
+  if (!strengths) return;
+  std::vector<float> death(span, 0.0f);
+  for (size_t i = 0; i < t.parent.size(); ++i)
+    if (t.size[i] == 1 && rank[t.parent[i]] >= 0 && t.lambda[i] > death[t.parent[i]])
+      death[t.parent[i]] = t.lambda[i];
+  for (size_t i = 0; i < t.parent.size(); ++i) {
+    const int64_t pt = t.child[i];
+    if (pt >= root || pt >= n) continue;
+    const int64_t lb = labels[pt];
+    if (lb < 0) continue;
+    const float ml = death[sorted[lb]];
+    const float lam = t.lambda[i];
+    if (ml == 0.0f || !std::isfinite(lam)) strengths[pt] = 1.0f;
+    else strengths[pt] = (lam < ml ? lam : ml) / ml;
+  }
+}
+
+namespace {
+
+struct Bar {
+  std::vector<double> birth, death, lam;
+};
+
+Bar barcodes(const CTree& t, int64_t min_size) {
+  Bar b;
+  std::vector<size_t> cn;
+  for (size_t i = 0; i < t.parent.size(); ++i)
+    if (t.size[i] > 1) cn.push_back(i);
+  if (cn.empty()) return b;
+  int64_t max_child = 0;
+  for (auto i : cn) max_child = std::max(max_child, t.child[i]);
+  const int64_t nn = max_child - t.n + 1;
+  b.birth.assign(nn, (double)min_size);
+  b.death.assign(nn, 0.0);
+  b.lam.assign(nn, 0.0);
+  b.death[0] = (double)t.n;
+  int64_t i = (int64_t)cn.size();
+  while (i >= 2) {
+    i -= 2;
+    const size_t a = cn[i], c = cn[i + 1];
+    const int64_t ia = t.child[a] - t.n, ib = t.child[c] - t.n;
+    const double lv = std::exp(-1.0 / (double)t.lambda[a]);
+    b.lam[ia] = lv;
+    b.lam[ib] = lv;
+    const double dt = (double)std::min(t.size[a], t.size[c]);
+    b.death[ia] = dt;
+    b.death[ib] = dt;
+    const int64_t pi = t.parent[a] - t.n;
+    const double mb = std::max(std::max(b.birth[ia], b.birth[ib]), dt);
+    b.birth[pi] = std::max(b.birth[pi], mb);
+  }
+  return b;
+}
+
+void total_persistence(const Bar& b, std::vector<double>& sizes, std::vector<double>& pers,
+                       hipStream_t s, DevPool& pool) {
+  sizes.assign(b.birth.begin(), b.birth.end());
+  std::sort(sizes.begin(), sizes.end());
+  sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+  std::vector<int64_t> bp, dp;
+  std::vector<double> contrib;
+  for (size_t i = 1; i < b.birth.size(); ++i) {
+    if (b.death[i] <= b.birth[i]) continue;
+    bp.push_back(std::lower_bound(sizes.begin(), sizes.end(), b.birth[i]) - sizes.begin());
+    dp.push_back(std::lower_bound(sizes.begin(), sizes.end(), b.death[i]) - sizes.begin());
+    contrib.push_back((b.death[i] - b.birth[i]) * b.lam[i]);
+  }
+  pers.assign(sizes.size(), 0.0);
+  persistence_bins_device(bp.data(), dp.data(), contrib.data(), (int64_t)bp.size(),
+                          (int64_t)sizes.size(), pers.data(), s, pool);
+}
+
+std::vector<int64_t> find_peaks(const std::vector<double>& x) {
+  std::vector<int64_t> pk;
+  if (x.size() < 3) return pk;
+  size_t i = 1;
+  const size_t imax = x.size() - 1;
+  while (i < imax) {
+    if (x[i - 1] < x[i]) {
+      size_t ia = i + 1;
+      while (ia < imax && x[ia] == x[i]) ++ia;
+      if (x[ia] < x[i]) {
+        pk.push_back((int64_t)((i + ia - 1) / 2));
+        i = ia;
+      }
+    }
+    ++i;
+  }
+  return pk;
+}
+
+double jaccard(const Bar& b, double x, double y) {
+  int64_t inter = 0, uni = 0;
+  for (size_t i = 0; i < b.birth.size(); ++i) {
+    const bool ia = b.birth[i] <= x && b.death[i] > x;
+    const bool ib = b.birth[i] <= y && b.death[i] > y;
+    inter += ia && ib;
+    uni += ia || ib;
+  }
+  return uni == 0 ? 0.0 : (double)inter / (double)uni;
+}
+
+int64_t n_clusters(const std::vector<int64_t>& l) {
+  int64_t m = -1;
+  for (auto x : l)
+    if (x >= 0) m = std::max(m, x);
+  return m + 1;
+}
+
+}  // namespace
+
+Layers cluster_layers_from_linkage(const Linkage& L, int64_t base_mcs, double min_sim,
+                                   int64_t max_layers, hipStream_t s, DevPool& pool) {
+  const int64_t n = L.n;
+  Layers out;
+  const CTree t = condense(L, base_mcs);
+  std::vector<std::vector<int64_t>> labs(1, std::vector<int64_t>(n));
+  std::vector<std::vector<float>> strs(1, std::vector<float>(n));
+  std::vector<double> pers(1, 0.0);
+  labels_and_strengths(t, leaves(t), labs[0].data(), strs[0].data());
+  const Bar b = barcodes(t, base_mcs);
+  if (!b.birth.empty()) {
+    std::vector<double> sizes, curve;
+    total_persistence(b, sizes, curve, s, pool);
+    const std::vector<int64_t> pk = find_peaks(curve);
+    std::vector<std::pair<int64_t, double>> idx;
+    for (auto p : pk) idx.push_back({p, curve[p]});
+    std::stable_sort(idx.begin(), idx.end(), [](const std::pair<int64_t, double>& a,
+                                                const std::pair<int64_t, double>& c) {
+      return c.second < a.second;
+    });
+    const int64_t cap = max_layers > 0 ? max_layers - 1 : 0;
+    std::vector<double> chosen;
+    for (auto& pp : idx) {
+      if ((int64_t)chosen.size() >= cap) break;
+      const double bs = sizes[pp.first];
+      bool diverse = true;
+      for (double c : chosen)
+        if (!(jaccard(b, bs, c) <= min_sim)) {
+          diverse = false;
+          break;
+        }
+      if (!diverse) continue;
+      chosen.push_back(bs);
+      std::vector<int64_t> act;
+      for (size_t i = 0; i < b.birth.size(); ++i)
+        if (b.birth[i] <= bs && b.death[i] > bs) act.push_back((int64_t)i + n);
+      labs.emplace_back(n);
+      strs.emplace_back(n);
+      labels_and_strengths(t, act, labs.back().data(), strs.back().data());
+      pers.push_back(curve[pp.first]);
+    }
+  }
+  std::vector<int64_t> order(labs.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::vector<int64_t> nc(labs.size());
+  for (size_t i = 0; i < labs.size(); ++i) nc[i] = n_clusters(labs[i]);
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t c) { return nc[c] < nc[a]; });
+  for (auto o : order) {
+    out.labels.push_back(std::move(labs[o]));
+    out.strengths.push_back(std::move(strs[o]));
+    out.persistence.push_back(pers[o]);
+  }
+  return out;
+}
+
+void search_n_clusters_from_linkage(const Linkage& L, int64_t target_k, int64_t* labels,
+                                    float* strengths) {
+  const int64_t n = L.n;
+  int64_t lo = 2, hi = n / 2;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) / 2;
+    if (mid == lo || mid == hi) break;
+    const int64_t mk = (int64_t)leaves(condense(L, mid)).size();
+    if (mk < target_k) hi = mid;
+    else lo = mid;
+  }
+  const CTree tl = condense(L, lo), th = condense(L, hi);
+  const std::vector<int64_t> vl = leaves(tl), vh = leaves(th);
+  std::vector<int64_t> ll(n), lh(n);
+  std::vector<float> sl(n), sh(n);
+  labels_and_strengths(tl, vl, ll.data(), sl.data());
+  labels_and_strengths(th, vh, lh.data(), sh.data());
+  const int64_t dl = std::llabs((int64_t)vl.size() - target_k);
+  const int64_t dh = std::llabs((int64_t)vh.size() - target_k);
+  bool use_lo;
+  if (dl < dh) use_lo = true;
+  else if (dh < dl) use_lo = false;
+  else {
+    int64_t al = 0, ah = 0;
+    for (auto x : ll) al += x >= 0;
+    for (auto x : lh) ah += x >= 0;
+    use_lo = al >= ah;
+  }
+  std::copy((use_lo ? ll : lh).begin(), (use_lo ? ll : lh).end(), labels);
+  std::copy((use_lo ? sl : sh).begin(), (use_lo ? sl : sh).end(), strengths);
+}
+
+int best_layer_index(const std::vector<double>& p) {
+  if (p.size() <= 1) return 0;
+  int best = 0;
+  for (size_t i = 1; i < p.size(); ++i)
+    if (!(p[i] < p[best])) best = (int)i;  // Iterator::max_by keeps the last maximum
+  return best;
+}
+
+}  // namespace evoc

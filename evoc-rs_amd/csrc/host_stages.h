@@ -1,0 +1,46 @@
+// Host-side stages of the pipeline (sequential tree logic and the label-propagation
+// initial layout) and their device helpers.
+#pragma once
+#include <vector>
+
+#include "kernels.h"
+
+namespace evoc {
+
+void label_prop_init_host(const int64_t* indptr, const int32_t* indices, const float* weights,
+                          int64_t n, int dim, const float* data, int64_t d, uint64_t seed,
+                          float* out, hipStream_t s, DevPool& pool, int verbose);
+
+// scipy-style single-linkage rows (src/clustering/linkage.rs:301-310)
+struct Linkage {
+  std::vector<int64_t> left, right, size;
+  std::vector<float> dist;
+  int64_t n = 0;
+};
+// edges must already be in (w, u, v) order
+Linkage linkage_from_sorted(const int32_t* u, const int32_t* v, const float* w, int64_t ne,
+                            int64_t n);
+
+// condensed tree rows (src/clustering/condensed_tree.rs:19-37)
+struct CTree {
+  std::vector<int64_t> parent, child, size;
+  std::vector<float> lambda;
+  int64_t n = 0;
+};
+CTree condense(const Linkage& L, int64_t mcs);
+std::vector<int64_t> leaves(const CTree& t);
+void labels_and_strengths(const CTree& t, const std::vector<int64_t>& clusters, int64_t* labels,
+                          float* strengths);
+
+struct Layers {
+  std::vector<std::vector<int64_t>> labels;
+  std::vector<std::vector<float>> strengths;
+  std::vector<double> persistence;
+};
+Layers cluster_layers_from_linkage(const Linkage& L, int64_t base_mcs, double min_sim,
+                                   int64_t max_layers, hipStream_t s, DevPool& pool);
+void search_n_clusters_from_linkage(const Linkage& L, int64_t target_k, int64_t* labels,
+                                    float* strengths);
+int best_layer_index(const std::vector<double>& persistence);
+
+}  // namespace evoc

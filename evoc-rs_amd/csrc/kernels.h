@@ -1,6 +1,7 @@
 // Internal (C++) interface between the host orchestration and the HIP kernels.
 #pragma once
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -143,6 +144,11 @@ void mst_device(const float* d_emb, int64_t n, int dim, int min_samples, int32_t
 void persistence_bins_device(const int64_t* h_bpos, const int64_t* h_dpos, const double* h_contrib,
                              int64_t nbc, int64_t nbins, double* h_out, hipStream_t s,
                              DevPool& pool);
+
+// ------------------------------------------------------------------ sort
+// stable LSD radix sort of (key, value) pairs on the low key_bits bits of key
+void radix_sort_pairs(uint64_t* d_key, uint32_t* d_val, int64_t n, int key_bits, hipStream_t s,
+                      DevPool& pool);
 
 // ------------------------------------------------------------------ scans
 void exclusive_scan_i64(const int64_t* d_in, int64_t* d_out, int64_t n, hipStream_t s,

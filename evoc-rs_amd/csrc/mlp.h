@@ -1,0 +1,21 @@
+// Learned warm start: a small MLP (input -> 256 -> 256 -> embedding dim, GELU) that
+// predicts the initial layout from the input vectors.
+#pragma once
+#include "../../include/evoc_hip.h"
+#include "kernels.h"
+
+namespace evoc {
+
+struct MlpWarmStart {
+  int64_t subsample = 50000;
+  int train_epochs = 0;
+  double train_ms = 0.0;
+  double final_loss = 0.0;
+};
+
+void warm_start_init(const float* dX, int64_t n, int d, int64_t ld, const Graph& g,
+                     const evoc_params_t& p, const evoc_hip_nn_params_t& nn, int dim,
+                     uint64_t seed, MlpWarmStart& ws, float* d_init, hipStream_t s,
+                     DevPool& pool, int verbose);
+
+}  // namespace evoc

@@ -1,0 +1,485 @@
+// Core distances and the mutual-reachability MST (build_mst, src/clustering/mst.rs:82-166)
+// on gfx950, without the KD-tree (src/clustering/kd_tree.rs): the embedding is ordered
+// along a Morton curve, cut into 64-point tiles with bounding boxes, and every query
+// wavefront (one tile of 64 queries) walks the tiles outward from its own position,
+// skipping a tile when every lane's point-to-box distance already exceeds its bound
+// (the AABB rule of kd_tree.rs:330-339) or when the tile lies wholly in the query
+// tile's component (same-component pruning, kd_tree.rs:336-339).
+//
+// Ties: core distances are order-free; MST edges are chosen under the strict total
+// order (w^2, min(u,v), max(u,v)) on original point ids, so the tree is the unique
+// minimum spanning tree for that order. Per Boruvka round each component's minimum
+// outgoing edge is reduced with two integer atomicMin passes (weight bits, then ids).
+#include <cstring>
+#include <numeric>
+
+#include "kernels.h"
+
+namespace evoc {
+namespace {
+
+constexpr int TS = 64;  // points per tile (one wavefront of queries)
+constexpr int MAXD = 16;
+constexpr int KCORE = 32;
+
+__device__ __forceinline__ unsigned ord_f32(float f) {  // order-preserving map to uint
+  unsigned b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float unord_f32(unsigned u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void bbox_kernel(const float* __restrict__ e, int64_t n, int dim,
+                            unsigned* __restrict__ lo, unsigned* __restrict__ hi) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < dim; ++c) {
+    float x = e[i * dim + c];
+    if (!(x == x)) x = 0.0f;
+    atomicMin(&lo[c], ord_f32(x));
+    atomicMax(&hi[c], ord_f32(x));
+  }
+}
+
+__global__ void morton_kernel(const float* __restrict__ e, int64_t n, int dim,
+                              const unsigned* __restrict__ lo, const unsigned* __restrict__ hi,
+                              int bits, uint64_t* __restrict__ code, uint32_t* __restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t q[MAXD];
+  const float maxq = (float)((1u << bits) - 1u);
+#pragma unroll
+  for (int c = 0; c < MAXD; ++c) {
+    q[c] = 0;
+    if (c < dim) {
+      const float l = unord_f32(lo[c]), h = unord_f32(hi[c]);
+      float x = e[i * dim + c];
+      if (!(x == x)) x = 0.0f;
+      const float r = h > l ? (x - l) / (h - l) : 0.0f;
+      q[c] = (uint32_t)fminf(fmaxf(r * maxq, 0.0f), maxq);
+    }
+  }
+  uint64_t m = 0;
+  for (int b = bits - 1; b >= 0; --b)
+#pragma unroll
+    for (int c = 0; c < MAXD; ++c)
+      if (c < dim) m = (m << 1) | ((q[c] >> b) & 1u);
+  code[i] = m;
+  idx[i] = (uint32_t)i;
+}
+
+__global__ void gather_kernel(const float* __restrict__ e, const uint32_t* __restrict__ perm,
+                              int64_t n, int dim, float* __restrict__ es) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * dim) return;
+  const int64_t p = q / dim;
+  const int c = (int)(q - p * dim);
+  es[q] = e[(int64_t)perm[p] * dim + c];
+}
+
+// one wavefront per tile: bounding box (lo, hi) per dimension
+__global__ void tile_box_kernel(const float* __restrict__ es, int64_t n, int dim, int64_t ntiles,
+                                float* __restrict__ blo, float* __restrict__ bhi) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t t = (int64_t)blockIdx.x * 4 + wave;
+  if (t >= ntiles) return;
+  const int64_t p = t * TS + lane;
+  for (int c = 0; c < dim; ++c) {
+    float l = p < n ? es[p * dim + c] : __builtin_huge_valf();
+    float h = p < n ? es[p * dim + c] : -__builtin_huge_valf();
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      l = fminf(l, __shfl_xor(l, o, 64));
+      h = fmaxf(h, __shfl_xor(h, o, 64));
+    }
+    if (lane == 0) {
+      blo[t * dim + c] = l;
+      bhi[t * dim + c] = h;
+    }
+  }
+}
+
+template <int D>
+__device__ __forceinline__ float box_dist2(const float (&q)[MAXD], const float* __restrict__ lo,
+                                           const float* __restrict__ hi, int dim) {
+  float r = 0.0f;
+#pragma unroll
+  for (int c = 0; c < MAXD; ++c)
+    if (c < (D > 0 ? D : dim)) {
+      const float l = lo[c], h = hi[c], x = q[c];
+      const float df = x < l ? l - x : (x > h ? x - h : 0.0f);
+      r = r + df * df;
+    }
+  return r;
+}
+
+template <int D>
+__device__ __forceinline__ float dsq_seq(const float (&q)[MAXD], const float* __restrict__ b,
+                                         int dim) {
+  float s = 0.0f;
+#pragma unroll
+  for (int c = 0; c < MAXD; ++c)
+    if (c < (D > 0 ? D : dim)) {
+      const float df = q[c] - b[c];
+      const float p = df * df;
+      s = s + p;
+    }
+  return s;
+}
+
+__device__ __forceinline__ int64_t spiral_tile(int64_t qt, int64_t s) {
+  const int64_t off = (s + 1) >> 1;
+  return (s & 1) ? qt + off : qt - off;
+}
+
+// ------------------------------------------------------------------ core distances
+template <int D>
+__global__ __launch_bounds__(64) void core_kernel(const float* __restrict__ es, int64_t n,
+                                                  int dim, int64_t ntiles,
+                                                  const float* __restrict__ blo,
+                                                  const float* __restrict__ bhi, int k,
+                                                  float* __restrict__ core) {
+  __shared__ float tp[TS * MAXD];
+  const int lane = threadIdx.x;
+  const int64_t qt = blockIdx.x;
+  const int64_t p = qt * TS + lane;
+  const bool valid = p < n;
+  const int nd = D > 0 ? D : dim;
+  float q[MAXD];
+#pragma unroll
+  for (int c = 0; c < MAXD; ++c) q[c] = (valid && c < nd) ? es[p * nd + c] : 0.0f;
+  // ascending list of the k smallest squared distances; kth = best[k-1]
+  float best[KCORE];
+#pragma unroll
+  for (int j = 0; j < KCORE; ++j) best[j] = __builtin_huge_valf();
+  float kth = __builtin_huge_valf();
+  for (int64_t s = 0; s < 2 * ntiles; ++s) {
+    const int64_t t = spiral_tile(qt, s);
+    if (t < 0 || t >= ntiles) continue;
+    const bool prune = !valid || box_dist2<D>(q, blo + t * nd, bhi + t * nd, nd) > kth;
+    if (__all(prune)) continue;
+    const int64_t pt = t * TS + lane;
+    __syncthreads();
+    for (int c = 0; c < nd; ++c) tp[lane * nd + c] = pt < n ? es[pt * nd + c] : 0.0f;
+    __syncthreads();
+    const int cnt = (int)min<int64_t>(TS, n - t * TS);
+    if (valid) {
+      for (int j = 0; j < cnt; ++j) {
+        if (t * TS + j == p) continue;
+        float x = dsq_seq<D>(q, tp + j * nd, nd);
+        if (x < kth) {
+#pragma unroll
+          for (int m = 0; m < KCORE; ++m)
+            if (m < k) {
+              const float lo = fminf(x, best[m]);
+              x = fmaxf(x, best[m]);
+              best[m] = lo;
+            }
+#pragma unroll
+          for (int m = 0; m < KCORE; ++m)
+            if (m == k - 1) kth = best[m];
+        }
+      }
+    }
+  }
+  if (valid) core[p] = kth;
+}
+
+// ------------------------------------------------------------------ Boruvka search
+struct Best {
+  float w;
+  int32_t a, b;  // original ids, a < b
+};
+__device__ __forceinline__ bool better(float w, int32_t a, int32_t b, const Best& o) {
+  if (w != o.w) return w < o.w;
+  if (a != o.a) return a < o.a;
+  return b < o.b;
+}
+
+template <int D>
+__global__ __launch_bounds__(64) void boruvka_search_kernel(
+    const float* __restrict__ es, const float* __restrict__ core, const int32_t* __restrict__ comp,
+    const uint32_t* __restrict__ perm, int64_t n, int dim, int64_t ntiles,
+    const float* __restrict__ blo, const float* __restrict__ bhi,
+    const int32_t* __restrict__ tile_comp, unsigned* __restrict__ comp_w,
+    float* __restrict__ best_w, int32_t* __restrict__ best_a, int32_t* __restrict__ best_b) {
+  __shared__ float tp[TS * MAXD];
+  __shared__ float tc[TS];
+  __shared__ int32_t tcomp[TS];
+  __shared__ int32_t tid_[TS];
+  const int lane = threadIdx.x;
+  const int64_t qt = blockIdx.x;
+  const int64_t p = qt * TS + lane;
+  const bool valid = p < n;
+  const int nd = D > 0 ? D : dim;
+  float q[MAXD];
+#pragma unroll
+  for (int c = 0; c < MAXD; ++c) q[c] = (valid && c < nd) ? es[p * nd + c] : 0.0f;
+  const float cq = valid ? core[p] : 0.0f;
+  const int32_t mycomp = valid ? comp[p] : -1;
+  const int32_t myid = valid ? (int32_t)perm[p] : 0;
+  const int32_t qtc = tile_comp[qt];
+  Best bst{__builtin_huge_valf(), INT32_MAX, INT32_MAX};
+  for (int64_t s = 0; s < 2 * ntiles; ++s) {
+    const int64_t t = spiral_tile(qt, s);
+    if (t < 0 || t >= ntiles) continue;
+    const int32_t ttc = tile_comp[t];
+    if (qtc >= 0 && ttc == qtc) continue;
+    float bound = bst.w;
+    if (valid) bound = fminf(bound, __uint_as_float(comp_w[mycomp]));
+    const bool prune = !valid || cq > bound ||
+                       box_dist2<D>(q, blo + t * nd, bhi + t * nd, nd) > bound ||
+                       (ttc >= 0 && ttc == mycomp);
+    if (__all(prune)) continue;
+    const int64_t pt = t * TS + lane;
+    __syncthreads();
+    for (int c = 0; c < nd; ++c) tp[lane * nd + c] = pt < n ? es[pt * nd + c] : 0.0f;
+    tc[lane] = pt < n ? core[pt] : __builtin_huge_valf();
+    tcomp[lane] = pt < n ? comp[pt] : mycomp;
+    tid_[lane] = pt < n ? (int32_t)perm[pt] : 0;
+    __syncthreads();
+    if (!prune) {
+      const int cnt = (int)min<int64_t>(TS, n - t * TS);
+      for (int j = 0; j < cnt; ++j) {
+        if (tcomp[j] == mycomp) continue;
+        const float cj = tc[j];
+        if (cj > bound) continue;
+        const float dd = dsq_seq<D>(q, tp + j * nd, nd);
+        const float mr = fmaxf(fmaxf(dd, cq), cj);
+        const int32_t oj = tid_[j];
+        const int32_t a = min(myid, oj), b = max(myid, oj);
+        if (better(mr, a, b, bst)) {
+          bst = Best{mr, a, b};
+          bound = fminf(bound, mr);
+        }
+      }
+      if (bst.w < __builtin_huge_valf()) atomicMin(&comp_w[mycomp], __float_as_uint(bst.w));
+    }
+  }
+  if (valid) {
+    best_w[p] = bst.w;
+    best_a[p] = bst.a;
+    best_b[p] = bst.b;
+  }
+}
+
+__global__ void boruvka_select_kernel(int64_t n, const int32_t* __restrict__ comp,
+                                      const unsigned* __restrict__ comp_w,
+                                      const float* __restrict__ best_w,
+                                      const int32_t* __restrict__ best_a,
+                                      const int32_t* __restrict__ best_b,
+                                      unsigned long long* __restrict__ comp_key) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float w = best_w[p];
+  if (!(w < __builtin_huge_valf())) return;
+  const int32_t c = comp[p];
+  if (__float_as_uint(w) != comp_w[c]) return;
+  const unsigned long long key = ((unsigned long long)(uint32_t)best_a[p] << 32) | (uint32_t)best_b[p];
+  atomicMin(&comp_key[c], key);
+}
+
+__global__ void tile_comp_kernel(const int32_t* __restrict__ comp, int64_t n, int64_t ntiles,
+                                 int32_t* __restrict__ tile_comp) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t t = (int64_t)blockIdx.x * 4 + wave;
+  if (t >= ntiles) return;
+  const int64_t p = t * TS + lane;
+  const int32_t c = p < n ? comp[p] : comp[t * TS];
+  const int32_t c0 = __shfl(c, 0, 64);
+  const bool same = __all(c == c0);
+  if (lane == 0) tile_comp[t] = same ? c0 : -1;
+}
+
+struct Sorted {
+  Buf<float> es;
+  Buf<uint32_t> perm;
+  Buf<float> blo, bhi;
+  int64_t ntiles = 0;
+};
+
+void morton_sort(const float* d_emb, int64_t n, int dim, Sorted& S, hipStream_t s,
+                 DevPool& pool) {
+  Buf<unsigned> lo(pool, dim), hi(pool, dim);
+  std::vector<unsigned> init_lo(dim, 0xffffffffu), init_hi(dim, 0u);
+  EVOC_HIP(hipMemcpyAsync(lo.p, init_lo.data(), dim * sizeof(unsigned), hipMemcpyHostToDevice, s));
+  EVOC_HIP(hipMemcpyAsync(hi.p, init_hi.data(), dim * sizeof(unsigned), hipMemcpyHostToDevice, s));
+  const unsigned g = (unsigned)ceil_div(n, 256);
+  bbox_kernel<<<g, 256, 0, s>>>(d_emb, n, dim, lo.p, hi.p);
+  Buf<uint64_t> code(pool, n);
+  S.perm.alloc(pool, n);
+  const int bits = std::min(21, 64 / dim);
+  morton_kernel<<<g, 256, 0, s>>>(d_emb, n, dim, lo.p, hi.p, bits, code.p, S.perm.p);
+  radix_sort_pairs(code.p, S.perm.p, n, bits * dim, s, pool);
+  S.es.alloc(pool, n * dim);
+  gather_kernel<<<(unsigned)ceil_div(n * dim, 256), 256, 0, s>>>(d_emb, S.perm.p, n, dim, S.es.p);
+  S.ntiles = ceil_div(n, TS);
+  S.blo.alloc(pool, S.ntiles * dim);
+  S.bhi.alloc(pool, S.ntiles * dim);
+  tile_box_kernel<<<(unsigned)ceil_div(S.ntiles, 4), 256, 0, s>>>(S.es.p, n, dim, S.ntiles,
+                                                                  S.blo.p, S.bhi.p);
+  EVOC_HIP(hipGetLastError());
+}
+
+void core_sorted(const Sorted& S, int64_t n, int dim, int min_samples, float* d_core_sorted,
+                 hipStream_t s) {
+  const int k = (int)std::min<int64_t>(min_samples, n - 1);
+  EVOC_CHECK(k <= KCORE, "min_samples must be <= 32");
+  if (k <= 0) {
+    EVOC_HIP(hipMemsetAsync(d_core_sorted, 0, n * sizeof(float), s));
+    return;
+  }
+  const unsigned g = (unsigned)S.ntiles;
+  if (dim == 8)
+    core_kernel<8><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.blo.p, S.bhi.p, k, d_core_sorted);
+  else
+    core_kernel<0><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.blo.p, S.bhi.p, k, d_core_sorted);
+  EVOC_HIP(hipGetLastError());
+}
+
+__global__ void unpermute_kernel(const float* __restrict__ src, const uint32_t* __restrict__ perm,
+                                 int64_t n, float* __restrict__ dst) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) dst[perm[p]] = src[p];
+}
+
+struct HostDsu {
+  std::vector<int32_t> p, r;
+  explicit HostDsu(int64_t n) : p(n), r(n, 0) { std::iota(p.begin(), p.end(), 0); }
+  int32_t find(int32_t x) {
+    while (p[x] != x) {
+      p[x] = p[p[x]];
+      x = p[x];
+    }
+    return x;
+  }
+  bool unite(int32_t a, int32_t b) {
+    a = find(a);
+    b = find(b);
+    if (a == b) return false;
+    if (r[a] < r[b]) std::swap(a, b);
+    p[b] = a;
+    if (r[a] == r[b]) ++r[a];
+    return true;
+  }
+};
+
+}  // namespace
+
+void core_sq_device(const float* d_emb, int64_t n, int dim, int min_samples, float* d_core,
+                    hipStream_t s, DevPool& pool) {
+  EVOC_CHECK(dim >= 1 && dim <= MAXD, "core distances: dim must be in [1, 16]");
+  if (n <= 0) return;
+  if (n == 1) {
+    EVOC_HIP(hipMemsetAsync(d_core, 0, sizeof(float), s));
+    return;
+  }
+  Sorted S;
+  morton_sort(d_emb, n, dim, S, s, pool);
+  Buf<float> cs(pool, n);
+  core_sorted(S, n, dim, min_samples, cs.p, s);
+  unpermute_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(cs.p, S.perm.p, n, d_core);
+  EVOC_HIP(hipGetLastError());
+}
+
+void mst_device(const float* d_emb, int64_t n, int dim, int min_samples, int32_t* h_u,
+                int32_t* h_v, float* h_w, hipStream_t s, DevPool& pool, int* rounds) {
+  EVOC_CHECK(dim >= 1 && dim <= MAXD, "mst: dim must be in [1, 16]");
+  if (rounds) *rounds = 0;
+  if (n <= 1) return;
+  Sorted S;
+  morton_sort(d_emb, n, dim, S, s, pool);
+  Buf<float> core(pool, n);
+  core_sorted(S, n, dim, min_samples, core.p, s);
+  std::vector<uint32_t> perm(n);
+  EVOC_HIP(hipMemcpyAsync(perm.data(), S.perm.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  Buf<int32_t> comp(pool, n), tile_comp(pool, S.ntiles), ba(pool, n), bb(pool, n);
+  Buf<float> bw(pool, n);
+  Buf<unsigned> comp_w(pool, n);
+  Buf<unsigned long long> comp_key(pool, n);
+  std::vector<int32_t> hcomp(n);
+  for (int64_t p = 0; p < n; ++p) hcomp[p] = (int32_t)p;  // every point its own component
+  EVOC_HIP(hipMemcpyAsync(comp.p, hcomp.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  EVOC_HIP(hipStreamSynchronize(s));
+  HostDsu dsu(n);
+  struct E { float w2; int32_t a, b; };
+  std::vector<E> edges;
+  edges.reserve(n - 1);
+  int64_t ncomp = n;
+  std::vector<unsigned> hw;
+  std::vector<unsigned long long> hk;
+  std::vector<int32_t> root_to_comp(n, -1);
+  int r = 0;
+  while (ncomp > 1) {
+    ++r;
+    tile_comp_kernel<<<(unsigned)ceil_div(S.ntiles, 4), 256, 0, s>>>(comp.p, n, S.ntiles,
+                                                                     tile_comp.p);
+    EVOC_HIP(hipMemsetAsync(comp_w.p, 0x7f, ncomp * sizeof(unsigned), s));  // 0x7f7f7f7f > any finite w
+    EVOC_HIP(hipMemsetAsync(comp_key.p, 0xff, ncomp * sizeof(unsigned long long), s));
+    const unsigned g = (unsigned)S.ntiles;
+    if (dim == 8)
+      boruvka_search_kernel<8><<<g, 64, 0, s>>>(S.es.p, core.p, comp.p, S.perm.p, n, dim,
+                                                S.ntiles, S.blo.p, S.bhi.p, tile_comp.p, comp_w.p,
+                                                bw.p, ba.p, bb.p);
+    else
+      boruvka_search_kernel<0><<<g, 64, 0, s>>>(S.es.p, core.p, comp.p, S.perm.p, n, dim,
+                                                S.ntiles, S.blo.p, S.bhi.p, tile_comp.p, comp_w.p,
+                                                bw.p, ba.p, bb.p);
+    boruvka_select_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(n, comp.p, comp_w.p, bw.p,
+                                                                     ba.p, bb.p, comp_key.p);
+    EVOC_HIP(hipGetLastError());
+    hw.resize(ncomp);
+    hk.resize(ncomp);
+    EVOC_HIP(hipMemcpyAsync(hw.data(), comp_w.p, ncomp * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipMemcpyAsync(hk.data(), comp_key.p, ncomp * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipStreamSynchronize(s));
+    bool merged = false;
+    for (int64_t c = 0; c < ncomp; ++c) {
+      if (hk[c] == ~0ull) continue;
+      const int32_t a = (int32_t)(hk[c] >> 32), b = (int32_t)(hk[c] & 0xffffffffu);
+      if (dsu.unite(a, b)) {
+        float w2;
+        std::memcpy(&w2, &hw[c], sizeof(float));
+        edges.push_back({w2, a, b});
+        merged = true;
+      }
+    }
+    if (!merged) break;
+    // relabel components compactly in sorted-position order
+    int64_t nc = 0;
+    std::fill(root_to_comp.begin(), root_to_comp.end(), -1);
+    for (int64_t p = 0; p < n; ++p) {
+      const int32_t rt = dsu.find((int32_t)perm[p]);
+      if (root_to_comp[rt] < 0) root_to_comp[rt] = (int32_t)nc++;
+      hcomp[p] = root_to_comp[rt];
+    }
+    ncomp = nc;
+    if (ncomp <= 1) break;
+    EVOC_HIP(hipMemcpyAsync(comp.p, hcomp.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  }
+  if (rounds) *rounds = r;
+  // weights are sqrt'd; order by (w, u, v) for the linkage
+  std::vector<int64_t> ord(edges.size());
+  std::vector<float> ws(edges.size());
+  for (size_t i = 0; i < edges.size(); ++i) ws[i] = std::sqrt(edges[i].w2);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) {
+    if (ws[x] != ws[y]) return ws[x] < ws[y];
+    if (edges[x].a != edges[y].a) return edges[x].a < edges[y].a;
+    return edges[x].b < edges[y].b;
+  });
+  for (size_t i = 0; i < ord.size(); ++i) {
+    h_u[i] = edges[ord[i]].a;
+    h_v[i] = edges[ord[i]].b;
+    h_w[i] = ws[ord[i]];
+  }
+  for (int64_t i = (int64_t)ord.size(); i < n - 1; ++i) {
+    h_u[i] = -1;
+    h_v[i] = -1;
+    h_w[i] = __builtin_huge_valf();
+  }
+}
+
+}  // namespace evoc

@@ -1,0 +1,530 @@
+// Pipeline orchestration (evoc(), src/lib.rs:199-352) and the stage entry points.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <unordered_set>
+
+#include "api_internal.h"
+#include "host_stages.h"
+#include "mlp.h"
+
+namespace evoc {
+namespace {
+
+using clk = std::chrono::steady_clock;
+double ms_since(clk::time_point t) {
+  return std::chrono::duration<double, std::milli>(clk::now() - t).count();
+}
+
+template <typename T>
+void to_host(std::vector<T>& h, const T* d, size_t n, hipStream_t s) {
+  h.resize(n);
+  if (n) EVOC_HIP(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+template <typename T>
+void to_dev(Buf<T>& b, const T* h, size_t n, DevPool& pool, hipStream_t s) {
+  b.alloc(pool, n);
+  if (n) EVOC_HIP(hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+// Embedding slots for an arbitrary adjacency (evoc_embedding, src/graph/embedding.rs:79-135):
+// edges (i, j, w) with i < j are taken from row i in row order; each endpoint gets one slot,
+// slots of a node ordered by edge index. For a symmetric graph with column-sorted rows
+// this is the graph itself.
+void build_slots(const std::vector<int64_t>& ptr, const std::vector<int32_t>& col,
+                 const std::vector<float>& w, std::vector<int64_t>& sp, std::vector<int32_t>& so,
+                 std::vector<float>& sw) {
+  const int64_t n = (int64_t)ptr.size() - 1;
+  std::vector<int64_t> deg(n, 0);
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t p = ptr[i]; p < ptr[i + 1]; ++p)
+      if (i < col[p]) {
+        deg[i]++;
+        deg[col[p]]++;
+      }
+  sp.assign(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) sp[i + 1] = sp[i] + deg[i];
+  so.resize(sp[n]);
+  sw.resize(sp[n]);
+  std::vector<int64_t> cur(sp.begin(), sp.end() - 1);
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t p = ptr[i]; p < ptr[i + 1]; ++p) {
+      const int32_t j = col[p];
+      if (i < j) {
+        so[cur[i]] = j;
+        sw[cur[i]++] = w[p];
+        so[cur[j]] = (int32_t)i;
+        sw[cur[j]++] = w[p];
+      }
+    }
+}
+
+bool symmetric_sorted(const std::vector<int64_t>& ptr, const std::vector<int32_t>& col) {
+  const int64_t n = (int64_t)ptr.size() - 1;
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t p = ptr[i]; p < ptr[i + 1]; ++p) {
+      if (col[p] == i || col[p] < 0 || col[p] >= n) return false;
+      if (p > ptr[i] && col[p] <= col[p - 1]) return false;
+      const int32_t* b = col.data() + ptr[col[p]];
+      const int32_t* e = col.data() + ptr[col[p] + 1];
+      const int32_t* f = std::lower_bound(b, e, (int32_t)i);
+      if (f == e || *f != i) return false;
+    }
+  return true;
+}
+
+int auto_dim(const evoc_params_t& p) {
+  return p.embedding_dim > 0 ? p.embedding_dim : std::min(std::max(p.n_neighbours / 4, 4), 16);
+}
+
+void validate_params(const evoc_params_t& p) {
+  EVOC_CHECK(p.n_neighbours >= 1 && p.n_neighbours <= 32, "n_neighbours must be in [1, 32]");
+  EVOC_CHECK(p.n_epochs >= 0, "n_epochs must be >= 0");
+  EVOC_CHECK(p.embedding_dim >= 0 && p.embedding_dim <= 16, "embedding_dim must be in [0, 16]");
+  EVOC_CHECK(p.min_samples >= 0 && p.min_samples <= 32, "min_samples must be in [0, 32]");
+  EVOC_CHECK(p.base_min_cluster_size >= 0, "base_min_cluster_size must be >= 0");
+  EVOC_CHECK(p.approx_n_clusters >= 0, "approx_n_clusters must be >= 0");
+  EVOC_CHECK(p.max_layers >= 0, "max_layers must be >= 0");
+}
+
+evoc_result_t* alloc_result(int64_t n, int k, int dim, int nl) {
+  auto* r = static_cast<evoc_result_t*>(std::calloc(1, sizeof(evoc_result_t)));
+  if (!r) throw std::bad_alloc();
+  r->n = n;
+  r->k = k;
+  r->dim = dim;
+  r->n_layers = nl;
+  auto al = [](size_t bytes) {
+    void* p = std::malloc(bytes ? bytes : 1);
+    if (!p) throw std::bad_alloc();
+    return p;
+  };
+  r->cluster_layers = static_cast<int64_t*>(al((size_t)nl * n * sizeof(int64_t)));
+  r->membership_strengths = static_cast<float*>(al((size_t)nl * n * sizeof(float)));
+  r->persistence_scores = static_cast<double*>(al((size_t)nl * sizeof(double)));
+  r->nn_indices = static_cast<int32_t*>(al((size_t)n * k * sizeof(int32_t)));
+  r->nn_distances = static_cast<float*>(al((size_t)n * k * sizeof(float)));
+  r->embedding = static_cast<float*>(al((size_t)n * dim * sizeof(float)));
+  return r;
+}
+
+void free_result(evoc_result_t* r) {
+  if (!r) return;
+  std::free(r->cluster_layers);
+  std::free(r->membership_strengths);
+  std::free(r->persistence_scores);
+  std::free(r->nn_indices);
+  std::free(r->nn_distances);
+  std::free(r->embedding);
+  std::free(r);
+}
+
+// The clustering stage on a device embedding: MST -> linkage -> layers or search.
+Layers cluster_device(const float* d_emb, int64_t n, int dim, const evoc_params_t& p,
+                      hipStream_t s, DevPool& pool, double* mst_ms, double* layer_ms) {
+  Layers out;
+  auto t0 = clk::now();
+  std::vector<int32_t> u(std::max<int64_t>(n - 1, 1)), v(std::max<int64_t>(n - 1, 1));
+  std::vector<float> w(std::max<int64_t>(n - 1, 1));
+  mst_device(d_emb, n, dim, p.min_samples, u.data(), v.data(), w.data(), s, pool);
+  int64_t ne = std::max<int64_t>(n - 1, 0);
+  while (ne > 0 && u[ne - 1] < 0) --ne;  // disconnected (non-finite input): fewer edges
+  const Linkage L = linkage_from_sorted(u.data(), v.data(), w.data(), ne, n);
+  if (mst_ms) *mst_ms = ms_since(t0);
+  t0 = clk::now();
+  if (p.approx_n_clusters > 0) {
+    out.labels.assign(1, std::vector<int64_t>(n));
+    out.strengths.assign(1, std::vector<float>(n));
+    search_n_clusters_from_linkage(L, p.approx_n_clusters, out.labels[0].data(),
+                                   out.strengths[0].data());
+    out.persistence.assign(1, 0.0);
+  } else {
+    out = cluster_layers_from_linkage(L, p.base_min_cluster_size, p.min_similarity_threshold,
+                                      p.max_layers, s, pool);
+  }
+  if (layer_ms) *layer_ms = ms_since(t0);
+  return out;
+}
+
+evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, int64_t ld,
+                        const int32_t* pre_idx, const float* pre_dist, int pre_k,
+                        const evoc_params_t& p, const evoc_hip_nn_params_t& nn, uint64_t seed,
+                        int verbose, Ctx& c) {
+  validate_params(p);
+  hipStream_t s = c.stream;
+  DevPool& pool = c.pool;
+  const auto t_all = clk::now();
+  double stage[8] = {0};
+  const int k = pre_idx ? pre_k : p.n_neighbours;
+  // 1. kNN
+  auto t0 = clk::now();
+  Buf<int32_t> didx(pool, n * k);
+  Buf<float> ddist(pool, n * k);
+  int64_t fallback = 0;
+  double knn_kernel_ms = 0.0;
+  float embed_kernel_ms = 0.0f;
+  if (pre_idx) {
+    if (verbose) std::printf("Using precomputed kNN graph...\n");
+    for (int64_t i = 0; i < n; ++i) {
+      std::unordered_set<int32_t> seen;
+      for (int t = 0; t < k; ++t) {
+        const int32_t j = pre_idx[i * k + t];
+        EVOC_CHECK(j >= -1 && j < n, "precomputed kNN index out of range");
+        EVOC_CHECK(j < 0 || seen.insert(j).second, "precomputed kNN row repeats an index");
+      }
+    }
+    EVOC_HIP(hipMemcpyAsync(didx.p, pre_idx, n * k * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    EVOC_HIP(hipMemcpyAsync(ddist.p, pre_dist, n * k * sizeof(float), hipMemcpyHostToDevice, s));
+  } else {
+    if (verbose) std::printf("Running exhaustive nearest neighbour search on the GPU...\n");
+    KnnStats st;
+    knn_device(dX, n, (int)d, ld, nn.metric, k, nn.knn_precision, didx.p, ddist.p, &st, s, pool);
+    fallback = st.fallback_rows;
+    knn_kernel_ms = st.mfma_ms;
+  }
+  EVOC_HIP(hipStreamSynchronize(s));
+  stage[0] = ms_since(t0);
+  if (verbose) std::printf("kNN search done in %.2f ms.\n", stage[0]);
+  // 2. fuzzy simplicial set
+  t0 = clk::now();
+  if (verbose) std::printf("Constructing fuzzy simplicial set...\n");
+  const float eff_k = p.neighbour_scale * (float)p.n_neighbours;
+  Graph g;
+  fuzzy_device(didx.p, ddist.p, n, k, eff_k, p.symmetrise != 0, nullptr, nullptr, g, s, pool);
+  EVOC_HIP(hipStreamSynchronize(s));
+  stage[1] = ms_since(t0);
+  if (verbose) std::printf("... fuzzy simplicial set done in %.2f ms.\n", stage[1]);
+  const int dim = auto_dim(p);
+  // graph on the host: needed by the host label-propagation stage and by the
+  // non-symmetrised embedding layout
+  std::vector<int64_t> hptr;
+  std::vector<int32_t> hcol;
+  std::vector<float> hw;
+  // 3. initial layout
+  t0 = clk::now();
+  Buf<float> demb(pool, n * dim);
+  int epochs = p.n_epochs;
+  if (nn.warm_start) {
+    if (verbose) std::printf("Computing learned warm-start initialisation...\n");
+    MlpWarmStart ws;
+    ws.subsample = nn.mlp_subsample > 0 ? nn.mlp_subsample : 50000;
+    warm_start_init(dX, n, (int)d, ld, g, p, nn, dim, seed, ws, demb.p, s, pool, verbose);
+    epochs = nn.warm_epochs > 0 ? nn.warm_epochs : std::max(1, (int)(0.3 * p.n_epochs));
+  } else {
+    if (verbose) std::printf("Computing label propagation initialisation...\n");
+    to_host(hptr, g.indptr.p, n + 1, s);
+    to_host(hcol, g.indices.p, g.nnz, s);
+    to_host(hw, g.weights.p, g.nnz, s);
+    std::vector<float> xh;
+    const float* data_h = hX;
+    if (!data_h) {
+      xh.resize(n * d);
+      EVOC_HIP(hipMemcpy2DAsync(xh.data(), d * sizeof(float), dX, ld * sizeof(float),
+                                d * sizeof(float), n, hipMemcpyDeviceToHost, s));
+      data_h = xh.data();
+    }
+    EVOC_HIP(hipStreamSynchronize(s));
+    std::vector<float> init(n * dim);
+    if (p.symmetrise) {
+      label_prop_init_host(hptr.data(), hcol.data(), hw.data(), n, dim, data_h, d, seed,
+                           init.data(), s, pool, verbose);
+    } else {
+      // Csr::from_coo (src/utils/sparse.rs:195-235) orders every row by column
+      std::vector<int32_t> c2(hcol);
+      std::vector<float> w2(hw);
+      for (int64_t i = 0; i < n; ++i) {
+        std::vector<std::pair<int32_t, float>> row;
+        for (int64_t q = hptr[i]; q < hptr[i + 1]; ++q) row.push_back({hcol[q], hw[q]});
+        std::sort(row.begin(), row.end(), [](const std::pair<int32_t, float>& a,
+                                             const std::pair<int32_t, float>& b) {
+          return a.first < b.first;
+        });
+        for (size_t q = 0; q < row.size(); ++q) {
+          c2[hptr[i] + q] = row[q].first;
+          w2[hptr[i] + q] = row[q].second;
+        }
+      }
+      label_prop_init_host(hptr.data(), c2.data(), w2.data(), n, dim, data_h, d, seed,
+                           init.data(), s, pool, verbose);
+    }
+    EVOC_HIP(hipMemcpyAsync(demb.p, init.data(), n * dim * sizeof(float), hipMemcpyHostToDevice, s));
+  }
+  EVOC_HIP(hipStreamSynchronize(s));
+  stage[2] = ms_since(t0);
+  if (verbose) std::printf("Label prop init done in %.2f ms.\n", stage[2]);
+  // 4. node embedding
+  t0 = clk::now();
+  if (verbose) std::printf("Computing %d-d node embedding (%d epochs)...\n", dim, epochs);
+  EmbedParams ep;
+  ep.n_epochs = epochs;
+  ep.noise_level = p.noise_level;
+  ep.initial_alpha = 0.1f;
+  if (p.symmetrise) {
+    embed_device(g, dim, ep, demb.p, false, seed, s, pool, &embed_kernel_ms);
+  } else {
+    if (hptr.empty()) {
+      to_host(hptr, g.indptr.p, n + 1, s);
+      to_host(hcol, g.indices.p, g.nnz, s);
+      to_host(hw, g.weights.p, g.nnz, s);
+      EVOC_HIP(hipStreamSynchronize(s));
+    }
+    std::vector<int64_t> sp;
+    std::vector<int32_t> so;
+    std::vector<float> sw;
+    build_slots(hptr, hcol, hw, sp, so, sw);
+    Graph eg;
+    eg.n = n;
+    eg.nnz = sp[n];
+    to_dev(eg.indptr, sp.data(), n + 1, pool, s);
+    to_dev(eg.indices, so.data(), eg.nnz, pool, s);
+    to_dev(eg.weights, sw.data(), eg.nnz, pool, s);
+    embed_device(eg, dim, ep, demb.p, false, seed, s, pool, &embed_kernel_ms);
+  }
+  EVOC_HIP(hipStreamSynchronize(s));
+  stage[3] = ms_since(t0);
+  if (verbose) std::printf(" ... embedding done in %.2f ms.\n", stage[3]);
+  // 5./6. clustering
+  if (verbose) std::printf("Running density-based clustering...\n");
+  Layers lay = cluster_device(demb.p, n, dim, p, s, pool, &stage[4], &stage[5]);
+  const int nl = (int)lay.labels.size();
+  evoc_result_t* r = alloc_result(n, k, dim, nl);
+  try {
+    for (int l = 0; l < nl; ++l) {
+      std::memcpy(r->cluster_layers + (size_t)l * n, lay.labels[l].data(), n * sizeof(int64_t));
+      std::memcpy(r->membership_strengths + (size_t)l * n, lay.strengths[l].data(), n * sizeof(float));
+      r->persistence_scores[l] = lay.persistence[l];
+    }
+    EVOC_HIP(hipMemcpyAsync(r->nn_indices, didx.p, n * k * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipMemcpyAsync(r->nn_distances, ddist.p, n * k * sizeof(float), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipMemcpyAsync(r->embedding, demb.p, n * dim * sizeof(float), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    free_result(r);
+    throw;
+  }
+  r->best_layer = best_layer_index(lay.persistence);
+  r->knn_fallback_rows = fallback;
+  r->knn_kernel_ms = knn_kernel_ms;
+  r->embed_kernel_ms = embed_kernel_ms;
+  stage[6] = ms_since(t_all);
+  for (int i = 0; i < 8; ++i) r->stage_ms[i] = stage[i];
+  if (verbose) {
+    std::printf("Clustering done in %.2f ms: %d layer(s).\n", stage[4] + stage[5], nl);
+    std::printf("EVoC (HIP) total: %.2f ms.\n", stage[6]);
+  }
+  return r;
+}
+
+}  // namespace
+}  // namespace evoc
+
+using namespace evoc;
+
+extern "C" {
+
+int evoc_hip_run(const float* data, int64_t n, int64_t d, int64_t row_stride, int64_t col_stride,
+                 const int32_t* pre_idx, const float* pre_dist, int32_t pre_k,
+                 const evoc_params_t* params, const evoc_hip_nn_params_t* nn, uint64_t seed,
+                 int32_t verbose, evoc_result_t** out) {
+  return guarded([&] {
+    EVOC_CHECK(out, "out is NULL");
+    *out = nullptr;
+    EVOC_CHECK(data && params, "data or params is NULL");
+    EVOC_CHECK(n >= 2 && d >= 1, "need at least 2 points and 1 feature");
+    EVOC_CHECK(!pre_idx == !pre_dist, "pre_idx and pre_dist must both be given");
+    EVOC_CHECK(!pre_idx || (pre_k >= 1 && pre_k <= 32), "pre_k must be in [1, 32]");
+    evoc_hip_nn_params_t q;
+    if (nn) q = *nn;
+    else evoc_hip_default_nn_params(&q);
+    Ctx& c = context(q.device_id);
+    const auto t0 = std::chrono::steady_clock::now();
+    // pack the (possibly strided, e.g. column-major) matrix row-major
+    std::vector<float> packed;
+    const float* xr = data;
+    if (!(row_stride == d && col_stride == 1)) {
+      packed.resize(n * d);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < d; ++j) packed[i * d + j] = data[i * row_stride + j * col_stride];
+      xr = packed.data();
+    }
+    for (int64_t i = 0; i < n * d; ++i)
+      EVOC_CHECK(std::isfinite(xr[i]), "input contains a non-finite value");
+    Buf<float> dx(c.pool, n * d);
+    EVOC_HIP(hipMemcpyAsync(dx.p, xr, n * d * sizeof(float), hipMemcpyHostToDevice, c.stream));
+    EVOC_HIP(hipStreamSynchronize(c.stream));
+    const double up = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    evoc_result_t* r = run_impl(dx.p, xr, n, d, d, pre_idx, pre_dist, pre_k, *params, q, seed,
+                                verbose, c);
+    r->stage_ms[7] = up;
+    *out = r;
+  });
+}
+
+int evoc_hip_run_device(const float* d_data, int64_t n, int64_t d, int64_t ld,
+                        const evoc_params_t* params, const evoc_hip_nn_params_t* nn,
+                        uint64_t seed, int32_t verbose, evoc_result_t** out) {
+  return guarded([&] {
+    EVOC_CHECK(out, "out is NULL");
+    *out = nullptr;
+    EVOC_CHECK(d_data && params, "data or params is NULL");
+    EVOC_CHECK(n >= 2 && d >= 1 && ld >= d, "need n >= 2, d >= 1, ld >= d");
+    evoc_hip_nn_params_t q;
+    if (nn) q = *nn;
+    else evoc_hip_default_nn_params(&q);
+    Ctx& c = context(q.device_id);
+    *out = run_impl(d_data, nullptr, n, d, ld, nullptr, nullptr, 0, *params, q, seed, verbose, c);
+  });
+}
+
+void evoc_hip_result_free(evoc_result_t* r) { free_result(r); }
+
+int evoc_hip_fuzzy(const int32_t* knn_idx, const float* knn_dist, int64_t n, int32_t k,
+                   float effective_k, int32_t symmetrise, float* sigma, float* rho,
+                   int64_t* indptr, int32_t* indices, float* weights, int64_t* nnz) {
+  return guarded([&] {
+    EVOC_CHECK(knn_idx && knn_dist && indptr && indices && weights && nnz, "fuzzy: NULL buffer");
+    EVOC_CHECK(n >= 1 && k >= 1 && k <= 32, "fuzzy: need n >= 1 and k in [1, 32]");
+    Ctx& c = context(0);
+    Buf<int32_t> di;
+    Buf<float> dd, ds(c.pool, n), dr(c.pool, n);
+    to_dev(di, knn_idx, n * k, c.pool, c.stream);
+    to_dev(dd, knn_dist, n * k, c.pool, c.stream);
+    Graph g;
+    fuzzy_device(di.p, dd.p, n, k, effective_k, symmetrise != 0, ds.p, dr.p, g, c.stream, c.pool);
+    if (sigma) EVOC_HIP(hipMemcpyAsync(sigma, ds.p, n * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    if (rho) EVOC_HIP(hipMemcpyAsync(rho, dr.p, n * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    EVOC_HIP(hipMemcpyAsync(indptr, g.indptr.p, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+    if (g.nnz) {
+      EVOC_HIP(hipMemcpyAsync(indices, g.indices.p, g.nnz * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+      EVOC_HIP(hipMemcpyAsync(weights, g.weights.p, g.nnz * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    }
+    EVOC_HIP(hipStreamSynchronize(c.stream));
+    *nnz = g.nnz;
+  });
+}
+
+int evoc_hip_embed(const int64_t* indptr, const int32_t* indices, const float* weights, int64_t n,
+                   int32_t dim, int32_t n_epochs, float noise_level, float negative_sample_rate,
+                   float initial_alpha, float beta1, float beta2, float eps, const float* init,
+                   uint64_t seed, float* out) {
+  return guarded([&] {
+    EVOC_CHECK(indptr && out, "embed: NULL buffer");
+    EVOC_CHECK(n >= 0 && dim >= 1 && dim <= 16, "embed: dim must be in [1, 16]");
+    if (n == 0) return;
+    Ctx& c = context(0);
+    std::vector<int64_t> ptr(indptr, indptr + n + 1);
+    std::vector<int32_t> col(indices, indices + ptr[n]);
+    std::vector<float> w(weights, weights + ptr[n]);
+    std::vector<int64_t> sp;
+    std::vector<int32_t> so;
+    std::vector<float> sw;
+    build_slots(ptr, col, w, sp, so, sw);
+    Graph g;
+    g.n = n;
+    g.nnz = sp[n];
+    to_dev(g.indptr, sp.data(), n + 1, c.pool, c.stream);
+    to_dev(g.indices, so.data(), g.nnz, c.pool, c.stream);
+    to_dev(g.weights, sw.data(), g.nnz, c.pool, c.stream);
+    Buf<float> de(c.pool, n * dim);
+    if (init) EVOC_HIP(hipMemcpyAsync(de.p, init, n * dim * sizeof(float), hipMemcpyHostToDevice, c.stream));
+    EmbedParams ep;
+    ep.n_epochs = n_epochs;
+    ep.noise_level = noise_level;
+    ep.negative_sample_rate = negative_sample_rate;
+    ep.initial_alpha = initial_alpha;
+    ep.beta1 = beta1;
+    ep.beta2 = beta2;
+    ep.eps = eps;
+    embed_device(g, dim, ep, de.p, init == nullptr, seed, c.stream, c.pool);
+    EVOC_HIP(hipMemcpyAsync(out, de.p, n * dim * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    EVOC_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int evoc_hip_label_prop_init(const int64_t* indptr, const int32_t* indices, const float* weights,
+                             int64_t n, int32_t dim, const float* data, int64_t d, uint64_t seed,
+                             float* out) {
+  return guarded([&] {
+    EVOC_CHECK(indptr && out, "label_prop_init: NULL buffer");
+    EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "label_prop_init: dim must be in [1, 16]");
+    Ctx& c = context(0);
+    label_prop_init_host(indptr, indices, weights, n, dim, data, d, seed, out, c.stream, c.pool, 0);
+  });
+}
+
+int evoc_hip_core_sq(const float* emb, int64_t n, int32_t dim, int32_t min_samples,
+                     float* core_sq) {
+  return guarded([&] {
+    EVOC_CHECK(emb && core_sq, "core_sq: NULL buffer");
+    EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "core_sq: dim must be in [1, 16]");
+    EVOC_CHECK(min_samples >= 0 && min_samples <= 32, "min_samples must be in [0, 32]");
+    Ctx& c = context(0);
+    Buf<float> de, dc(c.pool, n);
+    to_dev(de, emb, n * dim, c.pool, c.stream);
+    core_sq_device(de.p, n, dim, min_samples, dc.p, c.stream, c.pool);
+    EVOC_HIP(hipMemcpyAsync(core_sq, dc.p, n * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    EVOC_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int evoc_hip_mst(const float* emb, int64_t n, int32_t dim, int32_t min_samples, int32_t* u,
+                 int32_t* v, float* w) {
+  return guarded([&] {
+    EVOC_CHECK(emb && u && v && w, "mst: NULL buffer");
+    EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "mst: dim must be in [1, 16]");
+    EVOC_CHECK(min_samples >= 0 && min_samples <= 32, "min_samples must be in [0, 32]");
+    Ctx& c = context(0);
+    Buf<float> de;
+    to_dev(de, emb, n * dim, c.pool, c.stream);
+    mst_device(de.p, n, dim, min_samples, u, v, w, c.stream, c.pool);
+  });
+}
+
+int evoc_hip_cluster_layers(const float* emb, int64_t n, int32_t dim, int32_t min_samples,
+                            int64_t base_mcs, double min_sim, int64_t max_layers, int64_t* labels,
+                            float* strengths, double* persistence, int64_t* n_layers) {
+  return guarded([&] {
+    EVOC_CHECK(emb && labels && strengths && persistence && n_layers, "layers: NULL buffer");
+    EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "layers: dim must be in [1, 16]");
+    Ctx& c = context(0);
+    Buf<float> de;
+    to_dev(de, emb, n * dim, c.pool, c.stream);
+    evoc_params_t p;
+    evoc_hip_default_params(&p);
+    p.min_samples = min_samples;
+    p.base_min_cluster_size = (int32_t)base_mcs;
+    p.min_similarity_threshold = min_sim;
+    p.max_layers = (int32_t)max_layers;
+    Layers lay = cluster_device(de.p, n, dim, p, c.stream, c.pool, nullptr, nullptr);
+    const int64_t nl = std::min<int64_t>((int64_t)lay.labels.size(), std::max<int64_t>(max_layers, 1));
+    for (int64_t l = 0; l < nl; ++l) {
+      std::memcpy(labels + l * n, lay.labels[l].data(), n * sizeof(int64_t));
+      std::memcpy(strengths + l * n, lay.strengths[l].data(), n * sizeof(float));
+      persistence[l] = lay.persistence[l];
+    }
+    *n_layers = nl;
+  });
+}
+
+int evoc_hip_search_n_clusters(const float* emb, int64_t n, int32_t dim, int32_t min_samples,
+                               int64_t target_k, int64_t* labels, float* strengths) {
+  return guarded([&] {
+    EVOC_CHECK(emb && labels && strengths, "search: NULL buffer");
+    EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "search: dim must be in [1, 16]");
+    Ctx& c = context(0);
+    Buf<float> de;
+    to_dev(de, emb, n * dim, c.pool, c.stream);
+    evoc_params_t p;
+    evoc_hip_default_params(&p);
+    p.min_samples = min_samples;
+    p.approx_n_clusters = (int32_t)std::max<int64_t>(target_k, 1);
+    Layers lay = cluster_device(de.p, n, dim, p, c.stream, c.pool, nullptr, nullptr);
+    std::memcpy(labels, lay.labels[0].data(), n * sizeof(int64_t));
+    std::memcpy(strengths, lay.strengths[0].data(), n * sizeof(float));
+  });
+}
+
+}  // extern "C"

@@ -72,6 +72,8 @@ class _Result(C.Structure):
         ("embedding", C.POINTER(C.c_float)),
         ("stage_ms", C.c_double * 8),
         ("knn_fallback_rows", C.c_int64),
+        ("knn_kernel_ms", C.c_double),
+        ("embed_kernel_ms", C.c_double),
     ]
 
 
@@ -198,6 +200,8 @@ class EvocResult:
         self.stage_ms = dict(zip(["knn", "fuzzy", "init", "embed", "mst", "layers", "total",
                                   "upload"], list(r.stage_ms)))
         self.knn_fallback_rows = r.knn_fallback_rows
+        self.knn_kernel_ms = r.knn_kernel_ms
+        self.embed_kernel_ms = r.embed_kernel_ms
 
     def best_labels(self):
         return self.cluster_layers[self.best_layer]
@@ -232,7 +236,7 @@ def evoc(data, params=None, nn_params=None, seed=42, verbose=0, precomputed_knn=
         pd = np.ascontiguousarray(precomputed_knn[1], np.float32)
         pk = pi.shape[1]
     out = C.POINTER(_Result)()
-    flat = np.ndarray((base.size,), np.float32, buffer=base, order="A") if base.flags.f_contiguous and not base.flags.c_contiguous else base.reshape(-1)
+    flat = base.ravel(order="K")  # memory order; strides describe the layout
     _check(lib().evoc_hip_run(flat, n, d, rs, cs,
                               None if pi is None else pi.ctypes.data_as(vp),
                               None if pd is None else pd.ctypes.data_as(vp), pk,

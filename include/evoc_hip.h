@@ -79,6 +79,8 @@ typedef struct {
   float* embedding;                /* n * dim, the node embedding clustered */
   double stage_ms[8];              /* knn, fuzzy, init, embed, mst, layers, total, upload */
   int64_t knn_fallback_rows;       /* rows re-searched exactly after a failed certificate */
+  double knn_kernel_ms;            /* MFMA top-candidate kernel, hipEvent-timed on its stream */
+  double embed_kernel_ms;          /* all embedding epochs, hipEvent-timed */
 } evoc_result_t;
 
 void evoc_hip_default_params(evoc_params_t* p);

@@ -164,6 +164,41 @@ extern "C" int orc_knn_exhaustive(const float* x, int64_t n, int64_t d, int metr
   return 0;
 }
 
+// Exhaustive kNN of selected query rows against the whole matrix (same canonical
+// distance and (dist, idx) order as orc_knn_exhaustive); used to time the CPU on a
+// bounded sample of rows.
+extern "C" int orc_knn_rows(const float* x, int64_t n, int64_t d, int metric, int k,
+                            const int64_t* rows, int64_t nrows, int32_t* idx_out,
+                            float* dist_out) {
+  std::vector<float> nrm;
+  if (metric == 1) {
+    nrm.resize(n);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) nrm[i] = std::sqrt(canon_dot(x + i * d, x + i * d, d));
+  }
+  const int kk = (int)std::min<int64_t>(k, n - 1);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t r = 0; r < nrows; ++r) {
+    const int64_t i = rows[r];
+    std::vector<std::pair<float, int32_t>> best;
+    const float* xi = x + i * d;
+    for (int64_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      float dist = metric == 1 ? cos_dist(canon_dot(xi, x + j * d, d), nrm[i], nrm[j])
+                               : std::sqrt(canon_sq(xi, x + j * d, d));
+      std::pair<float, int32_t> c(dist, (int32_t)j);
+      if ((int)best.size() == kk && !(c < best.back())) continue;
+      best.insert(std::upper_bound(best.begin(), best.end(), c), c);
+      if ((int)best.size() > kk) best.pop_back();
+    }
+    for (int t = 0; t < k; ++t) {
+      idx_out[r * k + t] = t < kk ? best[t].second : -1;
+      dist_out[r * k + t] = t < kk ? best[t].first : INFINITY;
+    }
+  }
+  return 0;
+}
+
 // ================================================================ A2-A5: fuzzy set
 // smooth_knn_dist: src/graph/fuzzy_graph.rs:33-98 (local_connectivity = 1, n_iter = 64)
 static void smooth_knn_dist(const float* dists, int64_t n, int k, size_t k_target,

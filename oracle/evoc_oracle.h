@@ -24,6 +24,8 @@ extern "C" {
 // metric: 0 = euclidean (sqrt of squared L2), 1 = cosine (1 - cos)
 int orc_knn_exhaustive(const float* x, int64_t n, int64_t d, int metric, int k,
                        int32_t* idx_out, float* dist_out);
+int orc_knn_rows(const float* x, int64_t n, int64_t d, int metric, int k, const int64_t* rows,
+                 int64_t nrows, int32_t* idx_out, float* dist_out);
 // canonical f32 distance between two rows (same routine the kNN uses)
 float orc_canon_dist(const float* a, const float* b, int64_t d, int metric);
 

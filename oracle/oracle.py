@@ -49,6 +49,8 @@ def lib():
             build()
         L = C.CDLL(_LIB_PATH)
         L.orc_knn_exhaustive.argtypes = [f32p, C.c_int64, C.c_int64, C.c_int, C.c_int, i32p, f32p]
+        L.orc_knn_rows.argtypes = [f32p, C.c_int64, C.c_int64, C.c_int, C.c_int, i64p, C.c_int64,
+                                   i32p, f32p]
         L.orc_canon_dist.argtypes = [f32p, f32p, C.c_int64, C.c_int]
         L.orc_canon_dist.restype = C.c_float
         L.orc_fuzzy_graph.argtypes = [i32p, f32p, C.c_int64, C.c_int, C.c_float, C.c_int,
@@ -124,6 +126,16 @@ def knn_exhaustive(x, k, metric=0):
     idx = np.empty((n, k), np.int32)
     dist = np.empty((n, k), np.float32)
     lib().orc_knn_exhaustive(x, n, d, metric, k, idx, dist)
+    return idx, dist
+
+
+def knn_rows(x, rows, k, metric=0):
+    x = _f32(x)
+    n, d = x.shape
+    rows = np.ascontiguousarray(rows, np.int64)
+    idx = np.empty((len(rows), k), np.int32)
+    dist = np.empty((len(rows), k), np.float32)
+    lib().orc_knn_rows(x, n, d, metric, k, rows, len(rows), idx, dist)
     return idx, dist
 
 

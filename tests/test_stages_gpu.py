@@ -1,0 +1,187 @@
+"""Stage-by-stage parity of the HIP library against the CPU oracle on identical inputs.
+
+Bars (from the task's north star): fuzzy weights within 1e-5 relative; embedding, label
+propagation layout, core distances, MST, condensed tree, labels, strengths and fp64
+persistence bit-exact when both sides get the same input."""
+import numpy as np
+import pytest
+from sklearn.metrics import adjusted_rand_score as ari
+
+import datagen
+import evoc_hip as H
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def graph():
+    x, gt = datagen.gaussian_mixture(3000, 32, 12, 4.0, 1.0, seed=13)
+    idx, dist = O.knn_exhaustive(x, 15)
+    return x, gt, idx, dist
+
+
+def test_fuzzy_graph_parity(graph):
+    x, gt, idx, dist = graph
+    for sym in (True, False):
+        gp, gi, gw, gs, gr = H.fuzzy_graph(idx, dist, 15.0, symmetrise=sym)
+        op, oi, ow, os_, orh = O.fuzzy_graph(idx, dist, 15.0, symmetrise=sym)
+        assert np.array_equal(gp, op) and np.array_equal(gi, oi)
+        np.testing.assert_allclose(gw, ow, rtol=1e-5, atol=0)
+        np.testing.assert_array_equal(gr, orh)
+        np.testing.assert_allclose(gs, os_, rtol=1e-5)
+        print("weights bit-identical: %.4f" % np.mean(gw.view(np.uint32) == ow.view(np.uint32)))
+
+
+def test_fuzzy_graph_hub_rows():
+    # one node in every kNN list: its symmetrised row is longer than a wavefront
+    n, k = 400, 5
+    rng = np.random.default_rng(0)
+    idx = np.stack([rng.permutation(np.delete(np.arange(n), i))[:k] for i in range(n)]).astype(np.int32)
+    idx[1:, 0] = 0
+    for i in range(1, n):
+        if len(set(idx[i])) < k:
+            idx[i] = [0] + [j for j in rng.permutation(n) if j not in (0, i)][: k - 1]
+    dist = np.sort(rng.uniform(0.1, 2.0, size=(n, k)), axis=1).astype(np.float32)
+    gp, gi, gw, _, _ = H.fuzzy_graph(idx, dist, float(k))
+    op, oi, ow, _, _ = O.fuzzy_graph(idx, dist, float(k))
+    assert gp[1] - gp[0] > 64
+    assert np.array_equal(gp, op) and np.array_equal(gi, oi)
+    np.testing.assert_allclose(gw, ow, rtol=1e-5)
+
+
+def test_embedding_parity_bit_exact(graph):
+    x, gt, idx, dist = graph
+    op, oi, ow, _, _ = O.fuzzy_graph(idx, dist, 15.0)
+    init = O.label_prop_init(op, oi, ow, 8, data=x, seed=7)
+    for dim, ini in ((8, init), (4, None)):
+        g = H.embedding(op, oi, ow, dim, n_epochs=30, init=ini, seed=11)
+        o = O.embedding(op, oi, ow, dim, n_epochs=30, init=ini, seed=11)
+        assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+
+
+def test_embedding_asymmetric_graph_parity():
+    rng = np.random.default_rng(5)
+    n = 300
+    rows = [sorted(rng.choice(n, size=rng.integers(1, 8), replace=False).tolist()) for _ in range(n)]
+    ip = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64)
+    ind = np.concatenate(rows).astype(np.int32)
+    w = rng.uniform(0.05, 1.0, size=len(ind)).astype(np.float32)
+    g = H.embedding(ip, ind, w, 4, n_epochs=20, seed=3)
+    o = O.embedding(ip, ind, w, 4, n_epochs=20, seed=3)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+
+
+def test_label_prop_init_parity(graph):
+    x, gt, idx, dist = graph
+    op, oi, ow, _, _ = O.fuzzy_graph(idx, dist, 15.0)
+    g = H.label_prop_init(op, oi, ow, 8, data=x, seed=42)
+    o = O.label_prop_init(op, oi, ow, 8, data=x, seed=42)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    g2 = H.label_prop_init(op, oi, ow, 8, data=None, seed=3)
+    o2 = O.label_prop_init(op, oi, ow, 8, data=None, seed=3)
+    assert np.array_equal(g2.view(np.uint32), o2.view(np.uint32))
+
+
+def _embedding(n=4000, seed=3):
+    x, gt = datagen.gaussian_mixture(n, 8, 15, 3.0, 1.0, seed=seed)
+    return x, gt
+
+
+def test_core_distances_bit_exact():
+    e, _ = _embedding()
+    for ms in (1, 5, 16):
+        assert np.array_equal(H.core_sq(e, ms).view(np.uint32), O.core_sq(e, ms).view(np.uint32))
+
+
+def test_mst_bit_exact_with_ties():
+    e, _ = _embedding(3000)
+    for ms in (1, 5):
+        gu, gv, gw = H.mst(e, ms)
+        ou, ov, ow = O.mst(e, ms)
+        order = np.lexsort((ov, ou, ow))
+        assert np.array_equal(gu, ou[order]) and np.array_equal(gv, ov[order])
+        assert np.array_equal(gw.view(np.uint32), ow[order].view(np.uint32))
+    tied = np.round(np.random.default_rng(4).normal(size=(1500, 4)) * 2).astype(np.float32) / 2
+    gu, gv, gw = H.mst(tied, 3)
+    ou, ov, ow = O.mst(tied, 3)
+    assert sorted(zip(gu.tolist(), gv.tolist())) == sorted(zip(ou.tolist(), ov.tolist()))
+
+
+def test_cluster_layers_bit_exact():
+    e, gt = _embedding(3000, seed=8)
+    gl, gs, gp = H.cluster_layers(e, 5, 5, 0.2, 10)
+    ol, os_, op = O.cluster_layers(e, 5, 5, 0.2, 10, fp64=True)
+    assert gl.shape == ol.shape
+    assert np.array_equal(gl, ol)
+    assert np.array_equal(gs.view(np.uint32), os_.view(np.uint32))
+    assert np.array_equal(gp.view(np.uint64), op.view(np.uint64))
+
+
+def test_search_for_n_clusters_bit_exact():
+    e, gt = _embedding(3000, seed=9)
+    for target in (5, 15, 40):
+        gl, gs = H.search_for_n_clusters(e, 5, target)
+        ol, os_ = O.search_n_clusters(e, 5, target)
+        assert np.array_equal(gl, ol)
+        assert np.array_equal(gs.view(np.uint32), os_.view(np.uint32))
+
+
+def _acc(pred, gt):
+    c = 0
+    for p in set(pred.tolist()) - {-1}:
+        c += np.bincount(gt[pred == p]).max()
+    return c / len(pred)
+
+
+def test_end_to_end_same_knn_identical_labels(graph):
+    x, gt, idx, dist = graph
+    r = H.evoc(x, precomputed_knn=(idx, dist), seed=5)
+    o = O.evoc(x, O.params(), seed=5, pre_idx=idx, pre_dist=dist)
+    assert np.array_equal(r.embedding.view(np.uint32), o["embedding"].view(np.uint32))
+    assert np.array_equal(r.cluster_layers, o["labels"])
+    assert np.array_equal(r.persistence_scores, o["persistence"])
+
+
+def test_end_to_end_vs_oracle_ari():
+    x, gt = datagen.gaussian_mixture(4000, 64, 20, 5.0, 1.0, seed=42)
+    r = H.evoc(x, seed=42)
+    o = O.evoc(x, O.params(), seed=42)
+    assert np.array_equal(r.nn_indices, o["knn_idx"])
+    pers = o["persistence"]
+    ob = max(i for i in range(len(pers)) if pers[i] == pers.max()) if len(pers) > 1 else 0
+    score = ari(r.best_labels(), o["labels"][ob])
+    print("ARI GPU vs CPU labels:", score, "layers", len(r.cluster_layers), len(o["labels"]))
+    assert score >= 0.98
+
+
+def test_reference_gpu_integration_blobs():
+    # tests/integration_tests_gpu.rs:22-67 and :122-185
+    x, gt = datagen.make_blobs(100, 2, 8, 20.0, 5.0, seed=42)
+    r = H.evoc(x, seed=42)
+    assert _acc(r.best_labels(), gt) > 0.95
+    o = O.evoc(x.astype(np.float32), O.params(), seed=42)
+    pers = o["persistence"]
+    ob = max(i for i in range(len(pers)) if pers[i] == pers.max()) if len(pers) > 1 else 0
+    assert abs(r.n_clusters() - (o["labels"][ob].max() + 1)) <= 2
+    self_loops = sum(i in row for i, row in enumerate(r.nn_indices))
+    assert self_loops <= 0.01 * len(x)
+
+
+def test_reference_gpu_integration_approx_n_clusters_and_precomputed():
+    # tests/integration_tests_gpu.rs:187-268
+    x, gt = datagen.make_blobs(100, 3, 8, 20.0, 5.0, seed=42)
+    idx, dist = O.knn_exhaustive(x.astype(np.float32), 15)
+    r = H.evoc(x, precomputed_knn=(idx, dist), seed=42)
+    assert np.array_equal(r.nn_indices, idx)
+    r = H.evoc(x, params=H.EvocParams(approx_n_clusters=3), seed=42)
+    assert len(r.cluster_layers) == 1 and r.persistence_scores.tolist() == [0.0]
+    assert abs(r.n_clusters() - 3) <= 1
+
+
+def test_strided_input_matches_contiguous():
+    x, _ = datagen.make_blobs(80, 2, 6, 20.0, 3.0, seed=1)
+    xf = np.asfortranarray(x.astype(np.float32))
+    a = H.evoc(x.astype(np.float32), seed=3)
+    b = H.evoc(xf, seed=3)
+    assert np.array_equal(a.cluster_layers, b.cluster_layers)
