@@ -16,6 +16,10 @@ namespace {
 
 constexpr int KMAXF = 32;
 
+// canonical f32 exponential: exp evaluated in f64 and rounded once to f32 (correctly
+// rounded except in vanishingly rare double-rounding cases), identical on host and device
+__device__ __forceinline__ float cexp(float x) { return (float)exp((double)x); }
+
 __global__ __launch_bounds__(256) void sigma_kernel(const float* __restrict__ dist,
                                                     const int32_t* __restrict__ idx, int64_t n,
                                                     int k, double target, float* __restrict__ sigma,
@@ -35,7 +39,7 @@ __global__ __launch_bounds__(256) void sigma_kernel(const float* __restrict__ di
     for (int t = 0; t < KMAXF; ++t)
       if (t < k) {
         const float adjusted = fmaxf(d[t] - r, 0.0f);
-        val += expf(-(adjusted / mid));
+        val += cexp(-(adjusted / mid));
       }
     if (fabs((double)val - target) < 1e-5) break;
     if ((double)val > target) {
@@ -54,7 +58,7 @@ __global__ __launch_bounds__(256) void sigma_kernel(const float* __restrict__ di
     if (t < k) {
       const float adjusted = fmaxf(d[t] - r, 0.0f);
       float wt;
-      if (mid > 0.0f) wt = expf(-(adjusted / mid));
+      if (mid > 0.0f) wt = cexp(-(adjusted / mid));
       else if (adjusted > 0.0f) wt = 0.0f;
       else wt = 1.0f;
       w[i * k + t] = wt;

@@ -22,7 +22,6 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 constexpr int BM = 128;   // query rows per workgroup
 constexpr int BN = 128;   // database columns per tile
 constexpr int KC = 32;    // k-chunk staged through LDS
-constexpr int LDA = 40;   // padded LDS row stride (bf16 elements) of a staged chunk
 constexpr int SCS = 132;  // padded LDS row stride (floats) of the key tile
 constexpr float FINF = __builtin_huge_valf();
 
@@ -103,23 +102,45 @@ __device__ __forceinline__ void list_insert(float (&lk)[C], int (&li)[C], float 
   }
 }
 
+// vmcnt wait with a compile-time count (gfx9 encoding: vmcnt in bits 3:0 and 15:14,
+// expcnt and lgkmcnt left at their maxima so only vector-memory counts are awaited)
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  __builtin_amdgcn_s_waitcnt((N & 15) | (((N >> 4) & 3) << 14) | (7 << 4) | (15 << 8));
+}
+
+// 16-byte LDS slot of piece q (8 bf16 along k) of staged row r: XOR swizzle so that the
+// 16 rows a half-wavefront reads with one ds_read_b128 hit 16 distinct bank groups
+__device__ __forceinline__ int slot_of(int r, int q) { return r * 4 + (q ^ ((r >> 2) & 3)); }
+
+typedef __attribute__((address_space(3))) void lds_void;
+
 template <bool SPLIT, int C>  // C = list length per column half
 __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
     const uint16_t* __restrict__ Xh, const uint16_t* __restrict__ Xl, int64_t n, int64_t npad,
     int dpad, const float* __restrict__ colscale, int metric, int32_t* __restrict__ cand_idx,
     float* __restrict__ cand_key) {
-  constexpr int NOPS = SPLIT ? 4 : 2;  // A hi, B hi, (A lo, B lo)
-  constexpr int SE = BM * LDA;         // elements per staged operand
+  constexpr int NOPS = SPLIT ? 4 : 2;           // A hi, B hi, (A lo, B lo)
+  constexpr int NST = SPLIT ? 2 : 4;            // stages in the LDS ring
+  constexpr int OPB = BM * 4 * 16;              // bytes per staged operand chunk (8 KB)
+  constexpr int STB = NOPS * OPB;               // bytes per stage
+  constexpr int REGION = BM * SCS * 4;          // key tile / candidate queue
+  constexpr int QCAP = REGION / 8;
   extern __shared__ __attribute__((aligned(16))) char smem[];
-  uint16_t* stage = reinterpret_cast<uint16_t*>(smem);
-  float* sc = reinterpret_cast<float*>(smem + 2 * NOPS * SE * sizeof(uint16_t));
+  char* ring = smem;
+  float* sc = reinterpret_cast<float*>(smem + NST * STB);
+  float2* queue = reinterpret_cast<float2*>(sc);
+  float* thr = reinterpret_cast<float*>(smem + NST * STB + REGION);  // [2][BM]
+  int* qcount = reinterpret_cast<int*>(thr + 2 * BM);                 // [2]
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int64_t row0 = (int64_t)blockIdx.x * BM;
+  const int64_t rb = blockIdx.x;
+  const int64_t row0 = rb * BM;
   const int nck = dpad / KC;
   const int64_t ntile = npad / BN;
   const int64_t total = ntile * nck;
+  const int orow = tid & (BM - 1), ohalf = tid >> 7;  // list owned by this thread
 
   float lk[C];
   int li[C];
@@ -128,43 +149,43 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
     lk[q] = FINF;
     li[q] = -1;
   }
-  f32x16 acc[2][2];
-  u32x4 pre[NOPS][2];
-  float cs[2] = {0.0f, 0.0f};
+  thr[ohalf * BM + orow] = FINF;
+  if (tid < 2) qcount[tid] = 0;
 
-#define EVOC_GLOAD(G)                                                                    \
-  do {                                                                                   \
-    const int64_t ct_ = (G) / nck;                                                       \
-    const int kc_ = (int)((G) - ct_ * nck);                                              \
-    _Pragma("unroll") for (int op = 0; op < NOPS; ++op) {                                \
-      const uint16_t* src = (op < 2) ? Xh : Xl;                                          \
-      const int64_t rbase = (op & 1) ? ct_ * BN : row0;                                  \
-      _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                    \
-        const int item = tid + h * 256;                                                  \
-        const int r = item >> 2, q = item & 3;                                           \
-        pre[op][h] =                                                                     \
-            *reinterpret_cast<const u32x4*>(src + (rbase + r) * dpad + kc_ * KC + q * 8); \
-      }                                                                                  \
-    }                                                                                    \
-  } while (0)
-#define EVOC_LSTORE(BUF)                                                                    \
-  do {                                                                                      \
-    _Pragma("unroll") for (int op = 0; op < NOPS; ++op)                                     \
-        _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                     \
-      const int item = tid + h * 256;                                                       \
-      const int r = item >> 2, q = item & 3;                                                \
-      *reinterpret_cast<u32x4*>(stage + ((BUF) * NOPS + op) * SE + r * LDA + q * 8) = pre[op][h]; \
-    }                                                                                       \
-  } while (0)
-
-  EVOC_GLOAD(0);
-  EVOC_LSTORE(0);
-  __syncthreads();
-  for (int64_t g = 0; g < total; ++g) {
-    const int buf = (int)(g & 1);
+  // issue the direct-to-LDS loads of chunk g into its ring stage
+  auto issue = [&](int64_t g) {
     const int64_t ct = g / nck;
     const int kc = (int)(g - ct * nck);
-    if (g + 1 < total) EVOC_GLOAD(g + 1);
+    char* st = ring + (int)(g % NST) * STB;
+#pragma unroll
+    for (int op = 0; op < NOPS; ++op) {
+      const uint16_t* src = (op < 2) ? Xh : Xl;
+      const int64_t rbase = (op & 1) ? ct * BN : row0;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int blk = h * 4 + wave;          // 64 consecutive slots per instruction
+        const int s = blk * 64 + lane;
+        const int r = s >> 2, qs = s & 3;
+        const int q = qs ^ ((r >> 2) & 3);
+        const uint16_t* gp = src + (rbase + r) * dpad + kc * KC + q * 8;
+        __builtin_amdgcn_global_load_lds(gp, (lds_void*)(st + op * OPB + blk * 1024), 16, 0, 0);
+      }
+    }
+  };
+
+  f32x16 acc[2][2];
+  float cs[2] = {0.0f, 0.0f};
+  constexpr int PER_CHUNK = NOPS * 2;  // load instructions per thread per chunk
+  for (int64_t g = 0; g < NST - 1 && g < total; ++g) issue(g);
+  for (int64_t g = 0; g < total; ++g) {
+    const int64_t ahead = std::min<int64_t>(NST - 2, total - 1 - g);
+    if (ahead >= 2) wait_vm<2 * PER_CHUNK>();
+    else if (ahead == 1) wait_vm<PER_CHUNK>();
+    else wait_vm<0>();
+    __syncthreads();
+    if (g + NST - 1 < total) issue(g + NST - 1);
+    const int64_t ct = g / nck;
+    const int kc = (int)(g - ct * nck);
     if (kc == 0) {
 #pragma unroll
       for (int a = 0; a < 2; ++a)
@@ -173,33 +194,38 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
 #pragma unroll
       for (int tn = 0; tn < 2; ++tn) cs[tn] = colscale[ct * BN + wn * 64 + tn * 32 + (lane & 31)];
     }
-    const uint16_t* sA = stage + (buf * NOPS + 0) * SE;
-    const uint16_t* sB = stage + (buf * NOPS + 1) * SE;
+    const char* st = ring + (int)(g % NST) * STB;
 #pragma unroll
     for (int kk = 0; kk < KC / 16; ++kk) {
-      const int ko = kk * 16 + (lane >> 5) * 8;
+      const int q = kk * 2 + (lane >> 5);
       bf16x8 a[2], b[2];
 #pragma unroll
-      for (int tm = 0; tm < 2; ++tm)
-        a[tm] = *reinterpret_cast<const bf16x8*>(sA + (wm * 64 + tm * 32 + (lane & 31)) * LDA + ko);
+      for (int tm = 0; tm < 2; ++tm) {
+        const int r = wm * 64 + tm * 32 + (lane & 31);
+        a[tm] = *reinterpret_cast<const bf16x8*>(st + 0 * OPB + slot_of(r, q) * 16);
+      }
 #pragma unroll
-      for (int tn = 0; tn < 2; ++tn)
-        b[tn] = *reinterpret_cast<const bf16x8*>(sB + (wn * 64 + tn * 32 + (lane & 31)) * LDA + ko);
+      for (int tn = 0; tn < 2; ++tn) {
+        const int r = wn * 64 + tn * 32 + (lane & 31);
+        b[tn] = *reinterpret_cast<const bf16x8*>(st + 1 * OPB + slot_of(r, q) * 16);
+      }
 #pragma unroll
       for (int tm = 0; tm < 2; ++tm)
 #pragma unroll
         for (int tn = 0; tn < 2; ++tn)
           acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[tm], b[tn], acc[tm][tn], 0, 0, 0);
       if constexpr (SPLIT) {
-        const uint16_t* sAl = stage + (buf * NOPS + 2) * SE;
-        const uint16_t* sBl = stage + (buf * NOPS + 3) * SE;
         bf16x8 al[2], bl[2];
 #pragma unroll
-        for (int tm = 0; tm < 2; ++tm)
-          al[tm] = *reinterpret_cast<const bf16x8*>(sAl + (wm * 64 + tm * 32 + (lane & 31)) * LDA + ko);
+        for (int tm = 0; tm < 2; ++tm) {
+          const int r = wm * 64 + tm * 32 + (lane & 31);
+          al[tm] = *reinterpret_cast<const bf16x8*>(st + 2 * OPB + slot_of(r, q) * 16);
+        }
 #pragma unroll
-        for (int tn = 0; tn < 2; ++tn)
-          bl[tn] = *reinterpret_cast<const bf16x8*>(sBl + (wn * 64 + tn * 32 + (lane & 31)) * LDA + ko);
+        for (int tn = 0; tn < 2; ++tn) {
+          const int r = wn * 64 + tn * 32 + (lane & 31);
+          bl[tn] = *reinterpret_cast<const bf16x8*>(st + 3 * OPB + slot_of(r, q) * 16);
+        }
 #pragma unroll
         for (int tm = 0; tm < 2; ++tm)
 #pragma unroll
@@ -209,10 +235,95 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
           }
       }
     }
-    if (g + 1 < total) EVOC_LSTORE(buf ^ 1);
+    if (kc != nck - 1) continue;
+
+    // ---------------- epilogue of column tile ct: filter keys against the row thresholds
+    const int par = (int)(ct & 1);
+    const bool diag = ct == rb;
+    const bool tail = (ct + 1) * BN > n;
+#define EVOC_KEY(TM, TN, J, I, OUT)                                                    \
+  do {                                                                                   \
+    const float s_ = acc[TM][TN][(J) * 4 + (I)];                                         \
+    float k_ = metric == 1 ? -(s_ * cs[TN]) : (cs[TN] - 2.0f * s_);                      \
+    if (diag || tail) {                                                                  \
+      const int rl_ = wm * 64 + (TM) * 32 + (J) * 8 + (lane >> 5) * 4 + (I);             \
+      const int64_t gc_ = ct * BN + wn * 64 + (TN) * 32 + (lane & 31);                   \
+      if (gc_ >= n || gc_ == row0 + rl_) k_ = FINF;                                      \
+    }                                                                                    \
+    OUT = k_;                                                                            \
+  } while (0)
+    // thresholds of this lane's rows, four rows (one float4) at a time
+    int mine = 0;
+#pragma unroll
+    for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float4 t4 = *reinterpret_cast<const float4*>(
+            thr + wn * BM + wm * 64 + tm * 32 + j * 8 + (lane >> 5) * 4);
+        const float tv[4] = {t4.x, t4.y, t4.z, t4.w};
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            float key;
+            EVOC_KEY(tm, tn, j, i, key);
+            mine += key < tv[i];
+          }
+      }
+    // wave-exclusive prefix of the pass counts, one LDS atomic per wave
+    int incl = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    const int wtot = __shfl(incl, 63, 64);
+    int base = 0;
+    if (wtot > 0) {
+      if (lane == 0) base = atomicAdd(&qcount[par], wtot);
+      base = __shfl(base, 0, 64);
+    }
+    if (mine > 0) {
+      int pos = base + incl - mine;
+#pragma unroll
+      for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float4 t4 = *reinterpret_cast<const float4*>(
+              thr + wn * BM + wm * 64 + tm * 32 + j * 8 + (lane >> 5) * 4);
+          const float tv[4] = {t4.x, t4.y, t4.z, t4.w};
+#pragma unroll
+          for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              float key;
+              EVOC_KEY(tm, tn, j, i, key);
+              if (key < tv[i]) {
+                if (pos < QCAP) {
+                  const int rl = wm * 64 + tm * 32 + j * 8 + (lane >> 5) * 4 + i;
+                  const int cl = wn * 64 + tn * 32 + (lane & 31);
+                  queue[pos] = make_float2(key, __int_as_float((rl << 7) | cl));
+                }
+                ++pos;
+              }
+            }
+        }
+    }
+    if (tid == 0) qcount[par ^ 1] = 0;  // counter of the next tile (its readers are done)
     __syncthreads();
-    if (kc == nck - 1) {
-      // keys of this 128x128 tile -> LDS (smaller key = closer)
+    const int qn = qcount[par];
+    const int cbase = (int)(ct * BN);
+    if (qn <= QCAP) {
+      for (int e = 0; e < qn; ++e) {
+        const float2 en = queue[e];
+        const int pk = __float_as_int(en.y);
+        const int er = pk >> 7, ec = pk & 127;
+        if (er == orow && (ec >> 6) == ohalf && en.x < lk[C - 1])
+          list_insert<C>(lk, li, en.x, cbase + ec);
+      }
+    } else {
+      // queue overflow (the first tiles): scan the whole key tile row by row
+      __syncthreads();
 #pragma unroll
       for (int tm = 0; tm < 2; ++tm)
 #pragma unroll
@@ -223,33 +334,30 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
             for (int i = 0; i < 4; ++i) {
               const int rl = wm * 64 + tm * 32 + j * 8 + (lane >> 5) * 4 + i;
               const int cl = wn * 64 + tn * 32 + (lane & 31);
-              const float s = acc[tm][tn][j * 4 + i];
-              float key = metric == 1 ? -(s * cs[tn]) : (cs[tn] - 2.0f * s);
-              const int64_t gc = ct * BN + cl;
-              if (gc >= n || gc == row0 + rl) key = FINF;
+              float key;
+              EVOC_KEY(tm, tn, j, i, key);
               sc[rl * SCS + cl] = key;
             }
       __syncthreads();
-      const int r = tid & (BM - 1), hh = tid >> 7;
-      const float* srow = sc + r * SCS + hh * 64;
-      const int cbase = (int)(ct * BN) + hh * 64;
+      const float* srow = sc + orow * SCS + ohalf * 64;
+      const int cb = cbase + ohalf * 64;
       for (int q = 0; q < 64; q += 4) {
         const float4 v = *reinterpret_cast<const float4*>(srow + q);
-        if (v.x < lk[C - 1]) list_insert<C>(lk, li, v.x, cbase + q);
-        if (v.y < lk[C - 1]) list_insert<C>(lk, li, v.y, cbase + q + 1);
-        if (v.z < lk[C - 1]) list_insert<C>(lk, li, v.z, cbase + q + 2);
-        if (v.w < lk[C - 1]) list_insert<C>(lk, li, v.w, cbase + q + 3);
+        if (v.x < lk[C - 1]) list_insert<C>(lk, li, v.x, cb + q);
+        if (v.y < lk[C - 1]) list_insert<C>(lk, li, v.y, cb + q + 1);
+        if (v.z < lk[C - 1]) list_insert<C>(lk, li, v.z, cb + q + 2);
+        if (v.w < lk[C - 1]) list_insert<C>(lk, li, v.w, cb + q + 3);
       }
     }
+    thr[ohalf * BM + orow] = lk[C - 1];
   }
   // each (row, column-half) list is written as is: candidates per row = 2*C
-  const int r = tid & (BM - 1), hh = tid >> 7;
-  const int64_t grow = row0 + r;
+  const int64_t grow = row0 + orow;
   if (grow < n) {
 #pragma unroll
     for (int q = 0; q < C; ++q) {
-      cand_key[(grow * 2 + hh) * C + q] = lk[q];
-      cand_idx[(grow * 2 + hh) * C + q] = li[q];
+      cand_key[(grow * 2 + ohalf) * C + q] = lk[q];
+      cand_idx[(grow * 2 + ohalf) * C + q] = li[q];
     }
   }
 }
@@ -484,7 +592,9 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   const float e_in = split ? 4.0f * 1.5258789e-05f : (hflags[0] ? 2.0f * 0.0078125f : 0.0f);
   const float gamma = 2.0f * (float)dpad * u + e_in;
   const int nops = split ? 4 : 2;
-  const size_t smem = 2 * nops * BM * LDA * sizeof(uint16_t) + BM * SCS * sizeof(float);
+  const int nst = split ? 2 : 4;
+  const size_t smem = (size_t)nst * nops * BM * 4 * 16 + BM * SCS * sizeof(float) +
+                      2 * BM * sizeof(float) + 4 * sizeof(int);
   EVOC_HIP(hipEventRecord(e0, s));
   const unsigned grid = (unsigned)(npad / BM);
   static bool attr_set = false;

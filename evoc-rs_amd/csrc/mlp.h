@@ -18,4 +18,11 @@ void warm_start_init(const float* dX, int64_t n, int d, int64_t ld, const Graph&
                      uint64_t seed, MlpWarmStart& ws, float* d_init, hipStream_t s,
                      DevPool& pool, int verbose);
 
+// MSE loss, prediction and gradients of the MLP on one batch (weights given as host
+// fp32 arrays, layer l: W[l] (out x in), b[l] (out)); for gradient checks.
+void mlp_loss_and_grads(const float* x, const float* t, int rows, int D, int O,
+                        const float* const* w, const float* const* b, float* const* gw,
+                        float* const* gb, float* out_y, double* loss, hipStream_t s,
+                        DevPool& pool);
+
 }  // namespace evoc

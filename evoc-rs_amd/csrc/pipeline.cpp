@@ -527,4 +527,22 @@ int evoc_hip_search_n_clusters(const float* emb, int64_t n, int32_t dim, int32_t
   });
 }
 
+int evoc_hip_mlp_loss_grads(const float* x, const float* t, int32_t rows, int32_t d,
+                            int32_t out, const float* w0, const float* b0, const float* w1,
+                            const float* b1, const float* w2, const float* b2, float* gw0,
+                            float* gb0, float* gw1, float* gb1, float* gw2, float* gb2,
+                            float* y, double* loss) {
+  return guarded([&] {
+    EVOC_CHECK(x && t && w0 && b0 && w1 && b1 && w2 && b2 && gw0 && gb0 && gw1 && gb1 && gw2 &&
+                   gb2 && y && loss, "mlp: NULL buffer");
+    EVOC_CHECK(rows >= 1 && d >= 1 && out >= 1 && out <= 32, "mlp: bad shape");
+    Ctx& c = context(0);
+    const float* w[3] = {w0, w1, w2};
+    const float* b[3] = {b0, b1, b2};
+    float* gw[3] = {gw0, gw1, gw2};
+    float* gb[3] = {gb0, gb1, gb2};
+    mlp_loss_and_grads(x, t, rows, d, out, w, b, gw, gb, y, loss, c.stream, c.pool);
+  });
+}
+
 }  // extern "C"

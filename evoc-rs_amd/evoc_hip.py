@@ -109,6 +109,8 @@ _SIGS = {
                                           C.POINTER(C.c_int64)]),
     "evoc_hip_search_n_clusters": (C.c_int, [f32p, C.c_int64, C.c_int32, C.c_int32, C.c_int64,
                                              i64p, f32p]),
+    "evoc_hip_mlp_loss_grads": (C.c_int, [f32p, f32p, C.c_int32, C.c_int32, C.c_int32] + [f32p] * 12
+                                + [f32p, C.POINTER(C.c_double)]),
 }
 
 
@@ -359,3 +361,22 @@ def search_for_n_clusters(emb, min_samples, target_k):
     st = np.empty(n, np.float32)
     _check(lib().evoc_hip_search_n_clusters(emb, n, dim, min_samples, target_k, labels, st))
     return labels, st
+
+
+def mlp_loss_grads(x, t, weights, biases):
+    """One forward/backward pass of the warm-start MLP (bf16 MFMA, fp32 accumulation).
+    weights[l]: (out_l, in_l) float32, biases[l]: (out_l,). Returns (loss, y, gw, gb)."""
+    x = np.ascontiguousarray(x, np.float32)
+    t = np.ascontiguousarray(t, np.float32)
+    rows, d = x.shape
+    out = t.shape[1]
+    ws = [np.ascontiguousarray(w, np.float32) for w in weights]
+    bs = [np.ascontiguousarray(b, np.float32) for b in biases]
+    gw = [np.empty_like(w) for w in ws]
+    gb = [np.empty_like(b) for b in bs]
+    y = np.empty((rows, out), np.float32)
+    loss = C.c_double(0)
+    _check(lib().evoc_hip_mlp_loss_grads(x, t, rows, d, out, ws[0], bs[0], ws[1], bs[1], ws[2],
+                                         bs[2], gw[0], gb[0], gw[1], gb[1], gw[2], gb[2], y,
+                                         C.byref(loss)))
+    return loss.value, y, gw, gb

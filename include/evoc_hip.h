@@ -150,6 +150,17 @@ int evoc_hip_cluster_layers(const float* emb, int64_t n, int32_t dim, int32_t mi
 int evoc_hip_search_n_clusters(const float* emb, int64_t n, int32_t dim, int32_t min_samples,
                                int64_t target_k, int64_t* labels, float* strengths);
 
+/* Learned warm start (no counterpart in the reference): one forward/backward pass of
+ * the MLP  d -> 256 -> 256 -> out  (GELU; bf16 MFMA, fp32 accumulation) for the mean
+ * squared error against t (rows x out). Weights are fp32 host arrays, layer l has
+ * w[l] (out_l x in_l, row-major) and b[l]; gradients are written in the same layout,
+ * y receives the prediction, *loss the MSE. Used to check the backward kernels. */
+int evoc_hip_mlp_loss_grads(const float* x, const float* t, int32_t rows, int32_t d,
+                            int32_t out, const float* w0, const float* b0, const float* w1,
+                            const float* b1, const float* w2, const float* b2, float* gw0,
+                            float* gb0, float* gw1, float* gb1, float* gw2, float* gb2,
+                            float* y, double* loss);
+
 #ifdef __cplusplus
 }
 #endif

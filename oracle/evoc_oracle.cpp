@@ -89,6 +89,11 @@ inline float cos_dist(float dot, float na, float nb) {
   return 1.0f - c;
 }
 
+// Canonical f32 exponential of the fuzzy set: exp in f64, rounded once to f32. The
+// reference calls f32::exp (libm expf, within ~0.5 ulp of this); the correctly rounded
+// value is the platform-independent choice both sides share.
+inline float cexp(float x) { return (float)std::exp((double)x); }
+
 // Low-dimensional squared distance (embedding space): sequential sum.
 template <typename T>
 inline T dsq_seq(const T* a, const T* b, int dim) {
@@ -214,7 +219,7 @@ static void smooth_knn_dist(const float* dists, int64_t n, int k, size_t k_targe
       float val = 0.0f;
       for (int t = 0; t < k; ++t) {
         float adjusted = std::fmax(d[t] - r, 0.0f);
-        val += std::exp(-(adjusted / mid));
+        val += cexp(-(adjusted / mid));
       }
       if (std::fabs((double)val - target) < tol) break;
       if ((double)val > target) {
@@ -253,7 +258,7 @@ extern "C" int64_t orc_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist
       if (j == i || j < 0) continue;
       float adjusted = std::fmax(knn_dist[i * k + t] - rho[i], 0.0f);
       float w;
-      if (sigma[i] > 0.0f) w = std::exp(-(adjusted / sigma[i]));
+      if (sigma[i] > 0.0f) w = cexp(-(adjusted / sigma[i]));
       else if (adjusted > 0.0f) w = 0.0f;
       else w = 1.0f;
       rr.push_back(i);
