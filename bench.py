@@ -116,6 +116,9 @@ def main():
     ap.add_argument("--warm-start", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one device copy of the input (known bytes) before the timed "
+                         "region, to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -131,6 +134,12 @@ def main():
     nnp = H.NearestNeighbourParams(dist_metric="cosine", knn_precision="auto",
                                    warm_start=args.warm_start)
     ptr = x.data_ptr()
+    if args.calibrate_copy:
+        y = x.clone()
+        torch.cuda.synchronize()
+        print(f"# calibration copy: {x.numel() * 4} bytes read and written", file=sys.stderr,
+              flush=True)
+        del y
     for w in range(args.warmup):
         r = H.evoc_device(ptr, n, d, params=params, nn_params=nnp, seed=args.seed)
         print(f"# warmup {w}: {r.stage_ms}", file=sys.stderr, flush=True)

@@ -92,7 +92,7 @@ __device__ __forceinline__ void list_insert(float (&lk)[C], int (&li)[C], float 
   int ni = idx;
 #pragma unroll
   for (int q = 0; q < C; ++q) {
-    bool sw = (nk < lk[q]) || (nk == lk[q] && ni < li[q]);
+    const bool sw = (nk < lk[q]) | ((nk == lk[q]) & (ni < li[q]));  // no short-circuit branches
     float tk = sw ? lk[q] : nk;
     int ti = sw ? li[q] : ni;
     lk[q] = sw ? nk : lk[q];
@@ -114,6 +114,17 @@ __device__ __forceinline__ void wait_vm() {
 __device__ __forceinline__ int slot_of(int r, int q) { return r * 4 + (q ^ ((r >> 2) & 3)); }
 
 typedef __attribute__((address_space(3))) void lds_void;
+
+// Workgroup barrier for the direct-to-LDS ring: each wavefront has already waited for
+// its own loads of the stage with wait_vm<N>(), so no memory fence is wanted here
+// (__syncthreads() fences LDS, and with loads in flight that means vmcnt(0), draining
+// the whole prefetch). The empty asm statements keep the compiler from moving memory
+// operations across the barrier.
+__device__ __forceinline__ void ring_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
 
 template <bool SPLIT, int C>  // C = list length per column half
 __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
@@ -362,6 +373,303 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
   }
 }
 
+// ------------------------------------------------------------------ 2b. resident-A top-C
+// bf16-exact inputs (one MFMA per product). A 256-thread workgroup owns 256 query rows;
+// wavefront w keeps the MFMA A-fragments of its 64 rows (all of d <= 512) in registers
+// for the whole sweep, so the query block is read from memory once and only 32-column
+// database tiles stream, in 128-wide k-chunks, through an 8-stage LDS ring fed by direct
+// global->LDS loads; every B fragment read from LDS feeds two MFMAs. Lane l of
+// wavefront w owns the candidate list of row w*64 + l, so the whole epilogue of a tile
+// (keys, threshold filter, queue, list inserts) is private to the wavefront: the only
+// workgroup barrier is the one per ring stage.
+constexpr int RA_ROWS = 256;   // rows per workgroup
+constexpr int RA_MAXD = 512;   // largest d the resident-query kernel takes
+constexpr int RA_BN = 32;      // columns per tile
+constexpr int RA_KC = 128;     // k per chunk
+constexpr int RA_NST = 8;      // LDS ring stages
+constexpr int RA_SCS = 36;     // padded key-tile row stride (floats)
+constexpr int RA_STB_B = RA_BN * RA_KC * 2;              // 8 KB of database tile per stage
+constexpr int RA_STB = RA_STB_B + 4 * 256;               // + each wavefront's copy of the
+                                                         //   tile's column scales
+constexpr int RA_WREG = 64 * RA_SCS * 4;                 // per-wavefront key tile / buckets
+constexpr int RA_BKT = RA_WREG / (64 * 8);               // bucket slots per row (18)
+constexpr size_t RA_SMEM = (size_t)RA_NST * RA_STB;                // dynamic: the ring
+constexpr int RA_EPI_BYTES = 4 * RA_WREG + RA_ROWS * 8;             // static: epilogue
+
+// 16-byte LDS slot of piece q (8 bf16 along k) of staged column c: 16 consecutive
+// columns read with the same q land in 16 distinct slot positions mod 16
+__device__ __forceinline__ int ra_slot(int c, int q) { return c * 16 + (q ^ (c & 15)); }
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Sorted insert of (nk, ni) into an ascending (key, index) list, branch-free: the
+// tie-break is combined with bitwise operators so that the compiler emits compares and
+// selects rather than one exec-mask branch per element. Inserting a pair that is not
+// below the last element leaves the list unchanged.
+template <int C>
+__device__ __forceinline__ void cand_insert(float (&lk)[C], int (&li)[C], float nk, int ni) {
+#pragma unroll
+  for (int q = 0; q < C; ++q) {
+    const bool lt = nk < lk[q], eq = nk == lk[q], il = ni < li[q];
+    const bool sw = lt | (eq & il);
+    const float tk = sw ? lk[q] : nk;
+    const int ti = sw ? li[q] : ni;
+    lk[q] = sw ? nk : lk[q];
+    li[q] = sw ? ni : li[q];
+    nk = tk;
+    ni = ti;
+  }
+}
+
+// byte address of a shared-memory location, as LDS instructions take it
+template <typename T>
+__device__ __forceinline__ unsigned lds_addr(T* p) {
+  return (unsigned)(uintptr_t)(__attribute__((address_space(3))) char*)(p);
+}
+
+// LDS accesses of the tile epilogue, written as asm so that the compiler, which cannot
+// tell them apart from the ring's direct-to-LDS loads in flight, does not put a full
+// vmcnt(0) before them. Each load waits for itself inside its own block; stores need no
+// wait (LDS executes one wavefront's operations in order).
+__device__ __forceinline__ void lds_st_b64(unsigned a, int2 v) {
+  asm volatile("ds_write_b64 %0, %1" ::"v"(a), "v"(v) : "memory");
+}
+__device__ __forceinline__ void lds_st_b32(unsigned a, int v) {
+  asm volatile("ds_write_b32 %0, %1" ::"v"(a), "v"(v) : "memory");
+}
+__device__ __forceinline__ int lds_ld_b32(unsigned a) {
+  int v;
+  asm volatile("ds_read_b32 %0, %1\n s_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(a) : "memory");
+  return v;
+}
+__device__ __forceinline__ int2 lds_ld_b64(unsigned a) {
+  int2 v;
+  asm volatile("ds_read_b64 %0, %1\n s_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(a) : "memory");
+  return v;
+}
+
+template <int C, int NKC>  // C = list length, NKC = dpad / 128
+__global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
+    const uint16_t* __restrict__ Xh, int64_t n, int64_t npad, int dpad,
+    const float* __restrict__ colscale, int metric, int32_t* __restrict__ cand_idx,
+    float* __restrict__ cand_key) {
+  constexpr int NST = RA_NST;
+  // The ring (written by direct-to-LDS loads) and the epilogue state are separate LDS
+  // objects, so the compiler can see that epilogue LDS accesses never alias loads in
+  // flight and does not drain the prefetch before them.
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  __shared__ __attribute__((aligned(16))) char epi[RA_EPI_BYTES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  char* ring = smem;
+  float* sc = reinterpret_cast<float*>(epi + wave * RA_WREG);
+  int2* bucket = reinterpret_cast<int2*>(sc);  // [64 rows][RA_BKT] (key bits, column)
+  float* thr = reinterpret_cast<float*>(epi + 4 * RA_WREG) + wave * 64;
+  int* rcnt = reinterpret_cast<int*>(epi + 4 * RA_WREG + RA_ROWS * 4) + wave * 64;
+
+  const int64_t row0 = (int64_t)blockIdx.x * RA_ROWS;
+  const int64_t ntile = npad / RA_BN;
+  const int64_t total = ntile * NKC;
+
+  // resident A fragments: rows wave*64 + tm*32 + (lane & 31), k = f*16 + (lane >> 5)*8
+  bf16x8 afr[2][NKC * 8];
+#pragma unroll
+  for (int tm = 0; tm < 2; ++tm) {
+    const uint16_t* ar = Xh + (row0 + wave * 64 + tm * 32 + (lane & 31)) * dpad + (lane >> 5) * 8;
+#pragma unroll
+    for (int f = 0; f < NKC * 8; ++f) afr[tm][f] = *reinterpret_cast<const bf16x8*>(ar + f * 16);
+  }
+  wait_vm<0>();  // the sweep's vmcnt accounting starts from an empty queue
+  float lk[C];  // this lane's row: best C (key, column) pairs, ascending
+  int li[C];
+#pragma unroll
+  for (int q = 0; q < C; ++q) {
+    lk[q] = FINF;
+    li[q] = -1;
+  }
+  thr[lane] = FINF;
+  rcnt[lane] = 0;
+
+  // two 16-byte direct-to-LDS loads per thread per stage: 32 columns x 16 pieces
+  // Three direct-to-LDS loads per thread per stage: two 16-byte pieces of the 32-column
+  // x 128-k database chunk, and the tile's column scales (4 bytes per lane, lanes 32..63
+  // repeat lanes 0..31) into this wavefront's own 256-byte slot. Stages past the end
+  // reload the last stage into a slot no later chunk reads, so every chunk issues the
+  // same number of loads and all vmcnt counts are static.
+  auto issue = [&](int64_t g) {
+    const int64_t gs = g < total ? g : total - 1;
+    const int64_t ct = gs / NKC;
+    const int kc = (int)(gs - ct * NKC);
+    char* st = ring + (int)(g % NST) * RA_STB;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int blk = h * 4 + wave;
+      const int s = blk * 64 + lane;
+      const int c = s >> 4, qs = s & 15;
+      const int q = qs ^ (c & 15);
+      const uint16_t* gp = Xh + (ct * RA_BN + c) * dpad + kc * RA_KC + q * 8;
+      __builtin_amdgcn_global_load_lds(gp, (lds_void*)(st + blk * 1024), 16, 0, 0);
+    }
+    __builtin_amdgcn_global_load_lds(colscale + ct * RA_BN + (lane & 31),
+                                     (lds_void*)(st + RA_STB_B + wave * 256), 4, 0, 0);
+  };
+
+  for (int64_t g = 0; g < NST - 1; ++g) issue(g);
+  for (int64_t ct = 0; ct < ntile; ++ct) {
+    f32x16 acc[2] = {(f32x16){}, (f32x16){}};
+#pragma unroll
+    for (int kc = 0; kc < NKC; ++kc) {
+      const int64_t g = ct * NKC + kc;
+      wait_vm<3 * (NST - 2)>();  // stage g landed (three loads per stage per thread)
+      ring_barrier();
+      issue(g + NST - 1);
+      const char* st = ring + (int)(g % NST) * RA_STB;
+      bf16x8 bq[RA_KC / 16];  // all B fragments of the stage in flight at once
+#pragma unroll
+      for (int kk = 0; kk < RA_KC / 16; ++kk)
+        bq[kk] = *reinterpret_cast<const bf16x8*>(st + ra_slot(lane & 31, kk * 2 + (lane >> 5)) * 16);
+#pragma unroll
+      for (int kk = 0; kk < RA_KC / 16; ++kk)
+#pragma unroll
+        for (int tm = 0; tm < 2; ++tm)
+          acc[tm] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afr[tm][kc * 8 + kk], bq[kk], acc[tm], 0, 0, 0);
+    }
+
+    // ---------------- wavefront-private epilogue of column tile ct
+    // element (tm, e) of this lane: local row tm*32 + (e>>2)*8 + (lane>>5)*4 + (e&3) of
+    // the wavefront, column ct*32 + (lane & 31)
+    const bool diag = (ct * RA_BN < row0 + RA_ROWS) && (ct * RA_BN + RA_BN > row0);
+    const bool tail = (ct + 1) * RA_BN > n;
+    // This lane's 32 row thresholds and its column scale, read in one block with its
+    // own wait: the compiler, seeing LDS reads while direct-to-LDS loads are in flight,
+    // would otherwise wait for the whole prefetch (vmcnt(0)) although these addresses
+    // are never written by those loads. The scale comes from the last chunk's stage,
+    // complete since that chunk's barrier and not reused before the next chunk.
+    f32x4 tq[8];
+    float cs;
+    {
+      const unsigned ta = lds_addr(thr + ((lane >> 5) * 4));
+      const unsigned ca = lds_addr(ring + (int)((ct * NKC + NKC - 1) % NST) * RA_STB + RA_STB_B +
+                                   wave * 256 + lane * 4);
+      asm volatile(
+          "ds_read_b128 %0, %9\n"
+          "ds_read_b128 %1, %9 offset:32\n"
+          "ds_read_b128 %2, %9 offset:64\n"
+          "ds_read_b128 %3, %9 offset:96\n"
+          "ds_read_b128 %4, %9 offset:128\n"
+          "ds_read_b128 %5, %9 offset:160\n"
+          "ds_read_b128 %6, %9 offset:192\n"
+          "ds_read_b128 %7, %9 offset:224\n"
+          "ds_read_b32 %8, %10\n"
+          "s_waitcnt lgkmcnt(0)"
+          : "=v"(tq[0]), "=v"(tq[1]), "=v"(tq[2]), "=v"(tq[3]), "=v"(tq[4]), "=v"(tq[5]),
+            "=v"(tq[6]), "=v"(tq[7]), "=v"(cs)
+          : "v"(ta), "v"(ca)
+          : "memory");
+    }
+    // per-lane bases made opaque once per tile, so the per-element indices below are
+    // base + compile-time constant rather than values hoisted out of the sweep
+    const int lrow = (lane >> 5) * 4;
+    const int lcol = lane & 31;
+    int dv = (int)(ct * RA_BN - row0) - wave * 64 + lcol - lrow;  // column - row, locally
+    int rem = (int)std::min<int64_t>(n - ct * RA_BN, RA_BN) - lcol;
+    int sb = lrow * RA_SCS + lcol;
+    asm volatile("" : "+v"(dv), "+v"(rem), "+v"(sb));
+#pragma unroll
+    for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const float s_ = acc[tm][e];
+        float k_ = metric == 1 ? -(s_ * cs) : (cs - 2.0f * s_);
+        if (diag || tail) {
+          if (rem <= 0 || dv == tm * 32 + (e >> 2) * 8 + (e & 3)) k_ = FINF;
+        }
+        acc[tm][e] = k_;
+      }
+    // Keys below their row's threshold go to that row's bucket. Row r of the wavefront
+    // sits in one register slot (tm, j, i) of one lane half, one column per lane, so a
+    // ballot gives every passing key its bucket position and the row its count.
+    const int cbase = (int)(ct * RA_BN);
+    bool touched = false;
+    const bool lo_half = lane < 32;
+#pragma unroll
+    for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const f32x4 t4 = tq[tm * 4 + j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float key = acc[tm][j * 4 + i];
+          const float tv_i = t4[i];
+          const bool pass = key < tv_i;
+          const unsigned long long m = __ballot(pass);
+          if (m) {
+            touched = true;
+            const unsigned mlo = (unsigned)m, mhi = (unsigned)(m >> 32);
+            const int pos = lo_half ? __builtin_amdgcn_mbcnt_lo(mlo, 0u)
+                                    : __builtin_amdgcn_mbcnt_hi(mhi, 0u);
+            const int r = lrow + tm * 32 + j * 8 + i;
+            if (pass && pos < RA_BKT)
+              lds_st_b64(lds_addr(bucket + r * RA_BKT + pos), make_int2(__float_as_int(key), lcol));
+            if (lcol == 0) lds_st_b32(lds_addr(rcnt + r), __popc(lo_half ? mlo : mhi));
+          }
+        }
+      }
+    if (touched) {
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_wave_barrier();
+      const int cnt = lds_ld_b32(lds_addr(rcnt + lane));
+      lds_st_b32(lds_addr(rcnt + lane), 0);
+      if (__ballot(cnt > RA_BKT) == 0) {
+        // every lane drains its own row's bucket: cost is the largest count, not the sum
+#pragma unroll 1
+        for (int e = 0; e < cnt; ++e) {
+          const int2 en = lds_ld_b64(lds_addr(bucket + lane * RA_BKT + e));
+          // unconditional: a branch here would be if-converted into a second live copy
+          // of the list
+          cand_insert<C>(lk, li, __int_as_float(en.x), cbase + en.y);
+        }
+      } else {
+        // bucket overflow (the first tiles): dump the key tile, each lane scans its row
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+          for (int e = 0; e < 16; ++e)
+            sc[sb + (tm * 32 + (e >> 2) * 8 + (e & 3)) * RA_SCS] = acc[tm][e];
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+        const float* srow = sc + lane * RA_SCS;
+#pragma unroll 1
+        for (int q = 0; q < RA_BN; ++q) {
+          cand_insert<C>(lk, li, srow[q], cbase + q);
+        }
+      }
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_wave_barrier();
+      lds_st_b32(lds_addr(thr + lane), __float_as_int(lk[C - 1]));
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  wait_vm<0>();  // the tail reloads have landed before the wavefront ends
+  const int64_t grow = row0 + tid;
+  if (grow < n) {
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      cand_key[grow * C + q] = lk[q];
+      cand_idx[grow * C + q] = li[q];
+    }
+  }
+}
+
+template __global__ void knn_topc_ra_kernel<32, 1>(const uint16_t*, int64_t, int64_t, int,
+                                                   const float*, int, int32_t*, float*);
+template __global__ void knn_topc_ra_kernel<32, 2>(const uint16_t*, int64_t, int64_t, int,
+                                                   const float*, int, int32_t*, float*);
+template __global__ void knn_topc_ra_kernel<32, 4>(const uint16_t*, int64_t, int64_t, int,
+                                                   const float*, int, int32_t*, float*);
+
 // ------------------------------------------------------------------ 3. re-rank
 constexpr int MAXT = 16;  // d <= 1024 handled with the query row held in registers
 
@@ -369,7 +677,7 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     const float* __restrict__ X, int64_t n, int d, int64_t ld, const float* __restrict__ nrm,
     const float* __restrict__ sqn, const float* __restrict__ colscale,
     const unsigned* __restrict__ flags, const int32_t* __restrict__ cand_idx,
-    const float* __restrict__ cand_key, int C, int k, int metric, float gamma,
+    const float* __restrict__ cand_key, int C, int halves, int k, int metric, float gamma,
     int32_t* __restrict__ out_idx, float* __restrict__ out_dist, int* __restrict__ fail_count,
     int32_t* __restrict__ fail_rows) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -440,8 +748,9 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     const int src = __ffsll((long long)kb) - 1;
     const float dk = __shfl(my_d, src, 64);
     const float dk2 = __shfl(my_sq, src, 64);
-    const int ch = C / 2;  // each column half kept its ch best keys
-    const float kc = fminf(cand_key[row * C + (ch - 1)], cand_key[row * C + (C - 1)]);
+    // one sorted list of C keys, or two column halves that each kept their C/2 best
+    const float kc = halves == 2 ? fminf(cand_key[row * C + (C / 2 - 1)], cand_key[row * C + (C - 1)])
+                                 : cand_key[row * C + (C - 1)];
     const float u = 5.9604645e-08f;  // 2^-24
     if (metric == 1) {
       const float eps = 1.5f * gamma + 64.0f * u;
@@ -557,46 +866,6 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   EVOC_CHECK(n >= 1 && d >= 1, "knn: empty input");
   EVOC_CHECK(k >= 1 && k <= KMAX, "knn: k must be in [1, 32]");
   EVOC_CHECK(d <= MAXT * 64, "knn: d must be <= 1024");
-  const int CH = (k <= 16) ? 16 : 32;  // per column half
-  const int C = 2 * CH;
-  const int64_t npad = ceil_div(n, BM) * BM;
-  const int dpad = (int)(ceil_div(d, KC) * KC);
-  Buf<uint16_t> Xh(pool, npad * dpad), Xl;
-  Buf<float> nrm(pool, npad), sqn(pool, npad), colscale(pool, npad);
-  Buf<unsigned> flags(pool, 4);
-  Buf<int32_t> cidx(pool, n * C), frows(pool, n);
-  Buf<float> ckey(pool, n * C);
-  Buf<int> fcount(pool, 1);
-  EVOC_HIP(hipMemsetAsync(flags.p, 0, 4 * sizeof(unsigned), s));
-  EVOC_HIP(hipMemsetAsync(fcount.p, 0, sizeof(int), s));
-  // decide precision: exact bf16 inputs need one MFMA per product, others the split form
-  bool split;
-  if (precision == KNN_BF16) {
-    split = false;
-  } else {
-    Xl.alloc(pool, npad * dpad);
-    split = true;
-  }
-  hipEvent_t e0, e1;
-  EVOC_HIP(hipEventCreate(&e0));
-  EVOC_HIP(hipEventCreate(&e1));
-  knn_prep_kernel<<<(unsigned)ceil_div(npad, 4), 256, 0, s>>>(dX, n, d, ld, npad, dpad, Xh.p,
-                                                               Xl.p, nrm.p, sqn.p, colscale.p,
-                                                               metric, flags.p);
-  EVOC_HIP(hipGetLastError());
-  unsigned hflags[2];
-  EVOC_HIP(hipMemcpyAsync(hflags, flags.p, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
-  EVOC_HIP(hipStreamSynchronize(s));
-  if (split && precision == KNN_AUTO && hflags[0] == 0) split = false;  // input is bf16-exact
-  const float u = 5.9604645e-08f;
-  const float e_in = split ? 4.0f * 1.5258789e-05f : (hflags[0] ? 2.0f * 0.0078125f : 0.0f);
-  const float gamma = 2.0f * (float)dpad * u + e_in;
-  const int nops = split ? 4 : 2;
-  const int nst = split ? 2 : 4;
-  const size_t smem = (size_t)nst * nops * BM * 4 * 16 + BM * SCS * sizeof(float) +
-                      2 * BM * sizeof(float) + 4 * sizeof(int);
-  EVOC_HIP(hipEventRecord(e0, s));
-  const unsigned grid = (unsigned)(npad / BM);
   static bool attr_set = false;
   if (!attr_set) {
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<true, 16>,
@@ -607,38 +876,120 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<false, 32>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 1>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RA_SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 2>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RA_SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 4>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RA_SMEM));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_exact_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
     attr_set = true;
   }
-  if (split) {
-    if (CH == 16)
-      knn_topc_kernel<true, 16><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
-                                                        metric, cidx.p, ckey.p);
-    else
-      knn_topc_kernel<true, 32><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
-                                                        metric, cidx.p, ckey.p);
+  // Two tile kernels: the resident-query kernel for bf16-exact inputs (d <= 512, k <= 16,
+  // one list of 32 per row), else the 128x128 streaming kernel (split or not, two column
+  // halves of CH per row). The prep layout depends on the choice; an AUTO guess of
+  // bf16-exact that the prep flags refute is prepared again in the split layout.
+  const int CH = (k <= 16) ? 16 : 32;
+  const bool ra_possible = (precision != KNN_SPLIT) && d <= RA_MAXD && k <= 16;
+  Buf<float> nrm, sqn, colscale;
+  Buf<unsigned> flags(pool, 4);
+  Buf<uint16_t> Xh, Xl;
+  unsigned hflags[2] = {0, 0};
+  int64_t npad = 0;
+  int dpad = 0;
+  bool ra = false, split = false;
+  auto prep = [&](bool use_ra, bool want_lo) {
+    npad = use_ra ? ceil_div(n, RA_ROWS) * RA_ROWS : ceil_div(n, BM) * BM;
+    dpad = use_ra ? (d <= 128 ? 128 : d <= 256 ? 256 : 512) : (int)(ceil_div(d, KC) * KC);
+    Xh.alloc(pool, npad * dpad);
+    if (want_lo) Xl.alloc(pool, npad * dpad);
+    nrm.alloc(pool, npad);
+    sqn.alloc(pool, npad);
+    colscale.alloc(pool, npad);
+    EVOC_HIP(hipMemsetAsync(flags.p, 0, 4 * sizeof(unsigned), s));
+    knn_prep_kernel<<<(unsigned)ceil_div(npad, 4), 256, 0, s>>>(
+        dX, n, d, ld, npad, dpad, Xh.p, want_lo ? Xl.p : nullptr, nrm.p, sqn.p, colscale.p,
+        metric, flags.p);
+    EVOC_HIP(hipGetLastError());
+    EVOC_HIP(hipMemcpyAsync(hflags, flags.p, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipStreamSynchronize(s));
+  };
+  if (ra_possible) {
+    prep(true, false);
+    if (precision == KNN_BF16 || hflags[0] == 0) {
+      ra = true;
+    } else {  // AUTO and the input is not bf16-representable
+      Xh.reset();
+      nrm.reset();
+      sqn.reset();
+      colscale.reset();
+      prep(false, true);
+      split = true;
+    }
   } else {
-    if (CH == 16)
-      knn_topc_kernel<false, 16><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad, colscale.p,
-                                                         metric, cidx.p, ckey.p);
+    const bool want_lo = precision != KNN_BF16;
+    prep(false, want_lo);
+    split = want_lo && !(precision == KNN_AUTO && hflags[0] == 0);
+  }
+  const int C = ra ? 32 : 2 * CH;
+  Buf<int32_t> cidx(pool, n * C), frows(pool, n);
+  Buf<float> ckey(pool, n * C);
+  Buf<int> fcount(pool, 1);
+  EVOC_HIP(hipMemsetAsync(fcount.p, 0, sizeof(int), s));
+  const float u = 5.9604645e-08f;
+  const float e_in = split ? 4.0f * 1.5258789e-05f : (hflags[0] ? 2.0f * 0.0078125f : 0.0f);
+  const float gamma = 2.0f * (float)dpad * u + e_in;
+  hipEvent_t e0, e1;
+  EVOC_HIP(hipEventCreate(&e0));
+  EVOC_HIP(hipEventCreate(&e1));
+  EVOC_HIP(hipEventRecord(e0, s));
+  if (ra) {
+    const unsigned grid = (unsigned)(npad / RA_ROWS);
+    if (dpad == 128)
+      knn_topc_ra_kernel<32, 1><<<grid, 256, RA_SMEM, s>>>(Xh.p, n, npad, dpad, colscale.p, metric,
+                                                           cidx.p, ckey.p);
+    else if (dpad == 256)
+      knn_topc_ra_kernel<32, 2><<<grid, 256, RA_SMEM, s>>>(Xh.p, n, npad, dpad, colscale.p, metric,
+                                                           cidx.p, ckey.p);
     else
-      knn_topc_kernel<false, 32><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad, colscale.p,
-                                                         metric, cidx.p, ckey.p);
+      knn_topc_ra_kernel<32, 4><<<grid, 256, RA_SMEM, s>>>(Xh.p, n, npad, dpad, colscale.p, metric,
+                                                           cidx.p, ckey.p);
+  } else {
+    const int nops = split ? 4 : 2;
+    const int nst = split ? 2 : 4;
+    const size_t smem = (size_t)nst * nops * BM * 4 * 16 + BM * SCS * sizeof(float) +
+                        2 * BM * sizeof(float) + 4 * sizeof(int);
+    const unsigned grid = (unsigned)(npad / BM);
+    if (split) {
+      if (CH == 16)
+        knn_topc_kernel<true, 16><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
+                                                          metric, cidx.p, ckey.p);
+      else
+        knn_topc_kernel<true, 32><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
+                                                          metric, cidx.p, ckey.p);
+    } else {
+      if (CH == 16)
+        knn_topc_kernel<false, 16><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad,
+                                                           colscale.p, metric, cidx.p, ckey.p);
+      else
+        knn_topc_kernel<false, 32><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad,
+                                                           colscale.p, metric, cidx.p, ckey.p);
+    }
   }
   EVOC_HIP(hipGetLastError());
   EVOC_HIP(hipEventRecord(e1, s));
   knn_rerank_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(
-      dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, cidx.p, ckey.p, C, k, metric, gamma, d_idx,
-      d_dist, fcount.p, frows.p);
+      dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, cidx.p, ckey.p, C, ra ? 1 : 2, k, metric,
+      gamma, d_idx, d_dist, fcount.p, frows.p);
   EVOC_HIP(hipGetLastError());
   int nfail = 0;
   EVOC_HIP(hipMemcpyAsync(&nfail, fcount.p, sizeof(int), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
   if (nfail > 0) {
     const unsigned g2 = (unsigned)std::min(nfail, 4096);
-    knn_exact_kernel<<<g2, 256, 256 * KMAX * 8, s>>>(dX, n, d, ld, nrm.p, k, metric, fcount.p, frows.p, d_idx,
-                                        d_dist);
+    knn_exact_kernel<<<g2, 256, 256 * KMAX * 8, s>>>(dX, n, d, ld, nrm.p, k, metric, fcount.p,
+                                                     frows.p, d_idx, d_dist);
     EVOC_HIP(hipGetLastError());
   }
   float mfma_ms = 0.0f;

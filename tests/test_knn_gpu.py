@@ -51,3 +51,38 @@ def test_knn_zero_rows_cosine():
     x = rng.normal(size=(300, 33)).astype(np.float32)
     x[::17] = 0.0
     _check(x, 10, "cosine")
+
+
+@pytest.mark.parametrize("n,d,metric", [
+    (1000, 96, "euclidean"),     # d <= 128 tier, ragged n (not a multiple of 256)
+    (2049, 200, "cosine"),       # d <= 256 tier, one row past a 256-row block
+    (777, 512, "euclidean"),     # d == 512 tier
+    (300, 13, "cosine"),         # tiny d padded to 128
+])
+def test_knn_resident_query_tiers(n, d, metric):
+    """bf16-exact inputs with k <= 16 and d <= 512 run the resident-query tile kernel."""
+    x, _ = datagen.gaussian_mixture(n, d, 12, 4.0, 1.0, seed=n + d)
+    x = datagen.to_bf16_values(x)
+    _check(x, 15, metric)
+    _check(x, 7, metric, precision="bf16")
+
+
+def test_knn_resident_query_ties_zero_rows_tiny():
+    rng = np.random.default_rng(11)
+    base = datagen.to_bf16_values(rng.normal(size=(180, 40)).astype(np.float32))
+    x = np.concatenate([base, base[:60], base[:20]])  # exact duplicates -> distance ties
+    _check(x, 15, "euclidean")
+    _check(x, 16, "cosine")
+    z = base.copy()
+    z[::9] = 0.0
+    _check(z, 10, "cosine")
+    _check(base[:6], 15, "euclidean")   # k > n - 1
+    _check(base[:1], 2, "cosine")
+
+
+def test_knn_forced_bf16_on_inexact_input():
+    """precision='bf16' on float data: the kernel rounds to bf16 and the certificate
+    carries the input-rounding term; the exact re-rank keeps results bit-identical."""
+    x, _ = datagen.unit_mixture(3000, 256, 30, 0.2, seed=5)
+    _check(x, 15, "cosine", precision="bf16")
+    _check(x, 12, "euclidean", precision="bf16")
