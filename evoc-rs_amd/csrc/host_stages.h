@@ -7,9 +7,11 @@
 
 namespace evoc {
 
+// data_dev: the n x d input rows in device memory (row stride ld floats), or null for
+// the random base case
 void label_prop_init_host(const int64_t* indptr, const int32_t* indices, const float* weights,
-                          int64_t n, int dim, const float* data, int64_t d, uint64_t seed,
-                          float* out, hipStream_t s, DevPool& pool, int verbose);
+                          int64_t n, int dim, const float* data_dev, int64_t d, int64_t ld,
+                          uint64_t seed, float* out, hipStream_t s, DevPool& pool, int verbose);
 
 // scipy-style single-linkage rows (src/clustering/linkage.rs:301-310)
 struct Linkage {

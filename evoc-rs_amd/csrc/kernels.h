@@ -131,6 +131,13 @@ struct EmbedParams {
 void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, bool random_init,
                   uint64_t seed, hipStream_t s, DevPool& pool, float* kernel_ms = nullptr);
 
+// ------------------------------------------------------------------ label propagation
+// rows of (R^T_norm)_l1 X for a partition given as ascending member lists:
+// out[a] = coef[a] * sum over members m of a (in list order) of X[m]; host in/out
+void partition_means_device(const float* dX, int64_t ld, int64_t d, const int64_t* h_mptr,
+                            const int32_t* h_mem, const float* h_coef, int64_t np, int64_t n,
+                            float* h_out, hipStream_t s, DevPool& pool);
+
 // ------------------------------------------------------------------ core distances + MST
 void core_sq_device(const float* d_emb, int64_t n, int dim, int min_samples, float* d_core,
                     hipStream_t s, DevPool& pool);

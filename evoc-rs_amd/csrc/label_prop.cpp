@@ -4,6 +4,7 @@
 // reference's sparse algebra (src/utils/sparse.rs) so the layout is reproducible:
 //   * coarsening  R^T_norm G R   (per-partition dense accumulator, columns ascending),
 //   * expansion   ((G o G^T) R_norm)_l1 L  and  (R_norm)_l1 L, averaged, centred, x0.1.
+#include <chrono>
 #include <cstring>
 #include <numeric>
 #include <unordered_map>
@@ -344,16 +345,44 @@ void coarse_embedding(const Sp& g, int dim, int epochs, float alpha, std::vector
   EVOC_HIP(hipStreamSynchronize(s));
 }
 
-std::vector<float> level(const Sp& g, int dim, const std::vector<float>* data, int64_t d,
-                         int epochs, int64_t approx, uint64_t seed, hipStream_t s, DevPool& pool,
+// The data of a level: host rows (coarse levels) or device rows (the input level).
+struct Rows {
+  const std::vector<float>* host = nullptr;
+  const float* dev = nullptr;
+  int64_t ld = 0;
+  bool present() const { return host || dev; }
+};
+
+std::vector<float> level(const Sp& g, int dim, const Rows& data, int64_t d, int epochs,
+                         int64_t approx, uint64_t seed, hipStream_t s, DevPool& pool,
                          int verbose) {
   const int64_t n = g.rows;
   if (n < 64) {
-    if (data && (int64_t)data->size() == n * d && dim <= d) return pca(*data, n, d, dim);
+    if (data.present() && dim <= d) {
+      if (data.host && (int64_t)data.host->size() == n * d) return pca(*data.host, n, d, dim);
+      if (data.dev) {
+        std::vector<float> h(n * d);
+        EVOC_HIP(hipMemcpy2DAsync(h.data(), d * sizeof(float), data.dev, data.ld * sizeof(float),
+                                  d * sizeof(float), n, hipMemcpyDeviceToHost, s));
+        EVOC_HIP(hipStreamSynchronize(s));
+        return pca(h, n, d, dim);
+      }
+    }
     return sphere_init(n, dim, seed);
   }
+  using tclk = std::chrono::steady_clock;
+  auto tick = tclk::now();
+  auto lap = [&](const char* what) {
+    if (verbose >= 2) {
+      const auto now = tclk::now();
+      std::printf("    [n=%lld] %s: %.1f ms\n", (long long)n, what,
+                  std::chrono::duration<double, std::milli>(now - tick).count());
+      tick = now;
+    }
+  };
   int64_t np = 0;
   const std::vector<int32_t> part = lp_partition(g, approx, 20, seed, &np);
+  lap("label propagation");
   if (verbose) std::printf("  Label prop: %lld nodes -> %lld partitions\n", (long long)n, (long long)np);
   std::vector<int64_t> mptr;
   std::vector<int32_t> mem;
@@ -367,26 +396,40 @@ std::vector<float> level(const Sp& g, int dim, const std::vector<float>* data, i
     sa[a] = nrm > 0.0f ? 1.0f / nrm : 1.0f;
   }
   const Sp coarse = coarsen(g, part, np, mptr, mem, sa);
+  lap("coarsen graph");
   std::vector<float> cdata;
-  if (data) {
+  if (data.present()) {
     // rows of R^T_norm, L1-normalised (sparse.rs:479-501), times the data
-    cdata.assign(np * d, 0.0f);
-#pragma omp parallel for schedule(dynamic, 8)
+    std::vector<float> coef(np);
     for (int64_t a = 0; a < np; ++a) {
       float nrm = 0.0f;
       for (int64_t m = mptr[a]; m < mptr[a + 1]; ++m) nrm += std::fabs(sa[a]);
-      const float coef = nrm > 0.0f ? sa[a] * (1.0f / nrm) : sa[a];
-      float* out = &cdata[a * d];
-      for (int64_t m = mptr[a]; m < mptr[a + 1]; ++m) {
-        const float* x = &(*data)[(int64_t)mem[m] * d];
-        for (int64_t c = 0; c < d; ++c) out[c] = out[c] + coef * x[c];
+      coef[a] = nrm > 0.0f ? sa[a] * (1.0f / nrm) : sa[a];
+    }
+    cdata.assign(np * d, 0.0f);
+    if (data.dev) {
+      partition_means_device(data.dev, data.ld, d, mptr.data(), mem.data(), coef.data(), np, n,
+                             cdata.data(), s, pool);
+    } else {
+#pragma omp parallel for schedule(dynamic, 8)
+      for (int64_t a = 0; a < np; ++a) {
+        float* out = &cdata[a * d];
+        for (int64_t m = mptr[a]; m < mptr[a + 1]; ++m) {
+          const float* x = &(*data.host)[(int64_t)mem[m] * d];
+          for (int64_t c = 0; c < d; ++c) out[c] = out[c] + coef[a] * x[c];
+        }
       }
     }
   }
+  lap("coarsen data");
   const int ep = std::min(epochs, 255);
-  std::vector<float> layout = level(coarse, dim, data ? &cdata : nullptr, d, ep, approx / 4,
-                                    seed + 3000, s, pool, verbose);
+  Rows cr;
+  if (data.present()) cr.host = &cdata;
+  std::vector<float> layout = level(coarse, dim, cr, d, ep, approx / 4, seed + 3000, s, pool,
+                                    verbose);
+  tick = tclk::now();
   coarse_embedding(coarse, dim, ep, (float)(0.001 * (double)epochs), layout, seed + 4000, s, pool);
+  lap("coarse embedding");
   // expansion: rows of (G o G^T) R_norm, L1-normalised, times the coarse layout,
   // averaged with the layout of each node's own partition
   std::vector<float> res(n * dim);
@@ -445,14 +488,15 @@ std::vector<float> level(const Sp& g, int dim, const std::vector<float>* data, i
     for (int64_t i = 0; i < n; ++i) res[i * dim + c] = res[i * dim + c] - mean;
   }
   for (auto& v : res) v = 0.1f * v;
+  lap("expansion");
   return res;
 }
 
 }  // namespace
 
 void label_prop_init_host(const int64_t* indptr, const int32_t* indices, const float* weights,
-                          int64_t n, int dim, const float* data, int64_t d, uint64_t seed,
-                          float* out, hipStream_t s, DevPool& pool, int verbose) {
+                          int64_t n, int dim, const float* data_dev, int64_t d, int64_t ld,
+                          uint64_t seed, float* out, hipStream_t s, DevPool& pool, int verbose) {
   Sp g;
   g.rows = g.cols = n;
   g.ptr.assign(indptr, indptr + n + 1);
@@ -461,13 +505,12 @@ void label_prop_init_host(const int64_t* indptr, const int32_t* indices, const f
   int64_t approx = (int64_t)(8.0 * std::sqrt((double)n));
   approx = std::min<int64_t>(std::max<int64_t>(approx, 256), 16384);
   approx = std::min<int64_t>(approx, n);
-  std::vector<float> dv;
-  const std::vector<float>* dp = nullptr;
-  if (data) {
-    dv.assign(data, data + n * d);
-    dp = &dv;
+  Rows top;
+  if (data_dev) {
+    top.dev = data_dev;
+    top.ld = ld;
   }
-  std::vector<float> r = level(g, dim, dp, d, 50, approx, seed, s, pool, verbose);
+  std::vector<float> r = level(g, dim, top, d, 50, approx, seed, s, pool, verbose);
   std::memcpy(out, r.data(), n * dim * sizeof(float));
 }
 

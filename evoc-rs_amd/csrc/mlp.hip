@@ -360,10 +360,8 @@ void warm_start_init(const float* dX, int64_t n, int d, int64_t ld, const Graph&
   EVOC_HIP(hipMemcpyAsync(dsub.p, sub.data(), S * sizeof(int64_t), hipMemcpyHostToDevice, s));
   Buf<float> xs(pool, (size_t)S * d);
   {
-    std::vector<float> hx((size_t)S * d);
     gather_rows_kernel<<<(unsigned)ceil_div(S * d, 256), 256, 0, s>>>(dX, ld, d, dsub.p, S, xs.p);
-    EVOC_HIP(hipMemcpyAsync(hx.data(), xs.p, (size_t)S * d * sizeof(float), hipMemcpyDeviceToHost, s));
-    EVOC_HIP(hipStreamSynchronize(s));
+    EVOC_HIP(hipGetLastError());
     const int k = p.n_neighbours;
     Buf<int32_t> si(pool, S * k);
     Buf<float> sd(pool, S * k);
@@ -378,7 +376,7 @@ void warm_start_init(const float* dX, int64_t n, int d, int64_t ld, const Graph&
     EVOC_HIP(hipMemcpyAsync(w.data(), sg.weights.p, sg.nnz * sizeof(float), hipMemcpyDeviceToHost, s));
     EVOC_HIP(hipStreamSynchronize(s));
     std::vector<float> init((size_t)S * dim);
-    label_prop_init_host(ptr.data(), col.data(), w.data(), S, dim, hx.data(), d, seed, init.data(),
+    label_prop_init_host(ptr.data(), col.data(), w.data(), S, dim, xs.p, d, d, seed, init.data(),
                          s, pool, 0);
     Buf<float> lay(pool, (size_t)S * dim);
     EVOC_HIP(hipMemcpyAsync(lay.p, init.data(), init.size() * sizeof(float), hipMemcpyHostToDevice, s));

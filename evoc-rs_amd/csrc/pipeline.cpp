@@ -217,18 +217,11 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
     to_host(hptr, g.indptr.p, n + 1, s);
     to_host(hcol, g.indices.p, g.nnz, s);
     to_host(hw, g.weights.p, g.nnz, s);
-    std::vector<float> xh;
-    const float* data_h = hX;
-    if (!data_h) {
-      xh.resize(n * d);
-      EVOC_HIP(hipMemcpy2DAsync(xh.data(), d * sizeof(float), dX, ld * sizeof(float),
-                                d * sizeof(float), n, hipMemcpyDeviceToHost, s));
-      data_h = xh.data();
-    }
     EVOC_HIP(hipStreamSynchronize(s));
+    if (verbose >= 2) std::printf("    graph to host: %.1f ms\n", ms_since(t0));
     std::vector<float> init(n * dim);
     if (p.symmetrise) {
-      label_prop_init_host(hptr.data(), hcol.data(), hw.data(), n, dim, data_h, d, seed,
+      label_prop_init_host(hptr.data(), hcol.data(), hw.data(), n, dim, dX, d, ld, seed,
                            init.data(), s, pool, verbose);
     } else {
       // Csr::from_coo (src/utils/sparse.rs:195-235) orders every row by column
@@ -246,7 +239,7 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
           w2[hptr[i] + q] = row[q].second;
         }
       }
-      label_prop_init_host(hptr.data(), c2.data(), w2.data(), n, dim, data_h, d, seed,
+      label_prop_init_host(hptr.data(), c2.data(), w2.data(), n, dim, dX, d, ld, seed,
                            init.data(), s, pool, verbose);
     }
     EVOC_HIP(hipMemcpyAsync(demb.p, init.data(), n * dim * sizeof(float), hipMemcpyHostToDevice, s));
@@ -451,7 +444,10 @@ int evoc_hip_label_prop_init(const int64_t* indptr, const int32_t* indices, cons
     EVOC_CHECK(indptr && out, "label_prop_init: NULL buffer");
     EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "label_prop_init: dim must be in [1, 16]");
     Ctx& c = context(0);
-    label_prop_init_host(indptr, indices, weights, n, dim, data, d, seed, out, c.stream, c.pool, 0);
+    Buf<float> dx;
+    if (data) to_dev(dx, data, n * d, c.pool, c.stream);
+    label_prop_init_host(indptr, indices, weights, n, dim, data ? dx.p : nullptr, d, d, seed, out,
+                         c.stream, c.pool, 0);
   });
 }
 
