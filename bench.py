@@ -59,7 +59,7 @@ def read_traffic():
     """HBM bytes per launch of the kNN tile kernel from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, KiB per dispatch, scaled by the copy-kernel
     calibration factor recorded next to it), or None when no summary exists."""
-    path = os.path.join(REPO, "profiles", "knn_topc_pmc_1Mx512.txt")
+    path = os.path.join(REPO, "profiles", "knn_topc_ra_pmc_1Mx512.txt")
     if not os.path.exists(path):
         return None
     vals = {}
@@ -73,7 +73,7 @@ def read_traffic():
         return None
 
 
-def cpu_baseline(x_host, k, dim, knn_rows=32, sub=20000, seed=42):
+def cpu_baseline(x_host, k, dim, knn_rows=512, sub=100000, seed=42):
     """The oracle (C++ restatement of evoc-rs on the host cores) on a bounded sample:
     exact kNN of `knn_rows` query rows against all n rows (scaled by n / knn_rows),
     plus the remaining stages on a `sub`-point subsample given its exact kNN (scaled
@@ -159,7 +159,7 @@ def main():
     achieved = flops / (knn_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 4),
-                "traffic": read_traffic(), "kernel": "knn_topc_kernel",
+                "traffic": read_traffic(), "kernel": "knn_topc_ra_kernel",
                 "kernel_ms": round(knn_ms, 3)}
     cpu = None
     if not args.no_cpu_baseline:
