@@ -391,10 +391,12 @@ constexpr int RA_SCS = 36;     // padded key-tile row stride (floats)
 constexpr int RA_STB_B = RA_BN * RA_KC * 2;              // 8 KB of database tile per stage
 constexpr int RA_STB = RA_STB_B + 4 * 256;               // + each wavefront's copy of the
                                                          //   tile's column scales
-constexpr int RA_WREG = 64 * RA_SCS * 4;                 // per-wavefront key tile / buckets
-constexpr int RA_BKT = RA_WREG / (64 * 8);               // bucket slots per row (18)
+constexpr int RA_BKT = 32;                               // bucket slots per row
+constexpr int RA_DRAIN = 16;                             // drain once a row holds this many
+constexpr int RA_WREG = 64 * RA_BKT * 8;                 // per-wavefront buckets / key tile
+static_assert(RA_WREG >= 64 * RA_SCS * 4, "the key-tile dump must fit the bucket region");
 constexpr size_t RA_SMEM = (size_t)RA_NST * RA_STB;                // dynamic: the ring
-constexpr int RA_EPI_BYTES = 4 * RA_WREG + RA_ROWS * 8;             // static: epilogue
+constexpr int RA_EPI_BYTES = 4 * RA_WREG + RA_ROWS * 4;             // static: epilogue
 
 // 16-byte LDS slot of piece q (8 bf16 along k) of staged column c: 16 consecutive
 // columns read with the same q land in 16 distinct slot positions mod 16
@@ -402,21 +404,20 @@ __device__ __forceinline__ int ra_slot(int c, int q) { return c * 16 + (q ^ (c &
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Sorted insert of (nk, ni) into an ascending (key, index) list, branch-free: the
-// tie-break is combined with bitwise operators so that the compiler emits compares and
-// selects rather than one exec-mask branch per element. Inserting a pair that is not
-// below the last element leaves the list unchanged.
+// Sorted insert of (nk, ni) into an ascending key list, branch-free. Equal keys keep
+// their arrival order (a new key goes after its equals), which is deterministic because
+// each row receives its columns in a fixed order. What the certificate needs holds for
+// any arrival order: every pair not in the list has a key >= the list's last key.
 template <int C>
 __device__ __forceinline__ void cand_insert(float (&lk)[C], int (&li)[C], float nk, int ni) {
 #pragma unroll
   for (int q = 0; q < C; ++q) {
-    const bool lt = nk < lk[q], eq = nk == lk[q], il = ni < li[q];
-    const bool sw = lt | (eq & il);
-    const float tk = sw ? lk[q] : nk;
+    const bool sw = nk < lk[q];
     const int ti = sw ? li[q] : ni;
-    lk[q] = sw ? nk : lk[q];
     li[q] = sw ? ni : li[q];
-    nk = tk;
+    const float lo = fminf(nk, lk[q]), hi = fmaxf(nk, lk[q]);
+    lk[q] = lo;
+    nk = hi;
     ni = ti;
   }
 }
@@ -464,7 +465,6 @@ __global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
   float* sc = reinterpret_cast<float*>(epi + wave * RA_WREG);
   int2* bucket = reinterpret_cast<int2*>(sc);  // [64 rows][RA_BKT] (key bits, column)
   float* thr = reinterpret_cast<float*>(epi + 4 * RA_WREG) + wave * 64;
-  int* rcnt = reinterpret_cast<int*>(epi + 4 * RA_WREG + RA_ROWS * 4) + wave * 64;
 
   const int64_t row0 = (int64_t)blockIdx.x * RA_ROWS;
   const int64_t ntile = npad / RA_BN;
@@ -487,7 +487,7 @@ __global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
     li[q] = -1;
   }
   thr[lane] = FINF;
-  rcnt[lane] = 0;
+  int bcnt = 0;  // entries waiting in this lane's row bucket
 
   // two 16-byte direct-to-LDS loads per thread per stage: 32 columns x 16 pieces
   // Three direct-to-LDS loads per thread per stage: two 16-byte pieces of the 32-column
@@ -585,12 +585,17 @@ __global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
         }
         acc[tm][e] = k_;
       }
-    // Keys below their row's threshold go to that row's bucket. Row r of the wavefront
-    // sits in one register slot (tm, j, i) of one lane half, one column per lane, so a
-    // ballot gives every passing key its bucket position and the row its count.
+    // Keys below their row's threshold are appended to that row's LDS bucket, which
+    // keeps filling across tiles; the lists take the buckets in one parallel drain once
+    // some row holds RA_DRAIN entries, so each drain inserts for many rows at once. Row r
+    // of the wavefront sits in one register slot (tm, j, i) of one lane half, one column
+    // per lane: a ballot gives each passing key its position after the row's waiting
+    // entries (whose count the owner lane r holds). Thresholds only change at drains,
+    // which keeps the filter valid: every key left out is >= its row's list end.
     const int cbase = (int)(ct * RA_BN);
-    bool touched = false;
     const bool lo_half = lane < 32;
+    bool touched = false, overflow = false;
+    int inc = 0;  // entries this tile adds to this lane's row
 #pragma unroll
     for (int tm = 0; tm < 2; ++tm)
 #pragma unroll
@@ -599,57 +604,83 @@ __global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const float key = acc[tm][j * 4 + i];
-          const float tv_i = t4[i];
-          const bool pass = key < tv_i;
+          const bool pass = key < t4[i];
           const unsigned long long m = __ballot(pass);
           if (m) {
             touched = true;
+            const int r_lo = tm * 32 + j * 8 + i, r_hi = r_lo + 4;
             const unsigned mlo = (unsigned)m, mhi = (unsigned)(m >> 32);
-            const int pos = lo_half ? __builtin_amdgcn_mbcnt_lo(mlo, 0u)
-                                    : __builtin_amdgcn_mbcnt_hi(mhi, 0u);
-            const int r = lrow + tm * 32 + j * 8 + i;
+            const int c_lo = __builtin_amdgcn_readlane(bcnt, r_lo);
+            const int c_hi = __builtin_amdgcn_readlane(bcnt, r_hi);
+            const int n_lo = __popc(mlo), n_hi = __popc(mhi);
+            overflow |= (c_lo + n_lo > RA_BKT) | (c_hi + n_hi > RA_BKT);
+            const int pos = lo_half ? c_lo + (int)__builtin_amdgcn_mbcnt_lo(mlo, 0u)
+                                    : c_hi + (int)__builtin_amdgcn_mbcnt_hi(mhi, 0u);
+            const int r = lrow + r_lo;
             if (pass && pos < RA_BKT)
-              lds_st_b64(lds_addr(bucket + r * RA_BKT + pos), make_int2(__float_as_int(key), lcol));
-            if (lcol == 0) lds_st_b32(lds_addr(rcnt + r), __popc(lo_half ? mlo : mhi));
+              lds_st_b64(lds_addr(bucket + r * RA_BKT + pos),
+                         make_int2(__float_as_int(key), cbase + lcol));  // absolute column
+            inc += (lane == r_lo) ? n_lo : 0;
+            inc += (lane == r_hi) ? n_hi : 0;
           }
         }
       }
     if (touched) {
       asm volatile("" ::: "memory");
       __builtin_amdgcn_wave_barrier();
-      const int cnt = lds_ld_b32(lds_addr(rcnt + lane));
-      lds_st_b32(lds_addr(rcnt + lane), 0);
-      if (__ballot(cnt > RA_BKT) == 0) {
-        // every lane drains its own row's bucket: cost is the largest count, not the sum
-#pragma unroll 1
-        for (int e = 0; e < cnt; ++e) {
-          const int2 en = lds_ld_b64(lds_addr(bucket + lane * RA_BKT + e));
-          // unconditional: a branch here would be if-converted into a second live copy
-          // of the list
-          cand_insert<C>(lk, li, __int_as_float(en.x), cbase + en.y);
-        }
-      } else {
-        // bucket overflow (the first tiles): dump the key tile, each lane scans its row
-        asm volatile("" ::: "memory");
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int tm = 0; tm < 2; ++tm)
-#pragma unroll
-          for (int e = 0; e < 16; ++e)
-            sc[sb + (tm * 32 + (e >> 2) * 8 + (e & 3)) * RA_SCS] = acc[tm][e];
-        asm volatile("" ::: "memory");
-        __builtin_amdgcn_wave_barrier();
-        const float* srow = sc + lane * RA_SCS;
-#pragma unroll 1
-        for (int q = 0; q < RA_BN; ++q) {
-          cand_insert<C>(lk, li, srow[q], cbase + q);
-        }
+      bool drain = overflow;
+      if (!overflow) {
+        bcnt += inc;
+        drain = __ballot(bcnt >= RA_DRAIN) != 0;
       }
-      asm volatile("" ::: "memory");
-      __builtin_amdgcn_wave_barrier();
-      lds_st_b32(lds_addr(thr + lane), __float_as_int(lk[C - 1]));
-      asm volatile("" ::: "memory");
-      __builtin_amdgcn_wave_barrier();
+      if (drain) {
+        // every lane inserts its own row's waiting entries: the cost is the largest
+        // count, not the sum. Inserts are unconditional (a branch here would be
+        // if-converted into a second live copy of the list); the next entry is read
+        // while this one is inserted. After an overflow only the entries of earlier
+        // tiles are taken here and the current tile is scanned whole below.
+        const unsigned ba = lds_addr(bucket + lane * RA_BKT);
+        int2 en = lds_ld_b64(ba);
+#pragma unroll 1
+        for (int e = 0; e < bcnt; ++e) {
+          const int2 cur = en;
+          en = lds_ld_b64(ba + (unsigned)(e + 1) * 8u);
+          cand_insert<C>(lk, li, __int_as_float(cur.x), cur.y);
+        }
+        bcnt = 0;
+        if (overflow) {
+          // bucket overflow (the first tiles): dump the key tile over the buckets (all
+          // reads above have completed), each lane scans its row
+          asm volatile("" ::: "memory");
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+            for (int e = 0; e < 16; ++e)
+              sc[sb + (tm * 32 + (e >> 2) * 8 + (e & 3)) * RA_SCS] = acc[tm][e];
+          asm volatile("" ::: "memory");
+          __builtin_amdgcn_wave_barrier();
+          const float* srow = sc + lane * RA_SCS;
+#pragma unroll 1
+          for (int q = 0; q < RA_BN; ++q) cand_insert<C>(lk, li, srow[q], cbase + q);
+        }
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+        lds_st_b32(lds_addr(thr + lane), __float_as_int(lk[C - 1]));
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+  }
+  // entries still waiting after the last tile
+  if (__ballot(bcnt > 0) != 0) {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+    const unsigned ba = lds_addr(bucket + lane * RA_BKT);
+#pragma unroll 1
+    for (int e = 0; e < bcnt; ++e) {
+      const int2 cur = lds_ld_b64(ba + (unsigned)e * 8u);
+      cand_insert<C>(lk, li, __int_as_float(cur.x), cur.y);
     }
   }
   wait_vm<0>();  // the tail reloads have landed before the wavefront ends
