@@ -385,22 +385,28 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
 constexpr int RA_ROWS = 256;   // rows per workgroup
 constexpr int RA_MAXD = 512;   // largest d the resident-query kernel takes
 constexpr int RA_BN = 32;      // columns per tile
-constexpr int RA_KC = 128;     // k per chunk
-constexpr int RA_NST = 8;      // LDS ring stages
 constexpr int RA_SCS = 36;     // padded key-tile row stride (floats)
-constexpr int RA_STB_B = RA_BN * RA_KC * 2;              // 8 KB of database tile per stage
-constexpr int RA_STB = RA_STB_B + 4 * 256;               // + each wavefront's copy of the
-                                                         //   tile's column scales
 constexpr int RA_BKT = 32;                               // bucket slots per row
 constexpr int RA_DRAIN = 16;                             // drain once a row holds this many
 constexpr int RA_WREG = 64 * RA_BKT * 8;                 // per-wavefront buckets / key tile
 static_assert(RA_WREG >= 64 * RA_SCS * 4, "the key-tile dump must fit the bucket region");
-constexpr size_t RA_SMEM = (size_t)RA_NST * RA_STB;                // dynamic: the ring
 constexpr int RA_EPI_BYTES = 4 * RA_WREG + RA_ROWS * 4;             // static: epilogue
 
-// 16-byte LDS slot of piece q (8 bf16 along k) of staged column c: 16 consecutive
-// columns read with the same q land in 16 distinct slot positions mod 16
-__device__ __forceinline__ int ra_slot(int c, int q) { return c * 16 + (q ^ (c & 15)); }
+// Ring geometry for a k-chunk of KC (128 for d <= 128, else 256): a stage holds the
+// 32 x KC bf16 database chunk (16-byte pieces, XOR-swizzled) and one 256-byte copy of
+// the tile's column scales per wavefront.
+template <int KC>
+struct RaRing {
+  static constexpr int STB_B = RA_BN * KC * 2;            // 8 or 16 KB
+  static constexpr int STB = STB_B + 4 * 256;
+  static constexpr int NST = KC == 128 ? 8 : 4;           // stages (power of two)
+  static constexpr int PIECES = KC / 8;                   // 16-byte pieces per column
+  static constexpr int LOADS = STB_B / (256 * 16);        // 16-byte loads per thread
+  static constexpr size_t SMEM = (size_t)NST * STB;       // dynamic LDS
+  // slot of piece q of column c: 16 consecutive columns read with the same q hit 16
+  // distinct slot positions mod 16 (conflict-free ds_read_b128)
+  static __device__ __forceinline__ int slot(int c, int q) { return c * PIECES + (q ^ (c & 15)); }
+};
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -449,12 +455,14 @@ __device__ __forceinline__ int2 lds_ld_b64(unsigned a) {
   return v;
 }
 
-template <int C, int NKC>  // C = list length, NKC = dpad / 128
+template <int C, int NKC, int KC>  // C = list length, NKC = dpad / KC
 __global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
     const uint16_t* __restrict__ Xh, int64_t n, int64_t npad, int dpad,
     const float* __restrict__ colscale, int metric, int32_t* __restrict__ cand_idx,
     float* __restrict__ cand_key) {
-  constexpr int NST = RA_NST;
+  using R = RaRing<KC>;
+  constexpr int NST = R::NST;
+  constexpr int LOADS = R::LOADS;
   // The ring (written by direct-to-LDS loads) and the epilogue state are separate LDS
   // objects, so the compiler can see that epilogue LDS accesses never alias loads in
   // flight and does not drain the prefetch before them.
@@ -471,12 +479,12 @@ __global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
   const int64_t total = ntile * NKC;
 
   // resident A fragments: rows wave*64 + tm*32 + (lane & 31), k = f*16 + (lane >> 5)*8
-  bf16x8 afr[2][NKC * 8];
+  bf16x8 afr[2][NKC * KC / 16];
 #pragma unroll
   for (int tm = 0; tm < 2; ++tm) {
     const uint16_t* ar = Xh + (row0 + wave * 64 + tm * 32 + (lane & 31)) * dpad + (lane >> 5) * 8;
 #pragma unroll
-    for (int f = 0; f < NKC * 8; ++f) afr[tm][f] = *reinterpret_cast<const bf16x8*>(ar + f * 16);
+    for (int f = 0; f < NKC * KC / 16; ++f) afr[tm][f] = *reinterpret_cast<const bf16x8*>(ar + f * 16);
   }
   wait_vm<0>();  // the sweep's vmcnt accounting starts from an empty queue
   float lk[C];  // this lane's row: best C (key, column) pairs, ascending
@@ -489,49 +497,71 @@ __global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
   thr[lane] = FINF;
   int bcnt = 0;  // entries waiting in this lane's row bucket
 
-  // two 16-byte direct-to-LDS loads per thread per stage: 32 columns x 16 pieces
-  // Three direct-to-LDS loads per thread per stage: two 16-byte pieces of the 32-column
-  // x 128-k database chunk, and the tile's column scales (4 bytes per lane, lanes 32..63
-  // repeat lanes 0..31) into this wavefront's own 256-byte slot. Stages past the end
-  // reload the last stage into a slot no later chunk reads, so every chunk issues the
-  // same number of loads and all vmcnt counts are static.
-  auto issue = [&](int64_t g) {
-    const int64_t gs = g < total ? g : total - 1;
-    const int64_t ct = gs / NKC;
-    const int kc = (int)(gs - ct * NKC);
-    char* st = ring + (int)(g % NST) * RA_STB;
+  // Stage loads, issued in stage order from running pointers: LOADS 16-byte pieces of
+  // the 32-column x KC database chunk per thread, and the tile's column scales (4 bytes
+  // per lane, lanes 32..63 repeat lanes 0..31) into this wavefront's 256-byte slot.
+  // Past the last stage the pointers stop, so the tail reloads the last stage into a
+  // slot no later chunk reads: every chunk issues the same number of loads and all
+  // vmcnt counts are static.
+  const uint16_t* pb[LOADS];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int blk = h * 4 + wave;
-      const int s = blk * 64 + lane;
-      const int c = s >> 4, qs = s & 15;
-      const int q = qs ^ (c & 15);
-      const uint16_t* gp = Xh + (ct * RA_BN + c) * dpad + kc * RA_KC + q * 8;
-      __builtin_amdgcn_global_load_lds(gp, (lds_void*)(st + blk * 1024), 16, 0, 0);
+  for (int h = 0; h < LOADS; ++h) {
+    const int s = (h * 4 + wave) * 64 + lane;
+    const int c = s / R::PIECES, qs = s % R::PIECES;
+    pb[h] = Xh + (int64_t)c * dpad + (qs ^ (c & 15)) * 8;
+  }
+  const float* pc = colscale + (lane & 31);
+  int64_t gi = 0;   // next stage to issue
+  int kci = 0;      // its k-chunk
+  int sloti = 0;    // its ring slot
+  const int64_t tile_step = (int64_t)RA_BN * dpad - (int64_t)(NKC - 1) * KC;
+  auto issue_next = [&]() {
+    char* st = ring + sloti * R::STB;
+#pragma unroll
+    for (int h = 0; h < LOADS; ++h) {
+      const uint16_t* gp = pb[h];  // the builtin rejects the captured element itself
+      __builtin_amdgcn_global_load_lds(gp, (lds_void*)(st + (h * 4 + wave) * 1024), 16, 0, 0);
     }
-    __builtin_amdgcn_global_load_lds(colscale + ct * RA_BN + (lane & 31),
-                                     (lds_void*)(st + RA_STB_B + wave * 256), 4, 0, 0);
+    {
+      const float* cp = pc;
+      __builtin_amdgcn_global_load_lds(cp, (lds_void*)(st + R::STB_B + wave * 256), 4, 0, 0);
+    }
+    if (gi + 1 < total) {
+      if (++kci == NKC) {
+        kci = 0;
+#pragma unroll
+        for (int h = 0; h < LOADS; ++h) pb[h] += tile_step;
+        pc += RA_BN;
+      } else {
+#pragma unroll
+        for (int h = 0; h < LOADS; ++h) pb[h] += KC;
+      }
+    }
+    ++gi;
+    sloti = (sloti + 1) & (NST - 1);
   };
 
-  for (int64_t g = 0; g < NST - 1; ++g) issue(g);
+  for (int g = 0; g < NST - 1; ++g) issue_next();
   for (int64_t ct = 0; ct < ntile; ++ct) {
     f32x16 acc[2] = {(f32x16){}, (f32x16){}};
 #pragma unroll
     for (int kc = 0; kc < NKC; ++kc) {
       const int64_t g = ct * NKC + kc;
-      wait_vm<3 * (NST - 2)>();  // stage g landed (three loads per stage per thread)
-      ring_barrier();
-      issue(g + NST - 1);
-      const char* st = ring + (int)(g % NST) * RA_STB;
-      bf16x8 bq[RA_KC / 16];  // all B fragments of the stage in flight at once
+      wait_vm<(LOADS + 1) * (NST - 2)>();  // this thread's loads of stage g landed
+      ring_barrier();                       // ... and everyone else's
+      const char* st = ring + (int)(g & (NST - 1)) * R::STB;
+      bf16x8 bq[KC / 16];  // all B fragments of the stage in flight at once
 #pragma unroll
-      for (int kk = 0; kk < RA_KC / 16; ++kk)
-        bq[kk] = *reinterpret_cast<const bf16x8*>(st + ra_slot(lane & 31, kk * 2 + (lane >> 5)) * 16);
+      for (int kk = 0; kk < KC / 16; ++kk)
+        bq[kk] = *reinterpret_cast<const bf16x8*>(st + R::slot(lane & 31, kk * 2 + (lane >> 5)) * 16);
 #pragma unroll
-      for (int kk = 0; kk < RA_KC / 16; ++kk)
+      for (int kk = 0; kk < KC / 16; ++kk) {
 #pragma unroll
         for (int tm = 0; tm < 2; ++tm)
-          acc[tm] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afr[tm][kc * 8 + kk], bq[kk], acc[tm], 0, 0, 0);
+          acc[tm] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afr[tm][kc * (KC / 16) + kk], bq[kk],
+                                                           acc[tm], 0, 0, 0);
+        if (kk == 0) issue_next();  // stage g + NST - 1, into the slot stage g - 1 left
+      }
     }
 
     // ---------------- wavefront-private epilogue of column tile ct
@@ -548,8 +578,8 @@ __global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
     float cs;
     {
       const unsigned ta = lds_addr(thr + ((lane >> 5) * 4));
-      const unsigned ca = lds_addr(ring + (int)((ct * NKC + NKC - 1) % NST) * RA_STB + RA_STB_B +
-                                   wave * 256 + lane * 4);
+      const unsigned ca = lds_addr(ring + (int)((ct * NKC + NKC - 1) & (NST - 1)) * R::STB +
+                                   R::STB_B + wave * 256 + lane * 4);
       asm volatile(
           "ds_read_b128 %0, %9\n"
           "ds_read_b128 %1, %9 offset:32\n"
@@ -694,12 +724,12 @@ __global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
   }
 }
 
-template __global__ void knn_topc_ra_kernel<32, 1>(const uint16_t*, int64_t, int64_t, int,
-                                                   const float*, int, int32_t*, float*);
-template __global__ void knn_topc_ra_kernel<32, 2>(const uint16_t*, int64_t, int64_t, int,
-                                                   const float*, int, int32_t*, float*);
-template __global__ void knn_topc_ra_kernel<32, 4>(const uint16_t*, int64_t, int64_t, int,
-                                                   const float*, int, int32_t*, float*);
+template __global__ void knn_topc_ra_kernel<32, 1, 128>(const uint16_t*, int64_t, int64_t, int,
+                                                        const float*, int, int32_t*, float*);
+template __global__ void knn_topc_ra_kernel<32, 1, 256>(const uint16_t*, int64_t, int64_t, int,
+                                                        const float*, int, int32_t*, float*);
+template __global__ void knn_topc_ra_kernel<32, 2, 256>(const uint16_t*, int64_t, int64_t, int,
+                                                        const float*, int, int32_t*, float*);
 
 // ------------------------------------------------------------------ 3. re-rank
 constexpr int MAXT = 16;  // d <= 1024 handled with the query row held in registers
@@ -907,12 +937,12 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<false, 32>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 1>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RA_SMEM));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 2>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RA_SMEM));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 4>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RA_SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 1, 128>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<128>::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 1, 256>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<256>::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 2, 256>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<256>::SMEM));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_exact_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
     attr_set = true;
@@ -978,14 +1008,14 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   if (ra) {
     const unsigned grid = (unsigned)(npad / RA_ROWS);
     if (dpad == 128)
-      knn_topc_ra_kernel<32, 1><<<grid, 256, RA_SMEM, s>>>(Xh.p, n, npad, dpad, colscale.p, metric,
-                                                           cidx.p, ckey.p);
+      knn_topc_ra_kernel<32, 1, 128><<<grid, 256, RaRing<128>::SMEM, s>>>(
+          Xh.p, n, npad, dpad, colscale.p, metric, cidx.p, ckey.p);
     else if (dpad == 256)
-      knn_topc_ra_kernel<32, 2><<<grid, 256, RA_SMEM, s>>>(Xh.p, n, npad, dpad, colscale.p, metric,
-                                                           cidx.p, ckey.p);
+      knn_topc_ra_kernel<32, 1, 256><<<grid, 256, RaRing<256>::SMEM, s>>>(
+          Xh.p, n, npad, dpad, colscale.p, metric, cidx.p, ckey.p);
     else
-      knn_topc_ra_kernel<32, 4><<<grid, 256, RA_SMEM, s>>>(Xh.p, n, npad, dpad, colscale.p, metric,
-                                                           cidx.p, ckey.p);
+      knn_topc_ra_kernel<32, 2, 256><<<grid, 256, RaRing<256>::SMEM, s>>>(
+          Xh.p, n, npad, dpad, colscale.p, metric, cidx.p, ckey.p);
   } else {
     const int nops = split ? 4 : 2;
     const int nst = split ? 2 : 4;
