@@ -1,10 +1,11 @@
 // Core distances and the mutual-reachability MST (build_mst, src/clustering/mst.rs:82-166)
 // on gfx950, without the KD-tree (src/clustering/kd_tree.rs): the embedding is ordered
-// along a Morton curve, cut into 64-point tiles with bounding boxes, and every query
-// wavefront (one tile of 64 queries) walks the tiles outward from its own position,
-// skipping a tile when every lane's point-to-box distance already exceeds its bound
-// (the AABB rule of kd_tree.rs:330-339) or when the tile lies wholly in the query
-// tile's component (same-component pruning, kd_tree.rs:336-339).
+// along a Morton curve and cut into 64-point tiles with bounding boxes, and the tiles
+// are the leaves of an implicit bounding-volume tree (heap order, node boxes merged
+// bottom-up). Every query wavefront (one tile of 64 queries) walks that tree depth
+// first without a stack, skipping a subtree when every lane's point-to-box distance
+// already exceeds its bound (the AABB rule of kd_tree.rs:330-339) or when the subtree
+// lies wholly in the lane's component (same-component pruning, kd_tree.rs:336-339).
 //
 // Ties: core distances are order-free; MST edges are chosen under the strict total
 // order (w^2, min(u,v), max(u,v)) on original point ids, so the tree is the unique
@@ -128,17 +129,52 @@ __device__ __forceinline__ float dsq_seq(const float (&q)[MAXD], const float* __
   return s;
 }
 
-__device__ __forceinline__ int64_t spiral_tile(int64_t qt, int64_t s) {
-  const int64_t off = (s + 1) >> 1;
-  return (s & 1) ? qt + off : qt - off;
+// Depth-first successor in the implicit tree (root 1, children 2v and 2v+1) after the
+// subtree of v is done or skipped: strip v's trailing right-child steps, then move to the
+// right sibling. Returns 1 when the walk is over.
+__device__ __forceinline__ int64_t tree_next(int64_t v) {
+  return (v >> __builtin_ctzll(~(unsigned long long)v)) + 1;
+}
+
+// leaves L..2L-1 hold the tile boxes (empty boxes past the last tile)
+__global__ void bvh_leaf_kernel(const float* __restrict__ tlo, const float* __restrict__ thi,
+                                int64_t ntiles, int64_t L, int dim, float* __restrict__ nlo,
+                                float* __restrict__ nhi) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L) return;
+  for (int c = 0; c < dim; ++c) {
+    nlo[(L + t) * dim + c] = t < ntiles ? tlo[t * dim + c] : __builtin_huge_valf();
+    nhi[(L + t) * dim + c] = t < ntiles ? thi[t * dim + c] : -__builtin_huge_valf();
+  }
+}
+__global__ void bvh_merge_kernel(int64_t lev, int dim, float* __restrict__ nlo,
+                                 float* __restrict__ nhi) {
+  const int64_t v = lev + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= 2 * lev) return;
+  for (int c = 0; c < dim; ++c) {
+    nlo[v * dim + c] = fminf(nlo[2 * v * dim + c], nlo[(2 * v + 1) * dim + c]);
+    nhi[v * dim + c] = fmaxf(nhi[2 * v * dim + c], nhi[(2 * v + 1) * dim + c]);
+  }
+}
+// component of a subtree: c if all its points are in component c, -1 if mixed, -2 empty
+__global__ void bvh_comp_leaf_kernel(const int32_t* __restrict__ tile_comp, int64_t ntiles,
+                                     int64_t L, int32_t* __restrict__ ncomp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < L) ncomp[L + t] = t < ntiles ? tile_comp[t] : -2;
+}
+__global__ void bvh_comp_merge_kernel(int64_t lev, int32_t* __restrict__ ncomp) {
+  const int64_t v = lev + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= 2 * lev) return;
+  const int32_t a = ncomp[2 * v], b = ncomp[2 * v + 1];
+  ncomp[v] = a == -2 ? b : (b == -2 ? a : (a == b ? a : -1));
 }
 
 // ------------------------------------------------------------------ core distances
 template <int D>
 __global__ __launch_bounds__(64) void core_kernel(const float* __restrict__ es, int64_t n,
-                                                  int dim, int64_t ntiles,
-                                                  const float* __restrict__ blo,
-                                                  const float* __restrict__ bhi, int k,
+                                                  int dim, int64_t ntiles, int64_t L,
+                                                  const float* __restrict__ nlo,
+                                                  const float* __restrict__ nhi, int k,
                                                   float* __restrict__ core) {
   __shared__ float tp[TS * MAXD];
   const int lane = threadIdx.x;
@@ -154,11 +190,7 @@ __global__ __launch_bounds__(64) void core_kernel(const float* __restrict__ es, 
 #pragma unroll
   for (int j = 0; j < KCORE; ++j) best[j] = __builtin_huge_valf();
   float kth = __builtin_huge_valf();
-  for (int64_t s = 0; s < 2 * ntiles; ++s) {
-    const int64_t t = spiral_tile(qt, s);
-    if (t < 0 || t >= ntiles) continue;
-    const bool prune = !valid || box_dist2<D>(q, blo + t * nd, bhi + t * nd, nd) > kth;
-    if (__all(prune)) continue;
+  auto scan_tile = [&](int64_t t) {
     const int64_t pt = t * TS + lane;
     __syncthreads();
     for (int c = 0; c < nd; ++c) tp[lane * nd + c] = pt < n ? es[pt * nd + c] : 0.0f;
@@ -182,6 +214,24 @@ __global__ __launch_bounds__(64) void core_kernel(const float* __restrict__ es, 
         }
       }
     }
+  };
+  scan_tile(qt);  // the own tile first, for a starting bound
+  for (int64_t v = 1;;) {
+    const bool prune = !valid || box_dist2<D>(q, nlo + v * nd, nhi + v * nd, nd) > kth;
+    bool down = false;
+    if (!__all(prune)) {
+      if (v >= L) {
+        if (v - L != qt) scan_tile(v - L);
+      } else {
+        down = true;
+      }
+    }
+    if (down) {
+      v = 2 * v;
+    } else {
+      v = tree_next(v);
+      if (v == 1) break;
+    }
   }
   if (valid) core[p] = kth;
 }
@@ -200,9 +250,9 @@ __device__ __forceinline__ bool better(float w, int32_t a, int32_t b, const Best
 template <int D>
 __global__ __launch_bounds__(64) void boruvka_search_kernel(
     const float* __restrict__ es, const float* __restrict__ core, const int32_t* __restrict__ comp,
-    const uint32_t* __restrict__ perm, int64_t n, int dim, int64_t ntiles,
-    const float* __restrict__ blo, const float* __restrict__ bhi,
-    const int32_t* __restrict__ tile_comp, unsigned* __restrict__ comp_w,
+    const uint32_t* __restrict__ perm, int64_t n, int dim, int64_t L,
+    const float* __restrict__ nlo, const float* __restrict__ nhi,
+    const int32_t* __restrict__ ncomp, unsigned* __restrict__ comp_w,
     float* __restrict__ best_w, int32_t* __restrict__ best_a, int32_t* __restrict__ best_b) {
   __shared__ float tp[TS * MAXD];
   __shared__ float tc[TS];
@@ -219,42 +269,50 @@ __global__ __launch_bounds__(64) void boruvka_search_kernel(
   const float cq = valid ? core[p] : 0.0f;
   const int32_t mycomp = valid ? comp[p] : -1;
   const int32_t myid = valid ? (int32_t)perm[p] : 0;
-  const int32_t qtc = tile_comp[qt];
   Best bst{__builtin_huge_valf(), INT32_MAX, INT32_MAX};
-  for (int64_t s = 0; s < 2 * ntiles; ++s) {
-    const int64_t t = spiral_tile(qt, s);
-    if (t < 0 || t >= ntiles) continue;
-    const int32_t ttc = tile_comp[t];
-    if (qtc >= 0 && ttc == qtc) continue;
+  for (int64_t v = 1;;) {
+    const int32_t vc = ncomp[v];
     float bound = bst.w;
     if (valid) bound = fminf(bound, __uint_as_float(comp_w[mycomp]));
-    const bool prune = !valid || cq > bound ||
-                       box_dist2<D>(q, blo + t * nd, bhi + t * nd, nd) > bound ||
-                       (ttc >= 0 && ttc == mycomp);
-    if (__all(prune)) continue;
-    const int64_t pt = t * TS + lane;
-    __syncthreads();
-    for (int c = 0; c < nd; ++c) tp[lane * nd + c] = pt < n ? es[pt * nd + c] : 0.0f;
-    tc[lane] = pt < n ? core[pt] : __builtin_huge_valf();
-    tcomp[lane] = pt < n ? comp[pt] : mycomp;
-    tid_[lane] = pt < n ? (int32_t)perm[pt] : 0;
-    __syncthreads();
-    if (!prune) {
-      const int cnt = (int)min<int64_t>(TS, n - t * TS);
-      for (int j = 0; j < cnt; ++j) {
-        if (tcomp[j] == mycomp) continue;
-        const float cj = tc[j];
-        if (cj > bound) continue;
-        const float dd = dsq_seq<D>(q, tp + j * nd, nd);
-        const float mr = fmaxf(fmaxf(dd, cq), cj);
-        const int32_t oj = tid_[j];
-        const int32_t a = min(myid, oj), b = max(myid, oj);
-        if (better(mr, a, b, bst)) {
-          bst = Best{mr, a, b};
-          bound = fminf(bound, mr);
+    const bool prune = !valid || vc == -2 || (vc >= 0 && vc == mycomp) || cq > bound ||
+                       box_dist2<D>(q, nlo + v * nd, nhi + v * nd, nd) > bound;
+    bool down = false;
+    if (!__all(prune)) {
+      if (v >= L) {
+        const int64_t t = v - L;
+        const int64_t pt = t * TS + lane;
+        __syncthreads();
+        for (int c = 0; c < nd; ++c) tp[lane * nd + c] = pt < n ? es[pt * nd + c] : 0.0f;
+        tc[lane] = pt < n ? core[pt] : __builtin_huge_valf();
+        tcomp[lane] = pt < n ? comp[pt] : mycomp;
+        tid_[lane] = pt < n ? (int32_t)perm[pt] : 0;
+        __syncthreads();
+        if (!prune) {
+          const int cnt = (int)min<int64_t>(TS, n - t * TS);
+          for (int j = 0; j < cnt; ++j) {
+            if (tcomp[j] == mycomp) continue;
+            const float cj = tc[j];
+            if (cj > bound) continue;
+            const float dd = dsq_seq<D>(q, tp + j * nd, nd);
+            const float mr = fmaxf(fmaxf(dd, cq), cj);
+            const int32_t oj = tid_[j];
+            const int32_t a = min(myid, oj), b = max(myid, oj);
+            if (better(mr, a, b, bst)) {
+              bst = Best{mr, a, b};
+              bound = fminf(bound, mr);
+            }
+          }
+          if (bst.w < __builtin_huge_valf()) atomicMin(&comp_w[mycomp], __float_as_uint(bst.w));
         }
+      } else {
+        down = true;
       }
-      if (bst.w < __builtin_huge_valf()) atomicMin(&comp_w[mycomp], __float_as_uint(bst.w));
+    }
+    if (down) {
+      v = 2 * v;
+    } else {
+      v = tree_next(v);
+      if (v == 1) break;
     }
   }
   if (valid) {
@@ -262,6 +320,50 @@ __global__ __launch_bounds__(64) void boruvka_search_kernel(
     best_a[p] = bst.a;
     best_b[p] = bst.b;
   }
+}
+
+// Per-round bound seeding: each point tries the points of its own Morton tile and the
+// two adjacent tiles, and every component's bound starts at the best such
+// mutual-reachability edge to another component. These are real edges, so the bound is
+// an upper bound of the component's minimum outgoing edge; the search then only has to
+// certify within it (its pruning is strict, so edges equal to the bound are still seen)
+// instead of starting from an unbounded radius.
+template <int D>
+__global__ __launch_bounds__(64) void boruvka_seed_kernel(
+    const float* __restrict__ es, const float* __restrict__ core, const int32_t* __restrict__ comp,
+    int64_t n, int dim, int64_t ntiles, unsigned* __restrict__ comp_w) {
+  __shared__ float tp[3 * TS * MAXD];
+  __shared__ float tc[3 * TS];
+  __shared__ int32_t tcomp[3 * TS];
+  const int lane = threadIdx.x;
+  const int64_t qt = blockIdx.x;
+  const int64_t p = qt * TS + lane;
+  const bool valid = p < n;
+  const int nd = D > 0 ? D : dim;
+  for (int w = 0; w < 3; ++w) {
+    const int64_t t = qt - 1 + w;
+    const int64_t pt = t * TS + lane;
+    const bool ok = t >= 0 && t < ntiles && pt < n;
+    for (int c = 0; c < nd; ++c) tp[(w * TS + lane) * nd + c] = ok ? es[pt * nd + c] : 0.0f;
+    tc[w * TS + lane] = ok ? core[pt] : __builtin_huge_valf();
+    tcomp[w * TS + lane] = ok ? comp[pt] : -1;
+  }
+  __syncthreads();
+  if (!valid) return;
+  float q[MAXD];
+#pragma unroll
+  for (int c = 0; c < MAXD; ++c) q[c] = c < nd ? es[p * nd + c] : 0.0f;
+  const float cq = core[p];
+  const int32_t mycomp = comp[p];
+  float best = __builtin_huge_valf();
+  for (int j = 0; j < 3 * TS; ++j) {
+    const int32_t cj = tcomp[j];
+    if (cj < 0 || cj == mycomp) continue;
+    const float dd = dsq_seq<D>(q, tp + j * nd, nd);
+    const float mr = fmaxf(fmaxf(dd, cq), tc[j]);
+    best = fminf(best, mr);
+  }
+  if (best < __builtin_huge_valf()) atomicMin(&comp_w[mycomp], __float_as_uint(best));
 }
 
 __global__ void boruvka_select_kernel(int64_t n, const int32_t* __restrict__ comp,
@@ -295,8 +397,9 @@ __global__ void tile_comp_kernel(const int32_t* __restrict__ comp, int64_t n, in
 struct Sorted {
   Buf<float> es;
   Buf<uint32_t> perm;
-  Buf<float> blo, bhi;
-  int64_t ntiles = 0;
+  Buf<float> blo, bhi;  // tile boxes
+  Buf<float> nlo, nhi;  // tree node boxes, nodes 1 .. 2L-1 (leaves L .. 2L-1)
+  int64_t ntiles = 0, L = 1;
 };
 
 void morton_sort(const float* d_emb, int64_t n, int dim, Sorted& S, hipStream_t s,
@@ -319,6 +422,14 @@ void morton_sort(const float* d_emb, int64_t n, int dim, Sorted& S, hipStream_t 
   S.bhi.alloc(pool, S.ntiles * dim);
   tile_box_kernel<<<(unsigned)ceil_div(S.ntiles, 4), 256, 0, s>>>(S.es.p, n, dim, S.ntiles,
                                                                   S.blo.p, S.bhi.p);
+  S.L = 1;
+  while (S.L < S.ntiles) S.L <<= 1;
+  S.nlo.alloc(pool, 2 * S.L * dim);
+  S.nhi.alloc(pool, 2 * S.L * dim);
+  bvh_leaf_kernel<<<(unsigned)ceil_div(S.L, 256), 256, 0, s>>>(S.blo.p, S.bhi.p, S.ntiles, S.L,
+                                                               dim, S.nlo.p, S.nhi.p);
+  for (int64_t lev = S.L / 2; lev >= 1; lev /= 2)
+    bvh_merge_kernel<<<(unsigned)ceil_div(lev, 256), 256, 0, s>>>(lev, dim, S.nlo.p, S.nhi.p);
   EVOC_HIP(hipGetLastError());
 }
 
@@ -332,9 +443,11 @@ void core_sorted(const Sorted& S, int64_t n, int dim, int min_samples, float* d_
   }
   const unsigned g = (unsigned)S.ntiles;
   if (dim == 8)
-    core_kernel<8><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.blo.p, S.bhi.p, k, d_core_sorted);
+    core_kernel<8><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p, k,
+                                    d_core_sorted);
   else
-    core_kernel<0><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.blo.p, S.bhi.p, k, d_core_sorted);
+    core_kernel<0><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p, k,
+                                    d_core_sorted);
   EVOC_HIP(hipGetLastError());
 }
 
@@ -395,6 +508,7 @@ void mst_device(const float* d_emb, int64_t n, int dim, int min_samples, int32_t
   std::vector<uint32_t> perm(n);
   EVOC_HIP(hipMemcpyAsync(perm.data(), S.perm.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   Buf<int32_t> comp(pool, n), tile_comp(pool, S.ntiles), ba(pool, n), bb(pool, n);
+  Buf<int32_t> ncomp_node(pool, 2 * S.L);
   Buf<float> bw(pool, n);
   Buf<unsigned> comp_w(pool, n);
   Buf<unsigned long long> comp_key(pool, n);
@@ -415,17 +529,25 @@ void mst_device(const float* d_emb, int64_t n, int dim, int min_samples, int32_t
     ++r;
     tile_comp_kernel<<<(unsigned)ceil_div(S.ntiles, 4), 256, 0, s>>>(comp.p, n, S.ntiles,
                                                                      tile_comp.p);
+    bvh_comp_leaf_kernel<<<(unsigned)ceil_div(S.L, 256), 256, 0, s>>>(tile_comp.p, S.ntiles, S.L,
+                                                                      ncomp_node.p);
+    for (int64_t lev = S.L / 2; lev >= 1; lev /= 2)
+      bvh_comp_merge_kernel<<<(unsigned)ceil_div(lev, 256), 256, 0, s>>>(lev, ncomp_node.p);
     EVOC_HIP(hipMemsetAsync(comp_w.p, 0x7f, ncomp * sizeof(unsigned), s));  // 0x7f7f7f7f > any finite w
     EVOC_HIP(hipMemsetAsync(comp_key.p, 0xff, ncomp * sizeof(unsigned long long), s));
     const unsigned g = (unsigned)S.ntiles;
     if (dim == 8)
-      boruvka_search_kernel<8><<<g, 64, 0, s>>>(S.es.p, core.p, comp.p, S.perm.p, n, dim,
-                                                S.ntiles, S.blo.p, S.bhi.p, tile_comp.p, comp_w.p,
-                                                bw.p, ba.p, bb.p);
+      boruvka_seed_kernel<8><<<g, 64, 0, s>>>(S.es.p, core.p, comp.p, n, dim, S.ntiles, comp_w.p);
     else
-      boruvka_search_kernel<0><<<g, 64, 0, s>>>(S.es.p, core.p, comp.p, S.perm.p, n, dim,
-                                                S.ntiles, S.blo.p, S.bhi.p, tile_comp.p, comp_w.p,
-                                                bw.p, ba.p, bb.p);
+      boruvka_seed_kernel<0><<<g, 64, 0, s>>>(S.es.p, core.p, comp.p, n, dim, S.ntiles, comp_w.p);
+    if (dim == 8)
+      boruvka_search_kernel<8><<<g, 64, 0, s>>>(S.es.p, core.p, comp.p, S.perm.p, n, dim, S.L,
+                                                S.nlo.p, S.nhi.p, ncomp_node.p, comp_w.p, bw.p,
+                                                ba.p, bb.p);
+    else
+      boruvka_search_kernel<0><<<g, 64, 0, s>>>(S.es.p, core.p, comp.p, S.perm.p, n, dim, S.L,
+                                                S.nlo.p, S.nhi.p, ncomp_node.p, comp_w.p, bw.p,
+                                                ba.p, bb.p);
     boruvka_select_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(n, comp.p, comp_w.p, bw.p,
                                                                      ba.p, bb.p, comp_key.p);
     EVOC_HIP(hipGetLastError());
