@@ -8,12 +8,20 @@
 // without a separate schedule pass (embedding.rs:275-286). Negative samples are drawn
 // from Philox keyed by (seed, node, epoch, draw). Adam moments live in HBM next to the
 // layout. All arithmetic is f32 in the reference's operation order.
+//
+// Nodes with many edges (coarse graphs of the initial layout have thousands per node)
+// run one wavefront per node instead: lanes load the neighbours and compute every
+// gradient term (the pull of each sampled edge and each negative sample, with the same
+// per-node draw numbering), stage them in LDS, and the lanes of the layout dimensions
+// add them in the original order, so both kernels give bit-identical layouts.
 #include "kernels.h"
 
 namespace evoc {
 namespace {
 
 constexpr int MAXD = 16;
+constexpr int WIDE_DEG = 64;   // nodes with at least this many edges take the wide kernel
+constexpr int WIDE_CAP = 256;  // gradient terms staged per wavefront at a time
 
 __global__ void max_weight_kernel(const float* __restrict__ w, int64_t nnz,
                                   unsigned* __restrict__ out) {
@@ -55,10 +63,12 @@ __global__ __launch_bounds__(256) void epoch_kernel(
     float* __restrict__ next_s, const float* __restrict__ eps_s, float* __restrict__ next_n,
     float neg_factor, const float* __restrict__ e_old, float* __restrict__ e_new,
     float* __restrict__ m, float* __restrict__ v, float epoch_t, uint32_t epoch, float lr_ad,
-    float epsc, float gamma, float noise, float one_m_b1, float one_m_b2, uint64_t seed) {
+    float epsc, float gamma, float noise, float one_m_b1, float one_m_b2, uint64_t seed,
+    int64_t wide_deg) {
   constexpr int DD = D > 0 ? D : MAXD;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (indptr[i + 1] - indptr[i] >= wide_deg) return;  // done by epoch_wide_kernel
   const int nd = D > 0 ? D : dim;
   float ei[DD], g[DD];
 #pragma unroll
@@ -150,13 +160,187 @@ __global__ __launch_bounds__(256) void epoch_kernel(
   }
 }
 
+// One wavefront per high-degree node; identical arithmetic to epoch_kernel.
+template <int D>
+__global__ __launch_bounds__(256) void epoch_wide_kernel(
+    int64_t n, int dim, const int64_t* __restrict__ indptr, const int32_t* __restrict__ other,
+    float* __restrict__ next_s, const float* __restrict__ eps_s, float* __restrict__ next_n,
+    float neg_factor, const float* __restrict__ e_old, float* __restrict__ e_new,
+    float* __restrict__ m, float* __restrict__ v, float epoch_t, uint32_t epoch, float lr_ad,
+    float epsc, float gamma, float noise, float one_m_b1, float one_m_b2, uint64_t seed,
+    const int32_t* __restrict__ wide_nodes, int64_t nwide) {
+  constexpr int DD = D > 0 ? D : MAXD;
+  __shared__ float vbuf[4][WIDE_CAP * DD];
+  __shared__ unsigned char vok[4][WIDE_CAP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t w = (int64_t)blockIdx.x * 4 + wave;
+  if (w >= nwide) return;  // whole wavefronts only: no workgroup barrier below
+  const int64_t i = wide_nodes[w];
+  const int nd = D > 0 ? D : dim;
+  float ei[DD];
+#pragma unroll
+  for (int c = 0; c < DD; ++c) ei[c] = c < nd ? e_old[i * nd + c] : 0.0f;
+  float gl = 0.0f;  // lane c < nd: gradient component c
+  bool touched = false;
+  uint32_t draw_carry = 0;
+  const int64_t s0 = indptr[i], s1 = indptr[i + 1];
+  float* vb = vbuf[wave];
+  unsigned char* ok = vok[wave];
+  for (int64_t sb = s0; sb < s1; sb += 64) {
+    const int64_t s = sb + lane;
+    const bool valid = s < s1;
+    const float ns = valid ? next_s[s] : 0.0f;
+    const bool active = valid && !(ns > epoch_t);
+    int32_t j = 0;
+    float ps = 0.0f, pn = 0.0f, nn = 0.0f;
+    uint64_t cnt = 0;
+    if (active) {
+      j = other[s];
+      ps = eps_s[s];
+      pn = ps * neg_factor;
+      nn = next_n[s];
+      const float f = floorf((epoch_t - nn) / pn);
+      cnt = (f >= 0.0f && f < 1.8e19f) ? (uint64_t)f : 0;
+    }
+    touched |= __ballot(active) != 0;
+    const uint32_t nneg = (active && (int64_t)j > i) ? (uint32_t)cnt : 0u;
+    const uint32_t nvec = active ? 1u + nneg : 0u;
+    // exclusive prefix sums over the lanes (slot order) of draws and of staged terms
+    uint32_t dinc = nneg, vinc = nvec;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t td = __shfl_up(dinc, o, 64), tv = __shfl_up(vinc, o, 64);
+      if (lane >= o) {
+        dinc += td;
+        vinc += tv;
+      }
+    }
+    const uint32_t draw0 = draw_carry + dinc - nneg;
+    const uint32_t voff = vinc - nvec;
+    const uint32_t total = __shfl(vinc, 63, 64);
+    draw_carry += __shfl(dinc, 63, 64);
+    // this lane's positive term
+    float eo[DD];
+    float coeff = 0.0f;
+    bool pos_ok = false;
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < DD; ++c) eo[c] = c < nd ? e_old[(int64_t)j * nd + c] : 0.0f;
+      float dsq = 0.0f;
+#pragma unroll
+      for (int c = 0; c < DD; ++c)
+        if (c < nd) {
+          const float df = ei[c] - eo[c];
+          const float p = df * df;
+          dsq = dsq + p;
+        }
+      if (dsq >= 1e-8f) {
+        const float dist = sqrtf(dsq);
+        const float num = -((2.0f * noise) * dist) - 2.0f;
+        const float den = ((2.0f * dsq) - (0.5f * dist)) + 1.0f;
+        coeff = num / den;
+        pos_ok = true;
+      }
+    }
+    for (uint32_t w0 = 0; w0 < total; w0 += WIDE_CAP) {
+      // terms of this lane that fall in the window [w0, w0 + WIDE_CAP)
+      const uint32_t lo = voff > w0 ? voff : w0;
+      const uint32_t hi = min(voff + nvec, w0 + (uint32_t)WIDE_CAP);
+      for (uint32_t t = lo; t < hi; ++t) {
+        const uint32_t q = t - voff;
+        const int slot = (int)(t - w0);
+        bool good = false;
+        if (q == 0) {
+          good = pos_ok;
+          if (good) {
+#pragma unroll
+            for (int c = 0; c < DD; ++c)
+              if (c < nd) vb[slot * DD + c] = coeff * (ei[c] - eo[c]);
+          }
+        } else {
+          const uint32_t r = Philox::first(seed, draw0 + (q - 1), epoch, (uint32_t)i, TAG_NEG);
+          const int64_t kk = Philox::range(r, (uint64_t)n);
+          if (kk != i) {
+            float ek[DD];
+#pragma unroll
+            for (int c = 0; c < DD; ++c) ek[c] = c < nd ? e_old[kk * nd + c] : 0.0f;
+            float dk = 0.0f;
+#pragma unroll
+            for (int c = 0; c < DD; ++c)
+              if (c < nd) {
+                const float df = ei[c] - ek[c];
+                const float p = df * df;
+                dk = dk + p;
+              }
+            if (dk > 0.01f) {
+              good = true;
+              const float gc = (gamma * 4.0f) / ((1.0f + 0.25f * dk) * dk);
+#pragma unroll
+              for (int c = 0; c < DD; ++c)
+                if (c < nd) {
+                  const float diff = ei[c] - ek[c];
+                  vb[slot * DD + c] = fminf(fmaxf(gc * diff, -4.0f), 4.0f);
+                }
+            }
+          }
+        }
+        ok[slot] = good ? 1 : 0;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (lane < nd) {
+        const uint32_t cntw = min(total - w0, (uint32_t)WIDE_CAP);
+        for (uint32_t t = 0; t < cntw; ++t)
+          if (ok[t]) gl = gl + vb[t * DD + lane];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (active) {
+      next_s[s] = ns + ps;
+      next_n[s] = nn + (float)cnt * pn;
+    }
+  }
+  if (lane < nd) {
+    const int64_t qd = i * nd + lane;
+    const float eic = e_old[qd];
+    if (touched) {
+      float mq = m[qd], vq = v[qd];
+      mq = mq + one_m_b1 * (gl - mq);
+      vq = vq + one_m_b2 * (gl * gl - vq);
+      m[qd] = mq;
+      v[qd] = vq;
+      const float step = (lr_ad * mq) / (sqrtf(vq) + epsc);
+      e_new[qd] = eic + step;
+    } else {
+      e_new[qd] = eic;
+    }
+  }
+}
+
+// nodes with at least WIDE_DEG edges, in any order
+__global__ void wide_nodes_kernel(const int64_t* __restrict__ indptr, int64_t n,
+                                  int32_t* __restrict__ list, int* __restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (indptr[i + 1] - indptr[i] >= WIDE_DEG) list[atomicAdd(count, 1)] = (int32_t)i;
+}
+
 template <int D>
 void launch_epoch(unsigned grid, hipStream_t s, int64_t n, int dim, const int64_t* indptr,
                   const int32_t* other, float* ns, const float* es, float* nn, float nf,
                   const float* eo, float* en, float* m, float* v, float et, uint32_t ep, float la,
-                  float epsc, float gamma, float noise, float b1, float b2, uint64_t seed) {
+                  float epsc, float gamma, float noise, float b1, float b2, uint64_t seed,
+                  const int32_t* wide, int64_t nwide) {
   epoch_kernel<D><<<grid, 256, 0, s>>>(n, dim, indptr, other, ns, es, nn, nf, eo, en, m, v, et, ep,
-                                       la, epsc, gamma, noise, b1, b2, seed);
+                                       la, epsc, gamma, noise, b1, b2, seed,
+                                       nwide > 0 ? (int64_t)WIDE_DEG : INT64_MAX);
+  if (nwide > 0)
+    epoch_wide_kernel<D><<<(unsigned)ceil_div(nwide, 4), 256, 0, s>>>(
+        n, dim, indptr, other, ns, es, nn, nf, eo, en, m, v, et, ep, la, epsc, gamma, noise, b1,
+        b2, seed, wide, nwide);
 }
 
 }  // namespace
@@ -189,6 +373,15 @@ void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, b
   const float denom = (float)std::max(p.n_epochs - 1, 1);
   const float n_epochs_f = (float)p.n_epochs;
   const unsigned grid = (unsigned)ceil_div(n, 256);
+  // high-degree nodes for the wavefront-per-node kernel
+  Buf<int32_t> wide(pool, n);
+  Buf<int> nwide_d(pool, 1);
+  EVOC_HIP(hipMemsetAsync(nwide_d.p, 0, sizeof(int), s));
+  wide_nodes_kernel<<<grid, 256, 0, s>>>(g.indptr.p, n, wide.p, nwide_d.p);
+  int nwide_h = 0;
+  EVOC_HIP(hipMemcpyAsync(&nwide_h, nwide_d.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipStreamSynchronize(s));
+  const int64_t nwide = nwide_h;
   hipEvent_t e0, e1;
   EVOC_HIP(hipEventCreate(&e0));
   EVOC_HIP(hipEventCreate(&e1));
@@ -208,7 +401,7 @@ void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, b
 #define EVOC_EPOCH(D)                                                                         \
   launch_epoch<D>(grid, s, n, dim, g.indptr.p, g.indices.p, next_s.p, eps_s.p, next_n.p,    \
                   neg_factor, cur, nxt, m.p, v.p, epoch_t, (uint32_t)epoch, lr_ad, epsc, gamma, \
-                  p.noise_level, one_m_b1, one_m_b2, seed)
+                  p.noise_level, one_m_b1, one_m_b2, seed, wide.p, nwide)
     switch (dim) {
       case 2: EVOC_EPOCH(2); break;
       case 4: EVOC_EPOCH(4); break;

@@ -72,6 +72,42 @@ def test_embedding_asymmetric_graph_parity():
     assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
 
 
+def test_embedding_hub_nodes_parity():
+    """Nodes with 64 or more edges take the wavefront-per-node epoch kernel. Three hubs
+    with ~200 edges span several 64-slot chunks, and a high negative-sample rate stages
+    more than 256 gradient terms per chunk; the layout must equal the oracle's bit for
+    bit, for the specialised dims and a generic one."""
+    rng = np.random.default_rng(9)
+    n = 400
+    adj = [set() for _ in range(n)]
+    for i in range(n):
+        for j in rng.choice(n, size=4, replace=False):
+            if int(j) != i:
+                adj[i].add(int(j))
+                adj[int(j)].add(i)
+    for h in (0, 7, 123):
+        for j in rng.choice(n, size=200, replace=False):
+            if int(j) != h:
+                adj[h].add(int(j))
+                adj[int(j)].add(h)
+    wt = {}
+    rows = [sorted(a) for a in adj]
+    ip = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64)
+    ind = np.concatenate(rows).astype(np.int32)
+    w = np.empty(len(ind), np.float32)
+    for i, r in enumerate(rows):
+        for q, j in enumerate(r):
+            key = (min(i, j), max(i, j))
+            if key not in wt:
+                wt[key] = np.float32(rng.uniform(0.05, 1.0))
+            w[ip[i] + q] = wt[key]
+    assert max(len(r) for r in rows) >= 128
+    for dim, rate in ((8, 1.0), (16, 5.0), (5, 5.0)):
+        g = H.embedding(ip, ind, w, dim, n_epochs=25, negative_sample_rate=rate, seed=13)
+        o = O.embedding(ip, ind, w, dim, n_epochs=25, negative_sample_rate=rate, seed=13)
+        assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), (dim, rate)
+
+
 def test_label_prop_init_parity(graph):
     x, gt, idx, dist = graph
     op, oi, ow, _, _ = O.fuzzy_graph(idx, dist, 15.0)
