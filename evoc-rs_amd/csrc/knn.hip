@@ -455,10 +455,11 @@ __device__ __forceinline__ int2 lds_ld_b64(unsigned a) {
   return v;
 }
 
-template <int C, int NKC, int KC>  // C = list length, NKC = dpad / KC
+// C = list length, NKC = dpad / KC, COS = cosine keys (else squared-Euclidean)
+template <int C, int NKC, int KC, bool COS>
 __global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
     const uint16_t* __restrict__ Xh, int64_t n, int64_t npad, int dpad,
-    const float* __restrict__ colscale, int metric, int32_t* __restrict__ cand_idx,
+    const float* __restrict__ colscale, int32_t* __restrict__ cand_idx,
     float* __restrict__ cand_key) {
   using R = RaRing<KC>;
   constexpr int NST = R::NST;
@@ -609,12 +610,15 @@ __global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
         const float s_ = acc[tm][e];
-        float k_ = metric == 1 ? -(s_ * cs) : (cs - 2.0f * s_);
-        if (diag || tail) {
-          if (rem <= 0 || dv == tm * 32 + (e >> 2) * 8 + (e & 3)) k_ = FINF;
-        }
-        acc[tm][e] = k_;
+        acc[tm][e] = COS ? -(s_ * cs) : (cs - 2.0f * s_);
       }
+    if (__builtin_expect(diag || tail, 0)) {  // wave-uniform: the self pair and padding
+#pragma unroll
+      for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+          if (rem <= 0 || dv == tm * 32 + (e >> 2) * 8 + (e & 3)) acc[tm][e] = FINF;
+    }
     // Keys below their row's threshold are appended to that row's LDS bucket, which
     // keeps filling across tiles; the lists take the buckets in one parallel drain once
     // some row holds RA_DRAIN entries, so each drain inserts for many rows at once. Row r
@@ -724,12 +728,18 @@ __global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
   }
 }
 
-template __global__ void knn_topc_ra_kernel<32, 1, 128>(const uint16_t*, int64_t, int64_t, int,
-                                                        const float*, int, int32_t*, float*);
-template __global__ void knn_topc_ra_kernel<32, 1, 256>(const uint16_t*, int64_t, int64_t, int,
-                                                        const float*, int, int32_t*, float*);
-template __global__ void knn_topc_ra_kernel<32, 2, 256>(const uint16_t*, int64_t, int64_t, int,
-                                                        const float*, int, int32_t*, float*);
+template __global__ void knn_topc_ra_kernel<32, 1, 128, false>(const uint16_t*, int64_t, int64_t, int,
+                                                               const float*, int32_t*, float*);
+template __global__ void knn_topc_ra_kernel<32, 1, 128, true>(const uint16_t*, int64_t, int64_t, int,
+                                                               const float*, int32_t*, float*);
+template __global__ void knn_topc_ra_kernel<32, 1, 256, false>(const uint16_t*, int64_t, int64_t, int,
+                                                               const float*, int32_t*, float*);
+template __global__ void knn_topc_ra_kernel<32, 1, 256, true>(const uint16_t*, int64_t, int64_t, int,
+                                                               const float*, int32_t*, float*);
+template __global__ void knn_topc_ra_kernel<32, 2, 256, false>(const uint16_t*, int64_t, int64_t, int,
+                                                               const float*, int32_t*, float*);
+template __global__ void knn_topc_ra_kernel<32, 2, 256, true>(const uint16_t*, int64_t, int64_t, int,
+                                                               const float*, int32_t*, float*);
 
 // ------------------------------------------------------------------ 3. re-rank
 constexpr int MAXT = 16;  // d <= 1024 handled with the query row held in registers
@@ -937,11 +947,17 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<false, 32>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 1, 128>,
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 1, 128, false>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<128>::SMEM));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 1, 256>,
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 1, 128, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<128>::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 1, 256, false>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<256>::SMEM));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 2, 256>,
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 1, 256, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<256>::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 2, 256, false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<256>::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 2, 256, true>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<256>::SMEM));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_exact_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
@@ -1007,15 +1023,22 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   EVOC_HIP(hipEventRecord(e0, s));
   if (ra) {
     const unsigned grid = (unsigned)(npad / RA_ROWS);
+#define EVOC_RA_LAUNCH(NKC_, KC_)                                                              \
+  do {                                                                                         \
+    if (metric == 1)                                                                           \
+      knn_topc_ra_kernel<32, NKC_, KC_, true><<<grid, 256, RaRing<KC_>::SMEM, s>>>(            \
+          Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);                                    \
+    else                                                                                       \
+      knn_topc_ra_kernel<32, NKC_, KC_, false><<<grid, 256, RaRing<KC_>::SMEM, s>>>(           \
+          Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);                                    \
+  } while (0)
     if (dpad == 128)
-      knn_topc_ra_kernel<32, 1, 128><<<grid, 256, RaRing<128>::SMEM, s>>>(
-          Xh.p, n, npad, dpad, colscale.p, metric, cidx.p, ckey.p);
+      EVOC_RA_LAUNCH(1, 128);
     else if (dpad == 256)
-      knn_topc_ra_kernel<32, 1, 256><<<grid, 256, RaRing<256>::SMEM, s>>>(
-          Xh.p, n, npad, dpad, colscale.p, metric, cidx.p, ckey.p);
+      EVOC_RA_LAUNCH(1, 256);
     else
-      knn_topc_ra_kernel<32, 2, 256><<<grid, 256, RaRing<256>::SMEM, s>>>(
-          Xh.p, n, npad, dpad, colscale.p, metric, cidx.p, ckey.p);
+      EVOC_RA_LAUNCH(2, 256);
+#undef EVOC_RA_LAUNCH
   } else {
     const int nops = split ? 4 : 2;
     const int nst = split ? 2 : 4;
