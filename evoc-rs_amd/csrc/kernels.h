@@ -138,6 +138,15 @@ void partition_means_device(const float* dX, int64_t ld, int64_t d, const int64_
                             const int32_t* h_mem, const float* h_coef, int64_t np, int64_t n,
                             float* h_out, hipStream_t s, DevPool& pool);
 
+// label votes (n_iter Jacobi steps) on a device CSR; h_lab in/out
+void lp_steps_device(const int64_t* d_indptr, const int32_t* d_col, const float* d_val, int64_t n,
+                     int n_iter, int64_t* h_lab, hipStream_t s, DevPool& pool);
+// expansion of the input level, centred and scaled, into d_out (n x dim)
+void expansion_device(const int64_t* d_indptr, const int32_t* d_col, const float* d_val,
+                      int64_t n, int64_t nnz, int dim, const int32_t* h_part, const float* h_sa,
+                      int64_t np, const float* h_layout, float* d_out, hipStream_t s,
+                      DevPool& pool);
+
 // ------------------------------------------------------------------ core distances + MST
 void core_sq_device(const float* d_emb, int64_t n, int dim, int min_samples, float* d_core,
                     hipStream_t s, DevPool& pool);

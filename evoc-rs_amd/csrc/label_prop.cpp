@@ -95,15 +95,24 @@ void lp_outliers(const Sp& g, std::vector<int64_t>& lab, uint64_t seed) {
   }
 }
 
+// the same CSR in device memory (the input level only)
+struct DevG {
+  const int64_t* ptr = nullptr;
+  const int32_t* col = nullptr;
+  const float* val = nullptr;
+};
+
 // label_prop_loop (label_prop.rs:202-227): seeds, 20 steps, outliers, remap
-std::vector<int32_t> lp_partition(const Sp& g, int64_t approx, int n_iter, uint64_t seed,
-                                  int64_t* n_parts) {
+std::vector<int32_t> lp_partition(const Sp& g, const DevG* dg, int64_t approx, int n_iter,
+                                  uint64_t seed, int64_t* n_parts, hipStream_t s, DevPool& pool) {
   std::vector<int64_t> lab(g.rows, -1);
   for (int64_t t = 0; t < approx; ++t) {
     const uint32_t r = Philox::first(seed, (uint32_t)t, (uint32_t)(t >> 32), 0, TAG_LPSD);
     lab[Philox::range(r, (uint64_t)g.rows)] = t;
   }
-  for (int it = 0; it < n_iter; ++it) lab = lp_step(g, lab);
+  if (dg) lp_steps_device(dg->ptr, dg->col, dg->val, g.rows, n_iter, lab.data(), s, pool);
+  else
+    for (int it = 0; it < n_iter; ++it) lab = lp_step(g, lab);
   lp_outliers(g, lab, seed + 2000);
   // remap_labels (label_prop.rs:155-177): labelled ids in sorted order, -1s appended
   std::vector<int64_t> ids;
@@ -353,9 +362,11 @@ struct Rows {
   bool present() const { return host || dev; }
 };
 
+// With a device graph (the input level) the expansion is done on the GPU and written to
+// d_out, and the returned vector is empty.
 std::vector<float> level(const Sp& g, int dim, const Rows& data, int64_t d, int epochs,
                          int64_t approx, uint64_t seed, hipStream_t s, DevPool& pool,
-                         int verbose) {
+                         int verbose, const DevG* dg = nullptr, float* d_out = nullptr) {
   const int64_t n = g.rows;
   if (n < 64) {
     if (data.present() && dim <= d) {
@@ -381,7 +392,7 @@ std::vector<float> level(const Sp& g, int dim, const Rows& data, int64_t d, int 
     }
   };
   int64_t np = 0;
-  const std::vector<int32_t> part = lp_partition(g, approx, 20, seed, &np);
+  const std::vector<int32_t> part = lp_partition(g, dg, approx, 20, seed, &np, s, pool);
   lap("label propagation");
   if (verbose) std::printf("  Label prop: %lld nodes -> %lld partitions\n", (long long)n, (long long)np);
   std::vector<int64_t> mptr;
@@ -432,6 +443,12 @@ std::vector<float> level(const Sp& g, int dim, const Rows& data, int64_t d, int 
   lap("coarse embedding");
   // expansion: rows of (G o G^T) R_norm, L1-normalised, times the coarse layout,
   // averaged with the layout of each node's own partition
+  if (dg && d_out) {
+    expansion_device(dg->ptr, dg->col, dg->val, n, g.ptr[n], dim, part.data(), sa.data(), np,
+                     layout.data(), d_out, s, pool);
+    lap("expansion");
+    return {};
+  }
   std::vector<float> res(n * dim);
 #pragma omp parallel
   {
@@ -494,9 +511,11 @@ std::vector<float> level(const Sp& g, int dim, const Rows& data, int64_t d, int 
 
 }  // namespace
 
-void label_prop_init_host(const int64_t* indptr, const int32_t* indices, const float* weights,
-                          int64_t n, int dim, const float* data_dev, int64_t d, int64_t ld,
-                          uint64_t seed, float* out, hipStream_t s, DevPool& pool, int verbose) {
+void label_prop_init_device(const int64_t* indptr, const int32_t* indices, const float* weights,
+                            const int64_t* d_indptr, const int32_t* d_indices,
+                            const float* d_weights, int64_t n, int dim, const float* data_dev,
+                            int64_t d, int64_t ld, uint64_t seed, float* d_out, hipStream_t s,
+                            DevPool& pool, int verbose) {
   Sp g;
   g.rows = g.cols = n;
   g.ptr.assign(indptr, indptr + n + 1);
@@ -510,8 +529,30 @@ void label_prop_init_host(const int64_t* indptr, const int32_t* indices, const f
     top.dev = data_dev;
     top.ld = ld;
   }
-  std::vector<float> r = level(g, dim, top, d, 50, approx, seed, s, pool, verbose);
-  std::memcpy(out, r.data(), n * dim * sizeof(float));
+  DevG dg{d_indptr, d_indices, d_weights};
+  std::vector<float> r = level(g, dim, top, d, 50, approx, seed, s, pool, verbose, &dg, d_out);
+  if (!r.empty()) {  // the base case (n < 64) returns host rows
+    EVOC_HIP(hipMemcpyAsync(d_out, r.data(), n * dim * sizeof(float), hipMemcpyHostToDevice, s));
+    EVOC_HIP(hipStreamSynchronize(s));
+  }
+}
+
+void label_prop_init_host(const int64_t* indptr, const int32_t* indices, const float* weights,
+                          int64_t n, int dim, const float* data_dev, int64_t d, int64_t ld,
+                          uint64_t seed, float* out, hipStream_t s, DevPool& pool, int verbose) {
+  const int64_t nnz = indptr[n];
+  Buf<int64_t> dp(pool, n + 1);
+  Buf<int32_t> dc(pool, std::max<int64_t>(nnz, 1));
+  Buf<float> dw(pool, std::max<int64_t>(nnz, 1)), dout(pool, n * dim);
+  EVOC_HIP(hipMemcpyAsync(dp.p, indptr, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  if (nnz) {
+    EVOC_HIP(hipMemcpyAsync(dc.p, indices, nnz * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    EVOC_HIP(hipMemcpyAsync(dw.p, weights, nnz * sizeof(float), hipMemcpyHostToDevice, s));
+  }
+  label_prop_init_device(indptr, indices, weights, dp.p, dc.p, dw.p, n, dim, data_dev, d, ld, seed,
+                         dout.p, s, pool, verbose);
+  EVOC_HIP(hipMemcpyAsync(out, dout.p, n * dim * sizeof(float), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace evoc

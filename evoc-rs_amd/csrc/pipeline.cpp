@@ -219,11 +219,11 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
     to_host(hw, g.weights.p, g.nnz, s);
     EVOC_HIP(hipStreamSynchronize(s));
     if (verbose >= 2) std::printf("    graph to host: %.1f ms\n", ms_since(t0));
-    std::vector<float> init(n * dim);
-    if (p.symmetrise) {
-      label_prop_init_host(hptr.data(), hcol.data(), hw.data(), n, dim, dX, d, ld, seed,
-                           init.data(), s, pool, verbose);
+    if (p.symmetrise) {  // the device CSR is the same column-sorted graph
+      label_prop_init_device(hptr.data(), hcol.data(), hw.data(), g.indptr.p, g.indices.p,
+                             g.weights.p, n, dim, dX, d, ld, seed, demb.p, s, pool, verbose);
     } else {
+      std::vector<float> init(n * dim);
       // Csr::from_coo (src/utils/sparse.rs:195-235) orders every row by column
       std::vector<int32_t> c2(hcol);
       std::vector<float> w2(hw);
@@ -241,8 +241,9 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
       }
       label_prop_init_host(hptr.data(), c2.data(), w2.data(), n, dim, dX, d, ld, seed,
                            init.data(), s, pool, verbose);
+      EVOC_HIP(hipMemcpyAsync(demb.p, init.data(), n * dim * sizeof(float), hipMemcpyHostToDevice,
+                              s));
     }
-    EVOC_HIP(hipMemcpyAsync(demb.p, init.data(), n * dim * sizeof(float), hipMemcpyHostToDevice, s));
   }
   EVOC_HIP(hipStreamSynchronize(s));
   stage[2] = ms_since(t0);

@@ -72,13 +72,10 @@ def test_embedding_asymmetric_graph_parity():
     assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
 
 
-def test_embedding_hub_nodes_parity():
-    """Nodes with 64 or more edges take the wavefront-per-node epoch kernel. Three hubs
-    with ~200 edges span several 64-slot chunks, and a high negative-sample rate stages
-    more than 256 gradient terms per chunk; the layout must equal the oracle's bit for
-    bit, for the specialised dims and a generic one."""
-    rng = np.random.default_rng(9)
-    n = 400
+def _hub_graph(seed=9, n=400):
+    """Symmetric column-sorted CSR: 4 random edges per node plus three hubs with ~200
+    edges each (several 64-slot chunks), symmetric weights in (0.05, 1)."""
+    rng = np.random.default_rng(seed)
     adj = [set() for _ in range(n)]
     for i in range(n):
         for j in rng.choice(n, size=4, replace=False):
@@ -102,10 +99,29 @@ def test_embedding_hub_nodes_parity():
                 wt[key] = np.float32(rng.uniform(0.05, 1.0))
             w[ip[i] + q] = wt[key]
     assert max(len(r) for r in rows) >= 128
+    return ip, ind, w
+
+
+def test_embedding_hub_nodes_parity():
+    """Nodes with 64 or more edges take the wavefront-per-node epoch kernel. The hubs
+    span several 64-slot chunks, and a high negative-sample rate stages more than 256
+    gradient terms per chunk; the layout must equal the oracle's bit for bit, for the
+    specialised dims and a generic one."""
+    ip, ind, w = _hub_graph()
     for dim, rate in ((8, 1.0), (16, 5.0), (5, 5.0)):
         g = H.embedding(ip, ind, w, dim, n_epochs=25, negative_sample_rate=rate, seed=13)
         o = O.embedding(ip, ind, w, dim, n_epochs=25, negative_sample_rate=rate, seed=13)
         assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), (dim, rate)
+
+
+def test_label_prop_init_hub_graph_parity():
+    """The input level's label votes and expansion run one wavefront per node on the
+    GPU; hubs with ~200 neighbours exercise their multi-chunk paths."""
+    ip, ind, w = _hub_graph(seed=21, n=600)
+    for seed in (3, 8):
+        g = H.label_prop_init(ip, ind, w, 8, data=None, seed=seed)
+        o = O.label_prop_init(ip, ind, w, 8, data=None, seed=seed)
+        assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), seed
 
 
 def test_label_prop_init_parity(graph):
