@@ -159,7 +159,7 @@ def main():
     achieved = flops / (knn_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 4),
-                "traffic": read_traffic(), "kernel": "knn_topc_ra_kernel",
+                "traffic": read_traffic(), "kernel": "knn_topc_rb_kernel",
                 "kernel_ms": round(knn_ms, 3)}
     cpu = None
     if not args.no_cpu_baseline:

@@ -741,6 +741,281 @@ template __global__ void knn_topc_ra_kernel<32, 2, 256, false>(const uint16_t*, 
 template __global__ void knn_topc_ra_kernel<32, 2, 256, true>(const uint16_t*, int64_t, int64_t, int,
                                                                const float*, int32_t*, float*);
 
+// ------------------------------------------------------------------ 2c. two waves per SIMD
+// bf16-exact inputs, d <= 512, k <= 16, as knn_topc_ra_kernel but with eight 32-row
+// wavefronts per 256-row workgroup (two per SIMD). A wavefront's resident A-fragments
+// (32 rows x d, 128 registers at d = 512) then fit the VGPRs that the MFMAs read
+// directly, and while one wavefront waits (LDS, barrier, epilogue) the other keeps the
+// SIMD's matrix core busy. Each B fragment feeds one MFMA, so the ring's LDS reads are
+// 1 KB per MFMA. Per tile a wavefront holds one 32 x 32 key tile: lane l has column
+// l & 31 of rows j*8 + (l >> 5)*4 + i. Candidates are kept per (row, column half):
+// lane l owns row l & 31, columns with bit 4 = l >> 5, in a sorted list of 16, so a
+// row has 32 candidates and its bound is the smaller of its two halves' last keys.
+constexpr int RB_ROWS = 256;
+constexpr int RB_WAVES = 8;
+constexpr int RB_BKT = 16;     // bucket slots per (row, half)
+constexpr int RB_DRAIN = 8;    // drain once some (row, half) holds this many
+constexpr int RB_BGRP = 2;     // B fragments read from LDS ahead of their MFMAs
+constexpr int RB_WREG = 64 * RB_BKT * 8;                // per-wavefront buckets / key tile
+static_assert(RB_WREG >= 32 * RA_SCS * 4, "the key-tile dump must fit the bucket region");
+constexpr int RB_EPI_BYTES = RB_WAVES * RB_WREG + RB_WAVES * 64 * 4;
+
+template <int KC>
+struct RbRing {
+  static constexpr int STB_B = RA_BN * KC * 2;              // 16 KB at KC = 256
+  static constexpr int STB = STB_B + RB_WAVES * 256;        // + column scales per wave
+  static constexpr int NST = 4;
+  static constexpr int PIECES = KC / 8;
+  static constexpr int LOADS = STB_B / (64 * RB_WAVES * 16);  // 16-byte loads per thread
+  static constexpr size_t SMEM = (size_t)NST * STB;
+  static __device__ __forceinline__ int slot(int c, int q) { return c * PIECES + (q ^ (c & 15)); }
+};
+
+template <int NKC, int KC, bool COS>
+__global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
+    const uint16_t* __restrict__ Xh, int64_t n, int64_t npad, int dpad,
+    const float* __restrict__ colscale, int32_t* __restrict__ cand_idx,
+    float* __restrict__ cand_key) {
+  constexpr int C = 16;
+  using R = RbRing<KC>;
+  constexpr int NST = R::NST;
+  constexpr int LOADS = R::LOADS;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  __shared__ __attribute__((aligned(16))) char epi[RB_EPI_BYTES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  char* ring = smem;
+  float* sc = reinterpret_cast<float*>(epi + wave * RB_WREG);
+  int2* bucket = reinterpret_cast<int2*>(sc);  // [64 owners][RB_BKT]
+  float* thr = reinterpret_cast<float*>(epi + RB_WAVES * RB_WREG) + wave * 64;  // [owner]
+
+  const int64_t row0 = (int64_t)blockIdx.x * RB_ROWS;
+  const int64_t ntile = npad / RA_BN;
+  const int64_t total = ntile * NKC;
+
+  bf16x8 afr[NKC * KC / 16];
+  {
+    const uint16_t* ar = Xh + (row0 + wave * 32 + (lane & 31)) * dpad + (lane >> 5) * 8;
+#pragma unroll
+    for (int f = 0; f < NKC * KC / 16; ++f) afr[f] = *reinterpret_cast<const bf16x8*>(ar + f * 16);
+  }
+  wait_vm<0>();
+  float lk[C];  // owner list: row lane & 31 of this wavefront, column half lane >> 5
+  int li[C];
+#pragma unroll
+  for (int q = 0; q < C; ++q) {
+    lk[q] = FINF;
+    li[q] = -1;
+  }
+  thr[lane] = FINF;
+  int bcnt = 0;
+
+  const uint16_t* pb[LOADS];
+#pragma unroll
+  for (int h = 0; h < LOADS; ++h) {
+    const int s = (h * RB_WAVES + wave) * 64 + lane;
+    const int c = s / R::PIECES, qs = s % R::PIECES;
+    pb[h] = Xh + (int64_t)c * dpad + (qs ^ (c & 15)) * 8;
+  }
+  const float* pc = colscale + (lane & 31);
+  int64_t gi = 0;
+  int kci = 0, sloti = 0;
+  const int64_t tile_step = (int64_t)RA_BN * dpad - (int64_t)(NKC - 1) * KC;
+  auto issue_next = [&]() {
+    char* st = ring + sloti * R::STB;
+#pragma unroll
+    for (int h = 0; h < LOADS; ++h) {
+      const uint16_t* gp = pb[h];  // the builtin rejects the captured element itself
+      __builtin_amdgcn_global_load_lds(gp, (lds_void*)(st + (h * RB_WAVES + wave) * 1024), 16, 0, 0);
+    }
+    {
+      const float* cp = pc;
+      __builtin_amdgcn_global_load_lds(cp, (lds_void*)(st + R::STB_B + wave * 256), 4, 0, 0);
+    }
+    if (gi + 1 < total) {
+      if (++kci == NKC) {
+        kci = 0;
+#pragma unroll
+        for (int h = 0; h < LOADS; ++h) pb[h] += tile_step;
+        pc += RA_BN;
+      } else {
+#pragma unroll
+        for (int h = 0; h < LOADS; ++h) pb[h] += KC;
+      }
+    }
+    ++gi;
+    sloti = (sloti + 1) & (NST - 1);
+  };
+
+  const int lcol = lane & 31, half = lcol >> 4, grp = lane >> 4;
+  const int lrow = (lane >> 5) * 4;
+  for (int g = 0; g < NST - 1; ++g) issue_next();
+  for (int64_t ct = 0; ct < ntile; ++ct) {
+    f32x16 acc = (f32x16){};
+#pragma unroll
+    for (int kc = 0; kc < NKC; ++kc) {
+      const int64_t g = ct * NKC + kc;
+      wait_vm<(LOADS + 1) * (NST - 2)>();
+      ring_barrier();
+      // B piece q = 2*kk + (lane >> 5) of column lcol sits at slot lcol*PIECES + (q ^ (lcol &
+      // 15)) = lcol*PIECES + (2*kk ^ m); the per-lane parts are made opaque per chunk so the
+      // 16 addresses are formed in the loop rather than held in registers across it
+      int bcol = lcol * R::PIECES * 16, bm = (lane >> 5) ^ (lcol & 15);
+      asm volatile("" : "+v"(bcol), "+v"(bm));
+      const char* st = ring + (int)(g & (NST - 1)) * R::STB + bcol;
+#pragma unroll
+      for (int k0 = 0; k0 < KC / 16; k0 += RB_BGRP) {
+        bf16x8 bq[RB_BGRP];
+#pragma unroll
+        for (int u = 0; u < RB_BGRP; ++u)
+          bq[u] = *reinterpret_cast<const bf16x8*>(st + ((2 * (k0 + u)) ^ bm) * 16);
+#pragma unroll
+        for (int u = 0; u < RB_BGRP; ++u) {
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afr[kc * (KC / 16) + k0 + u], bq[u], acc,
+                                                        0, 0, 0);
+          if (k0 + u == 0) issue_next();
+        }
+      }
+    }
+
+    // ---------------- wavefront-private epilogue of column tile ct
+    const bool diag = (ct * RA_BN < row0 + RB_ROWS) && (ct * RA_BN + RA_BN > row0);
+    const bool tail = (ct + 1) * RA_BN > n;
+    f32x4 tq[4];  // thresholds of this lane's 16 rows in its column half
+    float cs;
+    {
+      const unsigned ta = lds_addr(thr + 32 * half + lrow);
+      const unsigned ca = lds_addr(ring + (int)((ct * NKC + NKC - 1) & (NST - 1)) * R::STB +
+                                   R::STB_B + wave * 256 + lane * 4);
+      asm volatile(
+          "ds_read_b128 %0, %5\n"
+          "ds_read_b128 %1, %5 offset:32\n"
+          "ds_read_b128 %2, %5 offset:64\n"
+          "ds_read_b128 %3, %5 offset:96\n"
+          "ds_read_b32 %4, %6\n"
+          "s_waitcnt lgkmcnt(0)"
+          : "=v"(tq[0]), "=v"(tq[1]), "=v"(tq[2]), "=v"(tq[3]), "=v"(cs)
+          : "v"(ta), "v"(ca)
+          : "memory");
+    }
+    int dv = (int)(ct * RA_BN - row0) - wave * 32 + lcol - lrow;
+    int rem = (int)std::min<int64_t>(n - ct * RA_BN, RA_BN) - lcol;
+    int sb = lrow * RA_SCS + lcol;
+    asm volatile("" : "+v"(dv), "+v"(rem), "+v"(sb));
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const float s_ = acc[e];
+      acc[e] = COS ? -(s_ * cs) : (cs - 2.0f * s_);
+    }
+    if (__builtin_expect(diag || tail, 0)) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e)
+        if (rem <= 0 || dv == (e >> 2) * 8 + (e & 3)) acc[e] = FINF;
+    }
+    const int cbase = (int)(ct * RA_BN);
+    bool touched = false, overflow = false;
+    int inc = 0;
+    // owner of this lane's (row, half) group in slot (j, i): row j*8+i (+4 for lanes >= 32),
+    // half (lane >> 4) & 1; its bucket is ob + (j*8+i) * RB_BKT * 8 bytes
+    int ob = (int)lds_addr(bucket) + ((grp & 1) * 32 + (grp >> 1) * 4) * RB_BKT * 8;
+    asm volatile("" : "+v"(ob));
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float key = acc[j * 4 + i];
+        const bool pass = key < tq[j][i];
+        const unsigned long long m = __ballot(pass);
+        if (m) {
+          touched = true;
+          // the 4 lane groups of 16 are the (row, half) pairs (r0, 0), (r0, 1),
+          // (r0 + 4, 0), (r0 + 4, 1); owner lane of a pair = row + 32 * half
+          const int r0 = j * 8 + i;
+          const int o0 = r0, o1 = r0 + 32, o2 = r0 + 4, o3 = r0 + 36;
+          const int c0 = __builtin_amdgcn_readlane(bcnt, o0);
+          const int c1 = __builtin_amdgcn_readlane(bcnt, o1);
+          const int c2 = __builtin_amdgcn_readlane(bcnt, o2);
+          const int c3 = __builtin_amdgcn_readlane(bcnt, o3);
+          const int n0 = __popcll(m & 0xffffull), n1 = __popcll(m & 0xffff0000ull);
+          const int n2 = __popcll(m & 0xffff00000000ull), n3 = __popcll(m & 0xffff000000000000ull);
+          overflow |= (c0 + n0 > RB_BKT) | (c1 + n1 > RB_BKT) | (c2 + n2 > RB_BKT) |
+                      (c3 + n3 > RB_BKT);
+          const unsigned long long gm = m & (0xffffull << (16 * grp));
+          const int below = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(gm >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((unsigned)gm, 0u));
+          const int base = grp == 0 ? c0 : (grp == 1 ? c1 : (grp == 2 ? c2 : c3));
+          const int pos = base + below;
+          if (pass && pos < RB_BKT)
+            lds_st_b64((unsigned)(ob + r0 * RB_BKT * 8 + pos * 8),
+                       make_int2(__float_as_int(key), cbase + lcol));
+          inc += (lane == o0 ? n0 : 0) + (lane == o1 ? n1 : 0) + (lane == o2 ? n2 : 0) +
+                 (lane == o3 ? n3 : 0);
+        }
+      }
+    if (touched) {
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_wave_barrier();
+      bool drain = overflow;
+      if (!overflow) {
+        bcnt += inc;
+        drain = __ballot(bcnt >= RB_DRAIN) != 0;
+      }
+      if (drain) {
+        const unsigned ba = lds_addr(bucket + lane * RB_BKT);
+        int2 en = lds_ld_b64(ba);
+#pragma unroll 1
+        for (int e = 0; e < bcnt; ++e) {
+          const int2 cur = en;
+          en = lds_ld_b64(ba + (unsigned)(e + 1) * 8u);
+          cand_insert<C>(lk, li, __int_as_float(cur.x), cur.y);
+        }
+        bcnt = 0;
+        if (overflow) {
+          asm volatile("" ::: "memory");
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int e = 0; e < 16; ++e) sc[sb + ((e >> 2) * 8 + (e & 3)) * RA_SCS] = acc[e];
+          asm volatile("" ::: "memory");
+          __builtin_amdgcn_wave_barrier();
+          // owner lane: row lane & 31, columns 16 * (lane >> 5) .. + 15
+          const float* srow = sc + (lane & 31) * RA_SCS + 16 * (lane >> 5);
+#pragma unroll 1
+          for (int q = 0; q < 16; ++q) cand_insert<C>(lk, li, srow[q], cbase + 16 * (lane >> 5) + q);
+        }
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+        lds_st_b32(lds_addr(thr + lane), __float_as_int(lk[C - 1]));
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+  }
+  if (__ballot(bcnt > 0) != 0) {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+    const unsigned ba = lds_addr(bucket + lane * RB_BKT);
+#pragma unroll 1
+    for (int e = 0; e < bcnt; ++e) {
+      const int2 cur = lds_ld_b64(ba + (unsigned)e * 8u);
+      cand_insert<C>(lk, li, __int_as_float(cur.x), cur.y);
+    }
+  }
+  wait_vm<0>();
+  // list of (row, half) -> candidates (row * 2 + half) * 16
+  const int64_t grow = row0 + wave * 32 + (lane & 31);
+  if (grow < n) {
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      cand_key[(grow * 2 + (lane >> 5)) * C + q] = lk[q];
+      cand_idx[(grow * 2 + (lane >> 5)) * C + q] = li[q];
+    }
+  }
+}
+
+template __global__ void knn_topc_rb_kernel<2, 256, true>(const uint16_t*, int64_t, int64_t, int,
+                                                          const float*, int32_t*, float*);
+template __global__ void knn_topc_rb_kernel<2, 256, false>(const uint16_t*, int64_t, int64_t, int,
+                                                           const float*, int32_t*, float*);
+
 // ------------------------------------------------------------------ 3. re-rank
 constexpr int MAXT = 16;  // d <= 1024 handled with the query row held in registers
 
@@ -844,88 +1119,133 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
 
 // ------------------------------------------------------------------ 4. exact rows
 constexpr int KMAX = 32;
+
+// block-wide merge of 256 per-thread sorted lists (k entries each, in LDS) into list 0
+__device__ void exact_block_merge(float* sk, int* si, int k) {
+  const int tid = threadIdx.x;
+  for (int stride = 1; stride < 256; stride <<= 1) {
+    if ((tid & (2 * stride - 1)) == 0) {
+      float* ak = sk + tid * KMAX;
+      int* ai = si + tid * KMAX;
+      const float* bk = sk + (tid + stride) * KMAX;
+      const int* bi = si + (tid + stride) * KMAX;
+      float mk[KMAX];
+      int mi[KMAX];
+      int pa = 0, pb = 0;
+      for (int q = 0; q < k; ++q) {
+        const bool ta = (ak[pa] < bk[pb]) || (ak[pa] == bk[pb] && (unsigned)ai[pa] < (unsigned)bi[pb]);
+        mk[q] = ta ? ak[pa] : bk[pb];
+        mi[q] = ta ? ai[pa] : bi[pb];
+        pa += ta;
+        pb += !ta;
+      }
+      for (int q = 0; q < k; ++q) {
+        ak[q] = mk[q];
+        ai[q] = mi[q];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Exact rows, phase 1: block (f, part) scans columns [part*span, (part+1)*span) of
+// failing row f in the canonical order and keeps the block's best k by (dist, idx).
 __global__ __launch_bounds__(256) void knn_exact_kernel(
     const float* __restrict__ X, int64_t n, int d, int64_t ld, const float* __restrict__ nrm,
-    int k, int metric, const int* __restrict__ fail_count, const int32_t* __restrict__ fail_rows,
-    int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+    int k, int metric, const int32_t* __restrict__ fail_rows, int parts, int64_t span,
+    float* __restrict__ part_key, int32_t* __restrict__ part_idx) {
   extern __shared__ __attribute__((aligned(16))) char xsm[];
   float* sk = reinterpret_cast<float*>(xsm);
   int* si = reinterpret_cast<int*>(xsm + 256 * KMAX * sizeof(float));
   const int tid = threadIdx.x;
-  const int nf = *fail_count;
-  for (int f = blockIdx.x; f < nf; f += gridDim.x) {
-    const int64_t row = fail_rows[f];
-    float lk[KMAX];
-    int li[KMAX];
+  const int f = blockIdx.x / parts, part = blockIdx.x % parts;
+  const int64_t row = fail_rows[f];
+  float lk[KMAX];
+  int li[KMAX];
 #pragma unroll
-    for (int q = 0; q < KMAX; ++q) {
-      lk[q] = FINF;
-      li[q] = -1;
-    }
-    const float* xr = X + row * ld;
-    for (int64_t j = tid; j < n; j += 256) {
-      if (j == row) continue;
-      const float* xj = X + j * ld;
-      float acc[64];
+  for (int q = 0; q < KMAX; ++q) {
+    lk[q] = FINF;
+    li[q] = -1;
+  }
+  const float* xr = X + row * ld;
+  const int64_t j0 = (int64_t)part * span, j1 = min<int64_t>(n, j0 + span);
+  for (int64_t j = j0 + tid; j < j1; j += 256) {
+    if (j == row) continue;
+    const float* xj = X + j * ld;
+    float acc[64];
 #pragma unroll
-      for (int c = 0; c < 64; ++c) acc[c] = 0.0f;
-      for (int t0 = 0; t0 < d; t0 += 64) {
+    for (int c = 0; c < 64; ++c) acc[c] = 0.0f;
+    for (int t0 = 0; t0 < d; t0 += 64) {
 #pragma unroll
-        for (int c = 0; c < 64; ++c)
-          if (t0 + c < d) {
-            const float a = xr[t0 + c], b = xj[t0 + c];
-            if (metric == 1) {
-              const float p = a * b;
-              acc[c] = acc[c] + p;
-            } else {
-              const float df = a - b;
-              const float p = df * df;
-              acc[c] = acc[c] + p;
-            }
+      for (int c = 0; c < 64; ++c)
+        if (t0 + c < d) {
+          const float a = xr[t0 + c], b = xj[t0 + c];
+          if (metric == 1) {
+            const float p = a * b;
+            acc[c] = acc[c] + p;
+          } else {
+            const float df = a - b;
+            const float p = df * df;
+            acc[c] = acc[c] + p;
           }
-      }
-#pragma unroll
-      for (int s = 32; s >= 1; s >>= 1)
-#pragma unroll
-        for (int c = 0; c < s; ++c) acc[c] = acc[c] + acc[c + s];
-      const float dist = metric == 1 ? cos_dist(acc[0], nrm[row], nrm[j]) : sqrtf(acc[0]);
-      if (dist < lk[k - 1] || (dist == lk[k - 1] && (int)j < li[k - 1]))
-        list_insert<KMAX>(lk, li, dist, (int)j);
-    }
-#pragma unroll
-    for (int q = 0; q < KMAX; ++q) {
-      sk[tid * KMAX + q] = lk[q];
-      si[tid * KMAX + q] = li[q];
-    }
-    __syncthreads();
-    for (int stride = 1; stride < 256; stride <<= 1) {
-      if ((tid & (2 * stride - 1)) == 0) {
-        float* ak = sk + tid * KMAX;
-        int* ai = si + tid * KMAX;
-        const float* bk = sk + (tid + stride) * KMAX;
-        const int* bi = si + (tid + stride) * KMAX;
-        float mk[KMAX];
-        int mi[KMAX];
-        int pa = 0, pb = 0;
-        for (int q = 0; q < k; ++q) {
-          const bool ta = (ak[pa] < bk[pb]) || (ak[pa] == bk[pb] && (unsigned)ai[pa] < (unsigned)bi[pb]);
-          mk[q] = ta ? ak[pa] : bk[pb];
-          mi[q] = ta ? ai[pa] : bi[pb];
-          pa += ta;
-          pb += !ta;
         }
-        for (int q = 0; q < k; ++q) {
-          ak[q] = mk[q];
-          ai[q] = mi[q];
-        }
-      }
-      __syncthreads();
     }
-    for (int q = tid; q < k; q += 256) {
-      out_idx[row * k + q] = si[q];
-      out_dist[row * k + q] = sk[q];
-    }
-    __syncthreads();
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1)
+#pragma unroll
+      for (int c = 0; c < s; ++c) acc[c] = acc[c] + acc[c + s];
+    const float dist = metric == 1 ? cos_dist(acc[0], nrm[row], nrm[j]) : sqrtf(acc[0]);
+    if (dist < lk[k - 1] || (dist == lk[k - 1] && (int)j < li[k - 1]))
+      list_insert<KMAX>(lk, li, dist, (int)j);
+  }
+#pragma unroll
+  for (int q = 0; q < KMAX; ++q) {
+    sk[tid * KMAX + q] = lk[q];
+    si[tid * KMAX + q] = li[q];
+  }
+  __syncthreads();
+  exact_block_merge(sk, si, k);
+  for (int q = tid; q < k; q += 256) {
+    part_key[(int64_t)blockIdx.x * k + q] = sk[q];
+    part_idx[(int64_t)blockIdx.x * k + q] = si[q];
+  }
+}
+
+// Exact rows, phase 2: one block per failing row merges its parts' lists by (dist, idx).
+__global__ __launch_bounds__(256) void knn_exact_merge_kernel(
+    int k, const int32_t* __restrict__ fail_rows, int parts, const float* __restrict__ part_key,
+    const int32_t* __restrict__ part_idx, int32_t* __restrict__ out_idx,
+    float* __restrict__ out_dist) {
+  extern __shared__ __attribute__((aligned(16))) char xsm[];
+  float* sk = reinterpret_cast<float*>(xsm);
+  int* si = reinterpret_cast<int*>(xsm + 256 * KMAX * sizeof(float));
+  const int tid = threadIdx.x;
+  const int f = blockIdx.x;
+  const int64_t row = fail_rows[f];
+  float lk[KMAX];
+  int li[KMAX];
+#pragma unroll
+  for (int q = 0; q < KMAX; ++q) {
+    lk[q] = FINF;
+    li[q] = -1;
+  }
+  const int64_t m = (int64_t)parts * k;
+  for (int64_t e = tid; e < m; e += 256) {
+    const float dist = part_key[(int64_t)f * m + e];
+    const int j = part_idx[(int64_t)f * m + e];
+    if (j < 0) continue;
+    if (dist < lk[k - 1] || (dist == lk[k - 1] && j < li[k - 1])) list_insert<KMAX>(lk, li, dist, j);
+  }
+#pragma unroll
+  for (int q = 0; q < KMAX; ++q) {
+    sk[tid * KMAX + q] = lk[q];
+    si[tid * KMAX + q] = li[q];
+  }
+  __syncthreads();
+  exact_block_merge(sk, si, k);
+  for (int q = tid; q < k; q += 256) {
+    out_idx[row * k + q] = si[q];
+    out_dist[row * k + q] = sk[q];
   }
 }
 
@@ -959,6 +1279,10 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<256>::SMEM));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 2, 256, true>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<256>::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<256>::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<256>::SMEM));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_exact_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
     attr_set = true;
@@ -1032,12 +1356,18 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       knn_topc_ra_kernel<32, NKC_, KC_, false><<<grid, 256, RaRing<KC_>::SMEM, s>>>(           \
           Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);                                    \
   } while (0)
-    if (dpad == 128)
+    if (dpad == 128) {
       EVOC_RA_LAUNCH(1, 128);
-    else if (dpad == 256)
+    } else if (dpad == 256) {
       EVOC_RA_LAUNCH(1, 256);
-    else
-      EVOC_RA_LAUNCH(2, 256);
+    } else {  // d in (256, 512]: eight 32-row wavefronts, two per SIMD
+      if (metric == 1)
+        knn_topc_rb_kernel<2, 256, true><<<grid, 512, RbRing<256>::SMEM, s>>>(
+            Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);
+      else
+        knn_topc_rb_kernel<2, 256, false><<<grid, 512, RbRing<256>::SMEM, s>>>(
+            Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);
+    }
 #undef EVOC_RA_LAUNCH
   } else {
     const int nops = split ? 4 : 2;
@@ -1064,16 +1394,23 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   EVOC_HIP(hipGetLastError());
   EVOC_HIP(hipEventRecord(e1, s));
   knn_rerank_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(
-      dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, cidx.p, ckey.p, C, ra ? 1 : 2, k, metric,
+      dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, cidx.p, ckey.p, C, (ra && dpad < 512) ? 1 : 2,
+      k, metric,
       gamma, d_idx, d_dist, fcount.p, frows.p);
   EVOC_HIP(hipGetLastError());
   int nfail = 0;
   EVOC_HIP(hipMemcpyAsync(&nfail, fcount.p, sizeof(int), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
   if (nfail > 0) {
-    const unsigned g2 = (unsigned)std::min(nfail, 4096);
-    knn_exact_kernel<<<g2, 256, 256 * KMAX * 8, s>>>(dX, n, d, ld, nrm.p, k, metric, fcount.p,
-                                                     frows.p, d_idx, d_dist);
+    // each failing row's columns are split over `parts` workgroups, then merged
+    const int parts = (int)std::max<int64_t>(1, std::min<int64_t>(512, 4096 / nfail));
+    const int64_t span = ceil_div(n, (int64_t)parts);
+    Buf<float> pk(pool, (size_t)nfail * parts * k);
+    Buf<int32_t> pi(pool, (size_t)nfail * parts * k);
+    knn_exact_kernel<<<(unsigned)(nfail * parts), 256, 256 * KMAX * 8, s>>>(
+        dX, n, d, ld, nrm.p, k, metric, frows.p, parts, span, pk.p, pi.p);
+    knn_exact_merge_kernel<<<(unsigned)nfail, 256, 256 * KMAX * 8, s>>>(k, frows.p, parts, pk.p,
+                                                                        pi.p, d_idx, d_dist);
     EVOC_HIP(hipGetLastError());
   }
   float mfma_ms = 0.0f;
