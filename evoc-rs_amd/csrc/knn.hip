@@ -755,7 +755,7 @@ constexpr int RB_ROWS = 256;
 constexpr int RB_WAVES = 8;
 constexpr int RB_BKT = 16;     // bucket slots per (row, half)
 constexpr int RB_DRAIN = 8;    // drain once some (row, half) holds this many
-constexpr int RB_BGRP = 2;     // B fragments read from LDS ahead of their MFMAs
+constexpr int RB_BGRP = 4;     // B fragments read from LDS ahead of their MFMAs
 constexpr int RB_WREG = 64 * RB_BKT * 8;                // per-wavefront buckets / key tile
 static_assert(RB_WREG >= 32 * RA_SCS * 4, "the key-tile dump must fit the bucket region");
 constexpr int RB_EPI_BYTES = RB_WAVES * RB_WREG + RB_WAVES * 64 * 4;
