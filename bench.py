@@ -59,7 +59,7 @@ def read_traffic():
     """HBM bytes per launch of the kNN tile kernel from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, KiB per dispatch, scaled by the copy-kernel
     calibration factor recorded next to it), or None when no summary exists."""
-    path = os.path.join(REPO, "profiles", "knn_topc_ra_pmc_1Mx512.txt")
+    path = os.path.join(REPO, "profiles", "knn_topc_rb_pmc_1Mx512.txt")
     if not os.path.exists(path):
         return None
     vals = {}
