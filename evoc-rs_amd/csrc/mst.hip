@@ -1,11 +1,13 @@
 // Core distances and the mutual-reachability MST (build_mst, src/clustering/mst.rs:82-166)
-// on gfx950, without the KD-tree (src/clustering/kd_tree.rs): the embedding is ordered
-// along a Morton curve and cut into 64-point tiles with bounding boxes, and the tiles
-// are the leaves of an implicit bounding-volume tree (heap order, node boxes merged
-// bottom-up). Every query wavefront (one tile of 64 queries) walks that tree depth
-// first without a stack, skipping a subtree when every lane's point-to-box distance
-// already exceeds its bound (the AABB rule of kd_tree.rs:330-339) or when the subtree
-// lies wholly in the lane's component (same-component pruning, kd_tree.rs:336-339).
+// on gfx950, with a kd-tree of its own layout in place of src/clustering/kd_tree.rs: the
+// embedding is cut into 64-point tiles, the tiles are the leaves of an implicit binary
+// tree in heap order, and level by level every node's points are ordered along the
+// widest dimension of its box, so that the count split at its middle tile is a median
+// split. Every query wavefront (one tile of 64 queries) walks that tree depth first,
+// nearer child first, over a wave-uniform stack, skipping a subtree when every lane's
+// point-to-box distance already exceeds its bound (the AABB rule of kd_tree.rs:330-339)
+// or when the subtree lies wholly in the lane's component (same-component pruning,
+// kd_tree.rs:336-339).
 //
 // Ties: core distances are order-free; MST edges are chosen under the strict total
 // order (w^2, min(u,v), max(u,v)) on original point ids, so the tree is the unique
@@ -23,51 +25,101 @@ constexpr int TS = 64;  // points per tile (one wavefront of queries)
 constexpr int MAXD = 16;
 constexpr int KCORE = 32;
 
-__device__ __forceinline__ unsigned ord_f32(float f) {  // order-preserving map to uint
-  unsigned b = __float_as_uint(f);
-  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
-__device__ __forceinline__ float unord_f32(unsigned u) {
-  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
-}
-
-__global__ void bbox_kernel(const float* __restrict__ e, int64_t n, int dim,
-                            unsigned* __restrict__ lo, unsigned* __restrict__ hi) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// kd ordering: positions are cut into 64-point tiles, and tile t is leaf L + t of the
+// implicit tree. Level by level, every node's points are ordered along the widest
+// dimension of the node's box, so that the fixed count split at the node's middle tile
+// boundary is a median split (a balanced kd-tree in the tree's own heap layout).
+__global__ void perm_tile_box_kernel(const float* __restrict__ e, const uint32_t* __restrict__ perm,
+                                     int64_t n, int dim, int64_t ntiles, float* __restrict__ blo,
+                                     float* __restrict__ bhi) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t t = (int64_t)blockIdx.x * 4 + wave;
+  if (t >= ntiles) return;
+  const int64_t p = t * TS + lane;
+  const int64_t row = p < n ? (int64_t)perm[p] : 0;
   for (int c = 0; c < dim; ++c) {
-    float x = e[i * dim + c];
+    float x = p < n ? e[row * dim + c] : 0.0f;
     if (!(x == x)) x = 0.0f;
-    atomicMin(&lo[c], ord_f32(x));
-    atomicMax(&hi[c], ord_f32(x));
-  }
-}
-
-__global__ void morton_kernel(const float* __restrict__ e, int64_t n, int dim,
-                              const unsigned* __restrict__ lo, const unsigned* __restrict__ hi,
-                              int bits, uint64_t* __restrict__ code, uint32_t* __restrict__ idx) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t q[MAXD];
-  const float maxq = (float)((1u << bits) - 1u);
+    float l = p < n ? x : __builtin_huge_valf();
+    float h = p < n ? x : -__builtin_huge_valf();
 #pragma unroll
-  for (int c = 0; c < MAXD; ++c) {
-    q[c] = 0;
-    if (c < dim) {
-      const float l = unord_f32(lo[c]), h = unord_f32(hi[c]);
-      float x = e[i * dim + c];
-      if (!(x == x)) x = 0.0f;
-      const float r = h > l ? (x - l) / (h - l) : 0.0f;
-      q[c] = (uint32_t)fminf(fmaxf(r * maxq, 0.0f), maxq);
+    for (int o = 32; o >= 1; o >>= 1) {
+      l = fminf(l, __shfl_xor(l, o, 64));
+      h = fmaxf(h, __shfl_xor(h, o, 64));
+    }
+    if (lane == 0) {
+      blo[t * dim + c] = l;
+      bhi[t * dim + c] = h;
     }
   }
-  uint64_t m = 0;
-  for (int b = bits - 1; b >= 0; --b)
+}
+
+// one workgroup per node of level `lev`: the box of the tiles under it
+__global__ __launch_bounds__(256) void kd_node_box_kernel(const float* __restrict__ blo,
+                                                          const float* __restrict__ bhi,
+                                                          int64_t ntiles, int64_t span, int dim,
+                                                          float* __restrict__ olo,
+                                                          float* __restrict__ ohi) {
+  __shared__ float rl[4], rh[4];
+  const int64_t node = blockIdx.x;
+  const int64_t t0 = node * span, t1 = min<int64_t>(t0 + span, ntiles);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int c = 0; c < dim; ++c) {
+    float l = __builtin_huge_valf(), h = -__builtin_huge_valf();
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += 256) {
+      l = fminf(l, blo[t * dim + c]);
+      h = fmaxf(h, bhi[t * dim + c]);
+    }
 #pragma unroll
-    for (int c = 0; c < MAXD; ++c)
-      if (c < dim) m = (m << 1) | ((q[c] >> b) & 1u);
-  code[i] = m;
-  idx[i] = (uint32_t)i;
+    for (int o = 32; o >= 1; o >>= 1) {
+      l = fminf(l, __shfl_xor(l, o, 64));
+      h = fmaxf(h, __shfl_xor(h, o, 64));
+    }
+    if (lane == 0) {
+      rl[wave] = l;
+      rh[wave] = h;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      olo[node * dim + c] = fminf(fminf(rl[0], rl[1]), fminf(rl[2], rl[3]));
+      ohi[node * dim + c] = fmaxf(fmaxf(rh[0], rh[1]), fmaxf(rh[2], rh[3]));
+    }
+    __syncthreads();
+  }
+}
+
+// sort key of position p at level `lev`: (node, coordinate along the node's widest
+// dimension quantised to 16 bits over the node's extent); the stable sort keeps the
+// previous order among equal keys
+__global__ void kd_key_kernel(const float* __restrict__ e, const uint32_t* __restrict__ perm,
+                              int64_t n, int dim, int shift, const float* __restrict__ olo,
+                              const float* __restrict__ ohi, uint64_t* __restrict__ key,
+                              uint32_t* __restrict__ val) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int64_t node = (p / TS) >> shift;
+  int best = 0;
+  float wb = ohi[node * dim] - olo[node * dim];
+  for (int c = 1; c < dim; ++c) {
+    const float w = ohi[node * dim + c] - olo[node * dim + c];
+    if (w > wb) {
+      wb = w;
+      best = c;
+    }
+  }
+  const uint32_t row = perm[p];
+  float x = e[(int64_t)row * dim + best];
+  if (!(x == x)) x = 0.0f;
+  const float lo = olo[node * dim + best];
+  const float r = wb > 0.0f ? (x - lo) / wb : 0.0f;
+  const uint32_t q = (uint32_t)fminf(fmaxf(r * 65535.0f, 0.0f), 65535.0f);
+  key[p] = ((uint64_t)node << 16) | q;
+  val[p] = row;
+}
+
+__global__ void iota_kernel(int64_t n, uint32_t* __restrict__ v) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) v[p] = (uint32_t)p;
 }
 
 __global__ void gather_kernel(const float* __restrict__ e, const uint32_t* __restrict__ perm,
@@ -129,11 +181,12 @@ __device__ __forceinline__ float dsq_seq(const float (&q)[MAXD], const float* __
   return s;
 }
 
-// Depth-first successor in the implicit tree (root 1, children 2v and 2v+1) after the
-// subtree of v is done or skipped: strip v's trailing right-child steps, then move to the
-// right sibling. Returns 1 when the walk is over.
-__device__ __forceinline__ int64_t tree_next(int64_t v) {
-  return (v >> __builtin_ctzll(~(unsigned long long)v)) + 1;
+constexpr int STACK = 64;  // > tree depth + 1 for any n < 2^31
+
+__device__ __forceinline__ float wave_min(float x) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) x = fminf(x, __shfl_xor(x, o, 64));
+  return x;
 }
 
 // leaves L..2L-1 hold the tile boxes (empty boxes past the last tile)
@@ -170,7 +223,7 @@ __global__ void bvh_comp_merge_kernel(int64_t lev, int32_t* __restrict__ ncomp) 
 }
 
 // ------------------------------------------------------------------ core distances
-template <int D>
+template <int D, int KC>
 __global__ __launch_bounds__(64) void core_kernel(const float* __restrict__ es, int64_t n,
                                                   int dim, int64_t ntiles, int64_t L,
                                                   const float* __restrict__ nlo,
@@ -186,9 +239,9 @@ __global__ __launch_bounds__(64) void core_kernel(const float* __restrict__ es, 
 #pragma unroll
   for (int c = 0; c < MAXD; ++c) q[c] = (valid && c < nd) ? es[p * nd + c] : 0.0f;
   // ascending list of the k smallest squared distances; kth = best[k-1]
-  float best[KCORE];
+  float best[KC];
 #pragma unroll
-  for (int j = 0; j < KCORE; ++j) best[j] = __builtin_huge_valf();
+  for (int j = 0; j < KC; ++j) best[j] = __builtin_huge_valf();
   float kth = __builtin_huge_valf();
   auto scan_tile = [&](int64_t t) {
     const int64_t pt = t * TS + lane;
@@ -202,36 +255,40 @@ __global__ __launch_bounds__(64) void core_kernel(const float* __restrict__ es, 
         float x = dsq_seq<D>(q, tp + j * nd, nd);
         if (x < kth) {
 #pragma unroll
-          for (int m = 0; m < KCORE; ++m)
+          for (int m = 0; m < KC; ++m)
             if (m < k) {
               const float lo = fminf(x, best[m]);
               x = fmaxf(x, best[m]);
               best[m] = lo;
             }
 #pragma unroll
-          for (int m = 0; m < KCORE; ++m)
+          for (int m = 0; m < KC; ++m)
             if (m == k - 1) kth = best[m];
         }
       }
     }
   };
   scan_tile(qt);  // the own tile first, for a starting bound
-  for (int64_t v = 1;;) {
-    const bool prune = !valid || box_dist2<D>(q, nlo + v * nd, nhi + v * nd, nd) > kth;
-    bool down = false;
-    if (!__all(prune)) {
-      if (v >= L) {
-        if (v - L != qt) scan_tile(v - L);
-      } else {
-        down = true;
-      }
+  // depth-first walk, nearer child first, over a wave-uniform stack in LDS
+  __shared__ int32_t stk[STACK];
+  int sp = 0;
+  stk[sp++] = 1;
+  while (sp > 0) {
+    const int64_t v = stk[--sp];
+    if (__all(!valid || box_dist2<D>(q, nlo + v * nd, nhi + v * nd, nd) > kth)) continue;
+    if (v >= L) {
+      if (v - L != qt) scan_tile(v - L);
+      continue;
     }
-    if (down) {
-      v = 2 * v;
-    } else {
-      v = tree_next(v);
-      if (v == 1) break;
-    }
+    const float d0 = box_dist2<D>(q, nlo + 2 * v * nd, nhi + 2 * v * nd, nd);
+    const float d1 = box_dist2<D>(q, nlo + (2 * v + 1) * nd, nhi + (2 * v + 1) * nd, nd);
+    const bool n0 = !__all(!valid || d0 > kth), n1 = !__all(!valid || d1 > kth);
+    const float m0 = wave_min(valid ? d0 : __builtin_huge_valf());
+    const float m1 = wave_min(valid ? d1 : __builtin_huge_valf());
+    const int32_t near = (int32_t)(m1 < m0 ? 2 * v + 1 : 2 * v);
+    const bool nn = m1 < m0 ? n1 : n0, nf = m1 < m0 ? n0 : n1;
+    if (nf) stk[sp++] = near ^ 1;
+    if (nn) stk[sp++] = near;
   }
   if (valid) core[p] = kth;
 }
@@ -270,50 +327,60 @@ __global__ __launch_bounds__(64) void boruvka_search_kernel(
   const int32_t mycomp = valid ? comp[p] : -1;
   const int32_t myid = valid ? (int32_t)perm[p] : 0;
   Best bst{__builtin_huge_valf(), INT32_MAX, INT32_MAX};
-  for (int64_t v = 1;;) {
+  __shared__ int32_t stk[STACK];
+  int sp = 0;
+  stk[sp++] = 1;
+  while (sp > 0) {
+    const int64_t v = stk[--sp];
     const int32_t vc = ncomp[v];
     float bound = bst.w;
     if (valid) bound = fminf(bound, __uint_as_float(comp_w[mycomp]));
     const bool prune = !valid || vc == -2 || (vc >= 0 && vc == mycomp) || cq > bound ||
                        box_dist2<D>(q, nlo + v * nd, nhi + v * nd, nd) > bound;
-    bool down = false;
-    if (!__all(prune)) {
-      if (v >= L) {
-        const int64_t t = v - L;
-        const int64_t pt = t * TS + lane;
-        __syncthreads();
-        for (int c = 0; c < nd; ++c) tp[lane * nd + c] = pt < n ? es[pt * nd + c] : 0.0f;
-        tc[lane] = pt < n ? core[pt] : __builtin_huge_valf();
-        tcomp[lane] = pt < n ? comp[pt] : mycomp;
-        tid_[lane] = pt < n ? (int32_t)perm[pt] : 0;
-        __syncthreads();
-        if (!prune) {
-          const int cnt = (int)min<int64_t>(TS, n - t * TS);
-          for (int j = 0; j < cnt; ++j) {
-            if (tcomp[j] == mycomp) continue;
-            const float cj = tc[j];
-            if (cj > bound) continue;
-            const float dd = dsq_seq<D>(q, tp + j * nd, nd);
-            const float mr = fmaxf(fmaxf(dd, cq), cj);
-            const int32_t oj = tid_[j];
-            const int32_t a = min(myid, oj), b = max(myid, oj);
-            if (better(mr, a, b, bst)) {
-              bst = Best{mr, a, b};
-              bound = fminf(bound, mr);
-            }
+    if (__all(prune)) continue;
+    if (v >= L) {
+      const int64_t t = v - L;
+      const int64_t pt = t * TS + lane;
+      __syncthreads();
+      for (int c = 0; c < nd; ++c) tp[lane * nd + c] = pt < n ? es[pt * nd + c] : 0.0f;
+      tc[lane] = pt < n ? core[pt] : __builtin_huge_valf();
+      tcomp[lane] = pt < n ? comp[pt] : mycomp;
+      tid_[lane] = pt < n ? (int32_t)perm[pt] : 0;
+      __syncthreads();
+      if (!prune) {
+        const int cnt = (int)min<int64_t>(TS, n - t * TS);
+        for (int j = 0; j < cnt; ++j) {
+          if (tcomp[j] == mycomp) continue;
+          const float cj = tc[j];
+          if (cj > bound) continue;
+          const float dd = dsq_seq<D>(q, tp + j * nd, nd);
+          const float mr = fmaxf(fmaxf(dd, cq), cj);
+          const int32_t oj = tid_[j];
+          const int32_t a = min(myid, oj), b = max(myid, oj);
+          if (better(mr, a, b, bst)) {
+            bst = Best{mr, a, b};
+            bound = fminf(bound, mr);
           }
-          if (bst.w < __builtin_huge_valf()) atomicMin(&comp_w[mycomp], __float_as_uint(bst.w));
         }
-      } else {
-        down = true;
+        if (bst.w < __builtin_huge_valf()) atomicMin(&comp_w[mycomp], __float_as_uint(bst.w));
       }
+      continue;
     }
-    if (down) {
-      v = 2 * v;
-    } else {
-      v = tree_next(v);
-      if (v == 1) break;
-    }
+    // children: skip those no lane needs, walk the nearer one first
+    const bool alive = valid && !(cq > bound);
+    const int32_t c0 = ncomp[2 * v], c1 = ncomp[2 * v + 1];
+    const bool o0 = alive && c0 != -2 && !(c0 >= 0 && c0 == mycomp);
+    const bool o1 = alive && c1 != -2 && !(c1 >= 0 && c1 == mycomp);
+    const float d0 = o0 ? box_dist2<D>(q, nlo + 2 * v * nd, nhi + 2 * v * nd, nd)
+                        : __builtin_huge_valf();
+    const float d1 = o1 ? box_dist2<D>(q, nlo + (2 * v + 1) * nd, nhi + (2 * v + 1) * nd, nd)
+                        : __builtin_huge_valf();
+    const bool n0 = !__all(!o0 || d0 > bound), n1 = !__all(!o1 || d1 > bound);
+    const float m0 = wave_min(d0), m1 = wave_min(d1);
+    const int32_t near = (int32_t)(m1 < m0 ? 2 * v + 1 : 2 * v);
+    const bool nn = m1 < m0 ? n1 : n0, nf = m1 < m0 ? n0 : n1;
+    if (nf) stk[sp++] = near ^ 1;
+    if (nn) stk[sp++] = near;
   }
   if (valid) {
     best_w[p] = bst.w;
@@ -402,28 +469,39 @@ struct Sorted {
   int64_t ntiles = 0, L = 1;
 };
 
-void morton_sort(const float* d_emb, int64_t n, int dim, Sorted& S, hipStream_t s,
-                 DevPool& pool) {
-  Buf<unsigned> lo(pool, dim), hi(pool, dim);
-  std::vector<unsigned> init_lo(dim, 0xffffffffu), init_hi(dim, 0u);
-  EVOC_HIP(hipMemcpyAsync(lo.p, init_lo.data(), dim * sizeof(unsigned), hipMemcpyHostToDevice, s));
-  EVOC_HIP(hipMemcpyAsync(hi.p, init_hi.data(), dim * sizeof(unsigned), hipMemcpyHostToDevice, s));
-  const unsigned g = (unsigned)ceil_div(n, 256);
-  bbox_kernel<<<g, 256, 0, s>>>(d_emb, n, dim, lo.p, hi.p);
-  Buf<uint64_t> code(pool, n);
+void kd_sort(const float* d_emb, int64_t n, int dim, Sorted& S, hipStream_t s, DevPool& pool) {
+  S.ntiles = ceil_div(n, TS);
+  S.L = 1;
+  int h = 0;
+  while (S.L < S.ntiles) {
+    S.L <<= 1;
+    ++h;
+  }
   S.perm.alloc(pool, n);
-  const int bits = std::min(21, 64 / dim);
-  morton_kernel<<<g, 256, 0, s>>>(d_emb, n, dim, lo.p, hi.p, bits, code.p, S.perm.p);
-  radix_sort_pairs(code.p, S.perm.p, n, bits * dim, s, pool);
+  const unsigned g = (unsigned)ceil_div(n, 256);
+  iota_kernel<<<g, 256, 0, s>>>(n, S.perm.p);
+  {
+    Buf<float> tlo(pool, S.ntiles * dim), thi(pool, S.ntiles * dim);
+    Buf<float> olo(pool, (S.L / 2) * dim + dim), ohi(pool, (S.L / 2) * dim + dim);
+    Buf<uint64_t> key(pool, n);
+    for (int lev = 0; lev < h; ++lev) {
+      perm_tile_box_kernel<<<(unsigned)ceil_div(S.ntiles, 4), 256, 0, s>>>(
+          d_emb, S.perm.p, n, dim, S.ntiles, tlo.p, thi.p);
+      const int64_t span = S.L >> lev;
+      const int64_t nodes = ceil_div(S.ntiles, span);
+      kd_node_box_kernel<<<(unsigned)nodes, 256, 0, s>>>(tlo.p, thi.p, S.ntiles, span, dim,
+                                                         olo.p, ohi.p);
+      kd_key_kernel<<<g, 256, 0, s>>>(d_emb, S.perm.p, n, dim, h - lev, olo.p, ohi.p, key.p,
+                                      S.perm.p);
+      radix_sort_pairs(key.p, S.perm.p, n, lev + 16, s, pool);
+    }
+  }
   S.es.alloc(pool, n * dim);
   gather_kernel<<<(unsigned)ceil_div(n * dim, 256), 256, 0, s>>>(d_emb, S.perm.p, n, dim, S.es.p);
-  S.ntiles = ceil_div(n, TS);
   S.blo.alloc(pool, S.ntiles * dim);
   S.bhi.alloc(pool, S.ntiles * dim);
   tile_box_kernel<<<(unsigned)ceil_div(S.ntiles, 4), 256, 0, s>>>(S.es.p, n, dim, S.ntiles,
                                                                   S.blo.p, S.bhi.p);
-  S.L = 1;
-  while (S.L < S.ntiles) S.L <<= 1;
   S.nlo.alloc(pool, 2 * S.L * dim);
   S.nhi.alloc(pool, 2 * S.L * dim);
   bvh_leaf_kernel<<<(unsigned)ceil_div(S.L, 256), 256, 0, s>>>(S.blo.p, S.bhi.p, S.ntiles, S.L,
@@ -442,12 +520,18 @@ void core_sorted(const Sorted& S, int64_t n, int dim, int min_samples, float* d_
     return;
   }
   const unsigned g = (unsigned)S.ntiles;
-  if (dim == 8)
-    core_kernel<8><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p, k,
-                                    d_core_sorted);
+  if (dim == 8 && k <= 8)
+    core_kernel<8, 8><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p, k,
+                                       d_core_sorted);
+  else if (dim == 8)
+    core_kernel<8, KCORE><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p, k,
+                                           d_core_sorted);
+  else if (k <= 8)
+    core_kernel<0, 8><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p, k,
+                                       d_core_sorted);
   else
-    core_kernel<0><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p, k,
-                                    d_core_sorted);
+    core_kernel<0, KCORE><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p, k,
+                                           d_core_sorted);
   EVOC_HIP(hipGetLastError());
 }
 
@@ -489,7 +573,7 @@ void core_sq_device(const float* d_emb, int64_t n, int dim, int min_samples, flo
     return;
   }
   Sorted S;
-  morton_sort(d_emb, n, dim, S, s, pool);
+  kd_sort(d_emb, n, dim, S, s, pool);
   Buf<float> cs(pool, n);
   core_sorted(S, n, dim, min_samples, cs.p, s);
   unpermute_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(cs.p, S.perm.p, n, d_core);
@@ -502,7 +586,7 @@ void mst_device(const float* d_emb, int64_t n, int dim, int min_samples, int32_t
   if (rounds) *rounds = 0;
   if (n <= 1) return;
   Sorted S;
-  morton_sort(d_emb, n, dim, S, s, pool);
+  kd_sort(d_emb, n, dim, S, s, pool);
   Buf<float> core(pool, n);
   core_sorted(S, n, dim, min_samples, core.p, s);
   std::vector<uint32_t> perm(n);

@@ -160,6 +160,21 @@ def test_mst_bit_exact_with_ties():
     assert sorted(zip(gu.tolist(), gv.tolist())) == sorted(zip(ou.tolist(), ov.tolist()))
 
 
+def test_mst_and_core_deep_tree_ragged():
+    # 20011 points: 313 tiles (the last one partial), nine levels of kd splits, and a
+    # block of exact duplicates so that equal coordinates straddle split boundaries
+    e, _ = _embedding(20011, seed=12)
+    e[5000:5400] = e[5000]
+    assert np.array_equal(H.core_sq(e, 5).view(np.uint32), O.core_sq(e, 5).view(np.uint32))
+    gu, gv, gw = H.mst(e, 5)
+    ou, ov, ow = O.mst(e, 5)
+    order = np.lexsort((ov, ou, ow))
+    assert np.array_equal(gu, ou[order]) and np.array_equal(gv, ov[order])
+    assert np.array_equal(gw.view(np.uint32), ow[order].view(np.uint32))
+    e3 = np.ascontiguousarray(e[:, :3])
+    assert np.array_equal(H.core_sq(e3, 12).view(np.uint32), O.core_sq(e3, 12).view(np.uint32))
+
+
 def test_cluster_layers_bit_exact():
     e, gt = _embedding(3000, seed=8)
     gl, gs, gp = H.cluster_layers(e, 5, 5, 0.2, 10)
