@@ -541,26 +541,109 @@ __global__ void unpermute_kernel(const float* __restrict__ src, const uint32_t* 
   if (p < n) dst[perm[p]] = src[p];
 }
 
-struct HostDsu {
-  std::vector<int32_t> p, r;
-  explicit HostDsu(int64_t n) : p(n), r(n, 0) { std::iota(p.begin(), p.end(), 0); }
-  int32_t find(int32_t x) {
-    while (p[x] != x) {
-      p[x] = p[p[x]];
-      x = p[x];
-    }
-    return x;
+// ---------------------------------------------------------------- Boruvka merge step
+// Each component's chosen edge hooks it onto the component at the other end. Under a
+// strict total order on edges the hooks form a forest whose only cycles are mutual
+// pairs (two components choosing the same edge); a pair is rooted at its smaller id,
+// and its edge is emitted once, by that root.
+__global__ void inv_perm_kernel(const uint32_t* __restrict__ perm, int64_t n,
+                                int32_t* __restrict__ inv) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) inv[perm[p]] = (int32_t)p;
+}
+
+__global__ void hook_kernel(int64_t ncomp, const unsigned long long* __restrict__ comp_key,
+                            const int32_t* __restrict__ comp, const int32_t* __restrict__ inv,
+                            int32_t* __restrict__ partner) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncomp) return;
+  const unsigned long long key = comp_key[c];
+  if (key == ~0ull) {
+    partner[c] = (int32_t)c;
+    return;
   }
-  bool unite(int32_t a, int32_t b) {
-    a = find(a);
-    b = find(b);
-    if (a == b) return false;
-    if (r[a] < r[b]) std::swap(a, b);
-    p[b] = a;
-    if (r[a] == r[b]) ++r[a];
-    return true;
+  const int32_t ca = comp[inv[(int32_t)(key >> 32)]];
+  const int32_t cb = comp[inv[(int32_t)(key & 0xffffffffu)]];
+  partner[c] = ca == (int32_t)c ? cb : ca;
+}
+
+__global__ void link_kernel(int64_t ncomp, const int32_t* __restrict__ partner,
+                            const unsigned long long* __restrict__ comp_key,
+                            const unsigned* __restrict__ comp_w, int64_t cap,
+                            int32_t* __restrict__ parent, unsigned* __restrict__ nedge,
+                            float* __restrict__ ew2, int32_t* __restrict__ ea,
+                            int32_t* __restrict__ eb) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncomp) return;
+  const int32_t o = partner[c];
+  if (o == (int32_t)c) {
+    parent[c] = o;
+    return;
   }
-};
+  const bool mutual = partner[o] == (int32_t)c;
+  parent[c] = (mutual && (int32_t)c < o) ? (int32_t)c : o;
+  if (mutual && (int32_t)c > o) return;
+  const unsigned idx = atomicAdd(nedge, 1u);
+  if (idx < cap) {
+    const unsigned long long key = comp_key[c];
+    ew2[idx] = __uint_as_float(comp_w[c]);
+    ea[idx] = (int32_t)(key >> 32);
+    eb[idx] = (int32_t)(key & 0xffffffffu);
+  }
+}
+
+// pointer jumping toward the roots; `busy` is set while some pointer still moves
+__global__ void jump_kernel(int64_t ncomp, int32_t* __restrict__ parent,
+                            unsigned* __restrict__ busy) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncomp) return;
+  int32_t pc = parent[c];
+  for (int it = 0; it < 32; ++it) {
+    const int32_t pp = parent[pc];
+    if (pp == pc) return;
+    pc = pp;
+    parent[c] = pc;
+  }
+  if (parent[pc] != pc) *busy = 1u;
+}
+
+__global__ void root_flag_kernel(int64_t ncomp, const int32_t* __restrict__ parent,
+                                 int64_t* __restrict__ flag) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c <= ncomp) flag[c] = (c < ncomp && parent[c] == (int32_t)c) ? 1 : 0;
+}
+
+__global__ void relabel_kernel(int64_t n, const int32_t* __restrict__ parent,
+                               const int64_t* __restrict__ rank, int32_t* __restrict__ comp) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) comp[p] = (int32_t)rank[parent[comp[p]]];
+}
+
+// final order of the edges: (sqrt(w^2), u, v) ascending, by two stable radix sorts
+__global__ void edge_ab_key_kernel(int64_t m, const int32_t* __restrict__ ea,
+                                   const int32_t* __restrict__ eb, uint64_t* __restrict__ key,
+                                   uint32_t* __restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  key[i] = ((uint64_t)(uint32_t)ea[i] << 32) | (uint32_t)eb[i];
+  idx[i] = (uint32_t)i;
+}
+__global__ void edge_w_key_kernel(int64_t m, const float* __restrict__ ew2,
+                                  const uint32_t* __restrict__ idx, uint64_t* __restrict__ key) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) key[i] = __float_as_uint(sqrtf(ew2[idx[i]]));
+}
+__global__ void edge_gather_kernel(int64_t m, const float* __restrict__ ew2,
+                                   const int32_t* __restrict__ ea, const int32_t* __restrict__ eb,
+                                   const uint32_t* __restrict__ idx, int32_t* __restrict__ u,
+                                   int32_t* __restrict__ v, float* __restrict__ w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t j = idx[i];
+  u[i] = ea[j];
+  v[i] = eb[j];
+  w[i] = sqrtf(ew2[j]);
+}
 
 }  // namespace
 
@@ -589,25 +672,19 @@ void mst_device(const float* d_emb, int64_t n, int dim, int min_samples, int32_t
   kd_sort(d_emb, n, dim, S, s, pool);
   Buf<float> core(pool, n);
   core_sorted(S, n, dim, min_samples, core.p, s);
-  std::vector<uint32_t> perm(n);
-  EVOC_HIP(hipMemcpyAsync(perm.data(), S.perm.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   Buf<int32_t> comp(pool, n), tile_comp(pool, S.ntiles), ba(pool, n), bb(pool, n);
-  Buf<int32_t> ncomp_node(pool, 2 * S.L);
-  Buf<float> bw(pool, n);
-  Buf<unsigned> comp_w(pool, n);
+  Buf<int32_t> ncomp_node(pool, 2 * S.L), inv(pool, n), partner(pool, n), parent(pool, n);
+  Buf<float> bw(pool, n), ew2(pool, n);
+  Buf<int32_t> ea(pool, n), eb(pool, n);
+  Buf<unsigned> comp_w(pool, n), ctr(pool, 2);  // ctr: edge count, jump flag
   Buf<unsigned long long> comp_key(pool, n);
-  std::vector<int32_t> hcomp(n);
-  for (int64_t p = 0; p < n; ++p) hcomp[p] = (int32_t)p;  // every point its own component
-  EVOC_HIP(hipMemcpyAsync(comp.p, hcomp.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  EVOC_HIP(hipStreamSynchronize(s));
-  HostDsu dsu(n);
-  struct E { float w2; int32_t a, b; };
-  std::vector<E> edges;
-  edges.reserve(n - 1);
+  Buf<int64_t> flag(pool, n + 1), rank(pool, n + 1);
+  const unsigned gn = (unsigned)ceil_div(n, 256);
+  iota_kernel<<<gn, 256, 0, s>>>(n, reinterpret_cast<uint32_t*>(comp.p));  // own components
+  inv_perm_kernel<<<gn, 256, 0, s>>>(S.perm.p, n, inv.p);
+  EVOC_HIP(hipMemsetAsync(ctr.p, 0, 2 * sizeof(unsigned), s));
   int64_t ncomp = n;
-  std::vector<unsigned> hw;
-  std::vector<unsigned long long> hk;
-  std::vector<int32_t> root_to_comp(n, -1);
+  unsigned nedge = 0;
   int r = 0;
   while (ncomp > 1) {
     ++r;
@@ -635,53 +712,49 @@ void mst_device(const float* d_emb, int64_t n, int dim, int min_samples, int32_t
     boruvka_select_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(n, comp.p, comp_w.p, bw.p,
                                                                      ba.p, bb.p, comp_key.p);
     EVOC_HIP(hipGetLastError());
-    hw.resize(ncomp);
-    hk.resize(ncomp);
-    EVOC_HIP(hipMemcpyAsync(hw.data(), comp_w.p, ncomp * sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    EVOC_HIP(hipMemcpyAsync(hk.data(), comp_key.p, ncomp * sizeof(unsigned long long),
-                            hipMemcpyDeviceToHost, s));
-    EVOC_HIP(hipStreamSynchronize(s));
-    bool merged = false;
-    for (int64_t c = 0; c < ncomp; ++c) {
-      if (hk[c] == ~0ull) continue;
-      const int32_t a = (int32_t)(hk[c] >> 32), b = (int32_t)(hk[c] & 0xffffffffu);
-      if (dsu.unite(a, b)) {
-        float w2;
-        std::memcpy(&w2, &hw[c], sizeof(float));
-        edges.push_back({w2, a, b});
-        merged = true;
-      }
+    const unsigned gc = (unsigned)ceil_div(ncomp, 256);
+    hook_kernel<<<gc, 256, 0, s>>>(ncomp, comp_key.p, comp.p, inv.p, partner.p);
+    link_kernel<<<gc, 256, 0, s>>>(ncomp, partner.p, comp_key.p, comp_w.p, n - 1, parent.p,
+                                   ctr.p, ew2.p, ea.p, eb.p);
+    unsigned h[2] = {0, 1};
+    for (int it = 0; h[1]; ++it) {
+      // 32 jumps per launch per pointer: a forest of any depth below 2^31 settles in a
+      // few launches; a cycle (which a strict edge order rules out) would never settle
+      EVOC_CHECK(it < 64, "mst: Boruvka hooks do not form a forest");
+      EVOC_HIP(hipMemsetAsync(ctr.p + 1, 0, sizeof(unsigned), s));
+      jump_kernel<<<gc, 256, 0, s>>>(ncomp, parent.p, ctr.p + 1);
+      EVOC_HIP(hipMemcpyAsync(h, ctr.p, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+      EVOC_HIP(hipStreamSynchronize(s));
     }
-    if (!merged) break;
-    // relabel components compactly in sorted-position order
+    EVOC_CHECK(h[0] <= (unsigned)(n - 1), "mst: more Boruvka edges than n - 1");
+    if (h[0] == nedge) break;  // no component has an outgoing edge left
+    nedge = h[0];
+    root_flag_kernel<<<(unsigned)ceil_div(ncomp + 1, 256), 256, 0, s>>>(ncomp, parent.p, flag.p);
+    exclusive_scan_i64(flag.p, rank.p, ncomp + 1, s, pool);
+    relabel_kernel<<<gn, 256, 0, s>>>(n, parent.p, rank.p, comp.p);
     int64_t nc = 0;
-    std::fill(root_to_comp.begin(), root_to_comp.end(), -1);
-    for (int64_t p = 0; p < n; ++p) {
-      const int32_t rt = dsu.find((int32_t)perm[p]);
-      if (root_to_comp[rt] < 0) root_to_comp[rt] = (int32_t)nc++;
-      hcomp[p] = root_to_comp[rt];
-    }
+    EVOC_HIP(hipMemcpyAsync(&nc, rank.p + ncomp, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipStreamSynchronize(s));
     ncomp = nc;
-    if (ncomp <= 1) break;
-    EVOC_HIP(hipMemcpyAsync(comp.p, hcomp.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
   }
   if (rounds) *rounds = r;
-  // weights are sqrt'd; order by (w, u, v) for the linkage
-  std::vector<int64_t> ord(edges.size());
-  std::vector<float> ws(edges.size());
-  for (size_t i = 0; i < edges.size(); ++i) ws[i] = std::sqrt(edges[i].w2);
-  std::iota(ord.begin(), ord.end(), 0);
-  std::sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) {
-    if (ws[x] != ws[y]) return ws[x] < ws[y];
-    if (edges[x].a != edges[y].a) return edges[x].a < edges[y].a;
-    return edges[x].b < edges[y].b;
-  });
-  for (size_t i = 0; i < ord.size(); ++i) {
-    h_u[i] = edges[ord[i]].a;
-    h_v[i] = edges[ord[i]].b;
-    h_w[i] = ws[ord[i]];
+  const int64_t m = nedge;
+  if (m > 0) {
+    Buf<uint64_t> key(pool, m);
+    Buf<uint32_t> idx(pool, m);
+    const unsigned gm = (unsigned)ceil_div(m, 256);
+    edge_ab_key_kernel<<<gm, 256, 0, s>>>(m, ea.p, eb.p, key.p, idx.p);
+    radix_sort_pairs(key.p, idx.p, m, 64, s, pool);
+    edge_w_key_kernel<<<gm, 256, 0, s>>>(m, ew2.p, idx.p, key.p);
+    radix_sort_pairs(key.p, idx.p, m, 32, s, pool);
+    edge_gather_kernel<<<gm, 256, 0, s>>>(m, ew2.p, ea.p, eb.p, idx.p, ba.p, bb.p, bw.p);
+    EVOC_HIP(hipGetLastError());
+    EVOC_HIP(hipMemcpyAsync(h_u, ba.p, m * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipMemcpyAsync(h_v, bb.p, m * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipMemcpyAsync(h_w, bw.p, m * sizeof(float), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipStreamSynchronize(s));
   }
-  for (int64_t i = (int64_t)ord.size(); i < n - 1; ++i) {
+  for (int64_t i = m; i < n - 1; ++i) {
     h_u[i] = -1;
     h_v[i] = -1;
     h_w[i] = __builtin_huge_valf();
