@@ -12,13 +12,12 @@ namespace evoc {
 void label_prop_init_host(const int64_t* indptr, const int32_t* indices, const float* weights,
                           int64_t n, int dim, const float* data_dev, int64_t d, int64_t ld,
                           uint64_t seed, float* out, hipStream_t s, DevPool& pool, int verbose);
-// same, with the graph also in device memory and the layout written to d_out (n x dim):
-// the input level's label votes and expansion run on the GPU
-void label_prop_init_device(const int64_t* indptr, const int32_t* indices, const float* weights,
-                            const int64_t* d_indptr, const int32_t* d_indices,
-                            const float* d_weights, int64_t n, int dim, const float* data_dev,
-                            int64_t d, int64_t ld, uint64_t seed, float* d_out, hipStream_t s,
-                            DevPool& pool, int verbose);
+// same, with the graph in device memory (rows column-sorted) and the layout written to
+// d_out (n x dim): the input level runs on the GPU, the coarse levels on the host
+void label_prop_init_device(const int64_t* d_indptr, const int32_t* d_indices,
+                            const float* d_weights, int64_t nnz, int64_t n, int dim,
+                            const float* data_dev, int64_t d, int64_t ld, uint64_t seed,
+                            float* d_out, hipStream_t s, DevPool& pool, int verbose);
 
 // scipy-style single-linkage rows (src/clustering/linkage.rs:301-310)
 struct Linkage {

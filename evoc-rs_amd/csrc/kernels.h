@@ -138,12 +138,30 @@ void partition_means_device(const float* dX, int64_t ld, int64_t d, const int64_
                             const int32_t* h_mem, const float* h_coef, int64_t np, int64_t n,
                             float* h_out, hipStream_t s, DevPool& pool);
 
-// label votes (n_iter Jacobi steps) on a device CSR; h_lab in/out
-void lp_steps_device(const int64_t* d_indptr, const int32_t* d_col, const float* d_val, int64_t n,
-                     int n_iter, int64_t* h_lab, hipStream_t s, DevPool& pool);
+// partition means with the member lists already in device memory (out: np x d)
+void partition_means_dev(const float* dX, int64_t ld, int64_t d, const int64_t* d_mptr,
+                         const int32_t* d_mem, const float* d_coef, int64_t np, float* d_out,
+                         hipStream_t s);
+// the input level's partition (label propagation, outliers, remap) in device memory
+struct LpTop {
+  Buf<int32_t> part;  // partition of every node
+  Buf<int32_t> mem;   // nodes by partition, ascending within one
+  Buf<int64_t> mptr;  // np + 1 offsets into mem
+  Buf<float> sa;      // 1/sqrt(|a|): column norms of the partition map
+  Buf<float> coef;    // L1 row normalisation of R^T_norm (partition means)
+  int64_t np = 0;
+};
+void lp_top_partition(const int64_t* d_ptr, const int32_t* d_col, const float* d_val, int64_t n,
+                      int64_t approx, int n_iter, uint64_t seed, LpTop& P, hipStream_t s,
+                      DevPool& pool);
+// R^T_norm G R of the input level, returned as a host CSR (np x np)
+void lp_top_coarsen(const int64_t* d_ptr, const int32_t* d_col, const float* d_val, int64_t n,
+                    int64_t nnz, const LpTop& P, std::vector<int64_t>& cptr,
+                    std::vector<int32_t>& ccol, std::vector<float>& cval, hipStream_t s,
+                    DevPool& pool);
 // expansion of the input level, centred and scaled, into d_out (n x dim)
 void expansion_device(const int64_t* d_indptr, const int32_t* d_col, const float* d_val,
-                      int64_t n, int64_t nnz, int dim, const int32_t* h_part, const float* h_sa,
+                      int64_t n, int64_t nnz, int dim, const int32_t* d_part, const float* d_sa,
                       int64_t np, const float* h_layout, float* d_out, hipStream_t s,
                       DevPool& pool);
 

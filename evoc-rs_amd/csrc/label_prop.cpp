@@ -1,5 +1,7 @@
 // Label-propagation initial layout (label_propagation_init, src/graph/label_prop.rs:537-564)
-// for the plain form of evoc_hip. Host C++ with OpenMP; every coarse-level node
+// for the plain form of evoc_hip. The input level (label votes, outliers, partition,
+// coarse graph, partition means, expansion) runs on the GPU; the coarse levels, a few
+// thousand nodes and fewer, run in host C++ with OpenMP, and every coarse-level node
 // embedding runs on the GPU through embed_device. Accumulation orders follow the
 // reference's sparse algebra (src/utils/sparse.rs) so the layout is reproducible:
 //   * coarsening  R^T_norm G R   (per-partition dense accumulator, columns ascending),
@@ -95,24 +97,15 @@ void lp_outliers(const Sp& g, std::vector<int64_t>& lab, uint64_t seed) {
   }
 }
 
-// the same CSR in device memory (the input level only)
-struct DevG {
-  const int64_t* ptr = nullptr;
-  const int32_t* col = nullptr;
-  const float* val = nullptr;
-};
-
 // label_prop_loop (label_prop.rs:202-227): seeds, 20 steps, outliers, remap
-std::vector<int32_t> lp_partition(const Sp& g, const DevG* dg, int64_t approx, int n_iter,
-                                  uint64_t seed, int64_t* n_parts, hipStream_t s, DevPool& pool) {
+std::vector<int32_t> lp_partition(const Sp& g, int64_t approx, int n_iter, uint64_t seed,
+                                  int64_t* n_parts) {
   std::vector<int64_t> lab(g.rows, -1);
   for (int64_t t = 0; t < approx; ++t) {
     const uint32_t r = Philox::first(seed, (uint32_t)t, (uint32_t)(t >> 32), 0, TAG_LPSD);
     lab[Philox::range(r, (uint64_t)g.rows)] = t;
   }
-  if (dg) lp_steps_device(dg->ptr, dg->col, dg->val, g.rows, n_iter, lab.data(), s, pool);
-  else
-    for (int it = 0; it < n_iter; ++it) lab = lp_step(g, lab);
+  for (int it = 0; it < n_iter; ++it) lab = lp_step(g, lab);
   lp_outliers(g, lab, seed + 2000);
   // remap_labels (label_prop.rs:155-177): labelled ids in sorted order, -1s appended
   std::vector<int64_t> ids;
@@ -362,11 +355,10 @@ struct Rows {
   bool present() const { return host || dev; }
 };
 
-// With a device graph (the input level) the expansion is done on the GPU and written to
-// d_out, and the returned vector is empty.
+// One level of label_propagation_init_inner on a host graph (the coarse levels).
 std::vector<float> level(const Sp& g, int dim, const Rows& data, int64_t d, int epochs,
                          int64_t approx, uint64_t seed, hipStream_t s, DevPool& pool,
-                         int verbose, const DevG* dg = nullptr, float* d_out = nullptr) {
+                         int verbose) {
   const int64_t n = g.rows;
   if (n < 64) {
     if (data.present() && dim <= d) {
@@ -392,7 +384,7 @@ std::vector<float> level(const Sp& g, int dim, const Rows& data, int64_t d, int 
     }
   };
   int64_t np = 0;
-  const std::vector<int32_t> part = lp_partition(g, dg, approx, 20, seed, &np, s, pool);
+  const std::vector<int32_t> part = lp_partition(g, approx, 20, seed, &np);
   lap("label propagation");
   if (verbose) std::printf("  Label prop: %lld nodes -> %lld partitions\n", (long long)n, (long long)np);
   std::vector<int64_t> mptr;
@@ -443,12 +435,6 @@ std::vector<float> level(const Sp& g, int dim, const Rows& data, int64_t d, int 
   lap("coarse embedding");
   // expansion: rows of (G o G^T) R_norm, L1-normalised, times the coarse layout,
   // averaged with the layout of each node's own partition
-  if (dg && d_out) {
-    expansion_device(dg->ptr, dg->col, dg->val, n, g.ptr[n], dim, part.data(), sa.data(), np,
-                     layout.data(), d_out, s, pool);
-    lap("expansion");
-    return {};
-  }
   std::vector<float> res(n * dim);
 #pragma omp parallel
   {
@@ -509,18 +495,67 @@ std::vector<float> level(const Sp& g, int dim, const Rows& data, int64_t d, int 
   return res;
 }
 
+// The input level with its graph in device memory: partition, coarse graph, partition
+// means and expansion on the GPU; the coarse levels recurse through level() on the host.
+void top_level(const int64_t* d_ptr, const int32_t* d_col, const float* d_val, int64_t n,
+               int64_t nnz, int dim, const Rows& data, int64_t d, int epochs, int64_t approx,
+               uint64_t seed, float* d_out, hipStream_t s, DevPool& pool, int verbose) {
+  if (n < 64) {  // the base case needs no graph
+    Sp g;
+    g.rows = g.cols = n;
+    std::vector<float> r = level(g, dim, data, d, epochs, approx, seed, s, pool, verbose);
+    EVOC_HIP(hipMemcpyAsync(d_out, r.data(), n * dim * sizeof(float), hipMemcpyHostToDevice, s));
+    EVOC_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  using tclk = std::chrono::steady_clock;
+  auto tick = tclk::now();
+  auto lap = [&](const char* what) {
+    if (verbose >= 2) {
+      const auto now = tclk::now();
+      std::printf("    [n=%lld] %s: %.1f ms\n", (long long)n, what,
+                  std::chrono::duration<double, std::milli>(now - tick).count());
+      tick = now;
+    }
+  };
+  LpTop P;
+  lp_top_partition(d_ptr, d_col, d_val, n, approx, 20, seed, P, s, pool);
+  EVOC_HIP(hipStreamSynchronize(s));
+  const int64_t np = P.np;
+  lap("label propagation");
+  if (verbose) std::printf("  Label prop: %lld nodes -> %lld partitions\n", (long long)n, (long long)np);
+  Sp coarse;
+  coarse.rows = coarse.cols = np;
+  lp_top_coarsen(d_ptr, d_col, d_val, n, nnz, P, coarse.ptr, coarse.col, coarse.val, s, pool);
+  lap("coarsen graph");
+  std::vector<float> cdata;
+  if (data.present()) {
+    cdata.resize(np * d);
+    Buf<float> dm(pool, np * d);
+    partition_means_dev(data.dev, data.ld, d, P.mptr.p, P.mem.p, P.coef.p, np, dm.p, s);
+    EVOC_HIP(hipMemcpyAsync(cdata.data(), dm.p, np * d * sizeof(float), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipStreamSynchronize(s));
+  }
+  lap("coarsen data");
+  const int ep = std::min(epochs, 255);
+  Rows cr;
+  if (data.present()) cr.host = &cdata;
+  std::vector<float> layout = level(coarse, dim, cr, d, ep, approx / 4, seed + 3000, s, pool,
+                                    verbose);
+  tick = tclk::now();
+  coarse_embedding(coarse, dim, ep, (float)(0.001 * (double)epochs), layout, seed + 4000, s, pool);
+  lap("coarse embedding");
+  expansion_device(d_ptr, d_col, d_val, n, nnz, dim, P.part.p, P.sa.p, np, layout.data(), d_out,
+                   s, pool);
+  lap("expansion");
+}
+
 }  // namespace
 
-void label_prop_init_device(const int64_t* indptr, const int32_t* indices, const float* weights,
-                            const int64_t* d_indptr, const int32_t* d_indices,
-                            const float* d_weights, int64_t n, int dim, const float* data_dev,
-                            int64_t d, int64_t ld, uint64_t seed, float* d_out, hipStream_t s,
-                            DevPool& pool, int verbose) {
-  Sp g;
-  g.rows = g.cols = n;
-  g.ptr.assign(indptr, indptr + n + 1);
-  g.col.assign(indices, indices + indptr[n]);
-  g.val.assign(weights, weights + indptr[n]);
+void label_prop_init_device(const int64_t* d_indptr, const int32_t* d_indices,
+                            const float* d_weights, int64_t nnz, int64_t n, int dim,
+                            const float* data_dev, int64_t d, int64_t ld, uint64_t seed,
+                            float* d_out, hipStream_t s, DevPool& pool, int verbose) {
   int64_t approx = (int64_t)(8.0 * std::sqrt((double)n));
   approx = std::min<int64_t>(std::max<int64_t>(approx, 256), 16384);
   approx = std::min<int64_t>(approx, n);
@@ -529,12 +564,8 @@ void label_prop_init_device(const int64_t* indptr, const int32_t* indices, const
     top.dev = data_dev;
     top.ld = ld;
   }
-  DevG dg{d_indptr, d_indices, d_weights};
-  std::vector<float> r = level(g, dim, top, d, 50, approx, seed, s, pool, verbose, &dg, d_out);
-  if (!r.empty()) {  // the base case (n < 64) returns host rows
-    EVOC_HIP(hipMemcpyAsync(d_out, r.data(), n * dim * sizeof(float), hipMemcpyHostToDevice, s));
-    EVOC_HIP(hipStreamSynchronize(s));
-  }
+  top_level(d_indptr, d_indices, d_weights, n, nnz, dim, top, d, 50, approx, seed, d_out, s, pool,
+            verbose);
 }
 
 void label_prop_init_host(const int64_t* indptr, const int32_t* indices, const float* weights,
@@ -549,8 +580,8 @@ void label_prop_init_host(const int64_t* indptr, const int32_t* indices, const f
     EVOC_HIP(hipMemcpyAsync(dc.p, indices, nnz * sizeof(int32_t), hipMemcpyHostToDevice, s));
     EVOC_HIP(hipMemcpyAsync(dw.p, weights, nnz * sizeof(float), hipMemcpyHostToDevice, s));
   }
-  label_prop_init_device(indptr, indices, weights, dp.p, dc.p, dw.p, n, dim, data_dev, d, ld, seed,
-                         dout.p, s, pool, verbose);
+  label_prop_init_device(dp.p, dc.p, dw.p, nnz, n, dim, data_dev, d, ld, seed, dout.p, s, pool,
+                         verbose);
   EVOC_HIP(hipMemcpyAsync(out, dout.p, n * dim * sizeof(float), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
 }

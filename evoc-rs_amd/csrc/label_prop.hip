@@ -45,6 +45,14 @@ __global__ __launch_bounds__(PM_THREADS) void partition_means_kernel(
 
 }  // namespace
 
+void partition_means_dev(const float* dX, int64_t ld, int64_t d, const int64_t* d_mptr,
+                         const int32_t* d_mem, const float* d_coef, int64_t np, float* d_out,
+                         hipStream_t s) {
+  partition_means_kernel<<<(unsigned)np, PM_THREADS, 0, s>>>(dX, ld, d, d_mptr, d_mem, d_coef,
+                                                              d_out);
+  EVOC_HIP(hipGetLastError());
+}
+
 void partition_means_device(const float* dX, int64_t ld, int64_t d, const int64_t* h_mptr,
                             const int32_t* h_mem, const float* h_coef, int64_t np, int64_t n,
                             float* h_out, hipStream_t s, DevPool& pool) {
@@ -56,9 +64,7 @@ void partition_means_device(const float* dX, int64_t ld, int64_t d, const int64_
   EVOC_HIP(hipMemcpyAsync(mptr.p, h_mptr, (np + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
   EVOC_HIP(hipMemcpyAsync(mem.p, h_mem, n * sizeof(int32_t), hipMemcpyHostToDevice, s));
   EVOC_HIP(hipMemcpyAsync(coef.p, h_coef, np * sizeof(float), hipMemcpyHostToDevice, s));
-  partition_means_kernel<<<(unsigned)np, PM_THREADS, 0, s>>>(dX, ld, d, mptr.p, mem.p, coef.p,
-                                                              out.p);
-  EVOC_HIP(hipGetLastError());
+  partition_means_dev(dX, ld, d, mptr.p, mem.p, coef.p, np, out.p, s);
   EVOC_HIP(hipMemcpyAsync(h_out, out.p, np * d * sizeof(float), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
 }
@@ -252,22 +258,54 @@ __global__ __launch_bounds__(256) void exp_node_kernel(
   }
 }
 
-// centring: per coordinate the mean of res summed in node order, then x0.1
-__global__ void exp_colsum_kernel(const float* __restrict__ res, int64_t n, int dim,
-                                  float inv_n, float* __restrict__ mean) {
-  const int c = threadIdx.x;
-  if (c >= dim) return;
+// Column means of the n x dim result, each column summed sequentially in row order
+// (the reference's iterator sum). The workgroup streams 64 KB chunks of rows into LDS,
+// transposed, double-buffered; one lane per column then runs the dependent adds out of
+// LDS while the next chunk's loads are in flight.
+constexpr int CS_THREADS = 256;
+constexpr int CS_FLOATS = 16384;  // floats per chunk buffer (64 KB)
+
+__global__ __launch_bounds__(CS_THREADS) void exp_colsum_kernel(const float* __restrict__ res,
+                                                                int64_t n, int dim, float inv_n,
+                                                                float* __restrict__ mean) {
+  extern __shared__ float cs_buf[];  // [2][dim][rows + 1]
+  const int rows = CS_FLOATS / dim;  // rows per chunk
+  const int pitch = rows + 1;
+  const int t = threadIdx.x;
+  const int64_t nchunks = (n + rows - 1) / rows;
+  constexpr int PER = CS_FLOATS / CS_THREADS;  // floats per thread per chunk
+  float stage[PER];
+  auto load = [&](int64_t k) {
+    const int64_t base = k * rows * dim;
+    const int64_t lim = n * dim - base;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int q = u * CS_THREADS + t;
+      stage[u] = q < lim ? res[base + q] : 0.0f;
+    }
+  };
+  auto store = [&](int buf) {
+    float* B = cs_buf + buf * dim * pitch;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int q = u * CS_THREADS + t;
+      if (q < rows * dim) B[(q % dim) * pitch + q / dim] = stage[u];
+    }
+  };
   float sum = 0.0f;
-  int64_t i = 0;
-  for (; i + 8 <= n; i += 8) {  // loads ahead of the dependent adds
-    float x[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) x[u] = res[(i + u) * dim + c];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) sum = sum + x[u];
+  load(0);
+  store(0);
+  for (int64_t k = 0; k < nchunks; ++k) {
+    __syncthreads();
+    if (k + 1 < nchunks) load(k + 1);
+    if (t < dim) {
+      const float* B = cs_buf + (k & 1) * dim * pitch + t * pitch;
+      const int cnt = (int)min<int64_t>(rows, n - k * rows);
+      for (int r = 0; r < cnt; ++r) sum = sum + B[r];
+    }
+    if (k + 1 < nchunks) store((k + 1) & 1);
   }
-  for (; i < n; ++i) sum = sum + res[i * dim + c];
-  mean[c] = sum * inv_n;
+  if (t < dim) mean[t] = sum * inv_n;
 }
 __global__ void exp_center_kernel(float* __restrict__ res, int64_t n, int dim,
                                   const float* __restrict__ mean, float* __restrict__ out) {
@@ -279,40 +317,372 @@ __global__ void exp_center_kernel(float* __restrict__ res, int64_t n, int dim,
 
 }  // namespace
 
-void lp_steps_device(const int64_t* d_indptr, const int32_t* d_col, const float* d_val, int64_t n,
-                     int n_iter, int64_t* h_lab, hipStream_t s, DevPool& pool) {
-  Buf<int64_t> a(pool, n), b(pool, n);
-  EVOC_HIP(hipMemcpyAsync(a.p, h_lab, n * sizeof(int64_t), hipMemcpyHostToDevice, s));
-  const unsigned g = (unsigned)ceil_div(n, LP_WAVES);
-  for (int it = 0; it < n_iter; ++it) {
-    lp_step_kernel<<<g, 64 * LP_WAVES, 0, s>>>(d_indptr, d_col, d_val, n, a.p, b.p);
-    std::swap(a.p, b.p);
-  }
-  EVOC_HIP(hipGetLastError());
-  EVOC_HIP(hipMemcpyAsync(h_lab, a.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  EVOC_HIP(hipStreamSynchronize(s));
-}
-
 void expansion_device(const int64_t* d_indptr, const int32_t* d_col, const float* d_val,
-                      int64_t n, int64_t nnz, int dim, const int32_t* h_part, const float* h_sa,
+                      int64_t n, int64_t nnz, int dim, const int32_t* d_part, const float* d_sa,
                       int64_t np, const float* h_layout, float* d_out, hipStream_t s,
                       DevPool& pool) {
   const int64_t m = std::max<int64_t>(nnz, 1);
-  Buf<int32_t> part(pool, n), ta(pool, m), tf(pool, m), ra(pool, m);
-  Buf<float> sa(pool, np), layout(pool, np * dim), tv(pool, m), tacc(pool, m), rv(pool, m),
-      res(pool, n * dim), mean(pool, dim);
-  EVOC_HIP(hipMemcpyAsync(part.p, h_part, n * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  EVOC_HIP(hipMemcpyAsync(sa.p, h_sa, np * sizeof(float), hipMemcpyHostToDevice, s));
+  Buf<int32_t> ta(pool, m), tf(pool, m), ra(pool, m);
+  Buf<float> layout(pool, np * dim), tv(pool, m), tacc(pool, m), rv(pool, m), res(pool, n * dim),
+      mean(pool, dim);
   EVOC_HIP(hipMemcpyAsync(layout.p, h_layout, np * dim * sizeof(float), hipMemcpyHostToDevice, s));
   const unsigned g = (unsigned)ceil_div(n, LP_WAVES);
-  exp_terms_kernel<<<g, 64 * LP_WAVES, 0, s>>>(d_indptr, d_col, d_val, n, part.p, sa.p, ta.p,
+  exp_terms_kernel<<<g, 64 * LP_WAVES, 0, s>>>(d_indptr, d_col, d_val, n, d_part, d_sa, ta.p,
                                                tv.p);
   exp_sums_kernel<<<g, 64 * LP_WAVES, 0, s>>>(d_indptr, n, ta.p, tv.p, tf.p, tacc.p);
-  exp_node_kernel<<<g, 64 * LP_WAVES, 0, s>>>(d_indptr, n, dim, part.p, sa.p, layout.p, ta.p, tf.p,
+  exp_node_kernel<<<g, 64 * LP_WAVES, 0, s>>>(d_indptr, n, dim, d_part, d_sa, layout.p, ta.p, tf.p,
                                               tacc.p, ra.p, rv.p, res.p);
-  exp_colsum_kernel<<<1, 64, 0, s>>>(res.p, n, dim, (float)(1.0 / (double)n), mean.p);
+  {
+    const size_t lds = 2 * (size_t)dim * (CS_FLOATS / dim + 1) * sizeof(float);
+    EVOC_HIP(hipFuncSetAttribute((const void*)exp_colsum_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    exp_colsum_kernel<<<1, CS_THREADS, lds, s>>>(res.p, n, dim, (float)(1.0 / (double)n), mean.p);
+  }
   exp_center_kernel<<<(unsigned)ceil_div(n * dim, 256), 256, 0, s>>>(res.p, n, dim, mean.p, d_out);
   EVOC_HIP(hipGetLastError());
+  EVOC_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace evoc
+
+// ---------------------------------------------------------------------------------
+// Partition and coarse graph of the input level in device memory: label_prop_loop
+// (label_prop.rs:202-227: seeds, Jacobi votes, label_outliers, remap_labels), the
+// member lists and column norms of the partition map, and R^T_norm G R (the
+// per-partition accumulation of coarsen, columns ascending) built from two stable
+// sorts, with every order-dependent f32 sum walked in the host formulation's order.
+namespace evoc {
+namespace {
+
+__global__ void lp_fill_kernel(int64_t n, int64_t* __restrict__ lab) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) lab[i] = -1;
+}
+__global__ void lp_seed_kernel(int64_t ns, const int64_t* __restrict__ pos,
+                               const int64_t* __restrict__ t, int64_t* __restrict__ lab) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < ns) lab[pos[q]] = t[q];
+}
+// flag[i] = 1 for unlabelled nodes; flag[n] = 0 so that the scan's last entry is the count
+__global__ void lp_unlabelled_kernel(int64_t n, const int64_t* __restrict__ lab,
+                                     int64_t* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= n) flag[i] = (i < n && lab[i] < 0) ? 1 : 0;
+}
+__global__ void lp_compact_kernel(int64_t n, const int64_t* __restrict__ flag,
+                                  const int64_t* __restrict__ rank, int64_t* __restrict__ list) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) list[rank[i]] = i;
+}
+
+// label_outliers (label_prop.rs:105-140) runs the outliers in node order, so when
+// outlier i searches, exactly the outliers below i already carry labels. Each thread
+// runs one outlier's depth-first search under that rule and records a label, or a
+// reference to the earlier outlier whose final label it takes, or (search exhausted)
+// its random label. Only the top 64 stack entries can ever be popped within the 64-pop
+// limit, so a 64-entry ring holds the stack exactly.
+__global__ void lp_outlier_walk_kernel(const int64_t* __restrict__ ptr,
+                                       const int32_t* __restrict__ col,
+                                       const int64_t* __restrict__ lab,
+                                       const int64_t* __restrict__ list, int64_t m,
+                                       uint64_t seed, int64_t maxl, int32_t* __restrict__ kind,
+                                       int64_t* __restrict__ val) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const int64_t i = list[k];
+  int64_t stk[64];
+  int top = 0, size = 0;
+  stk[0] = i;
+  top = 1;
+  size = 1;
+  int kd = 0;
+  int64_t v = 0;
+  for (int it = 0; it < 64 && size > 0 && kd == 0; ++it) {
+    --top;
+    --size;
+    const int64_t node = stk[top & 63];
+    for (int64_t p = ptr[node]; p < ptr[node + 1]; ++p) {
+      const int64_t j = col[p];
+      const int64_t lj = lab[j];
+      if (lj >= 0) {
+        kd = 1;
+        v = lj;
+        break;
+      }
+      if (j < i) {  // an earlier outlier: labelled by now
+        kd = 2;
+        v = j;
+        break;
+      }
+      stk[top & 63] = j;
+      ++top;
+      ++size;
+    }
+  }
+  if (kd == 0) {
+    const uint32_t r = Philox::first(seed, (uint32_t)i, (uint32_t)(i >> 32), 0, TAG_LPOL);
+    v = (int64_t)Philox::range(r, (uint64_t)maxl + 1);
+    kd = 1;
+  }
+  kind[k] = kd;
+  val[k] = v;
+}
+// references point to strictly smaller outliers, so every chain ends in a label
+__global__ void lp_outlier_resolve_kernel(const int64_t* __restrict__ list, int64_t m,
+                                          const int64_t* __restrict__ rank,
+                                          const int32_t* __restrict__ kind,
+                                          const int64_t* __restrict__ val,
+                                          int64_t* __restrict__ lab) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  int64_t q = k;
+  while (kind[q] == 2) q = rank[val[q]];
+  lab[list[k]] = val[q];
+}
+
+// remap_labels: the labels present, in ascending order, become 0..np-1
+__global__ void lp_present_kernel(int64_t n, const int64_t* __restrict__ lab,
+                                  int64_t* __restrict__ present) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && lab[i] >= 0) present[lab[i]] = 1;
+}
+__global__ void lp_part_kernel(int64_t n, const int64_t* __restrict__ lab,
+                               const int64_t* __restrict__ rank, int32_t* __restrict__ part,
+                               uint64_t* __restrict__ key, uint32_t* __restrict__ idx,
+                               int64_t* __restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t a = lab[i] >= 0 ? (int32_t)rank[lab[i]] : 0;  // every node is labelled here
+  part[i] = a;
+  key[i] = (uint64_t)a;
+  idx[i] = (uint32_t)i;
+  atomicAdd(reinterpret_cast<unsigned long long*>(&count[a]), 1ull);
+}
+__global__ void lp_idx_to_mem_kernel(int64_t n, const uint32_t* __restrict__ idx,
+                                     int32_t* __restrict__ mem) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) mem[i] = (int32_t)idx[i];
+}
+// column norm 1/sqrt(|a|) of the partition map (1.0f added |a| times, as the host sums
+// it) and the L1 row normalisation of R^T_norm (|sa| added |a| times in order)
+__global__ void lp_norms_kernel(int64_t np, const int64_t* __restrict__ mptr,
+                                float* __restrict__ sa, float* __restrict__ coef) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= np) return;
+  const int64_t cnt = mptr[a + 1] - mptr[a];
+  const float sq = (float)min<int64_t>(cnt, 16777216);  // f32 count saturates at 2^24
+  const float nrm = sqrtf(sq);
+  const float s = nrm > 0.0f ? 1.0f / nrm : 1.0f;
+  float l1 = 0.0f;
+  const float as = fabsf(s);
+  for (int64_t m = 0; m < cnt; ++m) l1 += as;
+  sa[a] = s;
+  coef[a] = l1 > 0.0f ? s * (1.0f / l1) : s;
+}
+
+// coarsening, step 1: entry p of row i gets key (part[i], col[p]); the stable sort
+// keeps CSR order (= member order, then row order) within a key
+__global__ void cg_key1_kernel(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                               int64_t n, const int32_t* __restrict__ part,
+                               uint64_t* __restrict__ key, uint32_t* __restrict__ idx) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const uint64_t a = (uint64_t)part[i] * (uint64_t)n;
+  for (int64_t p = ptr[i] + lane; p < ptr[i + 1]; p += 64) {
+    key[p] = a + (uint64_t)col[p];
+    idx[p] = (uint32_t)p;
+  }
+}
+// flag[q] = 1 where a run of equal keys starts; flag[m] = 0
+__global__ void cg_heads_kernel(int64_t m, const uint64_t* __restrict__ key,
+                                int64_t* __restrict__ flag) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q <= m) flag[q] = (q < m && (q == 0 || key[q] != key[q - 1])) ? 1 : 0;
+}
+__global__ void cg_starts_kernel(int64_t m, const int64_t* __restrict__ flag,
+                                 const int64_t* __restrict__ rank, int64_t* __restrict__ start) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < m && flag[q]) start[rank[q]] = q;
+}
+// step 2: per (a, j) run, the sum of sa[a] * G(i, j) in run order; the run then gets key
+// (a, part[j]) for the second stable sort (j stays ascending within it)
+__global__ void cg_run_sum_kernel(int64_t nrun, int64_t m, int64_t n, int64_t np,
+                                  const int64_t* __restrict__ start,
+                                  const uint64_t* __restrict__ key, const uint32_t* __restrict__ idx,
+                                  const float* __restrict__ val, const float* __restrict__ sa,
+                                  const int32_t* __restrict__ part, float* __restrict__ acc,
+                                  uint64_t* __restrict__ key2, uint32_t* __restrict__ idx2) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrun) return;
+  const int64_t q0 = start[r], q1 = r + 1 < nrun ? start[r + 1] : m;
+  const uint64_t k = key[q0];
+  const int64_t a = (int64_t)(k / (uint64_t)n), j = (int64_t)(k % (uint64_t)n);
+  const float s = sa[a];
+  float x = s * val[idx[q0]];
+  for (int64_t q = q0 + 1; q < q1; ++q) {
+    const float pr = s * val[idx[q]];
+    x = x + pr;
+  }
+  acc[r] = x;
+  key2[r] = (uint64_t)a * (uint64_t)np + (uint64_t)part[j];
+  idx2[r] = (uint32_t)r;
+}
+// step 3: per (a, b) run, the sum of the (a, j) sums in ascending j, clipped to [0, 1]
+__global__ void cg_entry_kernel(int64_t nent, int64_t nrun, int64_t np,
+                                const int64_t* __restrict__ start,
+                                const uint64_t* __restrict__ key2,
+                                const uint32_t* __restrict__ idx2, const float* __restrict__ acc,
+                                int32_t* __restrict__ ccol, float* __restrict__ cval,
+                                int64_t* __restrict__ count) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nent) return;
+  const int64_t r0 = start[e], r1 = e + 1 < nent ? start[e + 1] : nrun;
+  const uint64_t k = key2[r0];
+  float v = acc[idx2[r0]] * 1.0f;
+  for (int64_t r = r0 + 1; r < r1; ++r) {
+    const float pr = acc[idx2[r]] * 1.0f;
+    v = v + pr;
+  }
+  v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+  ccol[e] = (int32_t)(k % (uint64_t)np);
+  cval[e] = v;
+  atomicAdd(reinterpret_cast<unsigned long long*>(&count[k / (uint64_t)np]), 1ull);
+}
+
+int bit_length(uint64_t x) {
+  int b = 0;
+  while (x) {
+    ++b;
+    x >>= 1;
+  }
+  return std::max(b, 1);
+}
+
+}  // namespace
+
+void lp_top_partition(const int64_t* d_ptr, const int32_t* d_col, const float* d_val, int64_t n,
+                      int64_t approx, int n_iter, uint64_t seed, LpTop& P, hipStream_t s,
+                      DevPool& pool) {
+  EVOC_CHECK(approx >= 1 && approx <= n, "label propagation: seed count out of range");
+  EVOC_CHECK(n < ((int64_t)1 << 31), "label propagation: n must be below 2^31");
+  // seeds: seed t lands on a random node; a later seed on the same node wins
+  std::map<int64_t, int64_t> seeds;
+  for (int64_t t = 0; t < approx; ++t) {
+    const uint32_t r = Philox::first(seed, (uint32_t)t, (uint32_t)(t >> 32), 0, TAG_LPSD);
+    seeds[(int64_t)Philox::range(r, (uint64_t)n)] = t;
+  }
+  std::vector<int64_t> hp, ht;
+  int64_t maxl = -1;
+  for (auto& kv : seeds) {
+    hp.push_back(kv.first);
+    ht.push_back(kv.second);
+    maxl = std::max(maxl, kv.second);
+  }
+  const int64_t ns = (int64_t)hp.size();
+  EVOC_CHECK(maxl >= 0 && maxl < approx, "label propagation: no seeds");
+  Buf<int64_t> dpos(pool, ns), dt(pool, ns), la(pool, n), lb(pool, n);
+  EVOC_HIP(hipMemcpyAsync(dpos.p, hp.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  EVOC_HIP(hipMemcpyAsync(dt.p, ht.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  const unsigned gn = (unsigned)ceil_div(n, 256);
+  lp_fill_kernel<<<gn, 256, 0, s>>>(n, la.p);
+  lp_seed_kernel<<<(unsigned)ceil_div(ns, 256), 256, 0, s>>>(ns, dpos.p, dt.p, la.p);
+  // Jacobi votes (labelled nodes keep their label, so every seed label survives and
+  // maxl is the largest seed label)
+  const unsigned gw = (unsigned)ceil_div(n, LP_WAVES);
+  for (int it = 0; it < n_iter; ++it) {
+    lp_step_kernel<<<gw, 64 * LP_WAVES, 0, s>>>(d_ptr, d_col, d_val, n, la.p, lb.p);
+    std::swap(la.p, lb.p);
+  }
+  // outliers
+  Buf<int64_t> flag(pool, n + 1), rank(pool, n + 1);
+  lp_unlabelled_kernel<<<(unsigned)ceil_div(n + 1, 256), 256, 0, s>>>(n, la.p, flag.p);
+  exclusive_scan_i64(flag.p, rank.p, n + 1, s, pool);
+  int64_t m = 0;
+  EVOC_HIP(hipMemcpyAsync(&m, rank.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipStreamSynchronize(s));
+  if (m > 0) {
+    Buf<int64_t> list(pool, m), oval(pool, m);
+    Buf<int32_t> okind(pool, m);
+    lp_compact_kernel<<<gn, 256, 0, s>>>(n, flag.p, rank.p, list.p);
+    const unsigned gm = (unsigned)ceil_div(m, 64);
+    lp_outlier_walk_kernel<<<gm, 64, 0, s>>>(d_ptr, d_col, la.p, list.p, m, seed + 2000, maxl,
+                                             okind.p, oval.p);
+    lp_outlier_resolve_kernel<<<gm, 64, 0, s>>>(list.p, m, rank.p, okind.p, oval.p, la.p);
+  }
+  // remap: labels are seed ids below approx
+  Buf<int64_t> present(pool, approx + 1), lrank(pool, approx + 1);
+  EVOC_HIP(hipMemsetAsync(present.p, 0, (approx + 1) * sizeof(int64_t), s));
+  lp_present_kernel<<<gn, 256, 0, s>>>(n, la.p, present.p);
+  exclusive_scan_i64(present.p, lrank.p, approx + 1, s, pool);
+  EVOC_HIP(hipMemcpyAsync(&P.np, lrank.p + approx, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipStreamSynchronize(s));
+  const int64_t np = P.np;
+  // members of each partition in ascending node order: stable sort by partition
+  P.part.alloc(pool, n);
+  P.mem.alloc(pool, n);
+  P.mptr.alloc(pool, np + 1);
+  P.sa.alloc(pool, np);
+  P.coef.alloc(pool, np);
+  Buf<uint64_t> key(pool, n);
+  Buf<uint32_t> idx(pool, n);
+  Buf<int64_t> cnt(pool, np + 1);
+  EVOC_HIP(hipMemsetAsync(cnt.p, 0, (np + 1) * sizeof(int64_t), s));
+  lp_part_kernel<<<gn, 256, 0, s>>>(n, la.p, lrank.p, P.part.p, key.p, idx.p, cnt.p);
+  radix_sort_pairs(key.p, idx.p, n, bit_length((uint64_t)(np - 1)), s, pool);
+  lp_idx_to_mem_kernel<<<gn, 256, 0, s>>>(n, idx.p, P.mem.p);
+  exclusive_scan_i64(cnt.p, P.mptr.p, np + 1, s, pool);
+  lp_norms_kernel<<<(unsigned)ceil_div(np, 256), 256, 0, s>>>(np, P.mptr.p, P.sa.p, P.coef.p);
+  EVOC_HIP(hipGetLastError());
+}
+
+void lp_top_coarsen(const int64_t* d_ptr, const int32_t* d_col, const float* d_val, int64_t n,
+                    int64_t nnz, const LpTop& P, std::vector<int64_t>& cptr,
+                    std::vector<int32_t>& ccol, std::vector<float>& cval, hipStream_t s,
+                    DevPool& pool) {
+  const int64_t np = P.np;
+  cptr.assign(np + 1, 0);
+  ccol.clear();
+  cval.clear();
+  if (nnz == 0) return;
+  EVOC_CHECK(nnz < ((int64_t)1 << 32), "coarsening: more than 2^32 graph entries");
+  Buf<uint64_t> key(pool, nnz);
+  Buf<uint32_t> idx(pool, nnz);
+  cg_key1_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(d_ptr, d_col, n, P.part.p, key.p, idx.p);
+  radix_sort_pairs(key.p, idx.p, nnz, bit_length((uint64_t)np * (uint64_t)n - 1), s, pool);
+  Buf<int64_t> flag(pool, nnz + 1), rank(pool, nnz + 1);
+  cg_heads_kernel<<<(unsigned)ceil_div(nnz + 1, 256), 256, 0, s>>>(nnz, key.p, flag.p);
+  exclusive_scan_i64(flag.p, rank.p, nnz + 1, s, pool);
+  int64_t nrun = 0;
+  EVOC_HIP(hipMemcpyAsync(&nrun, rank.p + nnz, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipStreamSynchronize(s));
+  Buf<int64_t> start(pool, nrun);
+  cg_starts_kernel<<<(unsigned)ceil_div(nnz, 256), 256, 0, s>>>(nnz, flag.p, rank.p, start.p);
+  Buf<float> acc(pool, nrun);
+  Buf<uint64_t> key2(pool, nrun);
+  Buf<uint32_t> idx2(pool, nrun);
+  cg_run_sum_kernel<<<(unsigned)ceil_div(nrun, 256), 256, 0, s>>>(
+      nrun, nnz, n, np, start.p, key.p, idx.p, d_val, P.sa.p, P.part.p, acc.p, key2.p, idx2.p);
+  radix_sort_pairs(key2.p, idx2.p, nrun, bit_length((uint64_t)np * (uint64_t)np - 1), s, pool);
+  cg_heads_kernel<<<(unsigned)ceil_div(nrun + 1, 256), 256, 0, s>>>(nrun, key2.p, flag.p);
+  exclusive_scan_i64(flag.p, rank.p, nrun + 1, s, pool);
+  int64_t nent = 0;
+  EVOC_HIP(hipMemcpyAsync(&nent, rank.p + nrun, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipStreamSynchronize(s));
+  Buf<int64_t> start2(pool, nent), cnt(pool, np + 1), dptr(pool, np + 1);
+  cg_starts_kernel<<<(unsigned)ceil_div(nrun, 256), 256, 0, s>>>(nrun, flag.p, rank.p, start2.p);
+  Buf<int32_t> dcol(pool, nent);
+  Buf<float> dval(pool, nent);
+  EVOC_HIP(hipMemsetAsync(cnt.p, 0, (np + 1) * sizeof(int64_t), s));
+  cg_entry_kernel<<<(unsigned)ceil_div(nent, 256), 256, 0, s>>>(
+      nent, nrun, np, start2.p, key2.p, idx2.p, acc.p, dcol.p, dval.p, cnt.p);
+  exclusive_scan_i64(cnt.p, dptr.p, np + 1, s, pool);
+  EVOC_HIP(hipGetLastError());
+  ccol.resize(nent);
+  cval.resize(nent);
+  EVOC_HIP(hipMemcpyAsync(cptr.data(), dptr.p, (np + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipMemcpyAsync(ccol.data(), dcol.p, nent * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipMemcpyAsync(cval.data(), dval.p, nent * sizeof(float), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
 }
 

@@ -197,8 +197,7 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
   stage[1] = ms_since(t0);
   if (verbose) std::printf("... fuzzy simplicial set done in %.2f ms.\n", stage[1]);
   const int dim = auto_dim(p);
-  // graph on the host: needed by the host label-propagation stage and by the
-  // non-symmetrised embedding layout
+  // graph on the host: needed by the non-symmetrised initial layout and embedding
   std::vector<int64_t> hptr;
   std::vector<int32_t> hcol;
   std::vector<float> hw;
@@ -214,15 +213,14 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
     epochs = nn.warm_epochs > 0 ? nn.warm_epochs : std::max(1, (int)(0.3 * p.n_epochs));
   } else {
     if (verbose) std::printf("Computing label propagation initialisation...\n");
-    to_host(hptr, g.indptr.p, n + 1, s);
-    to_host(hcol, g.indices.p, g.nnz, s);
-    to_host(hw, g.weights.p, g.nnz, s);
-    EVOC_HIP(hipStreamSynchronize(s));
-    if (verbose >= 2) std::printf("    graph to host: %.1f ms\n", ms_since(t0));
-    if (p.symmetrise) {  // the device CSR is the same column-sorted graph
-      label_prop_init_device(hptr.data(), hcol.data(), hw.data(), g.indptr.p, g.indices.p,
-                             g.weights.p, n, dim, dX, d, ld, seed, demb.p, s, pool, verbose);
+    if (p.symmetrise) {  // the device CSR is the column-sorted graph
+      label_prop_init_device(g.indptr.p, g.indices.p, g.weights.p, g.nnz, n, dim, dX, d, ld, seed,
+                             demb.p, s, pool, verbose);
     } else {
+      to_host(hptr, g.indptr.p, n + 1, s);
+      to_host(hcol, g.indices.p, g.nnz, s);
+      to_host(hw, g.weights.p, g.nnz, s);
+      EVOC_HIP(hipStreamSynchronize(s));
       std::vector<float> init(n * dim);
       // Csr::from_coo (src/utils/sparse.rs:195-235) orders every row by column
       std::vector<int32_t> c2(hcol);

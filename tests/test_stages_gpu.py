@@ -124,6 +124,50 @@ def test_label_prop_init_hub_graph_parity():
         assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), seed
 
 
+def _outlier_graph(seed=5, n_a=1500, n_b=900, n_c=400):
+    """Symmetric column-sorted CSR built so that label propagation leaves outliers: a
+    cluster of weight-1.0 edges, chains of weight-0.3 edges (no vote can pass 1.0)
+    hooked to the cluster by a few weak edges, and isolated pairs; node ids shuffled so
+    that outliers reach earlier and later outliers in their searches."""
+    rng = np.random.default_rng(seed)
+    n = n_a + n_b + n_c
+    wt = {}
+
+    def edge(i, j, w):
+        if i != j:
+            wt[(min(i, j), max(i, j))] = np.float32(w)
+
+    for i in range(n_a):
+        for j in rng.choice(n_a, size=8, replace=False):
+            edge(i, int(j), 1.0)
+    for q in range(n_b - 1):
+        edge(n_a + q, n_a + q + 1, 0.3)
+    for q in range(0, n_b, 50):
+        edge(n_a + q, int(rng.integers(n_a)), 0.2)
+    for q in range(0, n_c - 1, 2):
+        edge(n_a + n_b + q, n_a + n_b + q + 1, 0.4)
+    perm = rng.permutation(n)
+    adj = [dict() for _ in range(n)]
+    for (i, j), w in wt.items():
+        a, b = int(perm[i]), int(perm[j])
+        adj[a][b] = w
+        adj[b][a] = w
+    rows = [sorted(a.items()) for a in adj]
+    ip = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64)
+    ind = np.array([j for r in rows for j, _ in r], np.int32)
+    w = np.array([x for r in rows for _, x in r], np.float32)
+    return ip, ind, w
+
+
+def test_label_prop_init_outliers_parity():
+    ip, ind, w = _outlier_graph()
+    x = np.random.default_rng(2).normal(size=(ip.shape[0] - 1, 12)).astype(np.float32)
+    for seed, data in ((1, None), (7, x)):
+        g = H.label_prop_init(ip, ind, w, 8, data=data, seed=seed)
+        o = O.label_prop_init(ip, ind, w, 8, data=data, seed=seed)
+        assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), seed
+
+
 def test_label_prop_init_parity(graph):
     x, gt, idx, dist = graph
     op, oi, ow, _, _ = O.fuzzy_graph(idx, dist, 15.0)
