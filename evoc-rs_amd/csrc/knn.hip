@@ -53,36 +53,41 @@ __global__ __launch_bounds__(256) void knn_prep_kernel(
     uint16_t* __restrict__ Xh, uint16_t* __restrict__ Xl, float* __restrict__ nrm,
     float* __restrict__ sqn, float* __restrict__ colscale, int metric,
     unsigned* __restrict__ flags /* [0]=inexact, [1]=max norm bits */) {
+  // one wavefront per row, rows strided over the grid; the flags are folded per
+  // wavefront so that each wavefront issues one atomic of each kind
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t row = (int64_t)blockIdx.x * 4 + wave;
-  if (row >= npad) return;
-  float acc = 0.0f;
+  const int64_t nwaves = (int64_t)gridDim.x * 4;
   bool any_lo = false;
-  for (int t = lane; t < dpad; t += 64) {
-    float v = (row < n && t < d) ? X[row * ld + t] : 0.0f;
-    float p = v * v;
-    acc = acc + p;
-    uint16_t h = f2bf(v);
-    float r = v - bf2f(h);
-    uint16_t lo = f2bf(r);
-    Xh[row * dpad + t] = h;
-    if (Xl) Xl[row * dpad + t] = lo;
-    any_lo |= (lo & 0x7fffu) != 0;
-  }
-  acc = tree64(acc);
-  if (__any(any_lo) && lane == 0) atomicOr(&flags[0], 1u);
-  if (lane == 0) {
-    float nv = sqrtf(acc);
-    if (row < n) {
-      nrm[row] = nv;
-      sqn[row] = acc;
-      atomicMax(&flags[1], __float_as_uint(nv));
+  unsigned maxbits = 0;
+  for (int64_t row = (int64_t)blockIdx.x * 4 + wave; row < npad; row += nwaves) {
+    float acc = 0.0f;
+    for (int t = lane; t < dpad; t += 64) {
+      float v = (row < n && t < d) ? X[row * ld + t] : 0.0f;
+      float p = v * v;
+      acc = acc + p;
+      uint16_t h = f2bf(v);
+      float r = v - bf2f(h);
+      uint16_t lo = f2bf(r);
+      Xh[row * dpad + t] = h;
+      if (Xl) Xl[row * dpad + t] = lo;
+      any_lo |= (lo & 0x7fffu) != 0;
     }
-    float cs;
-    if (metric == 1) cs = (row < n && nv > 0.0f) ? 1.0f / nv : 0.0f;
-    else cs = row < n ? acc : 0.0f;
-    colscale[row] = cs;
+    acc = tree64(acc);
+    if (lane == 0) {
+      float nv = sqrtf(acc);
+      if (row < n) {
+        nrm[row] = nv;
+        sqn[row] = acc;
+        maxbits = max(maxbits, __float_as_uint(nv));
+      }
+      float cs;
+      if (metric == 1) cs = (row < n && nv > 0.0f) ? 1.0f / nv : 0.0f;
+      else cs = row < n ? acc : 0.0f;
+      colscale[row] = cs;
+    }
   }
+  if (__any(any_lo) && lane == 0) atomicOr(&flags[0], 1u);
+  if (lane == 0 && maxbits) atomicMax(&flags[1], maxbits);
 }
 
 // ------------------------------------------------------------------ 2. MFMA top-C
@@ -1036,45 +1041,85 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     const int c = t * 64 + lane;
     q[t] = (t < nt && c < d) ? X[row * ld + c] : 0.0f;
   }
+  // Candidate c sits in lane c. Its approximate key bounds its exact distance from below
+  // (the same bound the certificate applies to non-candidates), so only candidates
+  // whose bound does not clear the exact k-th distance of the k smallest-key
+  // candidates need an exact distance: the rest are strictly farther than the k-th.
+  const float u = 5.9604645e-08f;  // 2^-24
+  const float dt = (float)(d / 64 + 12);
+  float E = 0.0f, eps = 0.0f;
+  if (metric == 1) {
+    eps = 1.5f * gamma + 64.0f * u;
+  } else {
+    const float nmax = __uint_as_float(flags[1]);
+    E = 1.5f * (dt * u * (sqn[row] + nmax * nmax) + 2.0f * (gamma + 2.0f * u) * nrm[row] * nmax);
+  }
+  const int cj = lane < C ? cand_idx[row * C + lane] : -1;
+  const float ckey = cj >= 0 ? cand_key[row * C + lane] : FINF;
+  const bool valid = cj >= 0;
+  const int nvalid = __popcll(__ballot(valid));
+  int krank = 0;
+  for (int c = 0; c < C; ++c) {
+    const float ok = __shfl(ckey, c, 64);
+    const int oj = __shfl(cj, c, 64);
+    if (oj >= 0 && (ok < ckey || (ok == ckey && c < lane))) ++krank;
+  }
   float my_d = FINF, my_sq = FINF;
   int my_j = -1;
-  for (int c = 0; c < C; ++c) {
-    const int j = cand_idx[row * C + c];
-    if (j < 0) continue;
-    float acc = 0.0f;
+  auto evaluate = [&](unsigned long long mask) {
+    for (; mask; mask &= mask - 1) {
+      const int c = __ffsll((long long)mask) - 1;
+      const int j = __shfl(cj, c, 64);
+      float acc = 0.0f;
 #pragma unroll
-    for (int t = 0; t < MAXT; ++t) {
-      const int cc = t * 64 + lane;
-      if (t < nt && cc < d) {
-        const float v = X[(int64_t)j * ld + cc];
-        if (metric == 1) {
-          const float p = q[t] * v;
-          acc = acc + p;
-        } else {
-          const float df = q[t] - v;
-          const float p = df * df;
-          acc = acc + p;
+      for (int t = 0; t < MAXT; ++t) {
+        const int cc = t * 64 + lane;
+        if (t < nt && cc < d) {
+          const float v = X[(int64_t)j * ld + cc];
+          if (metric == 1) {
+            const float p = q[t] * v;
+            acc = acc + p;
+          } else {
+            const float df = q[t] - v;
+            const float p = df * df;
+            acc = acc + p;
+          }
         }
       }
+      acc = tree64(acc);
+      acc = __shfl(acc, 0, 64);
+      const float dist = metric == 1 ? cos_dist(acc, nrm[row], nrm[j]) : sqrtf(acc);
+      if (lane == c) {
+        my_d = dist;
+        my_sq = acc;
+        my_j = j;
+      }
     }
-    acc = tree64(acc);
-    acc = __shfl(acc, 0, 64);
-    const float dist = metric == 1 ? cos_dist(acc, nrm[row], nrm[j]) : sqrtf(acc);
-    if (lane == c) {
-      my_d = dist;
-      my_sq = acc;
-      my_j = j;
+  };
+  auto exact_rank = [&]() {
+    int r = 0;
+    for (int c = 0; c < C; ++c) {
+      const float od = __shfl(my_d, c, 64);
+      const int oj = __shfl(my_j, c, 64);
+      if (oj >= 0 && (od < my_d || (od == my_d && oj < my_j))) ++r;
     }
+    return r;
+  };
+  evaluate(__ballot(valid && krank < k));
+  if (nvalid > k) {
+    const int r0 = exact_rank();
+    const unsigned long long kb0 = __ballot(my_j >= 0 && r0 == k - 1);
+    const int src0 = __ffsll((long long)kb0) - 1;
+    const float dk0 = __shfl(my_d, src0, 64);
+    const float dk20 = __shfl(my_sq, src0, 64);
+    bool need;
+    if (metric == 1) need = !((1.0f + ckey * colscale[row]) - eps > dk0);
+    else need = !(sqn[row] + ckey - E > dk20 * (1.0f + (dt + 4.0f) * u) + 1e-30f);
+    evaluate(__ballot(valid && krank >= k && need));
   }
-  int rank = 0;
-  for (int c = 0; c < C; ++c) {
-    const float od = __shfl(my_d, c, 64);
-    const int oj = __shfl(my_j, c, 64);
-    if (oj >= 0 && (od < my_d || (od == my_d && oj < my_j))) ++rank;
-  }
-  const bool valid = my_j >= 0;
-  const int nvalid = __popcll(__ballot(valid));
-  if (valid && rank < k) {
+  const int rank = exact_rank();
+  const bool kept = my_j >= 0;
+  if (kept && rank < k) {
     out_idx[row * k + rank] = my_j;
     out_dist[row * k + rank] = my_d;
   }
@@ -1090,23 +1135,17 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
   } else if (nvalid < k) {
     pass = false;
   } else {
-    const unsigned long long kb = __ballot(valid && rank == k - 1);
+    const unsigned long long kb = __ballot(kept && rank == k - 1);
     const int src = __ffsll((long long)kb) - 1;
     const float dk = __shfl(my_d, src, 64);
     const float dk2 = __shfl(my_sq, src, 64);
     // one sorted list of C keys, or two column halves that each kept their C/2 best
     const float kc = halves == 2 ? fminf(cand_key[row * C + (C / 2 - 1)], cand_key[row * C + (C - 1)])
                                  : cand_key[row * C + (C - 1)];
-    const float u = 5.9604645e-08f;  // 2^-24
     if (metric == 1) {
-      const float eps = 1.5f * gamma + 64.0f * u;
       const float lb = 1.0f + kc * colscale[row];
       pass = (lb - eps) > dk;
     } else {
-      const float nmax = __uint_as_float(flags[1]);
-      const float ni = nrm[row];
-      const float dt = (float)(d / 64 + 12);
-      const float E = 1.5f * (dt * u * (sqn[row] + nmax * nmax) + 2.0f * (gamma + 2.0f * u) * ni * nmax);
       const float lb2 = sqn[row] + kc - E;
       pass = lb2 > dk2 * (1.0f + (dt + 4.0f) * u) + 1e-30f;
     }
@@ -1150,68 +1189,84 @@ __device__ void exact_block_merge(float* sk, int* si, int k) {
 
 // Exact rows, phase 1: block (f, part) scans columns [part*span, (part+1)*span) of
 // failing row f in the canonical order and keeps the block's best k by (dist, idx).
+// Exact scan for rows whose certificate failed: up to EX_ROWS failing rows per launch,
+// held in registers, against the columns of one part per workgroup. One wavefront per
+// column at a time: lanes hold the column's dimensions (the canonical strided partial
+// sums and halving tree of the rerank), and lane f keeps the sorted list of failing
+// row f. Each wavefront writes its lists; the merge kernel combines them.
+constexpr int EX_ROWS = 8;
+
 __global__ __launch_bounds__(256) void knn_exact_kernel(
     const float* __restrict__ X, int64_t n, int d, int64_t ld, const float* __restrict__ nrm,
-    int k, int metric, const int32_t* __restrict__ fail_rows, int parts, int64_t span,
+    int k, int metric, const int32_t* __restrict__ fail_rows, int nrows, int parts, int64_t span,
     float* __restrict__ part_key, int32_t* __restrict__ part_idx) {
-  extern __shared__ __attribute__((aligned(16))) char xsm[];
-  float* sk = reinterpret_cast<float*>(xsm);
-  int* si = reinterpret_cast<int*>(xsm + 256 * KMAX * sizeof(float));
-  const int tid = threadIdx.x;
-  const int f = blockIdx.x / parts, part = blockIdx.x % parts;
-  const int64_t row = fail_rows[f];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int part = blockIdx.x;
+  const int nt = (d + 63) / 64;
+  float q[EX_ROWS][MAXT];
+  int64_t rows[EX_ROWS];
+#pragma unroll
+  for (int f = 0; f < EX_ROWS; ++f) {
+    rows[f] = f < nrows ? fail_rows[f] : -1;
+#pragma unroll
+    for (int t = 0; t < MAXT; ++t) {
+      const int c = t * 64 + lane;
+      q[f][t] = (f < nrows && t < nt && c < d) ? X[rows[f] * ld + c] : 0.0f;
+    }
+  }
   float lk[KMAX];
   int li[KMAX];
 #pragma unroll
-  for (int q = 0; q < KMAX; ++q) {
-    lk[q] = FINF;
-    li[q] = -1;
+  for (int r = 0; r < KMAX; ++r) {
+    lk[r] = FINF;
+    li[r] = -1;
   }
-  const float* xr = X + row * ld;
+  const float my_nrm = lane < nrows ? nrm[fail_rows[lane]] : 0.0f;
+  const int64_t my_row = lane < nrows ? fail_rows[lane] : -1;
   const int64_t j0 = (int64_t)part * span, j1 = min<int64_t>(n, j0 + span);
-  for (int64_t j = j0 + tid; j < j1; j += 256) {
-    if (j == row) continue;
-    const float* xj = X + j * ld;
-    float acc[64];
+  for (int64_t j = j0 + wave; j < j1; j += 4) {
+    float v[MAXT];
 #pragma unroll
-    for (int c = 0; c < 64; ++c) acc[c] = 0.0f;
-    for (int t0 = 0; t0 < d; t0 += 64) {
+    for (int t = 0; t < MAXT; ++t) {
+      const int c = t * 64 + lane;
+      v[t] = (t < nt && c < d) ? X[j * ld + c] : 0.0f;
+    }
+    const float nj = nrm[j];
 #pragma unroll
-      for (int c = 0; c < 64; ++c)
-        if (t0 + c < d) {
-          const float a = xr[t0 + c], b = xj[t0 + c];
+    for (int f = 0; f < EX_ROWS; ++f) {
+      if (f >= nrows) break;
+      float acc = 0.0f;
+#pragma unroll
+      for (int t = 0; t < MAXT; ++t) {
+        const int c = t * 64 + lane;
+        if (t < nt && c < d) {
           if (metric == 1) {
-            const float p = a * b;
-            acc[c] = acc[c] + p;
+            const float p = q[f][t] * v[t];
+            acc = acc + p;
           } else {
-            const float df = a - b;
+            const float df = q[f][t] - v[t];
             const float p = df * df;
-            acc[c] = acc[c] + p;
+            acc = acc + p;
           }
         }
+      }
+      acc = tree64(acc);
+      acc = __shfl(acc, 0, 64);
+      if (lane == f && j != my_row) {
+        const float dist = metric == 1 ? cos_dist(acc, my_nrm, nj) : sqrtf(acc);
+        if (dist < lk[k - 1] || (dist == lk[k - 1] && (int)j < li[k - 1]))
+          list_insert<KMAX>(lk, li, dist, (int)j);
+      }
     }
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1)
-#pragma unroll
-      for (int c = 0; c < s; ++c) acc[c] = acc[c] + acc[c + s];
-    const float dist = metric == 1 ? cos_dist(acc[0], nrm[row], nrm[j]) : sqrtf(acc[0]);
-    if (dist < lk[k - 1] || (dist == lk[k - 1] && (int)j < li[k - 1]))
-      list_insert<KMAX>(lk, li, dist, (int)j);
   }
-#pragma unroll
-  for (int q = 0; q < KMAX; ++q) {
-    sk[tid * KMAX + q] = lk[q];
-    si[tid * KMAX + q] = li[q];
-  }
-  __syncthreads();
-  exact_block_merge(sk, si, k);
-  for (int q = tid; q < k; q += 256) {
-    part_key[(int64_t)blockIdx.x * k + q] = sk[q];
-    part_idx[(int64_t)blockIdx.x * k + q] = si[q];
+  if (lane < nrows) {
+    const int64_t base = (((int64_t)lane * parts + part) * 4 + wave) * k;
+    for (int r = 0; r < k; ++r) {
+      part_key[base + r] = lk[r];
+      part_idx[base + r] = li[r];
+    }
   }
 }
-
-// Exact rows, phase 2: one block per failing row merges its parts' lists by (dist, idx).
 __global__ __launch_bounds__(256) void knn_exact_merge_kernel(
     int k, const int32_t* __restrict__ fail_rows, int parts, const float* __restrict__ part_key,
     const int32_t* __restrict__ part_idx, int32_t* __restrict__ out_idx,
@@ -1309,7 +1364,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     sqn.alloc(pool, npad);
     colscale.alloc(pool, npad);
     EVOC_HIP(hipMemsetAsync(flags.p, 0, 4 * sizeof(unsigned), s));
-    knn_prep_kernel<<<(unsigned)ceil_div(npad, 4), 256, 0, s>>>(
+    knn_prep_kernel<<<(unsigned)std::min<int64_t>(ceil_div(npad, 4), 4096), 256, 0, s>>>(
         dX, n, d, ld, npad, dpad, Xh.p, want_lo ? Xl.p : nullptr, nrm.p, sqn.p, colscale.p,
         metric, flags.p);
     EVOC_HIP(hipGetLastError());
@@ -1402,15 +1457,21 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   EVOC_HIP(hipMemcpyAsync(&nfail, fcount.p, sizeof(int), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
   if (nfail > 0) {
-    // each failing row's columns are split over `parts` workgroups, then merged
-    const int parts = (int)std::max<int64_t>(1, std::min<int64_t>(512, 4096 / nfail));
+    // the columns are split over `parts` workgroups of four wavefronts, EX_ROWS failing
+    // rows per launch; the per-wavefront lists of each row are then merged
+    const int parts = 1024;
     const int64_t span = ceil_div(n, (int64_t)parts);
-    Buf<float> pk(pool, (size_t)nfail * parts * k);
-    Buf<int32_t> pi(pool, (size_t)nfail * parts * k);
-    knn_exact_kernel<<<(unsigned)(nfail * parts), 256, 256 * KMAX * 8, s>>>(
-        dX, n, d, ld, nrm.p, k, metric, frows.p, parts, span, pk.p, pi.p);
-    knn_exact_merge_kernel<<<(unsigned)nfail, 256, 256 * KMAX * 8, s>>>(k, frows.p, parts, pk.p,
-                                                                        pi.p, d_idx, d_dist);
+    const int nb = std::min(nfail, EX_ROWS);
+    Buf<float> pk(pool, (size_t)nb * parts * 4 * k);
+    Buf<int32_t> pi(pool, (size_t)nb * parts * 4 * k);
+    for (int f0 = 0; f0 < nfail; f0 += EX_ROWS) {
+      const int rows = std::min(EX_ROWS, nfail - f0);
+      knn_exact_kernel<<<(unsigned)parts, 256, 0, s>>>(dX, n, d, ld, nrm.p, k, metric,
+                                                        frows.p + f0, rows, parts, span, pk.p,
+                                                        pi.p);
+      knn_exact_merge_kernel<<<(unsigned)rows, 256, 256 * KMAX * 8, s>>>(
+          k, frows.p + f0, parts * 4, pk.p, pi.p, d_idx, d_dist);
+    }
     EVOC_HIP(hipGetLastError());
   }
   float mfma_ms = 0.0f;
