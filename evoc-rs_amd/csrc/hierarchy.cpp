@@ -28,30 +28,40 @@ Linkage linkage_from_sorted(const int32_t* u, const int32_t* v, const float* w, 
   L.right.resize(ne);
   L.size.resize(ne);
   L.dist.resize(ne);
-  std::vector<int64_t> par(n), lab(n), sz(n + ne, 1), rk(n, 0);
-  std::iota(par.begin(), par.end(), 0);
-  std::iota(lab.begin(), lab.end(), 0);
+  // one record per vertex, so that a union-find step costs one cache line: the parent,
+  // and at a root its rank, the linkage label of its component and the component size
+  struct Vtx {
+    int32_t par, rank;
+    int64_t lab, size;
+  };
+  std::vector<Vtx> V(n);
+  for (int64_t x = 0; x < n; ++x) V[x] = Vtx{(int32_t)x, 0, x, 1};
   auto find = [&](int64_t x) {
-    while (par[x] != x) {
-      par[x] = par[par[x]];
-      x = par[x];
+    while (V[x].par != x) {
+      V[x].par = V[V[x].par].par;
+      x = V[x].par;
     }
     return x;
   };
+  constexpr int64_t AHEAD = 16;  // edges are in weight order: their vertices are scattered
   for (int64_t i = 0; i < ne; ++i) {
+    if (i + AHEAD < ne) {
+      __builtin_prefetch(&V[u[i + AHEAD]]);
+      __builtin_prefetch(&V[v[i + AHEAD]]);
+    }
     int64_t ru = find(u[i]), rv = find(v[i]);
-    int64_t l = lab[ru], r = lab[rv];
-    const int64_t ns = sz[l] + sz[r];
+    int64_t l = V[ru].lab, r = V[rv].lab;
+    const int64_t ns = V[ru].size + V[rv].size;
     if (l < r) std::swap(l, r);
     L.left[i] = l;
     L.right[i] = r;
     L.dist[i] = w[i];
     L.size[i] = ns;
-    sz[n + i] = ns;
-    if (rk[ru] < rk[rv]) std::swap(ru, rv);
-    par[rv] = ru;
-    if (rk[ru] == rk[rv]) ++rk[ru];
-    lab[ru] = n + i;
+    if (V[ru].rank < V[rv].rank) std::swap(ru, rv);
+    V[rv].par = (int32_t)ru;
+    if (V[ru].rank == V[rv].rank) ++V[ru].rank;
+    V[ru].lab = n + i;
+    V[ru].size = ns;
   }
   return L;
 }

@@ -123,15 +123,25 @@ void free_result(evoc_result_t* r) {
 
 // The clustering stage on a device embedding: MST -> linkage -> layers or search.
 Layers cluster_device(const float* d_emb, int64_t n, int dim, const evoc_params_t& p,
-                      hipStream_t s, DevPool& pool, double* mst_ms, double* layer_ms) {
+                      hipStream_t s, DevPool& pool, double* mst_ms, double* layer_ms,
+                      int verbose = 0) {
   Layers out;
   auto t0 = clk::now();
+  auto tl = t0;
+  auto lap = [&](const char* what) {
+    if (verbose >= 2) {
+      std::printf("    %s: %.1f ms\n", what, ms_since(tl));
+      tl = clk::now();
+    }
+  };
   std::vector<int32_t> u(std::max<int64_t>(n - 1, 1)), v(std::max<int64_t>(n - 1, 1));
   std::vector<float> w(std::max<int64_t>(n - 1, 1));
   mst_device(d_emb, n, dim, p.min_samples, u.data(), v.data(), w.data(), s, pool);
+  lap("mst");
   int64_t ne = std::max<int64_t>(n - 1, 0);
   while (ne > 0 && u[ne - 1] < 0) --ne;  // disconnected (non-finite input): fewer edges
   const Linkage L = linkage_from_sorted(u.data(), v.data(), w.data(), ne, n);
+  lap("linkage");
   if (mst_ms) *mst_ms = ms_since(t0);
   t0 = clk::now();
   if (p.approx_n_clusters > 0) {
@@ -144,6 +154,7 @@ Layers cluster_device(const float* d_emb, int64_t n, int dim, const evoc_params_
     out = cluster_layers_from_linkage(L, p.base_min_cluster_size, p.min_similarity_threshold,
                                       p.max_layers, s, pool);
   }
+  lap("layers");
   if (layer_ms) *layer_ms = ms_since(t0);
   return out;
 }
@@ -279,7 +290,7 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
   if (verbose) std::printf(" ... embedding done in %.2f ms.\n", stage[3]);
   // 5./6. clustering
   if (verbose) std::printf("Running density-based clustering...\n");
-  Layers lay = cluster_device(demb.p, n, dim, p, s, pool, &stage[4], &stage[5]);
+  Layers lay = cluster_device(demb.p, n, dim, p, s, pool, &stage[4], &stage[5], verbose);
   const int nl = (int)lay.labels.size();
   evoc_result_t* r = alloc_result(n, k, dim, nl);
   try {

@@ -86,3 +86,18 @@ def test_knn_forced_bf16_on_inexact_input():
     x, _ = datagen.unit_mixture(3000, 256, 30, 0.2, seed=5)
     _check(x, 15, "cosine", precision="bf16")
     _check(x, 12, "euclidean", precision="bf16")
+
+
+@pytest.mark.parametrize("bf16,d", [(True, 300), (False, 64)])
+def test_knn_exact_fallback_many_rows(bf16, d):
+    """40 copies of one point: each copy has 39 neighbours at distance zero, more than the
+    candidate lists hold, so its certificate fails and the exact scan (several batches of
+    failing rows) decides it."""
+    rng = np.random.default_rng(21)
+    x = rng.normal(size=(1500, d)).astype(np.float32)
+    x[100:140] = x[7]
+    if bf16:
+        x = datagen.to_bf16_values(x)
+    for metric in ("euclidean", "cosine"):
+        st = _check(x, 15, metric)
+        assert st["fallback_rows"] >= 40, st
