@@ -433,6 +433,26 @@ __global__ __launch_bounds__(64) void boruvka_seed_kernel(
   if (best < __builtin_huge_valf()) atomicMin(&comp_w[mycomp], __float_as_uint(best));
 }
 
+// Later rounds: a point's best edge from the previous round is still a real edge out
+// of its (merged) component when the far end lies in another component now, so it
+// seeds that component's bound too.
+__global__ void boruvka_prev_seed_kernel(int64_t n, const int32_t* __restrict__ comp,
+                                         const uint32_t* __restrict__ perm,
+                                         const int32_t* __restrict__ inv,
+                                         const float* __restrict__ best_w,
+                                         const int32_t* __restrict__ best_a,
+                                         const int32_t* __restrict__ best_b,
+                                         unsigned* __restrict__ comp_w) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float w = best_w[p];
+  if (!(w < __builtin_huge_valf())) return;
+  const int32_t me = (int32_t)perm[p];
+  const int32_t other = best_a[p] == me ? best_b[p] : best_a[p];
+  const int32_t c = comp[p];
+  if (comp[inv[other]] != c) atomicMin(&comp_w[c], __float_as_uint(w));
+}
+
 __global__ void boruvka_select_kernel(int64_t n, const int32_t* __restrict__ comp,
                                       const unsigned* __restrict__ comp_w,
                                       const float* __restrict__ best_w,
@@ -697,6 +717,9 @@ void mst_device(const float* d_emb, int64_t n, int dim, int min_samples, int32_t
     EVOC_HIP(hipMemsetAsync(comp_w.p, 0x7f, ncomp * sizeof(unsigned), s));  // 0x7f7f7f7f > any finite w
     EVOC_HIP(hipMemsetAsync(comp_key.p, 0xff, ncomp * sizeof(unsigned long long), s));
     const unsigned g = (unsigned)S.ntiles;
+    if (r > 1)
+      boruvka_prev_seed_kernel<<<gn, 256, 0, s>>>(n, comp.p, S.perm.p, inv.p, bw.p, ba.p, bb.p,
+                                                  comp_w.p);
     if (dim == 8)
       boruvka_seed_kernel<8><<<g, 64, 0, s>>>(S.es.p, core.p, comp.p, n, dim, S.ntiles, comp_w.p);
     else
