@@ -55,11 +55,12 @@ def make_workload(n, d, n_centres=200, zipf=1.1, sd=0.15, noise_frac=0.05, seed=
     return x, labels
 
 
-def read_traffic():
+def read_traffic(symmetric):
     """HBM bytes per launch of the kNN tile kernel from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, KiB per dispatch, scaled by the copy-kernel
     calibration factor recorded next to it), or None when no summary exists."""
-    path = os.path.join(REPO, "profiles", "knn_topc_rb_pmc_1Mx512.txt")
+    name = "knn_topc_rb_sym_pmc_1Mx512.txt" if symmetric else "knn_topc_rb_pmc_1Mx512.txt"
+    path = os.path.join(REPO, "profiles", name)
     if not os.path.exists(path):
         return None
     vals = {}
@@ -145,22 +146,28 @@ def main():
         print(f"# warmup {w}: {r.stage_ms}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    results = []
+    results, kstats = [], []
     for s in range(args.steps):
         r = H.evoc_device(ptr, n, d, params=params, nn_params=nnp, seed=args.seed)
         results.append(r)
+        kstats.append(H.knn_last_stats())
         print(f"# step {s}: {r.stage_ms}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = n * args.steps / elapsed
     last = results[-1]
-    knn_ms = sum(r.knn_kernel_ms for r in results) / len(results)
-    flops = 2.0 * n * n * d
+    # the dominant kernel: the kNN tile kernel, HIP-event timed on its stream; its
+    # algorithmic work is the bf16 MFMA products it executes (2 d flop per pair swept;
+    # the symmetric sweep covers each unordered pair of 256-row blocks once)
+    knn_ms = sum(st["kernel_ms"] for st in kstats) / len(kstats)
+    flops = kstats[-1]["kernel_flop"]
+    sym = kstats[-1]["symmetric"]
     achieved = flops / (knn_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 4),
-                "traffic": read_traffic(), "kernel": "knn_topc_rb_kernel",
-                "kernel_ms": round(knn_ms, 3)}
+                "traffic": read_traffic(sym), "kernel": "knn_topc_rb_kernel"
+                + (" (symmetric sweep)" if sym else ""),
+                "kernel_ms": round(knn_ms, 3), "kernel_flop": flops}
     cpu = None
     if not args.no_cpu_baseline:
         print("# cpu baseline", file=sys.stderr, flush=True)

@@ -103,7 +103,10 @@ struct KnnStats {
   int candidates = 0;
   float mfma_ms = 0.0f;
   double flops = 0.0;
+  int symmetric = 0;  // the symmetric sweep ran
 };
+// the calling thread's last knn_device call
+KnnStats& last_knn_stats();
 void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k, int precision,
                 int32_t* d_idx, float* d_dist, KnnStats* stats, hipStream_t s, DevPool& pool);
 

@@ -10,6 +10,8 @@
 //                 (64 strided lane sums + halving tree), sort by (dist, idx), and a
 //                 per-row certificate that no non-candidate can enter the top k.
 //  4. knn_exact:  brute-force exact rows for the (rare) rows whose certificate fails.
+#include <cstdlib>
+
 #include "kernels.h"
 
 namespace evoc {
@@ -763,7 +765,27 @@ constexpr int RB_DRAIN = 8;    // drain once some (row, half) holds this many
 constexpr int RB_BGRP = 4;     // B fragments read from LDS ahead of their MFMAs
 constexpr int RB_WREG = 64 * RB_BKT * 8;                // per-wavefront buckets / key tile
 static_assert(RB_WREG >= 32 * RA_SCS * 4, "the key-tile dump must fit the bucket region");
-constexpr int RB_EPI_BYTES = RB_WAVES * RB_WREG + RB_WAVES * 64 * 4;
+constexpr int RB_EPI_BYTES = RB_WAVES * RB_WREG + RB_WAVES * 64 * 4 + RB_WAVES * 32 * 4;
+
+// The column side of one launch. The plain sweep sets X = the query rows, self = 1 and
+// covers every column tile. The symmetric sweep covers each unordered pair of 256-row
+// blocks once: block I takes the cyclic range I, I+1, ... of half the blocks, keeps the
+// forward keys (rows of I) in its lists and queues the transposed keys (rows of the
+// other block) that fall below that row's threshold thr0, one queue per wavefront. The
+// threshold pass runs the plain sweep against a sample of rows (self = 0).
+struct RbCols {
+  const uint16_t* X;    // column rows, bf16, dpad wide, ntile * 32 of them (zero padded)
+  const float* scale;   // their key scales
+  const float* thr0;    // symmetric sweep: reverse-candidate threshold of every row
+  int64_t ncol;         // valid columns
+  int64_t ntile;        // 32-column tiles
+  int self;             // columns are the query rows: mask the diagonal
+  int32_t* qj;          // symmetric sweep: queued reverse candidates (row, column, key)
+  int32_t* qr;
+  float* qk;
+  int64_t qcap;         // entries per wavefront queue
+  int32_t* qcount;      // entries written per wavefront
+};
 
 template <int KC>
 struct RbRing {
@@ -775,12 +797,17 @@ struct RbRing {
   static constexpr size_t SMEM = (size_t)NST * STB;
   static __device__ __forceinline__ int slot(int c, int q) { return c * PIECES + (q ^ (c & 15)); }
 };
+// symmetric sweep: each wavefront stages its reverse candidates (column, row, key) in LDS
+// and copies them to its global queue in batches, with no vector-memory operation of the
+// ring in flight (stores mixed with the ring's loads would void its counted waits)
+constexpr int RB_QL = 192;                                      // LDS queue entries per wavefront
+constexpr size_t RB_QBYTES = (size_t)RB_WAVES * RB_QL * 12;     // after the ring
 
-template <int NKC, int KC, bool COS>
+template <int NKC, int KC, bool COS, bool SYM>
 __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     const uint16_t* __restrict__ Xh, int64_t n, int64_t npad, int dpad,
     const float* __restrict__ colscale, int32_t* __restrict__ cand_idx,
-    float* __restrict__ cand_key) {
+    float* __restrict__ cand_key, const RbCols cols) {
   constexpr int C = 16;
   using R = RbRing<KC>;
   constexpr int NST = R::NST;
@@ -792,10 +819,46 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   float* sc = reinterpret_cast<float*>(epi + wave * RB_WREG);
   int2* bucket = reinterpret_cast<int2*>(sc);  // [64 owners][RB_BKT]
   float* thr = reinterpret_cast<float*>(epi + RB_WAVES * RB_WREG) + wave * 64;  // [owner]
+  // key scales of this wavefront's 32 rows (the transposed keys of the symmetric sweep)
+  float* rsc = reinterpret_cast<float*>(epi + RB_WAVES * RB_WREG + RB_WAVES * 64 * 4) + wave * 32;
 
   const int64_t row0 = (int64_t)blockIdx.x * RB_ROWS;
-  const int64_t ntile = npad / RA_BN;
-  const int64_t total = ntile * NKC;
+  const int64_t ntile = cols.ntile;
+  int64_t tile0 = 0, nt = ntile;
+  if (SYM) {
+    const int64_t m = npad / RB_ROWS;  // 256-row blocks; ntile == 8 m
+    const int64_t I = blockIdx.x;
+    const int64_t nblk = (m & 1) ? (m + 1) / 2 : m / 2 + (I < m / 2 ? 1 : 0);
+    tile0 = I * (RB_ROWS / RA_BN);
+    nt = nblk * (RB_ROWS / RA_BN);
+  }
+  const int64_t total = nt * NKC;
+  int qn = 0;  // this wavefront's reverse candidates in its global queue
+  int lq = 0;  // and staged in LDS
+  const int64_t qbase = SYM ? ((int64_t)blockIdx.x * RB_WAVES + wave) * cols.qcap : 0;
+  int32_t* const qjw = SYM ? cols.qj + qbase : nullptr;
+  int32_t* const qrw = SYM ? cols.qr + qbase : nullptr;
+  float* const qkw = SYM ? cols.qk + qbase : nullptr;
+  const int qcap = (int)std::min<int64_t>(cols.qcap, INT32_MAX / 2);
+  // LDS queue: [RB_QL] columns, [RB_QL] rows, [RB_QL] keys per wavefront
+  const unsigned lqa = lds_addr(smem + R::SMEM + (size_t)wave * RB_QL * 12);
+  auto flush = [&]() {
+    wait_vm<0>();
+    for (int e = lane; e < lq; e += 64) {
+      const int cj = lds_ld_b32(lqa + 4u * e);
+      const int cr = lds_ld_b32(lqa + 4u * (RB_QL + e));
+      const int ck = lds_ld_b32(lqa + 4u * (2 * RB_QL + e));
+      const int pos = qn + e;
+      if (pos < qcap) {
+        qjw[pos] = cj;
+        qrw[pos] = cr;
+        qkw[pos] = __int_as_float(ck);
+      }
+    }
+    wait_vm<0>();
+    qn += lq;
+    lq = 0;
+  };
 
   bf16x8 afr[NKC * KC / 16];
   {
@@ -812,6 +875,10 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     li[q] = -1;
   }
   thr[lane] = FINF;
+  if (SYM && lane < 32) {
+    const int64_t r = row0 + wave * 32 + lane;
+    rsc[lane] = r < n ? colscale[r] : 0.0f;
+  }
   int bcnt = 0;
 
   const uint16_t* pb[LOADS];
@@ -819,12 +886,15 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   for (int h = 0; h < LOADS; ++h) {
     const int s = (h * RB_WAVES + wave) * 64 + lane;
     const int c = s / R::PIECES, qs = s % R::PIECES;
-    pb[h] = Xh + (int64_t)c * dpad + (qs ^ (c & 15)) * 8;
+    pb[h] = cols.X + (tile0 * RA_BN + c) * dpad + (qs ^ (c & 15)) * 8;
   }
-  const float* pc = colscale + (lane & 31);
-  int64_t gi = 0;
+  // per tile, lanes 0-31 bring the 32 column scales and (symmetric sweep) lanes 32-63 the
+  // columns' reverse thresholds into the ring
+  const float* pc = ((SYM && lane >= 32) ? cols.thr0 : cols.scale) + tile0 * RA_BN + (lane & 31);
+  int64_t gi = 0, tnext = tile0;
   int kci = 0, sloti = 0;
   const int64_t tile_step = (int64_t)RA_BN * dpad - (int64_t)(NKC - 1) * KC;
+  const int64_t wrap_step = (int64_t)ntile * RA_BN * dpad;
   auto issue_next = [&]() {
     char* st = ring + sloti * R::STB;
 #pragma unroll
@@ -842,6 +912,12 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
 #pragma unroll
         for (int h = 0; h < LOADS; ++h) pb[h] += tile_step;
         pc += RA_BN;
+        if (SYM && ++tnext == ntile) {  // the cyclic range wraps to column 0
+          tnext = 0;
+#pragma unroll
+          for (int h = 0; h < LOADS; ++h) pb[h] -= wrap_step;
+          pc -= ntile * RA_BN;
+        }
       } else {
 #pragma unroll
         for (int h = 0; h < LOADS; ++h) pb[h] += KC;
@@ -854,7 +930,7 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   const int lcol = lane & 31, half = lcol >> 4, grp = lane >> 4;
   const int lrow = (lane >> 5) * 4;
   for (int g = 0; g < NST - 1; ++g) issue_next();
-  for (int64_t ct = 0; ct < ntile; ++ct) {
+  for (int64_t ct = 0; ct < nt; ++ct) {
     f32x16 acc = (f32x16){};
 #pragma unroll
     for (int kc = 0; kc < NKC; ++kc) {
@@ -883,14 +959,59 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     }
 
     // ---------------- wavefront-private epilogue of column tile ct
-    const bool diag = (ct * RA_BN < row0 + RB_ROWS) && (ct * RA_BN + RA_BN > row0);
-    const bool tail = (ct + 1) * RA_BN > n;
+    const int64_t cta = SYM ? (tile0 + ct >= ntile ? tile0 + ct - ntile : tile0 + ct) : ct;
+    const bool diag = cols.self && (cta * RA_BN < row0 + RB_ROWS) && (cta * RA_BN + RA_BN > row0);
+    const bool tail = (cta + 1) * RA_BN > cols.ncol;
+    if (SYM && (cta >> 3) != (int64_t)blockIdx.x) {
+      // transposed keys: column cta*32 + lcol is a row too; its candidates are this
+      // wavefront's rows, keyed with the rows' own scales
+      float t0c;
+      {
+        const unsigned ca = lds_addr(ring + (int)((ct * NKC + NKC - 1) & (NST - 1)) * R::STB +
+                                     R::STB_B + wave * 256 + (32 + lcol) * 4);
+        asm volatile("ds_read_b32 %0, %1\ns_waitcnt lgkmcnt(0)" : "=v"(t0c) : "v"(ca) : "memory");
+      }
+      const int cabs = (int)(cta * RA_BN) + lcol;
+      if (!((int64_t)cabs < n)) t0c = -FINF;
+      const int rbase = (int)(row0 + wave * 32) + lrow;
+      const int nrow = (int)std::min<int64_t>(n, INT32_MAX);
+#pragma unroll 1
+      for (int j = 0; j < 4; ++j) {
+        f32x4 rs;
+        {
+          const unsigned ra_ = lds_addr(rsc + lrow + j * 8);
+          asm volatile("ds_read_b128 %0, %1\ns_waitcnt lgkmcnt(0)" : "=v"(rs) : "v"(ra_) : "memory");
+        }
+        float sj[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sj[i] = acc[j * 4 + i];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float rk = COS ? -(sj[i] * rs[i]) : (rs[i] - 2.0f * sj[i]);
+          const int rr = rbase + j * 8 + i;
+          const bool rp = (rk < t0c) & (rr < nrow);
+          const unsigned long long rm = __ballot(rp);
+          if (rm) {
+            if (lq > RB_QL - 64) flush();
+            const int below = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(rm >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((unsigned)rm, 0u));
+            const unsigned e = (unsigned)(lq + below);
+            if (rp) {
+              lds_st_b32(lqa + 4u * e, cabs);
+              lds_st_b32(lqa + 4u * (RB_QL + e), rr);
+              lds_st_b32(lqa + 4u * (2 * RB_QL + e), __float_as_int(rk));
+            }
+            lq += __popcll(rm);
+          }
+        }
+      }
+    }
     f32x4 tq[4];  // thresholds of this lane's 16 rows in its column half
     float cs;
     {
       const unsigned ta = lds_addr(thr + 32 * half + lrow);
       const unsigned ca = lds_addr(ring + (int)((ct * NKC + NKC - 1) & (NST - 1)) * R::STB +
-                                   R::STB_B + wave * 256 + lane * 4);
+                                   R::STB_B + wave * 256 + lcol * 4);
       asm volatile(
           "ds_read_b128 %0, %5\n"
           "ds_read_b128 %1, %5 offset:32\n"
@@ -902,8 +1023,8 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
           : "v"(ta), "v"(ca)
           : "memory");
     }
-    int dv = (int)(ct * RA_BN - row0) - wave * 32 + lcol - lrow;
-    int rem = (int)std::min<int64_t>(n - ct * RA_BN, RA_BN) - lcol;
+    int dv = (int)(cta * RA_BN - row0) - wave * 32 + lcol - lrow;
+    int rem = (int)std::min<int64_t>(cols.ncol - cta * RA_BN, RA_BN) - lcol;
     int sb = lrow * RA_SCS + lcol;
     asm volatile("" : "+v"(dv), "+v"(rem), "+v"(sb));
 #pragma unroll
@@ -916,7 +1037,7 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
       for (int e = 0; e < 16; ++e)
         if (rem <= 0 || dv == (e >> 2) * 8 + (e & 3)) acc[e] = FINF;
     }
-    const int cbase = (int)(ct * RA_BN);
+    const int cbase = (int)(cta * RA_BN);
     bool touched = false, overflow = false;
     int inc = 0;
     // owner of this lane's (row, half) group in slot (j, i): row j*8+i (+4 for lanes >= 32),
@@ -1005,6 +1126,10 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     }
   }
   wait_vm<0>();
+  if (SYM) {
+    if (lq > 0) flush();
+    if (lane == 0) cols.qcount[(int64_t)blockIdx.x * RB_WAVES + wave] = qn;
+  }
   // list of (row, half) -> candidates (row * 2 + half) * 16
   const int64_t grow = row0 + wave * 32 + (lane & 31);
   if (grow < n) {
@@ -1016,10 +1141,132 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   }
 }
 
-template __global__ void knn_topc_rb_kernel<2, 256, true>(const uint16_t*, int64_t, int64_t, int,
-                                                          const float*, int32_t*, float*);
-template __global__ void knn_topc_rb_kernel<2, 256, false>(const uint16_t*, int64_t, int64_t, int,
-                                                           const float*, int32_t*, float*);
+template __global__ void knn_topc_rb_kernel<2, 256, true, false>(const uint16_t*, int64_t, int64_t,
+                                                                 int, const float*, int32_t*,
+                                                                 float*, const RbCols);
+template __global__ void knn_topc_rb_kernel<2, 256, false, false>(const uint16_t*, int64_t, int64_t,
+                                                                  int, const float*, int32_t*,
+                                                                  float*, const RbCols);
+template __global__ void knn_topc_rb_kernel<2, 256, true, true>(const uint16_t*, int64_t, int64_t,
+                                                                int, const float*, int32_t*,
+                                                                float*, const RbCols);
+template __global__ void knn_topc_rb_kernel<2, 256, false, true>(const uint16_t*, int64_t, int64_t,
+                                                                 int, const float*, int32_t*,
+                                                                 float*, const RbCols);
+
+// ------------------------------------------------------------------ 2d. symmetric sweep
+constexpr int64_t KNN_SYM_MIN_N = 65536;  // automatic use of the symmetric sweep from this n
+// The sample of the threshold pass: rows q * n / S, zero padded to a multiple of 32.
+__global__ void rb_sample_kernel(const uint16_t* __restrict__ Xh, const float* __restrict__ scale,
+                                 int64_t n, int dpad, int64_t S, int64_t Spad,
+                                 uint16_t* __restrict__ Xs, float* __restrict__ ss) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = dpad / 8;  // 16-byte pieces per row
+  if (t >= Spad * per) return;
+  const int64_t q = t / per;
+  const int c = (int)(t % per);
+  const int64_t src = q < S ? q * n / S : -1;
+  int4 v = make_int4(0, 0, 0, 0);
+  if (src >= 0) v = reinterpret_cast<const int4*>(Xh + src * dpad)[c];
+  reinterpret_cast<int4*>(Xs + q * dpad)[c] = v;
+  if (c == 0) ss[q] = src >= 0 ? scale[src] : 0.0f;
+}
+// a row's reverse threshold: the bound of its two half-lists against the sample
+__global__ void rb_thr0_kernel(const float* __restrict__ ckey, int64_t n, int64_t npad,
+                               float* __restrict__ thr0) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= npad) return;
+  thr0[r] = r < n ? fminf(ckey[(r * 2) * 16 + 15], ckey[(r * 2 + 1) * 16 + 15]) : -FINF;
+}
+// queued reverse candidates into per-row buckets (order within a bucket is free: the
+// merge ranks by (key, index))
+__global__ __launch_bounds__(256) void rb_scatter_kernel(const int32_t* __restrict__ qj,
+                                                         const int32_t* __restrict__ qr,
+                                                         const float* __restrict__ qk,
+                                                         const int32_t* __restrict__ qcount,
+                                                         int64_t qcap, int rcap,
+                                                         int32_t* __restrict__ rcnt,
+                                                         float* __restrict__ rkey,
+                                                         int32_t* __restrict__ ridx) {
+  const int64_t w = blockIdx.x;
+  const int64_t cnt = min<int64_t>(qcount[w], qcap);
+  for (int64_t e = threadIdx.x; e < cnt; e += 256) {
+    const int64_t q = w * qcap + e;
+    const int32_t j = qj[q];
+    const int slot = atomicAdd(&rcnt[j], 1);
+    if (slot < rcap) {
+      rkey[(int64_t)j * rcap + slot] = qk[q];
+      ridx[(int64_t)j * rcap + slot] = qr[q];
+    }
+  }
+}
+// Per row: the 32 forward candidates (two half-lists) and its reverse bucket ranked by
+// (key, index); the 32 best are kept, sorted. Every column not kept has a key >= the
+// returned bound: forward non-candidates >= their half's last key, reverse keys never
+// queued >= thr0, entries ranked out >= the 32nd kept. A full bucket voids the bound.
+constexpr int RB_RCAP = 768;
+__global__ __launch_bounds__(256) void rb_merge_kernel(
+    const float* __restrict__ ckey, const int32_t* __restrict__ cidx, const int32_t* __restrict__ rcnt,
+    const float* __restrict__ rkey, const int32_t* __restrict__ ridx, const float* __restrict__ thr0,
+    int64_t n, float* __restrict__ mkey, int32_t* __restrict__ midx, float* __restrict__ bound) {
+  __shared__ float sk[4][32 + RB_RCAP];
+  __shared__ int32_t si[4][32 + RB_RCAP];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + wave;
+  if (row >= n) return;
+  const int cnt = rcnt[row];
+  const int m = min(cnt, RB_RCAP);
+  const int K = 32 + m;
+  for (int e = lane; e < K; e += 64) {
+    float key;
+    int32_t idx;
+    if (e < 32) {
+      key = ckey[row * 32 + e];
+      idx = cidx[row * 32 + e];
+    } else {
+      key = rkey[row * RB_RCAP + (e - 32)];
+      idx = ridx[row * RB_RCAP + (e - 32)];
+    }
+    if (idx < 0) key = FINF;
+    sk[wave][e] = key;
+    si[wave][e] = idx;
+  }
+  __builtin_amdgcn_wave_barrier();
+  __threadfence_block();
+  int nvalid = 0;
+  for (int e = lane; e < K; e += 64) nvalid += si[wave][e] >= 0 ? 1 : 0;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) nvalid += __shfl_xor(nvalid, o, 64);
+  float key31 = FINF;  // the 32nd kept key
+  for (int e = lane; e < K; e += 64) {
+    const float key = sk[wave][e];
+    const int32_t idx = si[wave][e];
+    if (idx < 0) continue;
+    int rank = 0;
+    for (int f = 0; f < K; ++f) {
+      const float k2 = sk[wave][f];
+      const int32_t i2 = si[wave][f];
+      rank += (i2 >= 0 && (k2 < key || (k2 == key && i2 < idx))) ? 1 : 0;
+    }
+    if (rank < 32) {
+      mkey[row * 32 + rank] = key;
+      midx[row * 32 + rank] = idx;
+    }
+    if (rank == 31) key31 = key;
+  }
+  for (int q = nvalid + lane; q < 32; q += 64) {
+    mkey[row * 32 + q] = FINF;
+    midx[row * 32 + q] = -1;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) key31 = fminf(key31, __shfl_xor(key31, o, 64));
+  if (lane == 0) {
+    float b = fminf(fminf(ckey[(row * 2) * 16 + 15], ckey[(row * 2 + 1) * 16 + 15]), thr0[row]);
+    if (nvalid > 32) b = fminf(b, key31);
+    if (cnt > RB_RCAP) b = -FINF;
+    bound[row] = b;
+  }
+}
 
 // ------------------------------------------------------------------ 3. re-rank
 constexpr int MAXT = 16;  // d <= 1024 handled with the query row held in registers
@@ -1028,9 +1275,9 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     const float* __restrict__ X, int64_t n, int d, int64_t ld, const float* __restrict__ nrm,
     const float* __restrict__ sqn, const float* __restrict__ colscale,
     const unsigned* __restrict__ flags, const int32_t* __restrict__ cand_idx,
-    const float* __restrict__ cand_key, int C, int halves, int k, int metric, float gamma,
-    int32_t* __restrict__ out_idx, float* __restrict__ out_dist, int* __restrict__ fail_count,
-    int32_t* __restrict__ fail_rows) {
+    const float* __restrict__ cand_key, int C, int halves, const float* __restrict__ row_bound,
+    int k, int metric, float gamma, int32_t* __restrict__ out_idx, float* __restrict__ out_dist,
+    int* __restrict__ fail_count, int32_t* __restrict__ fail_rows) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t row = (int64_t)blockIdx.x * 4 + wave;
   if (row >= n) return;
@@ -1139,9 +1386,11 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     const int src = __ffsll((long long)kb) - 1;
     const float dk = __shfl(my_d, src, 64);
     const float dk2 = __shfl(my_sq, src, 64);
-    // one sorted list of C keys, or two column halves that each kept their C/2 best
-    const float kc = halves == 2 ? fminf(cand_key[row * C + (C / 2 - 1)], cand_key[row * C + (C - 1)])
-                                 : cand_key[row * C + (C - 1)];
+    // one sorted list of C keys, or two column halves that each kept their C/2 best, or
+    // (symmetric sweep) the bound its merge computed
+    const float kc = row_bound ? row_bound[row]
+                     : halves == 2 ? fminf(cand_key[row * C + (C / 2 - 1)], cand_key[row * C + (C - 1)])
+                                   : cand_key[row * C + (C - 1)];
     if (metric == 1) {
       const float lb = 1.0f + kc * colscale[row];
       pass = (lb - eps) > dk;
@@ -1334,10 +1583,16 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<256>::SMEM));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 2, 256, true>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<256>::SMEM));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, true>,
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, true, false>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<256>::SMEM));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, false>,
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, false, false>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<256>::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, true, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(RbRing<256>::SMEM + RB_QBYTES)));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, false, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(RbRing<256>::SMEM + RB_QBYTES)));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_exact_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
     attr_set = true;
@@ -1396,6 +1651,35 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   const float u = 5.9604645e-08f;
   const float e_in = split ? 4.0f * 1.5258789e-05f : (hflags[0] ? 2.0f * 0.0078125f : 0.0f);
   const float gamma = 2.0f * (float)dpad * u + e_in;
+  // The symmetric sweep (each pair of row blocks once) for the d in (256, 512] kernel at
+  // large n; EVOC_HIP_KNN_SYMMETRIC=1 / 0 forces it on / off.
+  bool sym = false;
+  if (ra && dpad == 512) {
+    const char* env = std::getenv("EVOC_HIP_KNN_SYMMETRIC");
+    sym = env ? (env[0] == '1') : (n >= KNN_SYM_MIN_N);
+  }
+  int64_t sym_S = 0, sym_Spad = 0, sym_qcap = 0;
+  Buf<uint16_t> Xs;
+  Buf<float> ss, thr0, qk;
+  Buf<int32_t> qj, qr, qcount;
+  if (sym) {
+    sym_S = std::min<int64_t>(65536, std::max<int64_t>(256, n / 16 / 256 * 256));
+    sym_S = std::min<int64_t>(sym_S, n);
+    sym_Spad = ceil_div(sym_S, RA_BN) * RA_BN;
+    Xs.alloc(pool, sym_Spad * dpad);
+    ss.alloc(pool, sym_Spad);
+    thr0.alloc(pool, npad);
+    rb_sample_kernel<<<(unsigned)ceil_div(sym_Spad * (dpad / 8), 256), 256, 0, s>>>(
+        Xh.p, colscale.p, n, dpad, sym_S, sym_Spad, Xs.p, ss.p);
+    // about 32 rows x n/2 columns x 32/S reverse keys per wavefront; three times that
+    const int64_t expect = 512 * ceil_div(n, sym_S);
+    sym_qcap = std::max<int64_t>(4096, ceil_div(3 * expect, 256) * 256);
+    const int64_t nwaves = (npad / RB_ROWS) * RB_WAVES;
+    qj.alloc(pool, nwaves * sym_qcap);
+    qr.alloc(pool, nwaves * sym_qcap);
+    qk.alloc(pool, nwaves * sym_qcap);
+    qcount.alloc(pool, nwaves);
+  }
   hipEvent_t e0, e1;
   EVOC_HIP(hipEventCreate(&e0));
   EVOC_HIP(hipEventCreate(&e1));
@@ -1416,12 +1700,40 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     } else if (dpad == 256) {
       EVOC_RA_LAUNCH(1, 256);
     } else {  // d in (256, 512]: eight 32-row wavefronts, two per SIMD
-      if (metric == 1)
-        knn_topc_rb_kernel<2, 256, true><<<grid, 512, RbRing<256>::SMEM, s>>>(
-            Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);
-      else
-        knn_topc_rb_kernel<2, 256, false><<<grid, 512, RbRing<256>::SMEM, s>>>(
-            Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);
+      RbCols cols{Xh.p, colscale.p, nullptr, n, npad / RA_BN, 1, nullptr, nullptr, nullptr, 0,
+                  nullptr};
+      if (sym) {
+        // threshold pass against a sample of rows, then the symmetric sweep
+        RbCols sc{Xs.p, ss.p, nullptr, sym_S, sym_Spad / RA_BN, 0, nullptr, nullptr, nullptr, 0,
+                  nullptr};
+        if (metric == 1)
+          knn_topc_rb_kernel<2, 256, true, false><<<grid, 512, RbRing<256>::SMEM, s>>>(
+              Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, sc);
+        else
+          knn_topc_rb_kernel<2, 256, false, false><<<grid, 512, RbRing<256>::SMEM, s>>>(
+              Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, sc);
+        rb_thr0_kernel<<<(unsigned)ceil_div(npad, 256), 256, 0, s>>>(ckey.p, n, npad, thr0.p);
+        EVOC_HIP(hipEventRecord(e0, s));
+        cols.thr0 = thr0.p;
+        cols.qj = qj.p;
+        cols.qr = qr.p;
+        cols.qk = qk.p;
+        cols.qcap = sym_qcap;
+        cols.qcount = qcount.p;
+        if (metric == 1)
+          knn_topc_rb_kernel<2, 256, true, true><<<grid, 512, RbRing<256>::SMEM + RB_QBYTES, s>>>(
+              Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
+        else
+          knn_topc_rb_kernel<2, 256, false, true><<<grid, 512, RbRing<256>::SMEM + RB_QBYTES, s>>>(
+              Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
+      } else {
+        if (metric == 1)
+          knn_topc_rb_kernel<2, 256, true, false><<<grid, 512, RbRing<256>::SMEM, s>>>(
+              Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
+        else
+          knn_topc_rb_kernel<2, 256, false, false><<<grid, 512, RbRing<256>::SMEM, s>>>(
+              Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
+      }
     }
 #undef EVOC_RA_LAUNCH
   } else {
@@ -1448,9 +1760,45 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   }
   EVOC_HIP(hipGetLastError());
   EVOC_HIP(hipEventRecord(e1, s));
+  Buf<float> mkey, bnd;
+  Buf<int32_t> midx;
+  if (sym) {
+    const int64_t nwaves = (npad / RB_ROWS) * RB_WAVES;
+    std::vector<int32_t> hq(nwaves);
+    EVOC_HIP(hipMemcpyAsync(hq.data(), qcount.p, nwaves * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipStreamSynchronize(s));
+    int64_t qmax = 0;
+    for (auto q : hq) qmax = std::max<int64_t>(qmax, q);
+    if (qmax > sym_qcap) {  // a queue overflowed: sweep every pair instead
+      sym = false;
+      RbCols cols{Xh.p, colscale.p, nullptr, n, npad / RA_BN, 1, nullptr, nullptr, nullptr, 0,
+                  nullptr};
+      const unsigned grid = (unsigned)(npad / RA_ROWS);
+      if (metric == 1)
+        knn_topc_rb_kernel<2, 256, true, false><<<grid, 512, RbRing<256>::SMEM, s>>>(
+            Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
+      else
+        knn_topc_rb_kernel<2, 256, false, false><<<grid, 512, RbRing<256>::SMEM, s>>>(
+            Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
+      EVOC_HIP(hipGetLastError());
+    } else {
+      Buf<int32_t> rcnt(pool, n), ridx(pool, n * RB_RCAP);
+      Buf<float> rkey(pool, n * RB_RCAP);
+      EVOC_HIP(hipMemsetAsync(rcnt.p, 0, n * sizeof(int32_t), s));
+      rb_scatter_kernel<<<(unsigned)nwaves, 256, 0, s>>>(qj.p, qr.p, qk.p, qcount.p, sym_qcap,
+                                                         RB_RCAP, rcnt.p, rkey.p, ridx.p);
+      mkey.alloc(pool, n * 32);
+      midx.alloc(pool, n * 32);
+      bnd.alloc(pool, n);
+      rb_merge_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(ckey.p, cidx.p, rcnt.p, rkey.p,
+                                                              ridx.p, thr0.p, n, mkey.p, midx.p,
+                                                              bnd.p);
+      EVOC_HIP(hipGetLastError());
+    }
+  }
   knn_rerank_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(
-      dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, cidx.p, ckey.p, C, (ra && dpad < 512) ? 1 : 2,
-      k, metric,
+      dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, sym ? midx.p : cidx.p,
+      sym ? mkey.p : ckey.p, C, (ra && dpad < 512) ? 1 : 2, sym ? bnd.p : nullptr, k, metric,
       gamma, d_idx, d_dist, fcount.p, frows.p);
   EVOC_HIP(hipGetLastError());
   int nfail = 0;
@@ -1478,13 +1826,27 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   EVOC_HIP(hipEventElapsedTime(&mfma_ms, e0, e1));
   EVOC_HIP(hipEventDestroy(e0));
   EVOC_HIP(hipEventDestroy(e1));
-  if (stats) {
-    stats->split = split ? 1 : 0;
-    stats->fallback_rows = nfail;
-    stats->candidates = C;
-    stats->mfma_ms = mfma_ms;
-    stats->flops = 2.0 * (double)npad * (double)npad * (double)dpad * (split ? 3.0 : 1.0);
+  KnnStats st;
+  st.split = split ? 1 : 0;
+  st.fallback_rows = nfail;
+  st.candidates = C;
+  st.mfma_ms = mfma_ms;
+  double pairs = (double)npad * (double)npad;
+  if (sym) {  // block I swept nblk(I) blocks of 256 columns
+    const int64_t m = npad / RB_ROWS;
+    int64_t blocks = 0;
+    for (int64_t I = 0; I < m; ++I) blocks += (m & 1) ? (m + 1) / 2 : m / 2 + (I < m / 2 ? 1 : 0);
+    pairs = (double)blocks * RB_ROWS * RB_ROWS;
   }
+  st.flops = 2.0 * pairs * (double)dpad * (split ? 3.0 : 1.0);
+  st.symmetric = sym ? 1 : 0;
+  last_knn_stats() = st;
+  if (stats) *stats = st;
+}
+
+KnnStats& last_knn_stats() {
+  static thread_local KnnStats st;
+  return st;
 }
 
 }  // namespace evoc

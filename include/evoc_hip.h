@@ -111,6 +111,13 @@ int evoc_hip_knn(const float* x, int64_t n, int64_t d, int32_t metric, int32_t k
                  int32_t precision, int32_t* idx_out, float* dist_out, int64_t* fallback_rows,
                  double* kernel_ms);
 
+/* Diagnostics of the calling thread's last kNN search (evoc_hip_knn or the kNN stage of
+ * evoc_hip_run / evoc_hip_run_device): the MFMA flop the tile kernel executed, its
+ * hipEvent time in ms, and whether the symmetric sweep (each pair of 256-row blocks once,
+ * transposed keys queued for the other block) ran. No counterpart in the reference: it
+ * serves roofline accounting and tests. */
+int evoc_hip_knn_last_stats(double* kernel_flop, double* kernel_ms, int32_t* symmetric);
+
 /* build_fuzzy_simplicial_set (src/graph/fuzzy_graph.rs:275-295) as CSR. indices and
  * weights need room for 2*n*k entries; *nnz receives the stored count. */
 int evoc_hip_fuzzy(const int32_t* knn_idx, const float* knn_dist, int64_t n, int32_t k,

@@ -101,3 +101,46 @@ def test_knn_exact_fallback_many_rows(bf16, d):
     for metric in ("euclidean", "cosine"):
         st = _check(x, 15, metric)
         assert st["fallback_rows"] >= 40, st
+
+
+@pytest.mark.parametrize("n,d,metric", [
+    (3000, 300, "cosine"),      # 12 row blocks: even count, the middle pair taken once
+    (2800, 512, "euclidean"),   # 11 row blocks: odd count
+    (777, 400, "cosine"),       # 4 blocks, ragged last block
+    (513, 300, "euclidean"),    # 3 blocks, one row in the last
+    (250, 300, "cosine"),       # one block: the diagonal only
+])
+def test_knn_symmetric_sweep(n, d, metric, monkeypatch):
+    """The symmetric sweep (each pair of 256-row blocks once, transposed keys queued for
+    the other block's rows, merged and certified) gives the oracle's exact kNN."""
+    monkeypatch.setenv("EVOC_HIP_KNN_SYMMETRIC", "1")
+    x, _ = datagen.gaussian_mixture(n, d, 12, 4.0, 1.0, seed=n + d)
+    x = datagen.to_bf16_values(x)
+    st = _check(x, 15, metric)
+    assert st["symmetric"], st
+    _check(x, 7, metric, precision="bf16")
+
+
+def test_knn_symmetric_sweep_ties_zero_rows(monkeypatch):
+    monkeypatch.setenv("EVOC_HIP_KNN_SYMMETRIC", "1")
+    rng = np.random.default_rng(23)
+    x = datagen.to_bf16_values(rng.normal(size=(1700, 320)).astype(np.float32))
+    x[900:960] = x[5]           # 61 copies: certificates fail, exact rows decide
+    x[1200:1210] = 0.0          # zero rows: cosine distance 1 to everything
+    for metric in ("cosine", "euclidean"):
+        st = _check(x, 15, metric)
+        assert st["symmetric"] and st["fallback_rows"] >= 60, st
+
+
+def test_knn_symmetric_equals_plain_at_300k(monkeypatch):
+    """At a size the oracle cannot check quickly: the symmetric sweep and the plain sweep
+    (each bit-exact against the oracle at the test sizes) return identical graphs."""
+    x, _ = datagen.unit_mixture(300_000, 320, 60, 0.2, seed=8, bf16=True)
+    monkeypatch.setenv("EVOC_HIP_KNN_SYMMETRIC", "0")
+    pi, pd, ps = H.knn(x, 15, metric="cosine", return_stats=True)
+    monkeypatch.setenv("EVOC_HIP_KNN_SYMMETRIC", "1")
+    si, sd, ss = H.knn(x, 15, metric="cosine", return_stats=True)
+    assert not ps["symmetric"] and ss["symmetric"]
+    assert ss["kernel_flop"] < 0.51 * ps["kernel_flop"]
+    assert np.array_equal(pi, si)
+    assert np.array_equal(pd.view(np.uint32), sd.view(np.uint32))
