@@ -780,11 +780,12 @@ struct RbCols {
   int64_t ncol;         // valid columns
   int64_t ntile;        // 32-column tiles
   int self;             // columns are the query rows: mask the diagonal
-  int32_t* qj;          // symmetric sweep: queued reverse candidates (row, column, key)
+  int32_t* qj;          // symmetric sweep: queued reverse candidates (row j, column r, dot)
   int32_t* qr;
   float* qk;
   int64_t qcap;         // entries per wavefront queue
   int32_t* qcount;      // entries written per wavefront
+  int64_t group;        // workgroups resident at once (one per CU): walks start in step
 };
 
 template <int KC>
@@ -797,7 +798,7 @@ struct RbRing {
   static constexpr size_t SMEM = (size_t)NST * STB;
   static __device__ __forceinline__ int slot(int c, int q) { return c * PIECES + (q ^ (c & 15)); }
 };
-// symmetric sweep: each wavefront stages its reverse candidates (column, row, key) in LDS
+// symmetric sweep: each wavefront stages its reverse candidates (column, row, dot) in LDS
 // and copies them to its global queue in batches, with no vector-memory operation of the
 // ring in flight (stores mixed with the ring's loads would void its counted waits)
 constexpr int RB_QL = 192;                                      // LDS queue entries per wavefront
@@ -824,14 +825,28 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
 
   const int64_t row0 = (int64_t)blockIdx.x * RB_ROWS;
   const int64_t ntile = cols.ntile;
-  int64_t tile0 = 0, nt = ntile;
+  int64_t tile0 = 0, nt = ntile, tshift = 0;
   if (SYM) {
+    // Block I owns the cyclic range I, I+1, ... of nblk blocks. The workgroups resident
+    // together (consecutive I, `group` of them) start their walks at a common block, the
+    // last first-block of the group, so that they stream the shared part of their ranges
+    // in step and the L2 serves most column reads; each walk then wraps to its own start.
     const int64_t m = npad / RB_ROWS;  // 256-row blocks; ntile == 8 m
     const int64_t I = blockIdx.x;
     const int64_t nblk = (m & 1) ? (m + 1) / 2 : m / 2 + (I < m / 2 ? 1 : 0);
     tile0 = I * (RB_ROWS / RA_BN);
     nt = nblk * (RB_ROWS / RA_BN);
+    const int64_t g = cols.group > 0 ? cols.group : 1;
+    tshift = std::min<int64_t>(g - 1 - I % g, nblk - 1) * (RB_ROWS / RA_BN);
   }
+  // absolute column tile of the ct-th tile of this walk
+  auto tile_at = [&](int64_t ct) {
+    int64_t r = tshift + ct;
+    if (r >= nt) r -= nt;
+    int64_t a = tile0 + r;
+    if (a >= ntile) a -= ntile;
+    return a;
+  };
   const int64_t total = nt * NKC;
   int qn = 0;  // this wavefront's reverse candidates in its global queue
   int lq = 0;  // and staged in LDS
@@ -875,9 +890,22 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     li[q] = -1;
   }
   thr[lane] = FINF;
-  if (SYM && lane < 32) {
-    const int64_t r = row0 + wave * 32 + lane;
-    rsc[lane] = r < n ? colscale[r] : 0.0f;
+  // the smallest and largest row scale of this wavefront: a cheap bound that rules out
+  // transposed candidates for most tiles before any per-row work
+  float rs_min = 0.0f, rs_max = 0.0f;
+  if (SYM) {
+    const int64_t r = row0 + wave * 32 + (lane & 31);
+    const float v = r < n ? colscale[r] : 0.0f;
+    if (lane < 32) rsc[lane] = v;
+    rs_min = v;
+    rs_max = v;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      rs_min = fminf(rs_min, __shfl_xor(rs_min, o, 64));
+      rs_max = fmaxf(rs_max, __shfl_xor(rs_max, o, 64));
+    }
+    rs_min = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(rs_min)));
+    rs_max = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(rs_max)));
   }
   int bcnt = 0;
 
@@ -886,15 +914,14 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   for (int h = 0; h < LOADS; ++h) {
     const int s = (h * RB_WAVES + wave) * 64 + lane;
     const int c = s / R::PIECES, qs = s % R::PIECES;
-    pb[h] = cols.X + (tile0 * RA_BN + c) * dpad + (qs ^ (c & 15)) * 8;
+    pb[h] = cols.X + (tile_at(0) * RA_BN + c) * dpad + (qs ^ (c & 15)) * 8;
   }
   // per tile, lanes 0-31 bring the 32 column scales and (symmetric sweep) lanes 32-63 the
   // columns' reverse thresholds into the ring
-  const float* pc = ((SYM && lane >= 32) ? cols.thr0 : cols.scale) + tile0 * RA_BN + (lane & 31);
-  int64_t gi = 0, tnext = tile0;
+  const float* pc = ((SYM && lane >= 32) ? cols.thr0 : cols.scale) + tile_at(0) * RA_BN + (lane & 31);
+  int64_t gi = 0, tcur = 0;  // walk index of the tile being issued
   int kci = 0, sloti = 0;
   const int64_t tile_step = (int64_t)RA_BN * dpad - (int64_t)(NKC - 1) * KC;
-  const int64_t wrap_step = (int64_t)ntile * RA_BN * dpad;
   auto issue_next = [&]() {
     char* st = ring + sloti * R::STB;
 #pragma unroll
@@ -912,11 +939,15 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
 #pragma unroll
         for (int h = 0; h < LOADS; ++h) pb[h] += tile_step;
         pc += RA_BN;
-        if (SYM && ++tnext == ntile) {  // the cyclic range wraps to column 0
-          tnext = 0;
+        if (SYM) {  // jumps of the walk: the range end back to its start, the last tile to 0
+          const int64_t from = tile_at(tcur);
+          ++tcur;
+          const int64_t jump = tile_at(tcur) - (from + 1);
+          if (jump != 0) {
 #pragma unroll
-          for (int h = 0; h < LOADS; ++h) pb[h] -= wrap_step;
-          pc -= ntile * RA_BN;
+            for (int h = 0; h < LOADS; ++h) pb[h] += jump * RA_BN * dpad;
+            pc += jump * RA_BN;
+          }
         }
       } else {
 #pragma unroll
@@ -959,12 +990,17 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     }
 
     // ---------------- wavefront-private epilogue of column tile ct
-    const int64_t cta = SYM ? (tile0 + ct >= ntile ? tile0 + ct - ntile : tile0 + ct) : ct;
+    const int64_t cta = SYM ? tile_at(ct) : ct;  // absolute column tile
     const bool diag = cols.self && (cta * RA_BN < row0 + RB_ROWS) && (cta * RA_BN + RA_BN > row0);
     const bool tail = (cta + 1) * RA_BN > cols.ncol;
     if (SYM && (cta >> 3) != (int64_t)blockIdx.x) {
-      // transposed keys: column cta*32 + lcol is a row too; its candidates are this
-      // wavefront's rows, keyed with the rows' own scales
+      // Transposed keys: column cta*32 + lcol is a row too, and this wavefront's rows are
+      // its candidates; a (row, column) pair is queued when its key, formed with the row's
+      // scale, beats the column's threshold. For cosine keys a bound filters first: for
+      // dots s <= ms and any row scale r in [rs_min, rs_max], -(s r) >= -max(ms rs_min,
+      // ms rs_max) (f32 rounding is monotone, so it holds as computed), and a group of four
+      // rows whose bound clears the threshold skips the exact test. The queue keeps the
+      // raw dot; the key is formed again, identically, when it is scattered.
       float t0c;
       {
         const unsigned ca = lds_addr(ring + (int)((ct * NKC + NKC - 1) & (NST - 1)) * R::STB +
@@ -973,35 +1009,51 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
       }
       const int cabs = (int)(cta * RA_BN) + lcol;
       if (!((int64_t)cabs < n)) t0c = -FINF;
-      const int rbase = (int)(row0 + wave * 32) + lrow;
-      const int nrow = (int)std::min<int64_t>(n, INT32_MAX);
-#pragma unroll 1
-      for (int j = 0; j < 4; ++j) {
-        f32x4 rs;
-        {
-          const unsigned ra_ = lds_addr(rsc + lrow + j * 8);
-          asm volatile("ds_read_b128 %0, %1\ns_waitcnt lgkmcnt(0)" : "=v"(rs) : "v"(ra_) : "memory");
+      unsigned m16 = 0xffffu;
+      if (COS) {  // per row group of 4 slots: the bound at the group's largest dot
+        m16 = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float ms = fmaxf(fmaxf(acc[4 * j], acc[4 * j + 1]), fmaxf(acc[4 * j + 2], acc[4 * j + 3]));
+          m16 |= (-fmaxf(ms * rs_min, ms * rs_max) < t0c ? 15u : 0u) << (4 * j);
         }
-        float sj[4];
+      }
+      if (row0 + RB_ROWS > n) {  // the last block: rows past n are padding
 #pragma unroll
-        for (int i = 0; i < 4; ++i) sj[i] = acc[j * 4 + i];
+        for (int e = 0; e < 16; ++e)
+          if (!(row0 + wave * 32 + (e >> 2) * 8 + lrow + (e & 3) < n)) m16 &= ~(1u << e);
+      }
+      if (__ballot(m16 != 0) != 0) {
+        const int rbase = (int)(row0 + wave * 32) + lrow;
+#pragma unroll 1
+        for (int j = 0; j < 4; ++j) {
+          if (__ballot(((m16 >> (4 * j)) & 15u) != 0) == 0) continue;
+          // flagged slots of this row group: the exact key with the row's own scale
+          f32x4 rs;
+          {
+            const unsigned ra_ = lds_addr(rsc + lrow + j * 8);
+            asm volatile("ds_read_b128 %0, %1\ns_waitcnt lgkmcnt(0)" : "=v"(rs) : "v"(ra_) : "memory");
+          }
+          float sj[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const float rk = COS ? -(sj[i] * rs[i]) : (rs[i] - 2.0f * sj[i]);
-          const int rr = rbase + j * 8 + i;
-          const bool rp = (rk < t0c) & (rr < nrow);
-          const unsigned long long rm = __ballot(rp);
-          if (rm) {
-            if (lq > RB_QL - 64) flush();
-            const int below = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(rm >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo((unsigned)rm, 0u));
-            const unsigned e = (unsigned)(lq + below);
-            if (rp) {
-              lds_st_b32(lqa + 4u * e, cabs);
-              lds_st_b32(lqa + 4u * (RB_QL + e), rr);
-              lds_st_b32(lqa + 4u * (2 * RB_QL + e), __float_as_int(rk));
+          for (int i = 0; i < 4; ++i) sj[i] = acc[j * 4 + i];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float rk = COS ? -(sj[i] * rs[i]) : (rs[i] - 2.0f * sj[i]);
+            const bool rp = ((m16 >> (4 * j + i)) & 1u) && (rk < t0c);
+            const unsigned long long rm = __ballot(rp);
+            if (rm) {
+              if (lq > RB_QL - 64) flush();
+              const int below = (int)__builtin_amdgcn_mbcnt_hi(
+                  (unsigned)(rm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)rm, 0u));
+              const unsigned q = (unsigned)(lq + below);
+              if (rp) {
+                lds_st_b32(lqa + 4u * q, cabs);
+                lds_st_b32(lqa + 4u * (RB_QL + q), rbase + j * 8 + i);
+                lds_st_b32(lqa + 4u * (2 * RB_QL + q), __float_as_int(sj[i]));
+              }
+              lq += __popcll(rm);
             }
-            lq += __popcll(rm);
           }
         }
       }
@@ -1179,12 +1231,15 @@ __global__ void rb_thr0_kernel(const float* __restrict__ ckey, int64_t n, int64_
   thr0[r] = r < n ? fminf(ckey[(r * 2) * 16 + 15], ckey[(r * 2 + 1) * 16 + 15]) : -FINF;
 }
 // queued reverse candidates into per-row buckets (order within a bucket is free: the
-// merge ranks by (key, index))
+// merge ranks by (key, index)); the key of row j's candidate r is formed here from the
+// dot and r's scale, as the tile kernel forms forward keys
+template <bool COS>
 __global__ __launch_bounds__(256) void rb_scatter_kernel(const int32_t* __restrict__ qj,
                                                          const int32_t* __restrict__ qr,
-                                                         const float* __restrict__ qk,
+                                                         const float* __restrict__ qs,
                                                          const int32_t* __restrict__ qcount,
                                                          int64_t qcap, int rcap,
+                                                         const float* __restrict__ colscale,
                                                          int32_t* __restrict__ rcnt,
                                                          float* __restrict__ rkey,
                                                          int32_t* __restrict__ ridx) {
@@ -1193,10 +1248,14 @@ __global__ __launch_bounds__(256) void rb_scatter_kernel(const int32_t* __restri
   for (int64_t e = threadIdx.x; e < cnt; e += 256) {
     const int64_t q = w * qcap + e;
     const int32_t j = qj[q];
+    const int32_t r = qr[q];
+    const float s_ = qs[q];
+    const float cs = colscale[r];
+    const float key = COS ? -(s_ * cs) : (cs - 2.0f * s_);
     const int slot = atomicAdd(&rcnt[j], 1);
     if (slot < rcap) {
-      rkey[(int64_t)j * rcap + slot] = qk[q];
-      ridx[(int64_t)j * rcap + slot] = qr[q];
+      rkey[(int64_t)j * rcap + slot] = key;
+      ridx[(int64_t)j * rcap + slot] = r;
     }
   }
 }
@@ -1556,6 +1615,14 @@ __global__ __launch_bounds__(256) void knn_exact_merge_kernel(
 }  // namespace
 
 // ------------------------------------------------------------------ launcher
+// compute units of the current device: the symmetric sweep runs one workgroup per CU
+static int64_t cu_count() {
+  int dev = 0, cus = 0;
+  EVOC_HIP(hipGetDevice(&dev));
+  EVOC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return cus > 0 ? cus : 1;
+}
+
 void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k, int precision,
                 int32_t* d_idx, float* d_dist, KnnStats* stats, hipStream_t s, DevPool& pool) {
   EVOC_CHECK(n >= 1 && d >= 1, "knn: empty input");
@@ -1720,6 +1787,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
         cols.qk = qk.p;
         cols.qcap = sym_qcap;
         cols.qcount = qcount.p;
+        cols.group = cu_count();
         if (metric == 1)
           knn_topc_rb_kernel<2, 256, true, true><<<grid, 512, RbRing<256>::SMEM + RB_QBYTES, s>>>(
               Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
@@ -1785,8 +1853,12 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       Buf<int32_t> rcnt(pool, n), ridx(pool, n * RB_RCAP);
       Buf<float> rkey(pool, n * RB_RCAP);
       EVOC_HIP(hipMemsetAsync(rcnt.p, 0, n * sizeof(int32_t), s));
-      rb_scatter_kernel<<<(unsigned)nwaves, 256, 0, s>>>(qj.p, qr.p, qk.p, qcount.p, sym_qcap,
-                                                         RB_RCAP, rcnt.p, rkey.p, ridx.p);
+      if (metric == 1)
+        rb_scatter_kernel<true><<<(unsigned)nwaves, 256, 0, s>>>(
+            qj.p, qr.p, qk.p, qcount.p, sym_qcap, RB_RCAP, colscale.p, rcnt.p, rkey.p, ridx.p);
+      else
+        rb_scatter_kernel<false><<<(unsigned)nwaves, 256, 0, s>>>(
+            qj.p, qr.p, qk.p, qcount.p, sym_qcap, RB_RCAP, colscale.p, rcnt.p, rkey.p, ridx.p);
       mkey.alloc(pool, n * 32);
       midx.alloc(pool, n * 32);
       bnd.alloc(pool, n);
