@@ -1230,6 +1230,17 @@ __global__ void rb_thr0_kernel(const float* __restrict__ ckey, int64_t n, int64_
   if (r >= npad) return;
   thr0[r] = r < n ? fminf(ckey[(r * 2) * 16 + 15], ckey[(r * 2 + 1) * 16 + 15]) : -FINF;
 }
+// Largest forward key of every row (+inf when its lists are not full): a reverse
+// candidate above it is beaten by all 32 forward candidates and cannot be kept.
+__global__ void rb_fwdmax_kernel(const float* __restrict__ ckey, const int32_t* __restrict__ cidx,
+                                 int64_t n, float* __restrict__ fmax, int32_t* __restrict__ dropped) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  float m = -FINF;
+  for (int e = 0; e < 32; ++e) m = cidx[r * 32 + e] < 0 ? FINF : fmaxf(m, ckey[r * 32 + e]);
+  fmax[r] = m;
+  dropped[r] = 0;
+}
 // queued reverse candidates into per-row buckets (order within a bucket is free: the
 // merge ranks by (key, index)); the key of row j's candidate r is formed here from the
 // dot and r's scale, as the tile kernel forms forward keys
@@ -1240,6 +1251,8 @@ __global__ __launch_bounds__(256) void rb_scatter_kernel(const int32_t* __restri
                                                          const int32_t* __restrict__ qcount,
                                                          int64_t qcap, int rcap,
                                                          const float* __restrict__ colscale,
+                                                         const float* __restrict__ fmax,
+                                                         int32_t* __restrict__ dropped,
                                                          int32_t* __restrict__ rcnt,
                                                          float* __restrict__ rkey,
                                                          int32_t* __restrict__ ridx) {
@@ -1252,6 +1265,10 @@ __global__ __launch_bounds__(256) void rb_scatter_kernel(const int32_t* __restri
     const float s_ = qs[q];
     const float cs = colscale[r];
     const float key = COS ? -(s_ * cs) : (cs - 2.0f * s_);
+    if (key > fmax[j]) {  // ranked out by the row's forward candidates
+      dropped[j] = 1;
+      continue;
+    }
     const int slot = atomicAdd(&rcnt[j], 1);
     if (slot < rcap) {
       rkey[(int64_t)j * rcap + slot] = key;
@@ -1262,12 +1279,14 @@ __global__ __launch_bounds__(256) void rb_scatter_kernel(const int32_t* __restri
 // Per row: the 32 forward candidates (two half-lists) and its reverse bucket ranked by
 // (key, index); the 32 best are kept, sorted. Every column not kept has a key >= the
 // returned bound: forward non-candidates >= their half's last key, reverse keys never
-// queued >= thr0, entries ranked out >= the 32nd kept. A full bucket voids the bound.
+// queued >= thr0, entries ranked out (here, or dropped at the scatter above the largest
+// forward key) >= the 32nd kept. A full bucket voids the bound.
 constexpr int RB_RCAP = 768;
 __global__ __launch_bounds__(256) void rb_merge_kernel(
     const float* __restrict__ ckey, const int32_t* __restrict__ cidx, const int32_t* __restrict__ rcnt,
     const float* __restrict__ rkey, const int32_t* __restrict__ ridx, const float* __restrict__ thr0,
-    int64_t n, float* __restrict__ mkey, int32_t* __restrict__ midx, float* __restrict__ bound) {
+    const int32_t* __restrict__ dropped, int64_t n, float* __restrict__ mkey,
+    int32_t* __restrict__ midx, float* __restrict__ bound) {
   __shared__ float sk[4][32 + RB_RCAP];
   __shared__ int32_t si[4][32 + RB_RCAP];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1321,7 +1340,7 @@ __global__ __launch_bounds__(256) void rb_merge_kernel(
   for (int o = 32; o >= 1; o >>= 1) key31 = fminf(key31, __shfl_xor(key31, o, 64));
   if (lane == 0) {
     float b = fminf(fminf(ckey[(row * 2) * 16 + 15], ckey[(row * 2 + 1) * 16 + 15]), thr0[row]);
-    if (nvalid > 32) b = fminf(b, key31);
+    if (nvalid > 32 || dropped[row]) b = fminf(b, key31);  // some candidate was ranked out
     if (cnt > RB_RCAP) b = -FINF;
     bound[row] = b;
   }
@@ -1850,21 +1869,25 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
             Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
       EVOC_HIP(hipGetLastError());
     } else {
-      Buf<int32_t> rcnt(pool, n), ridx(pool, n * RB_RCAP);
-      Buf<float> rkey(pool, n * RB_RCAP);
+      Buf<int32_t> rcnt(pool, n), ridx(pool, n * RB_RCAP), dropped(pool, n);
+      Buf<float> rkey(pool, n * RB_RCAP), fmax(pool, n);
       EVOC_HIP(hipMemsetAsync(rcnt.p, 0, n * sizeof(int32_t), s));
+      rb_fwdmax_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(ckey.p, cidx.p, n, fmax.p,
+                                                                 dropped.p);
       if (metric == 1)
         rb_scatter_kernel<true><<<(unsigned)nwaves, 256, 0, s>>>(
-            qj.p, qr.p, qk.p, qcount.p, sym_qcap, RB_RCAP, colscale.p, rcnt.p, rkey.p, ridx.p);
+            qj.p, qr.p, qk.p, qcount.p, sym_qcap, RB_RCAP, colscale.p, fmax.p, dropped.p, rcnt.p,
+            rkey.p, ridx.p);
       else
         rb_scatter_kernel<false><<<(unsigned)nwaves, 256, 0, s>>>(
-            qj.p, qr.p, qk.p, qcount.p, sym_qcap, RB_RCAP, colscale.p, rcnt.p, rkey.p, ridx.p);
+            qj.p, qr.p, qk.p, qcount.p, sym_qcap, RB_RCAP, colscale.p, fmax.p, dropped.p, rcnt.p,
+            rkey.p, ridx.p);
       mkey.alloc(pool, n * 32);
       midx.alloc(pool, n * 32);
       bnd.alloc(pool, n);
       rb_merge_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(ckey.p, cidx.p, rcnt.p, rkey.p,
-                                                              ridx.p, thr0.p, n, mkey.p, midx.p,
-                                                              bnd.p);
+                                                              ridx.p, thr0.p, dropped.p, n,
+                                                              mkey.p, midx.p, bnd.p);
       EVOC_HIP(hipGetLastError());
     }
   }
