@@ -6,23 +6,26 @@
 // Row a of the result is coef[a] * sum of the member rows of partition a, accumulated
 // member by member in ascending node order, each term rounded as coef * x before the
 // add -- the order the sparse product walks a CSR row -- so the f32 values equal the
-// host formulation bit for bit. One workgroup per partition, one thread per column
-// stripe; eight member rows are loaded ahead of the dependent adds.
+// host formulation bit for bit. One workgroup per (partition, 256 columns), one thread
+// per column; sixteen member rows are loaded ahead of the dependent adds.
 #include "kernels.h"
 
 namespace evoc {
 namespace {
 
 constexpr int PM_THREADS = 256;
-constexpr int PM_AHEAD = 8;
+constexpr int PM_AHEAD = 16;
 
+// one workgroup per (partition, stripe of 256 columns)
 __global__ __launch_bounds__(PM_THREADS) void partition_means_kernel(
     const float* __restrict__ X, int64_t ld, int64_t d, const int64_t* __restrict__ mptr,
     const int32_t* __restrict__ mem, const float* __restrict__ coef, float* __restrict__ out) {
   const int64_t a = blockIdx.x;
   const int64_t m0 = mptr[a], m1 = mptr[a + 1];
   const float cf = coef[a];
-  for (int64_t c = threadIdx.x; c < d; c += PM_THREADS) {
+  {
+    const int64_t c = (int64_t)blockIdx.y * PM_THREADS + threadIdx.x;
+    if (c >= d) return;
     float acc = 0.0f;
     int64_t m = m0;
     for (; m + PM_AHEAD <= m1; m += PM_AHEAD) {
@@ -48,8 +51,8 @@ __global__ __launch_bounds__(PM_THREADS) void partition_means_kernel(
 void partition_means_dev(const float* dX, int64_t ld, int64_t d, const int64_t* d_mptr,
                          const int32_t* d_mem, const float* d_coef, int64_t np, float* d_out,
                          hipStream_t s) {
-  partition_means_kernel<<<(unsigned)np, PM_THREADS, 0, s>>>(dX, ld, d, d_mptr, d_mem, d_coef,
-                                                              d_out);
+  const dim3 grid((unsigned)np, (unsigned)ceil_div(d, PM_THREADS));
+  partition_means_kernel<<<grid, PM_THREADS, 0, s>>>(dX, ld, d, d_mptr, d_mem, d_coef, d_out);
   EVOC_HIP(hipGetLastError());
 }
 
@@ -265,12 +268,17 @@ __global__ __launch_bounds__(256) void exp_node_kernel(
 constexpr int CS_THREADS = 256;
 constexpr int CS_FLOATS = 16384;  // floats per chunk buffer (64 KB)
 
+// LDS pitch of one column: 4 mod 64 words, so that the summing lanes' 16-byte reads start
+// in distinct groups of four banks; rows per chunk = pitch - 4 (a multiple of 64)
+__host__ __device__ inline int cs_pitch(int dim) { return (CS_FLOATS / dim - 4) / 64 * 64 + 4; }
+__host__ __device__ inline int cs_rows(int dim) { return cs_pitch(dim) - 4; }
+
 __global__ __launch_bounds__(CS_THREADS) void exp_colsum_kernel(const float* __restrict__ res,
                                                                 int64_t n, int dim, float inv_n,
                                                                 float* __restrict__ mean) {
-  extern __shared__ float cs_buf[];  // [2][dim][rows + 1]
-  const int rows = CS_FLOATS / dim;  // rows per chunk
-  const int pitch = rows + 1;
+  extern __shared__ __attribute__((aligned(16))) float cs_buf[];  // [2][dim][pitch]
+  const int rows = cs_rows(dim);
+  const int pitch = cs_pitch(dim);
   const int t = threadIdx.x;
   const int64_t nchunks = (n + rows - 1) / rows;
   constexpr int PER = CS_FLOATS / CS_THREADS;  // floats per thread per chunk
@@ -281,7 +289,7 @@ __global__ __launch_bounds__(CS_THREADS) void exp_colsum_kernel(const float* __r
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int q = u * CS_THREADS + t;
-      stage[u] = q < lim ? res[base + q] : 0.0f;
+      stage[u] = (q < rows * dim && q < lim) ? res[base + q] : 0.0f;
     }
   };
   auto store = [&](int buf) {
@@ -299,9 +307,18 @@ __global__ __launch_bounds__(CS_THREADS) void exp_colsum_kernel(const float* __r
     __syncthreads();
     if (k + 1 < nchunks) load(k + 1);
     if (t < dim) {
-      const float* B = cs_buf + (k & 1) * dim * pitch + t * pitch;
+      const float4* B = reinterpret_cast<const float4*>(cs_buf + (k & 1) * dim * pitch + t * pitch);
       const int cnt = (int)min<int64_t>(rows, n - k * rows);
-      for (int r = 0; r < cnt; ++r) sum = sum + B[r];
+      int r = 0;
+      for (; r + 16 <= cnt; r += 16) {  // four 16-byte reads ahead of the dependent adds
+        const float4 a = B[r / 4], b = B[r / 4 + 1], c = B[r / 4 + 2], d = B[r / 4 + 3];
+        sum = sum + a.x; sum = sum + a.y; sum = sum + a.z; sum = sum + a.w;
+        sum = sum + b.x; sum = sum + b.y; sum = sum + b.z; sum = sum + b.w;
+        sum = sum + c.x; sum = sum + c.y; sum = sum + c.z; sum = sum + c.w;
+        sum = sum + d.x; sum = sum + d.y; sum = sum + d.z; sum = sum + d.w;
+      }
+      const float* Bs = reinterpret_cast<const float*>(B);
+      for (; r < cnt; ++r) sum = sum + Bs[r];
     }
     if (k + 1 < nchunks) store((k + 1) & 1);
   }
@@ -333,7 +350,7 @@ void expansion_device(const int64_t* d_indptr, const int32_t* d_col, const float
   exp_node_kernel<<<g, 64 * LP_WAVES, 0, s>>>(d_indptr, n, dim, d_part, d_sa, layout.p, ta.p, tf.p,
                                               tacc.p, ra.p, rv.p, res.p);
   {
-    const size_t lds = 2 * (size_t)dim * (CS_FLOATS / dim + 1) * sizeof(float);
+    const size_t lds = 2 * (size_t)dim * cs_pitch(dim) * sizeof(float);
     EVOC_HIP(hipFuncSetAttribute((const void*)exp_colsum_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     exp_colsum_kernel<<<1, CS_THREADS, lds, s>>>(res.p, n, dim, (float)(1.0 / (double)n), mean.p);
@@ -528,20 +545,36 @@ __global__ void cg_run_sum_kernel(int64_t nrun, int64_t m, int64_t n, int64_t np
   key2[r] = (uint64_t)a * (uint64_t)np + (uint64_t)part[j];
   idx2[r] = (uint32_t)r;
 }
+// the (a, j) sums in the order of the second sort, so that runs are read contiguously
+__global__ void cg_gather_kernel(int64_t nrun, const uint32_t* __restrict__ idx2,
+                                 const float* __restrict__ acc, float* __restrict__ accs) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nrun) accs[r] = acc[idx2[r]];
+}
 // step 3: per (a, b) run, the sum of the (a, j) sums in ascending j, clipped to [0, 1]
 __global__ void cg_entry_kernel(int64_t nent, int64_t nrun, int64_t np,
                                 const int64_t* __restrict__ start,
-                                const uint64_t* __restrict__ key2,
-                                const uint32_t* __restrict__ idx2, const float* __restrict__ acc,
+                                const uint64_t* __restrict__ key2, const float* __restrict__ accs,
                                 int32_t* __restrict__ ccol, float* __restrict__ cval,
                                 int64_t* __restrict__ count) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nent) return;
   const int64_t r0 = start[e], r1 = e + 1 < nent ? start[e + 1] : nrun;
   const uint64_t k = key2[r0];
-  float v = acc[idx2[r0]] * 1.0f;
-  for (int64_t r = r0 + 1; r < r1; ++r) {
-    const float pr = acc[idx2[r]] * 1.0f;
+  float v = accs[r0] * 1.0f;
+  int64_t r = r0 + 1;
+  for (; r + 8 <= r1; r += 8) {  // loads ahead of the dependent adds
+    float x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = accs[r + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const float pr = x[u] * 1.0f;
+      v = v + pr;
+    }
+  }
+  for (; r < r1; ++r) {
+    const float pr = accs[r] * 1.0f;
     v = v + pr;
   }
   v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
@@ -674,8 +707,10 @@ void lp_top_coarsen(const int64_t* d_ptr, const int32_t* d_col, const float* d_v
   Buf<int32_t> dcol(pool, nent);
   Buf<float> dval(pool, nent);
   EVOC_HIP(hipMemsetAsync(cnt.p, 0, (np + 1) * sizeof(int64_t), s));
+  Buf<float> accs(pool, nrun);
+  cg_gather_kernel<<<(unsigned)ceil_div(nrun, 256), 256, 0, s>>>(nrun, idx2.p, acc.p, accs.p);
   cg_entry_kernel<<<(unsigned)ceil_div(nent, 256), 256, 0, s>>>(
-      nent, nrun, np, start2.p, key2.p, idx2.p, acc.p, dcol.p, dval.p, cnt.p);
+      nent, nrun, np, start2.p, key2.p, accs.p, dcol.p, dval.p, cnt.p);
   exclusive_scan_i64(cnt.p, dptr.p, np + 1, s, pool);
   EVOC_HIP(hipGetLastError());
   ccol.resize(nent);
