@@ -1223,12 +1223,33 @@ __global__ void rb_sample_kernel(const uint16_t* __restrict__ Xh, const float* _
   reinterpret_cast<int4*>(Xs + q * dpad)[c] = v;
   if (c == 0) ss[q] = src >= 0 ? scale[src] : 0.0f;
 }
-// a row's reverse threshold: the bound of its two half-lists against the sample
+// A row's reverse threshold: the RB_THR_RANK-th smallest key of its two sorted half-lists
+// against the sample (any threshold is correct; this one queues about RB_THR_RANK / S of
+// the transposed keys, S the sample size).
+constexpr int RB_THR_RANK = 16;
 __global__ void rb_thr0_kernel(const float* __restrict__ ckey, int64_t n, int64_t npad,
                                float* __restrict__ thr0) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= npad) return;
-  thr0[r] = r < n ? fminf(ckey[(r * 2) * 16 + 15], ckey[(r * 2 + 1) * 16 + 15]) : -FINF;
+  if (r >= n) {
+    thr0[r] = -FINF;
+    return;
+  }
+  const float* a = ckey + (r * 2) * 16;
+  const float* b = ckey + (r * 2 + 1) * 16;
+  int ia = 0, ib = 0;
+  float v = FINF;
+  for (int t = 0; t < RB_THR_RANK; ++t) {
+    const float x = ia < 16 ? a[ia] : FINF, y = ib < 16 ? b[ib] : FINF;
+    if (x <= y) {
+      v = x;
+      ++ia;
+    } else {
+      v = y;
+      ++ib;
+    }
+  }
+  thr0[r] = v;
 }
 // Largest forward key of every row (+inf when its lists are not full): a reverse
 // candidate above it is beaten by all 32 forward candidates and cannot be kept.
@@ -1749,7 +1770,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   Buf<float> ss, thr0, qk;
   Buf<int32_t> qj, qr, qcount;
   if (sym) {
-    sym_S = std::min<int64_t>(65536, std::max<int64_t>(256, n / 16 / 256 * 256));
+    sym_S = std::min<int64_t>(65536, std::max<int64_t>(256, n / 32 / 256 * 256));
     sym_S = std::min<int64_t>(sym_S, n);
     sym_Spad = ceil_div(sym_S, RA_BN) * RA_BN;
     Xs.alloc(pool, sym_Spad * dpad);
@@ -1757,9 +1778,9 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     thr0.alloc(pool, npad);
     rb_sample_kernel<<<(unsigned)ceil_div(sym_Spad * (dpad / 8), 256), 256, 0, s>>>(
         Xh.p, colscale.p, n, dpad, sym_S, sym_Spad, Xs.p, ss.p);
-    // about 32 rows x n/2 columns x 32/S reverse keys per wavefront; three times that
-    const int64_t expect = 512 * ceil_div(n, sym_S);
-    sym_qcap = std::max<int64_t>(4096, ceil_div(3 * expect, 256) * 256);
+    // about 32 rows x n/2 columns x RB_THR_RANK/S reverse keys per wavefront; three times that
+    const int64_t expect = 16 * RB_THR_RANK * ceil_div(n, sym_S);
+    sym_qcap = std::max<int64_t>(16384, ceil_div(3 * expect, 256) * 256);
     const int64_t nwaves = (npad / RB_ROWS) * RB_WAVES;
     qj.alloc(pool, nwaves * sym_qcap);
     qr.alloc(pool, nwaves * sym_qcap);
