@@ -75,6 +75,10 @@ CTree condense(const Linkage& L, int64_t mcs) {
   std::vector<int64_t> relabel(root + 1, 0);
   relabel[root] = n;
   int64_t next = n + 1;
+  t.parent.reserve(n + n / 8 + 64);
+  t.child.reserve(n + n / 8 + 64);
+  t.lambda.reserve(n + n / 8 + 64);
+  t.size.reserve(n + n / 8 + 64);
   auto emit = [&](int64_t p, int64_t c, float lam, int64_t s) {
     t.parent.push_back(p);
     t.child.push_back(c);
@@ -150,45 +154,46 @@ void labels_and_strengths(const CTree& t, const std::vector<int64_t>& clusters, 
   std::fill(labels, labels + n, -1);
   if (strengths) std::fill(strengths, strengths + n, 0.0f);
   if (clusters.empty() || t.parent.empty()) return;
+  const size_t nr = t.parent.size();
   int64_t max_id = 0, root = INT64_MAX;
-  for (size_t i = 0; i < t.parent.size(); ++i) {
+  for (size_t i = 0; i < nr; ++i) {
     max_id = std::max({max_id, t.parent[i], t.child[i]});
     root = std::min(root, t.parent[i]);
   }
   std::vector<int64_t> sorted(clusters);
   std::sort(sorted.begin(), sorted.end());
-  const int64_t span = max_id + 1;
-  std::vector<int64_t> rank(span, -1);
+  // per-cluster arrays indexed by id - n (cluster ids are n .. max_id)
+  const int64_t nc = max_id + 1 - n;
+  if (nc <= 0) return;
+  std::vector<int64_t> rank(nc, -1);
   for (size_t q = 0; q < sorted.size(); ++q)
-    if (sorted[q] >= 0 && sorted[q] < span) rank[sorted[q]] = (int64_t)q;
+    if (sorted[q] >= n && sorted[q] <= max_id) rank[sorted[q] - n] = (int64_t)q;
   // nearest selected ancestor of every cluster id (parents precede children)
-  std::vector<int64_t> cpar(span, -1);
-  for (size_t i = 0; i < t.parent.size(); ++i)
-    if (t.child[i] >= n) cpar[t.child[i]] = t.parent[i];
-  std::vector<int64_t> sel(span, -1);
-  for (int64_t c = n; c < span; ++c) {
-    if (rank[c] >= 0) sel[c] = c;
-    else if (cpar[c] >= n) sel[c] = sel[cpar[c]];
+  std::vector<int64_t> cpar(nc, -1);
+  for (size_t i = 0; i < nr; ++i)
+    if (t.child[i] >= n) cpar[t.child[i] - n] = t.parent[i];
+  std::vector<int64_t> sel(nc, -1);
+  for (int64_t c = 0; c < nc; ++c) {
+    if (rank[c] >= 0) sel[c] = c + n;
+    else if (cpar[c] >= n) sel[c] = sel[cpar[c] - n];
   }
-  std::vector<int64_t> ppar(n, -1);
-  for (size_t i = 0; i < t.parent.size(); ++i)
-    if (t.child[i] < n) ppar[t.child[i]] = t.parent[i];
-  for (int64_t p = 0; p < n; ++p) {
-    if (ppar[p] < 0) continue;
-    const int64_t s = sel[ppar[p]];
-    labels[p] = (s > root) ? rank[s] : -1;
+  for (size_t i = 0; i < nr; ++i) {
+    const int64_t p = t.child[i];
+    if (p >= n) continue;
+    const int64_t s = sel[t.parent[i] - n];
+    labels[p] = (s > root) ? rank[s - n] : -1;
   }
   if (!strengths) return;
-  std::vector<float> death(span, 0.0f);
-  for (size_t i = 0; i < t.parent.size(); ++i)
-    if (t.size[i] == 1 && rank[t.parent[i]] >= 0 && t.lambda[i] > death[t.parent[i]])
-      death[t.parent[i]] = t.lambda[i];
-  for (size_t i = 0; i < t.parent.size(); ++i) {
+  std::vector<float> death(nc, 0.0f);
+  for (size_t i = 0; i < nr; ++i)
+    if (t.size[i] == 1 && rank[t.parent[i] - n] >= 0 && t.lambda[i] > death[t.parent[i] - n])
+      death[t.parent[i] - n] = t.lambda[i];
+  for (size_t i = 0; i < nr; ++i) {
     const int64_t pt = t.child[i];
     if (pt >= root || pt >= n) continue;
     const int64_t lb = labels[pt];
     if (lb < 0) continue;
-    const float ml = death[sorted[lb]];
+    const float ml = death[sorted[lb] - n];
     const float lam = t.lambda[i];
     if (ml == 0.0f || !std::isfinite(lam)) strengths[pt] = 1.0f;
     else strengths[pt] = (lam < ml ? lam : ml) / ml;
