@@ -223,12 +223,17 @@ __global__ void bvh_comp_merge_kernel(int64_t lev, int32_t* __restrict__ ncomp) 
 }
 
 // ------------------------------------------------------------------ core distances
-template <int D, int KC>
+// With LIST, the kernel also records each point's k nearest (sorted positions) and
+// whether another point lies at exactly its k-th distance (a tie outside the list): the
+// first Boruvka round resolves most points from these lists.
+template <int D, int KC, bool LIST>
 __global__ __launch_bounds__(64) void core_kernel(const float* __restrict__ es, int64_t n,
                                                   int dim, int64_t ntiles, int64_t L,
                                                   const float* __restrict__ nlo,
                                                   const float* __restrict__ nhi, int k,
-                                                  float* __restrict__ core) {
+                                                  float* __restrict__ core,
+                                                  int32_t* __restrict__ knn_pos,
+                                                  uint8_t* __restrict__ knn_tie) {
   __shared__ float tp[TS * MAXD];
   const int lane = threadIdx.x;
   const int64_t qt = blockIdx.x;
@@ -240,9 +245,15 @@ __global__ __launch_bounds__(64) void core_kernel(const float* __restrict__ es, 
   for (int c = 0; c < MAXD; ++c) q[c] = (valid && c < nd) ? es[p * nd + c] : 0.0f;
   // ascending list of the k smallest squared distances; kth = best[k-1]
   float best[KC];
+  int32_t bidx[LIST ? KC : 1];
 #pragma unroll
   for (int j = 0; j < KC; ++j) best[j] = __builtin_huge_valf();
+  if (LIST) {
+#pragma unroll
+    for (int j = 0; j < KC; ++j) bidx[LIST ? j : 0] = -1;
+  }
   float kth = __builtin_huge_valf();
+  float rej = __builtin_huge_valf();  // smallest distance not in the list
   auto scan_tile = [&](int64_t t) {
     const int64_t pt = t * TS + lane;
     __syncthreads();
@@ -254,16 +265,34 @@ __global__ __launch_bounds__(64) void core_kernel(const float* __restrict__ es, 
         if (t * TS + j == p) continue;
         float x = dsq_seq<D>(q, tp + j * nd, nd);
         if (x < kth) {
+          if (LIST) {
+            int32_t xi = (int32_t)(t * TS + j);
 #pragma unroll
-          for (int m = 0; m < KC; ++m)
-            if (m < k) {
-              const float lo = fminf(x, best[m]);
-              x = fmaxf(x, best[m]);
-              best[m] = lo;
-            }
+            for (int m = 0; m < KC; ++m)
+              if (m < k) {
+                const bool sw = x < best[m];
+                const float lo = sw ? x : best[m], hi = sw ? best[m] : x;
+                const int32_t li = sw ? xi : bidx[LIST ? m : 0], hi_i = sw ? bidx[LIST ? m : 0] : xi;
+                best[m] = lo;
+                bidx[LIST ? m : 0] = li;
+                x = hi;
+                xi = hi_i;
+              }
+            rej = fminf(rej, x);  // the entry pushed out of the list
+          } else {
+#pragma unroll
+            for (int m = 0; m < KC; ++m)
+              if (m < k) {
+                const float lo = fminf(x, best[m]);
+                x = fmaxf(x, best[m]);
+                best[m] = lo;
+              }
+          }
 #pragma unroll
           for (int m = 0; m < KC; ++m)
             if (m == k - 1) kth = best[m];
+        } else if (LIST) {
+          rej = fminf(rej, x);
         }
       }
     }
@@ -290,7 +319,15 @@ __global__ __launch_bounds__(64) void core_kernel(const float* __restrict__ es, 
     if (nf) stk[sp++] = near ^ 1;
     if (nn) stk[sp++] = near;
   }
-  if (valid) core[p] = kth;
+  if (valid) {
+    core[p] = kth;
+    if (LIST) {
+#pragma unroll
+      for (int m = 0; m < KC; ++m)
+        if (m < k) knn_pos[p * k + m] = bidx[LIST ? m : 0];
+      knn_tie[p] = rej == kth ? 1 : 0;  // boxes pruned held only points farther than kth
+    }
+  }
 }
 
 // ------------------------------------------------------------------ Boruvka search
@@ -310,7 +347,9 @@ __global__ __launch_bounds__(64) void boruvka_search_kernel(
     const uint32_t* __restrict__ perm, int64_t n, int dim, int64_t L,
     const float* __restrict__ nlo, const float* __restrict__ nhi,
     const int32_t* __restrict__ ncomp, unsigned* __restrict__ comp_w,
-    float* __restrict__ best_w, int32_t* __restrict__ best_a, int32_t* __restrict__ best_b) {
+    float* __restrict__ best_w, int32_t* __restrict__ best_a, int32_t* __restrict__ best_b,
+    const float* __restrict__ pre_w, const int32_t* __restrict__ pre_a,
+    const int32_t* __restrict__ pre_b, const uint8_t* __restrict__ pre_done) {
   __shared__ float tp[TS * MAXD];
   __shared__ float tc[TS];
   __shared__ int32_t tcomp[TS];
@@ -327,6 +366,15 @@ __global__ __launch_bounds__(64) void boruvka_search_kernel(
   const int32_t mycomp = valid ? comp[p] : -1;
   const int32_t myid = valid ? (int32_t)perm[p] : 0;
   Best bst{__builtin_huge_valf(), INT32_MAX, INT32_MAX};
+  // first round: the edge resolved from the core-distance neighbour list (a real edge,
+  // so a bound); a point marked done needs no search at all
+  bool done = false;
+  if (pre_w && valid) {
+    bst = Best{pre_w[p], pre_a[p], pre_b[p]};
+    done = pre_done[p] != 0;
+    if (bst.w < __builtin_huge_valf()) atomicMin(&comp_w[mycomp], __float_as_uint(bst.w));
+  }
+  const bool live = valid && !done;
   __shared__ int32_t stk[STACK];
   int sp = 0;
   stk[sp++] = 1;
@@ -335,7 +383,7 @@ __global__ __launch_bounds__(64) void boruvka_search_kernel(
     const int32_t vc = ncomp[v];
     float bound = bst.w;
     if (valid) bound = fminf(bound, __uint_as_float(comp_w[mycomp]));
-    const bool prune = !valid || vc == -2 || (vc >= 0 && vc == mycomp) || cq > bound ||
+    const bool prune = !live || vc == -2 || (vc >= 0 && vc == mycomp) || cq > bound ||
                        box_dist2<D>(q, nlo + v * nd, nhi + v * nd, nd) > bound;
     if (__all(prune)) continue;
     if (v >= L) {
@@ -367,7 +415,7 @@ __global__ __launch_bounds__(64) void boruvka_search_kernel(
       continue;
     }
     // children: skip those no lane needs, walk the nearer one first
-    const bool alive = valid && !(cq > bound);
+    const bool alive = live && !(cq > bound);
     const int32_t c0 = ncomp[2 * v], c1 = ncomp[2 * v + 1];
     const bool o0 = alive && c0 != -2 && !(c0 >= 0 && c0 == mycomp);
     const bool o1 = alive && c1 != -2 && !(c1 >= 0 && c1 == mycomp);
@@ -431,6 +479,36 @@ __global__ __launch_bounds__(64) void boruvka_seed_kernel(
     best = fminf(best, mr);
   }
   if (best < __builtin_huge_valf()) atomicMin(&comp_w[mycomp], __float_as_uint(best));
+}
+
+// First round, from each point's k nearest (positions in the sorted order): every listed
+// q has d(p, q) <= core(p), so its mutual-reachability weight is max(core(p), core(q)).
+// When some listed q has core(q) <= core(p), that weight is core(p), which no edge of p can
+// undercut; unlisted points are farther than the k-th distance unless one ties it. So
+// without such a tie, the best listed edge under (w, min id, max id) is p's best edge.
+__global__ void boruvka_round1_kernel(int64_t n, int k, const float* __restrict__ core,
+                                      const int32_t* __restrict__ knn_pos,
+                                      const uint8_t* __restrict__ knn_tie,
+                                      const uint32_t* __restrict__ perm, float* __restrict__ pre_w,
+                                      int32_t* __restrict__ pre_a, int32_t* __restrict__ pre_b,
+                                      uint8_t* __restrict__ pre_done) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float cp = core[p];
+  const int32_t me = (int32_t)perm[p];
+  Best b{__builtin_huge_valf(), INT32_MAX, INT32_MAX};
+  for (int m = 0; m < k; ++m) {
+    const int32_t qpos = knn_pos[p * k + m];
+    if (qpos < 0) continue;
+    const float w = fmaxf(cp, core[qpos]);
+    const int32_t o = (int32_t)perm[qpos];
+    const int32_t a = min(me, o), bb = max(me, o);
+    if (better(w, a, bb, b)) b = Best{w, a, bb};
+  }
+  pre_w[p] = b.w;
+  pre_a[p] = b.a;
+  pre_b[p] = b.b;
+  pre_done[p] = (b.w == cp && !knn_tie[p]) ? 1 : 0;
 }
 
 // Later rounds: a point's best edge from the previous round is still a real edge out
@@ -531,28 +609,40 @@ void kd_sort(const float* d_emb, int64_t n, int dim, Sorted& S, hipStream_t s, D
   EVOC_HIP(hipGetLastError());
 }
 
-void core_sorted(const Sorted& S, int64_t n, int dim, int min_samples, float* d_core_sorted,
-                 hipStream_t s) {
+// core distances in sorted order; with knn_pos / knn_tie (k <= 8 only: returns false
+// otherwise) also each point's k nearest and the tie flag
+bool core_sorted(const Sorted& S, int64_t n, int dim, int min_samples, float* d_core_sorted,
+                 hipStream_t s, int32_t* knn_pos = nullptr, uint8_t* knn_tie = nullptr) {
   const int k = (int)std::min<int64_t>(min_samples, n - 1);
   EVOC_CHECK(k <= KCORE, "min_samples must be <= 32");
   if (k <= 0) {
     EVOC_HIP(hipMemsetAsync(d_core_sorted, 0, n * sizeof(float), s));
-    return;
+    return false;
   }
   const unsigned g = (unsigned)S.ntiles;
-  if (dim == 8 && k <= 8)
-    core_kernel<8, 8><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p, k,
-                                       d_core_sorted);
-  else if (dim == 8)
-    core_kernel<8, KCORE><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p, k,
-                                           d_core_sorted);
-  else if (k <= 8)
-    core_kernel<0, 8><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p, k,
-                                       d_core_sorted);
-  else
-    core_kernel<0, KCORE><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p, k,
-                                           d_core_sorted);
+  const bool list = knn_pos && knn_tie && k <= 8;
+  if (list) {
+    if (dim == 8)
+      core_kernel<8, 8, true><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p,
+                                               k, d_core_sorted, knn_pos, knn_tie);
+    else
+      core_kernel<0, 8, true><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p,
+                                               k, d_core_sorted, knn_pos, knn_tie);
+  } else if (dim == 8 && k <= 8) {
+    core_kernel<8, 8, false><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p,
+                                              k, d_core_sorted, nullptr, nullptr);
+  } else if (dim == 8) {
+    core_kernel<8, KCORE, false><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p,
+                                                  S.nhi.p, k, d_core_sorted, nullptr, nullptr);
+  } else if (k <= 8) {
+    core_kernel<0, 8, false><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p,
+                                              k, d_core_sorted, nullptr, nullptr);
+  } else {
+    core_kernel<0, KCORE, false><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p,
+                                                  S.nhi.p, k, d_core_sorted, nullptr, nullptr);
+  }
   EVOC_HIP(hipGetLastError());
+  return list;
 }
 
 __global__ void unpermute_kernel(const float* __restrict__ src, const uint32_t* __restrict__ perm,
@@ -691,7 +781,21 @@ void mst_device(const float* d_emb, int64_t n, int dim, int min_samples, int32_t
   Sorted S;
   kd_sort(d_emb, n, dim, S, s, pool);
   Buf<float> core(pool, n);
-  core_sorted(S, n, dim, min_samples, core.p, s);
+  const int kc = (int)std::min<int64_t>(min_samples, n - 1);
+  Buf<int32_t> knn_pos(pool, n * std::max(kc, 1));
+  Buf<uint8_t> knn_tie(pool, n);
+  const bool have_list = core_sorted(S, n, dim, min_samples, core.p, s, knn_pos.p, knn_tie.p);
+  Buf<float> pre_w;
+  Buf<int32_t> pre_a, pre_b;
+  Buf<uint8_t> pre_done;
+  if (have_list) {
+    pre_w.alloc(pool, n);
+    pre_a.alloc(pool, n);
+    pre_b.alloc(pool, n);
+    pre_done.alloc(pool, n);
+    boruvka_round1_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(
+        n, kc, core.p, knn_pos.p, knn_tie.p, S.perm.p, pre_w.p, pre_a.p, pre_b.p, pre_done.p);
+  }
   Buf<int32_t> comp(pool, n), tile_comp(pool, S.ntiles), ba(pool, n), bb(pool, n);
   Buf<int32_t> ncomp_node(pool, 2 * S.L), inv(pool, n), partner(pool, n), parent(pool, n);
   Buf<float> bw(pool, n), ew2(pool, n);
@@ -724,14 +828,19 @@ void mst_device(const float* d_emb, int64_t n, int dim, int min_samples, int32_t
       boruvka_seed_kernel<8><<<g, 64, 0, s>>>(S.es.p, core.p, comp.p, n, dim, S.ntiles, comp_w.p);
     else
       boruvka_seed_kernel<0><<<g, 64, 0, s>>>(S.es.p, core.p, comp.p, n, dim, S.ntiles, comp_w.p);
+    const bool first = r == 1 && have_list;
+    const float* pw = first ? pre_w.p : nullptr;
+    const int32_t* pa = first ? pre_a.p : nullptr;
+    const int32_t* pbb = first ? pre_b.p : nullptr;
+    const uint8_t* pd = first ? pre_done.p : nullptr;
     if (dim == 8)
       boruvka_search_kernel<8><<<g, 64, 0, s>>>(S.es.p, core.p, comp.p, S.perm.p, n, dim, S.L,
                                                 S.nlo.p, S.nhi.p, ncomp_node.p, comp_w.p, bw.p,
-                                                ba.p, bb.p);
+                                                ba.p, bb.p, pw, pa, pbb, pd);
     else
       boruvka_search_kernel<0><<<g, 64, 0, s>>>(S.es.p, core.p, comp.p, S.perm.p, n, dim, S.L,
                                                 S.nlo.p, S.nhi.p, ncomp_node.p, comp_w.p, bw.p,
-                                                ba.p, bb.p);
+                                                ba.p, bb.p, pw, pa, pbb, pd);
     boruvka_select_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(n, comp.p, comp_w.p, bw.p,
                                                                      ba.p, bb.p, comp_key.p);
     EVOC_HIP(hipGetLastError());
