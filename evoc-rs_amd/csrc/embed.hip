@@ -171,7 +171,6 @@ __global__ __launch_bounds__(256) void epoch_wide_kernel(
     const int32_t* __restrict__ wide_nodes, int64_t nwide) {
   constexpr int DD = D > 0 ? D : MAXD;
   __shared__ float vbuf[4][WIDE_CAP * DD];
-  __shared__ unsigned char vok[4][WIDE_CAP];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t w = (int64_t)blockIdx.x * 4 + wave;
   if (w >= nwide) return;  // whole wavefronts only: no workgroup barrier below
@@ -185,7 +184,6 @@ __global__ __launch_bounds__(256) void epoch_wide_kernel(
   uint32_t draw_carry = 0;
   const int64_t s0 = indptr[i], s1 = indptr[i + 1];
   float* vb = vbuf[wave];
-  unsigned char* ok = vok[wave];
   for (int64_t sb = s0; sb < s1; sb += 64) {
     const int64_t s = sb + lane;
     const bool valid = s < s1;
@@ -284,15 +282,28 @@ __global__ __launch_bounds__(256) void epoch_wide_kernel(
             }
           }
         }
-        ok[slot] = good ? 1 : 0;
+        // a skipped term is staged as +0: the running sum starts at +0 and so is never -0,
+        // and x + (+0) == x for every other x, so adding it changes no bit
+        if (!good) {
+#pragma unroll
+          for (int c = 0; c < DD; ++c)
+            if (c < nd) vb[slot * DD + c] = 0.0f;
+        }
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       if (lane < nd) {
         const uint32_t cntw = min(total - w0, (uint32_t)WIDE_CAP);
-        for (uint32_t t = 0; t < cntw; ++t)
-          if (ok[t]) gl = gl + vb[t * DD + lane];
+        uint32_t t = 0;
+        for (; t + 8 <= cntw; t += 8) {  // reads ahead of the dependent adds
+          float x[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) x[u] = vb[(t + u) * DD + lane];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) gl = gl + x[u];
+        }
+        for (; t < cntw; ++t) gl = gl + vb[t * DD + lane];
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
