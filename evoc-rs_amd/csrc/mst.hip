@@ -189,6 +189,24 @@ __device__ __forceinline__ float wave_min(float x) {
   return x;
 }
 
+// Lowers each lane's component bound to its w (a real edge weight; +inf: nothing to
+// publish). Called by the whole wavefront. In late rounds a few large components take
+// every wavefront's updates, and one atomic per lane serialises on their addresses, so
+// the wavefront sends one atomic per distinct component among its lanes (usually one:
+// tiles are compact), carrying that component's minimum.
+__device__ __forceinline__ void publish_bound(unsigned* __restrict__ comp_w, bool valid,
+                                              int32_t mycomp, float w) {
+  const int lane = threadIdx.x % 64;
+  bool pend = valid && w < __builtin_huge_valf();
+  for (unsigned long long m = __ballot(pend); m != 0; m = __ballot(pend)) {
+    const int lead = __ffsll((long long)m) - 1;
+    const int32_t c = __shfl(mycomp, lead, 64);
+    const float cm = wave_min((pend && mycomp == c) ? w : __builtin_huge_valf());
+    if (lane == lead) atomicMin(&comp_w[c], __float_as_uint(cm));
+    pend = pend && mycomp != c;
+  }
+}
+
 // leaves L..2L-1 hold the tile boxes (empty boxes past the last tile)
 __global__ void bvh_leaf_kernel(const float* __restrict__ tlo, const float* __restrict__ thi,
                                 int64_t ntiles, int64_t L, int dim, float* __restrict__ nlo,
@@ -381,8 +399,8 @@ __global__ __launch_bounds__(64) void boruvka_search_kernel(
   while (sp > 0) {
     const int64_t v = stk[--sp];
     const int32_t vc = ncomp[v];
-    float bound = bst.w;
-    if (valid) bound = fminf(bound, __uint_as_float(comp_w[mycomp]));
+    const float cw = valid ? __uint_as_float(comp_w[mycomp]) : __builtin_huge_valf();
+    float bound = fminf(bst.w, cw);
     const bool prune = !live || vc == -2 || (vc >= 0 && vc == mycomp) || cq > bound ||
                        box_dist2<D>(q, nlo + v * nd, nhi + v * nd, nd) > bound;
     if (__all(prune)) continue;
@@ -410,8 +428,9 @@ __global__ __launch_bounds__(64) void boruvka_search_kernel(
             bound = fminf(bound, mr);
           }
         }
-        if (bst.w < __builtin_huge_valf()) atomicMin(&comp_w[mycomp], __float_as_uint(bst.w));
       }
+      // only an edge below the bound read at this node can lower it
+      publish_bound(comp_w, valid, mycomp, (!prune && bst.w < cw) ? bst.w : __builtin_huge_valf());
       continue;
     }
     // children: skip those no lane needs, walk the nearer one first
@@ -464,21 +483,22 @@ __global__ __launch_bounds__(64) void boruvka_seed_kernel(
     tcomp[w * TS + lane] = ok ? comp[pt] : -1;
   }
   __syncthreads();
-  if (!valid) return;
   float q[MAXD];
 #pragma unroll
-  for (int c = 0; c < MAXD; ++c) q[c] = c < nd ? es[p * nd + c] : 0.0f;
-  const float cq = core[p];
-  const int32_t mycomp = comp[p];
+  for (int c = 0; c < MAXD; ++c) q[c] = (valid && c < nd) ? es[p * nd + c] : 0.0f;
+  const float cq = valid ? core[p] : 0.0f;
+  const int32_t mycomp = valid ? comp[p] : -1;
   float best = __builtin_huge_valf();
-  for (int j = 0; j < 3 * TS; ++j) {
-    const int32_t cj = tcomp[j];
-    if (cj < 0 || cj == mycomp) continue;
-    const float dd = dsq_seq<D>(q, tp + j * nd, nd);
-    const float mr = fmaxf(fmaxf(dd, cq), tc[j]);
-    best = fminf(best, mr);
+  if (valid) {
+    for (int j = 0; j < 3 * TS; ++j) {
+      const int32_t cj = tcomp[j];
+      if (cj < 0 || cj == mycomp) continue;
+      const float dd = dsq_seq<D>(q, tp + j * nd, nd);
+      const float mr = fmaxf(fmaxf(dd, cq), tc[j]);
+      best = fminf(best, mr);
+    }
   }
-  if (best < __builtin_huge_valf()) atomicMin(&comp_w[mycomp], __float_as_uint(best));
+  publish_bound(comp_w, valid, mycomp, best);
 }
 
 // First round, from each point's k nearest (positions in the sorted order): every listed
