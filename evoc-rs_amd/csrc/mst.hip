@@ -182,6 +182,9 @@ __device__ __forceinline__ float dsq_seq(const float (&q)[MAXD], const float* __
 }
 
 constexpr int STACK = 64;  // > tree depth + 1 for any n < 2^31
+// a tile needed by more queries of a wavefront than this is scanned lane per query
+constexpr int CORE_DIRECT = 24;
+constexpr int SEARCH_DIRECT = 24;
 
 __device__ __forceinline__ float wave_min(float x) {
 #pragma unroll
@@ -240,6 +243,19 @@ __global__ void bvh_comp_merge_kernel(int64_t lev, int32_t* __restrict__ ncomp) 
   ncomp[v] = a == -2 ? b : (b == -2 ? a : (a == b ? a : -1));
 }
 
+__device__ __forceinline__ float rdlane(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const unsigned lo = __shfl_xor((unsigned)x, o, 64), hi = __shfl_xor((unsigned)(x >> 32), o, 64);
+    const unsigned long long y = ((unsigned long long)hi << 32) | lo;
+    x = y < x ? y : x;
+  }
+  return x;
+}
+
 // ------------------------------------------------------------------ core distances
 // With LIST, the kernel also records each point's k nearest (sorted positions) and
 // whether another point lies at exactly its k-th distance (a tie outside the list): the
@@ -252,7 +268,6 @@ __global__ __launch_bounds__(64) void core_kernel(const float* __restrict__ es, 
                                                   float* __restrict__ core,
                                                   int32_t* __restrict__ knn_pos,
                                                   uint8_t* __restrict__ knn_tie) {
-  __shared__ float tp[TS * MAXD];
   const int lane = threadIdx.x;
   const int64_t qt = blockIdx.x;
   const int64_t p = qt * TS + lane;
@@ -272,59 +287,96 @@ __global__ __launch_bounds__(64) void core_kernel(const float* __restrict__ es, 
   }
   float kth = __builtin_huge_valf();
   float rej = __builtin_huge_valf();  // smallest distance not in the list
-  auto scan_tile = [&](int64_t t) {
+  // One tile. When many queries need it (the own tile, or early in the walk), each lane
+  // scans the tile's points from LDS. Otherwise the scan is transposed: lane j holds the
+  // tile's point j, and the queries that need the tile (mine) are taken one at a time; the
+  // walk is shared by the wavefront, so most tiles it reaches are needed by a few of its
+  // queries only. Either way a query inserts its candidates below its k-th distance in
+  // tile order, as a scan of its own would (candidates it skips are farther than its k-th
+  // distance, so they could only lower rej by values above it).
+  __shared__ float tp[TS * MAXD];
+  auto scan_tile = [&](int64_t t, bool mine) {
     const int64_t pt = t * TS + lane;
-    __syncthreads();
-    for (int c = 0; c < nd; ++c) tp[lane * nd + c] = pt < n ? es[pt * nd + c] : 0.0f;
-    __syncthreads();
-    const int cnt = (int)min<int64_t>(TS, n - t * TS);
-    if (valid) {
-      for (int j = 0; j < cnt; ++j) {
-        if (t * TS + j == p) continue;
-        float x = dsq_seq<D>(q, tp + j * nd, nd);
-        if (x < kth) {
-          if (LIST) {
-            int32_t xi = (int32_t)(t * TS + j);
-#pragma unroll
-            for (int m = 0; m < KC; ++m)
-              if (m < k) {
-                const bool sw = x < best[m];
-                const float lo = sw ? x : best[m], hi = sw ? best[m] : x;
-                const int32_t li = sw ? xi : bidx[LIST ? m : 0], hi_i = sw ? bidx[LIST ? m : 0] : xi;
-                best[m] = lo;
-                bidx[LIST ? m : 0] = li;
-                x = hi;
-                xi = hi_i;
-              }
-            rej = fminf(rej, x);  // the entry pushed out of the list
-          } else {
-#pragma unroll
-            for (int m = 0; m < KC; ++m)
-              if (m < k) {
-                const float lo = fminf(x, best[m]);
-                x = fmaxf(x, best[m]);
-                best[m] = lo;
-              }
-          }
+    const bool cv = pt < n;
+    const unsigned long long need0 = __ballot(valid && mine);
+    auto insert = [&](float xv, int32_t xi) {
+      if (xv < kth) {
+        if (LIST) {
 #pragma unroll
           for (int m = 0; m < KC; ++m)
-            if (m == k - 1) kth = best[m];
-        } else if (LIST) {
-          rej = fminf(rej, x);
+            if (m < k) {
+              const bool sw = xv < best[m];
+              const float lo = sw ? xv : best[m], hi = sw ? best[m] : xv;
+              const int32_t li = sw ? xi : bidx[LIST ? m : 0], hi_i = sw ? bidx[LIST ? m : 0] : xi;
+              best[m] = lo;
+              bidx[LIST ? m : 0] = li;
+              xv = hi;
+              xi = hi_i;
+            }
+          rej = fminf(rej, xv);  // the entry pushed out of the list
+        } else {
+#pragma unroll
+          for (int m = 0; m < KC; ++m)
+            if (m < k) {
+              const float lo = fminf(xv, best[m]);
+              xv = fmaxf(xv, best[m]);
+              best[m] = lo;
+            }
         }
+#pragma unroll
+        for (int m = 0; m < KC; ++m)
+          if (m == k - 1) kth = best[m];
+      } else if (LIST) {
+        rej = fminf(rej, xv);
+      }
+    };
+    if (__popcll(need0) > CORE_DIRECT) {
+      __syncthreads();
+      for (int c = 0; c < nd; ++c) tp[lane * nd + c] = cv ? es[pt * nd + c] : 0.0f;
+      __syncthreads();
+      const int cnt = (int)min<int64_t>(TS, n - t * TS);
+      if (valid) {
+        for (int j = 0; j < cnt; ++j) {
+          if (t * TS + j == p) continue;
+          insert(dsq_seq<D>(q, tp + j * nd, nd), (int32_t)(t * TS + j));
+        }
+      }
+      return;
+    }
+    float cx[MAXD];
+#pragma unroll
+    for (int c = 0; c < MAXD; ++c) cx[c] = (cv && c < nd) ? es[pt * nd + c] : 0.0f;
+    for (unsigned long long need = need0; need != 0; need &= need - 1) {
+      const int ql = __ffsll((long long)need) - 1;
+      float qv[MAXD];
+#pragma unroll
+      for (int c = 0; c < MAXD; ++c) qv[c] = c < nd ? rdlane(q[c], ql) : 0.0f;
+      const float kq = rdlane(kth, ql);
+      const bool el = cv && pt != qt * TS + ql;
+      const float x = el ? dsq_seq<D>(qv, cx, nd) : __builtin_huge_valf();
+      const bool pass = el && x < kq;
+      if (LIST) {
+        const float r = wave_min((el && !pass) ? x : __builtin_huge_valf());
+        if (lane == ql) rej = fminf(rej, r);
+      }
+      for (unsigned long long pm = __ballot(pass); pm != 0; pm &= pm - 1) {
+        const int jl = __ffsll((long long)pm) - 1;
+        const float xv = rdlane(x, jl);
+        if (lane == ql) insert(xv, (int32_t)(t * TS + jl));
       }
     }
   };
-  scan_tile(qt);  // the own tile first, for a starting bound
+  scan_tile(qt, true);  // the own tile first, for a starting bound
   // depth-first walk, nearer child first, over a wave-uniform stack in LDS
   __shared__ int32_t stk[STACK];
   int sp = 0;
   stk[sp++] = 1;
   while (sp > 0) {
     const int64_t v = stk[--sp];
-    if (__all(!valid || box_dist2<D>(q, nlo + v * nd, nhi + v * nd, nd) > kth)) continue;
+    const bool far = !valid || box_dist2<D>(q, nlo + v * nd, nhi + v * nd, nd) > kth;
+    if (__all(far)) continue;
     if (v >= L) {
-      if (v - L != qt) scan_tile(v - L);
+      if (v - L != qt) scan_tile(v - L, !far);
       continue;
     }
     const float d0 = box_dist2<D>(q, nlo + 2 * v * nd, nhi + 2 * v * nd, nd);
@@ -407,26 +459,59 @@ __global__ __launch_bounds__(64) void boruvka_search_kernel(
     if (v >= L) {
       const int64_t t = v - L;
       const int64_t pt = t * TS + lane;
-      __syncthreads();
-      for (int c = 0; c < nd; ++c) tp[lane * nd + c] = pt < n ? es[pt * nd + c] : 0.0f;
-      tc[lane] = pt < n ? core[pt] : __builtin_huge_valf();
-      tcomp[lane] = pt < n ? comp[pt] : mycomp;
-      tid_[lane] = pt < n ? (int32_t)perm[pt] : 0;
-      __syncthreads();
-      if (!prune) {
-        const int cnt = (int)min<int64_t>(TS, n - t * TS);
-        for (int j = 0; j < cnt; ++j) {
-          if (tcomp[j] == mycomp) continue;
-          const float cj = tc[j];
-          if (cj > bound) continue;
-          const float dd = dsq_seq<D>(q, tp + j * nd, nd);
-          const float mr = fmaxf(fmaxf(dd, cq), cj);
-          const int32_t oj = tid_[j];
-          const int32_t a = min(myid, oj), b = max(myid, oj);
-          if (better(mr, a, b, bst)) {
-            bst = Best{mr, a, b};
-            bound = fminf(bound, mr);
+      const bool cv = pt < n;
+      const unsigned long long need0 = __ballot(!prune);
+      if (__popcll(need0) > SEARCH_DIRECT) {
+        __syncthreads();
+        for (int c = 0; c < nd; ++c) tp[lane * nd + c] = cv ? es[pt * nd + c] : 0.0f;
+        tc[lane] = cv ? core[pt] : __builtin_huge_valf();
+        tcomp[lane] = cv ? comp[pt] : mycomp;
+        tid_[lane] = cv ? (int32_t)perm[pt] : 0;
+        __syncthreads();
+        if (!prune) {
+          const int cnt = (int)min<int64_t>(TS, n - t * TS);
+          for (int j = 0; j < cnt; ++j) {
+            if (tcomp[j] == mycomp) continue;
+            const float cj = tc[j];
+            if (cj > bound) continue;
+            const float dd = dsq_seq<D>(q, tp + j * nd, nd);
+            const float mr = fmaxf(fmaxf(dd, cq), cj);
+            const int32_t oj = tid_[j];
+            const int32_t a = min(myid, oj), b = max(myid, oj);
+            if (better(mr, a, b, bst)) {
+              bst = Best{mr, a, b};
+              bound = fminf(bound, mr);
+            }
           }
+        }
+      } else {
+        // Transposed: lane j holds the tile's point j; each query that needs the tile
+        // takes the arg-min of its edges to the tile under the (w, a, b) order across the
+        // wavefront. Edges above the query's bound are left out, as in the lane scan.
+        float cx[MAXD];
+#pragma unroll
+        for (int c = 0; c < MAXD; ++c) cx[c] = (cv && c < nd) ? es[pt * nd + c] : 0.0f;
+        const float cj = cv ? core[pt] : __builtin_huge_valf();
+        const int32_t ccomp = cv ? comp[pt] : -1;
+        const int32_t cid = cv ? (int32_t)perm[pt] : 0;
+        for (unsigned long long need = need0; need != 0; need &= need - 1) {
+          const int ql = __ffsll((long long)need) - 1;
+          float qv[MAXD];
+#pragma unroll
+          for (int c = 0; c < MAXD; ++c) qv[c] = c < nd ? rdlane(q[c], ql) : 0.0f;
+          const float cqq = rdlane(cq, ql), bq = rdlane(bound, ql);
+          const int32_t compq = __builtin_amdgcn_readlane(mycomp, ql);
+          const int32_t idq = __builtin_amdgcn_readlane(myid, ql);
+          const bool el = cv && ccomp != compq && !(cj > bq);
+          const float mr = el ? fmaxf(fmaxf(dsq_seq<D>(qv, cx, nd), cqq), cj) : __builtin_huge_valf();
+          const float wm = wave_min(el ? mr : __builtin_huge_valf());
+          const bool at = el && mr == wm;
+          if (__ballot(at) == 0) continue;
+          const int32_t a = min(idq, cid), b = max(idq, cid);
+          const unsigned long long key = wave_min_u64(
+              at ? (((unsigned long long)(uint32_t)a << 32) | (uint32_t)b) : ~0ull);
+          const int32_t ka = (int32_t)(key >> 32), kb = (int32_t)(uint32_t)key;
+          if (lane == ql && better(wm, ka, kb, bst)) bst = Best{wm, ka, kb};
         }
       }
       // only an edge below the bound read at this node can lower it
