@@ -372,7 +372,7 @@ __global__ __launch_bounds__(64) void core_kernel(const float* __restrict__ es, 
   int sp = 0;
   stk[sp++] = 1;
   while (sp > 0) {
-    const int64_t v = stk[--sp];
+    const int64_t v = __builtin_amdgcn_readfirstlane(stk[--sp]);  // wave-uniform: scalar loads of the boxes
     const bool far = !valid || box_dist2<D>(q, nlo + v * nd, nhi + v * nd, nd) > kth;
     if (__all(far)) continue;
     if (v >= L) {
@@ -449,7 +449,7 @@ __global__ __launch_bounds__(64) void boruvka_search_kernel(
   int sp = 0;
   stk[sp++] = 1;
   while (sp > 0) {
-    const int64_t v = stk[--sp];
+    const int64_t v = __builtin_amdgcn_readfirstlane(stk[--sp]);  // wave-uniform: scalar loads of the boxes
     const int32_t vc = ncomp[v];
     const float cw = valid ? __uint_as_float(comp_w[mycomp]) : __builtin_huge_valf();
     float bound = fminf(bst.w, cw);
