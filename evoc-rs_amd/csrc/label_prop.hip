@@ -86,58 +86,64 @@ constexpr int LP_WAVES = 4;  // wavefronts per workgroup
 
 // One Jacobi step: an unlabelled node takes the label of largest summed edge weight
 // (labels in first-appearance order, each label's weights added in CSR order, the
-// running best starting at 1.0, ties to the later label), or stays unlabelled.
+// running best starting at 1.0, ties to the later label), or stays unlabelled. A
+// wavefront covers 64 consecutive nodes: labelled ones are copied lane-wise, and the
+// unlabelled ones are taken one at a time by the whole wavefront.
 __global__ __launch_bounds__(64 * LP_WAVES) void lp_step_kernel(
     const int64_t* __restrict__ indptr, const int32_t* __restrict__ col,
     const float* __restrict__ val, int64_t n, const int64_t* __restrict__ lab_in,
     int64_t* __restrict__ lab_out) {
   const int lane = threadIdx.x & 63;
-  const int64_t i = (int64_t)blockIdx.x * LP_WAVES + (threadIdx.x >> 6);
-  if (i >= n) return;  // whole wavefronts
-  const int64_t li0 = lab_in[i];
-  if (li0 >= 0) {
-    if (lane == 0) lab_out[i] = li0;
-    return;
-  }
-  const int64_t s0 = indptr[i], s1 = indptr[i + 1];
-  float top = 1.0f;
-  int64_t best = -1;
-  for (int64_t pb = s0; pb < s1; pb += 64) {
-    const int64_t p = pb + lane;
-    const bool pv = p < s1;
-    const int64_t lp = pv ? lab_in[col[p]] : -1;
-    bool first = lp >= 0;
-    float vw = pv ? val[p] : 0.0f;
-    for (int64_t qb = s0; qb < s1; qb += 64) {
-      const int64_t q = qb + lane;
-      const int64_t lq = q < s1 ? lab_in[col[q]] : -1;
-      const float wq = q < s1 ? val[q] : 0.0f;
-      const int tn = (int)min<int64_t>(64, s1 - qb);
-      for (int t = 0; t < tn; ++t) {
-        const int64_t lt = __shfl(lq, t, 64);
-        const float wt = __shfl(wq, t, 64);
-        const int64_t qi = qb + t;
-        if (lt == lp) {
-          if (qi < p) first = false;
-          else if (qi > p) vw = vw + wt;
+  const int64_t i0 = ((int64_t)blockIdx.x * LP_WAVES + (threadIdx.x >> 6)) * 64;
+  if (i0 >= n) return;  // whole wavefronts
+  const int64_t me = i0 + lane;
+  const int64_t lme = me < n ? lab_in[me] : 0;
+  if (me < n && lme >= 0) lab_out[me] = lme;
+  for (unsigned long long todo = __ballot(me < n && lme < 0); todo != 0; todo &= todo - 1) {
+    const int64_t i = i0 + (__ffsll((long long)todo) - 1);
+    const int64_t s0 = indptr[i], s1 = indptr[i + 1];
+    float top = 1.0f;
+    int64_t best = -1;
+    for (int64_t pb = s0; pb < s1; pb += 64) {
+      const int64_t p = pb + lane;
+      const bool pv = p < s1;
+      const int64_t lp = pv ? lab_in[col[p]] : -1;
+      if (__ballot(lp >= 0) == 0) continue;  // no label in this chunk: nothing to vote
+      bool first = lp >= 0;
+      float vw = pv ? val[p] : 0.0f;
+      for (int64_t qb = s0; qb < s1; qb += 64) {
+        const int64_t q = qb + lane;
+        const int64_t lq = q < s1 ? lab_in[col[q]] : -1;
+        const float wq = q < s1 ? val[q] : 0.0f;
+        // only labelled slots can match a lane's label (an unlabelled lane's sums are
+        // never read), so the walk visits those alone, in slot order
+        for (unsigned long long lm = __ballot(lq >= 0); lm != 0; lm &= lm - 1) {
+          const int t = __ffsll((long long)lm) - 1;
+          const int64_t lt = __shfl(lq, t, 64);
+          const float wt = __shfl(wq, t, 64);
+          const int64_t qi = qb + t;
+          if (lt == lp) {
+            if (qi < p) first = false;
+            else if (qi > p) vw = vw + wt;
+          }
+        }
+      }
+      // the chunk's first-appearance labels, in order: the sequential best update
+      const unsigned long long fm = __ballot(first);
+      for (unsigned long long m = fm; m; m &= m - 1) {
+        const int t = __ffsll((long long)m) - 1;
+        const float w = __shfl(vw, t, 64);
+        const int64_t l = __shfl(lp, t, 64);
+        if (w > top) {
+          top = w;
+          best = l;
+        } else if (w == top) {
+          best = l;
         }
       }
     }
-    // the chunk's first-appearance labels, in order: the sequential best update
-    const unsigned long long fm = __ballot(first);
-    for (unsigned long long m = fm; m; m &= m - 1) {
-      const int t = __ffsll((long long)m) - 1;
-      const float w = __shfl(vw, t, 64);
-      const int64_t l = __shfl(lp, t, 64);
-      if (w > top) {
-        top = w;
-        best = l;
-      } else if (w == top) {
-        best = l;
-      }
-    }
+    if (lane == 0) lab_out[i] = best;
   }
-  if (lane == 0) lab_out[i] = best;
 }
 
 // Expansion, pass 1 per node: for every slot the partition a of the neighbour and the
@@ -622,7 +628,7 @@ void lp_top_partition(const int64_t* d_ptr, const int32_t* d_col, const float* d
   lp_seed_kernel<<<(unsigned)ceil_div(ns, 256), 256, 0, s>>>(ns, dpos.p, dt.p, la.p);
   // Jacobi votes (labelled nodes keep their label, so every seed label survives and
   // maxl is the largest seed label)
-  const unsigned gw = (unsigned)ceil_div(n, LP_WAVES);
+  const unsigned gw = (unsigned)ceil_div(n, 64 * LP_WAVES);
   for (int it = 0; it < n_iter; ++it) {
     lp_step_kernel<<<gw, 64 * LP_WAVES, 0, s>>>(d_ptr, d_col, d_val, n, la.p, lb.p);
     std::swap(la.p, lb.p);
