@@ -776,7 +776,7 @@ constexpr int RB_EPI_BYTES = RB_WAVES * RB_WREG + RB_WAVES * 64 * 4 + RB_WAVES *
 struct RbCols {
   const uint16_t* X;    // column rows, bf16, dpad wide, ntile * 32 of them (zero padded)
   const float* scale;   // their key scales
-  const float* thr0;    // symmetric sweep: reverse-candidate threshold of every row
+  float* thr0;          // symmetric sweep: reverse-candidate threshold of every row
   int64_t ncol;         // valid columns
   int64_t ntile;        // 32-column tiles
   int self;             // columns are the query rows: mask the diagonal
@@ -823,21 +823,25 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   // key scales of this wavefront's 32 rows (the transposed keys of the symmetric sweep)
   float* rsc = reinterpret_cast<float*>(epi + RB_WAVES * RB_WREG + RB_WAVES * 64 * 4) + wave * 32;
 
-  const int64_t row0 = (int64_t)blockIdx.x * RB_ROWS;
+  // The symmetric sweep takes its blocks in descending order: the blocks a workgroup
+  // queues transposed keys for come after its own, so most of them have finished by then
+  // and replaced their rows' thresholds with the tighter largest forward key.
+  const int64_t bid = SYM ? (int64_t)gridDim.x - 1 - (int64_t)blockIdx.x : (int64_t)blockIdx.x;
+  const int64_t row0 = bid * RB_ROWS;
   const int64_t ntile = cols.ntile;
   int64_t tile0 = 0, nt = ntile, tshift = 0;
   if (SYM) {
     // Block I owns the cyclic range I, I+1, ... of nblk blocks. The workgroups resident
-    // together (consecutive I, `group` of them) start their walks at a common block, the
-    // last first-block of the group, so that they stream the shared part of their ranges
-    // in step and the L2 serves most column reads; each walk then wraps to its own start.
+    // together (consecutive blockIdx, `group` of them: blocks I0 - group + 1 .. I0) start
+    // their walks at I0, so that they stream the shared part of their ranges in step and
+    // the L2 serves most column reads; each walk then wraps to its own start.
     const int64_t m = npad / RB_ROWS;  // 256-row blocks; ntile == 8 m
-    const int64_t I = blockIdx.x;
+    const int64_t I = bid;
     const int64_t nblk = (m & 1) ? (m + 1) / 2 : m / 2 + (I < m / 2 ? 1 : 0);
     tile0 = I * (RB_ROWS / RA_BN);
     nt = nblk * (RB_ROWS / RA_BN);
     const int64_t g = cols.group > 0 ? cols.group : 1;
-    tshift = std::min<int64_t>(g - 1 - I % g, nblk - 1) * (RB_ROWS / RA_BN);
+    tshift = std::min<int64_t>((int64_t)blockIdx.x % g, nblk - 1) * (RB_ROWS / RA_BN);
   }
   // absolute column tile of the ct-th tile of this walk
   auto tile_at = [&](int64_t ct) {
@@ -993,7 +997,7 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     const int64_t cta = SYM ? tile_at(ct) : ct;  // absolute column tile
     const bool diag = cols.self && (cta * RA_BN < row0 + RB_ROWS) && (cta * RA_BN + RA_BN > row0);
     const bool tail = (cta + 1) * RA_BN > cols.ncol;
-    if (SYM && (cta >> 3) != (int64_t)blockIdx.x) {
+    if (SYM && (cta >> 3) != bid) {
       // Transposed keys: column cta*32 + lcol is a row too, and this wavefront's rows are
       // its candidates; a (row, column) pair is queued when its key, formed with the row's
       // scale, beats the column's threshold. For cosine keys a bound filters first: for
@@ -1178,12 +1182,18 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     }
   }
   wait_vm<0>();
+  // list of (row, half) -> candidates (row * 2 + half) * 16
+  const int64_t grow = row0 + wave * 32 + (lane & 31);
   if (SYM) {
     if (lq > 0) flush();
     if (lane == 0) cols.qcount[(int64_t)blockIdx.x * RB_WAVES + wave] = qn;
+    // Publish the row's largest forward key as its threshold when it is smaller. Any
+    // transposed key at or above it is beaten by all 32 forward candidates; workgroups
+    // still to come read the smaller value (or, from a stale cache line, the old one:
+    // every value a threshold takes is an upper bound the merge accounts for).
+    const float fwdmax = fmaxf(lk[C - 1], __shfl_xor(lk[C - 1], 32, 64));
+    if (lane < 32 && grow < n && fwdmax < cols.thr0[grow]) cols.thr0[grow] = fwdmax;
   }
-  // list of (row, half) -> candidates (row * 2 + half) * 16
-  const int64_t grow = row0 + wave * 32 + (lane & 31);
   if (grow < n) {
 #pragma unroll
     for (int q = 0; q < C; ++q) {
@@ -1300,7 +1310,7 @@ __global__ __launch_bounds__(256) void rb_scatter_kernel(const int32_t* __restri
 // Per row: the 32 forward candidates (two half-lists) and its reverse bucket ranked by
 // (key, index); the 32 best are kept, sorted. Every column not kept has a key >= the
 // returned bound: forward non-candidates >= their half's last key, reverse keys never
-// queued >= thr0, entries ranked out (here, or dropped at the scatter above the largest
+// queued >= thr0 (the smallest value the threshold took during the sweep), entries ranked out (here, or dropped at the scatter above the largest
 // forward key) >= the 32nd kept. A full bucket voids the bound.
 constexpr int RB_RCAP = 768;
 __global__ __launch_bounds__(256) void rb_merge_kernel(
