@@ -299,10 +299,10 @@ Layers cluster_layers_from_linkage(const Linkage& L, int64_t base_mcs, double mi
   const int64_t n = L.n;
   Layers out;
   const CTree t = condense(L, base_mcs);
-  std::vector<std::vector<int64_t>> labs(1, std::vector<int64_t>(n));
-  std::vector<std::vector<float>> strs(1, std::vector<float>(n));
+  // the selected clusters of every layer first (the leaves, then the diverse peaks of
+  // the total-persistence curve), then the layers' labels, independently in parallel
+  std::vector<std::vector<int64_t>> sel(1, leaves(t));
   std::vector<double> pers(1, 0.0);
-  labels_and_strengths(t, leaves(t), labs[0].data(), strs[0].data());
   const Bar b = barcodes(t, base_mcs);
   if (!b.birth.empty()) {
     std::vector<double> sizes, curve;
@@ -330,16 +330,21 @@ Layers cluster_layers_from_linkage(const Linkage& L, int64_t base_mcs, double mi
       std::vector<int64_t> act;
       for (size_t i = 0; i < b.birth.size(); ++i)
         if (b.birth[i] <= bs && b.death[i] > bs) act.push_back((int64_t)i + n);
-      labs.emplace_back(n);
-      strs.emplace_back(n);
-      labels_and_strengths(t, act, labs.back().data(), strs.back().data());
+      sel.push_back(std::move(act));
       pers.push_back(curve[pp.first]);
     }
   }
+  const int64_t nl = (int64_t)sel.size();
+  std::vector<std::vector<int64_t>> labs(nl, std::vector<int64_t>(n));
+  std::vector<std::vector<float>> strs(nl, std::vector<float>(n));
+  std::vector<int64_t> nc(nl);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t l = 0; l < nl; ++l) {
+    labels_and_strengths(t, sel[l], labs[l].data(), strs[l].data());
+    nc[l] = n_clusters(labs[l]);
+  }
   std::vector<int64_t> order(labs.size());
   std::iota(order.begin(), order.end(), 0);
-  std::vector<int64_t> nc(labs.size());
-  for (size_t i = 0; i < labs.size(); ++i) nc[i] = n_clusters(labs[i]);
   std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t c) { return nc[c] < nc[a]; });
   for (auto o : order) {
     out.labels.push_back(std::move(labs[o]));
