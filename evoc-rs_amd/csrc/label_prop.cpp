@@ -347,6 +347,22 @@ void coarse_embedding(const Sp& g, int dim, int epochs, float alpha, std::vector
   EVOC_HIP(hipStreamSynchronize(s));
 }
 
+// the same with the coarse graph in device memory: the slots are built there
+void coarse_embedding_dev(const Graph& g, int dim, int epochs, float alpha,
+                          std::vector<float>& layout, uint64_t seed, hipStream_t s, DevPool& pool) {
+  const int64_t n = g.n;
+  Graph dg;
+  coarse_slots_device(g, dg, s, pool);
+  Buf<float> de(pool, n * dim);
+  EVOC_HIP(hipMemcpyAsync(de.p, layout.data(), n * dim * sizeof(float), hipMemcpyHostToDevice, s));
+  EmbedParams ep;
+  ep.n_epochs = epochs;
+  ep.initial_alpha = alpha;
+  embed_device(dg, dim, ep, de.p, false, seed, s, pool);
+  EVOC_HIP(hipMemcpyAsync(layout.data(), de.p, n * dim * sizeof(float), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipStreamSynchronize(s));
+}
+
 // The data of a level: host rows (coarse levels) or device rows (the input level).
 struct Rows {
   const std::vector<float>* host = nullptr;
@@ -355,10 +371,11 @@ struct Rows {
   bool present() const { return host || dev; }
 };
 
-// One level of label_propagation_init_inner on a host graph (the coarse levels).
+// One level of label_propagation_init_inner on a host graph (the coarse levels). With
+// dev (the same graph in device memory) and short rows the expansion runs on the GPU.
 std::vector<float> level(const Sp& g, int dim, const Rows& data, int64_t d, int epochs,
                          int64_t approx, uint64_t seed, hipStream_t s, DevPool& pool,
-                         int verbose) {
+                         int verbose, const Graph* dev = nullptr) {
   const int64_t n = g.rows;
   if (n < 64) {
     if (data.present() && dim <= d) {
@@ -436,6 +453,21 @@ std::vector<float> level(const Sp& g, int dim, const Rows& data, int64_t d, int 
   // expansion: rows of (G o G^T) R_norm, L1-normalised, times the coarse layout,
   // averaged with the layout of each node's own partition
   std::vector<float> res(n * dim);
+  // the GPU expansion compares a row's slots pairwise: it suits short rows only
+  int64_t maxdeg = 0;
+  for (int64_t i = 0; i < n; ++i) maxdeg = std::max<int64_t>(maxdeg, g.ptr[i + 1] - g.ptr[i]);
+  if (dev && maxdeg <= 64) {
+    Buf<int32_t> dpart(pool, n);
+    Buf<float> dsa(pool, np), dres(pool, n * dim);
+    EVOC_HIP(hipMemcpyAsync(dpart.p, part.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    EVOC_HIP(hipMemcpyAsync(dsa.p, sa.data(), np * sizeof(float), hipMemcpyHostToDevice, s));
+    expansion_device(dev->indptr.p, dev->indices.p, dev->weights.p, n, dev->nnz, dim, dpart.p,
+                     dsa.p, np, layout.data(), dres.p, s, pool);
+    EVOC_HIP(hipMemcpyAsync(res.data(), dres.p, n * dim * sizeof(float), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipStreamSynchronize(s));
+    lap("expansion");
+    return res;
+  }
 #pragma omp parallel
   {
     std::vector<float> acc(np, 0.0f);
@@ -526,7 +558,9 @@ void top_level(const int64_t* d_ptr, const int32_t* d_col, const float* d_val, i
   if (verbose) std::printf("  Label prop: %lld nodes -> %lld partitions\n", (long long)n, (long long)np);
   Sp coarse;
   coarse.rows = coarse.cols = np;
-  lp_top_coarsen(d_ptr, d_col, d_val, n, nnz, P, coarse.ptr, coarse.col, coarse.val, s, pool);
+  Graph dcoarse;  // the coarse graph stays in device memory for its expansion and embedding
+  lp_top_coarsen(d_ptr, d_col, d_val, n, nnz, P, coarse.ptr, coarse.col, coarse.val, s, pool,
+                 &dcoarse);
   lap("coarsen graph");
   std::vector<float> cdata;
   if (data.present()) {
@@ -540,10 +574,16 @@ void top_level(const int64_t* d_ptr, const int32_t* d_col, const float* d_val, i
   const int ep = std::min(epochs, 255);
   Rows cr;
   if (data.present()) cr.host = &cdata;
+  const Graph* dv = dcoarse.n == np ? &dcoarse : nullptr;  // not kept for an empty graph
   std::vector<float> layout = level(coarse, dim, cr, d, ep, approx / 4, seed + 3000, s, pool,
-                                    verbose);
+                                    verbose, dv);
   tick = tclk::now();
-  coarse_embedding(coarse, dim, ep, (float)(0.001 * (double)epochs), layout, seed + 4000, s, pool);
+  if (dv)
+    coarse_embedding_dev(*dv, dim, ep, (float)(0.001 * (double)epochs), layout, seed + 4000, s,
+                         pool);
+  else
+    coarse_embedding(coarse, dim, ep, (float)(0.001 * (double)epochs), layout, seed + 4000, s,
+                     pool);
   lap("coarse embedding");
   expansion_device(d_ptr, d_col, d_val, n, nnz, dim, P.part.p, P.sa.p, np, layout.data(), d_out,
                    s, pool);

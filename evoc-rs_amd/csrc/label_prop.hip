@@ -589,6 +589,51 @@ __global__ void cg_entry_kernel(int64_t nent, int64_t nrun, int64_t np,
   atomicAdd(reinterpret_cast<unsigned long long*>(&count[k / (uint64_t)np]), 1ull);
 }
 
+// Embedding slots of a coarse graph (one per endpoint of every entry (i, j) with i < j,
+// as the host builds them in row order): node x lists first the rows i < x holding (i,
+// x), ascending, then its own entries j > x, ascending. Each slot gets the key
+// (2 x + group) * np + other, so one sort by key lays the lists out in that order.
+__global__ __launch_bounds__(256) void slot_key_kernel(const int64_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ col,
+                                                       int64_t np, uint64_t* __restrict__ key,
+                                                       uint32_t* __restrict__ idx,
+                                                       int32_t* __restrict__ erow,
+                                                       int64_t* __restrict__ deg) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * LP_WAVES + (threadIdx.x >> 6);
+  if (i >= np) return;
+  const uint64_t none = (uint64_t)(2 * np) * (uint64_t)np;
+  int64_t own = 0;
+  for (int64_t p = ptr[i] + lane; p < ptr[i + 1]; p += 64) {
+    const int32_t j = col[p];
+    erow[p] = (int32_t)i;
+    const bool up = (int64_t)j > i;
+    key[2 * p] = up ? (uint64_t)(2 * i + 1) * (uint64_t)np + (uint64_t)j : none;
+    key[2 * p + 1] = up ? (uint64_t)(2 * (int64_t)j) * (uint64_t)np + (uint64_t)i : none;
+    idx[2 * p] = (uint32_t)(2 * p);
+    idx[2 * p + 1] = (uint32_t)(2 * p + 1);
+    if (up) {
+      ++own;
+      atomicAdd(reinterpret_cast<unsigned long long*>(&deg[j]), 1ull);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) own += __shfl_xor(own, o, 64);
+  if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&deg[i]), (unsigned long long)own);
+}
+// slot s of the sorted order: the other endpoint and the entry's weight
+__global__ void slot_gather_kernel(int64_t m, const uint32_t* __restrict__ idx,
+                                   const int32_t* __restrict__ col, const int32_t* __restrict__ erow,
+                                   const float* __restrict__ val, int32_t* __restrict__ other,
+                                   float* __restrict__ w) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const uint32_t t = idx[q];
+  const int64_t p = t >> 1;
+  other[q] = (t & 1u) ? erow[p] : col[p];
+  w[q] = val[p];
+}
+
 int bit_length(uint64_t x) {
   int b = 0;
   while (x) {
@@ -678,7 +723,7 @@ void lp_top_partition(const int64_t* d_ptr, const int32_t* d_col, const float* d
 void lp_top_coarsen(const int64_t* d_ptr, const int32_t* d_col, const float* d_val, int64_t n,
                     int64_t nnz, const LpTop& P, std::vector<int64_t>& cptr,
                     std::vector<int32_t>& ccol, std::vector<float>& cval, hipStream_t s,
-                    DevPool& pool) {
+                    DevPool& pool, Graph* keep) {
   const int64_t np = P.np;
   cptr.assign(np + 1, 0);
   ccol.clear();
@@ -725,6 +770,47 @@ void lp_top_coarsen(const int64_t* d_ptr, const int32_t* d_col, const float* d_v
   EVOC_HIP(hipMemcpyAsync(ccol.data(), dcol.p, nent * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipMemcpyAsync(cval.data(), dval.p, nent * sizeof(float), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
+  if (keep) {
+    keep->n = np;
+    keep->nnz = nent;
+    keep->indptr = std::move(dptr);
+    keep->indices = std::move(dcol);
+    keep->weights = std::move(dval);
+  }
+}
+
+void coarse_slots_device(const Graph& g, Graph& out, hipStream_t s, DevPool& pool) {
+  const int64_t np = g.n, nnz = g.nnz;
+  out.n = np;
+  out.indptr.alloc(pool, np + 1);
+  Buf<int64_t> deg(pool, np + 1);
+  EVOC_HIP(hipMemsetAsync(deg.p, 0, (np + 1) * sizeof(int64_t), s));
+  if (nnz == 0 || np == 0) {
+    exclusive_scan_i64(deg.p, out.indptr.p, np + 1, s, pool);
+    out.nnz = 0;
+    out.indices.alloc(pool, 1);
+    out.weights.alloc(pool, 1);
+    return;
+  }
+  EVOC_CHECK(2 * nnz < ((int64_t)1 << 32), "coarse embedding: more than 2^31 graph entries");
+  Buf<uint64_t> key(pool, 2 * nnz);
+  Buf<uint32_t> idx(pool, 2 * nnz);
+  Buf<int32_t> erow(pool, nnz);
+  slot_key_kernel<<<(unsigned)ceil_div(np, LP_WAVES), 64 * LP_WAVES, 0, s>>>(
+      g.indptr.p, g.indices.p, np, key.p, idx.p, erow.p, deg.p);
+  radix_sort_pairs(key.p, idx.p, 2 * nnz, bit_length((uint64_t)(2 * np) * (uint64_t)np), s, pool);
+  exclusive_scan_i64(deg.p, out.indptr.p, np + 1, s, pool);
+  int64_t m = 0;
+  EVOC_HIP(hipMemcpyAsync(&m, out.indptr.p + np, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipStreamSynchronize(s));
+  out.nnz = m;
+  out.indices.alloc(pool, std::max<int64_t>(m, 1));
+  out.weights.alloc(pool, std::max<int64_t>(m, 1));
+  if (m > 0)
+    slot_gather_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, s>>>(m, idx.p, g.indices.p, erow.p,
+                                                                 g.weights.p, out.indices.p,
+                                                                 out.weights.p);
+  EVOC_HIP(hipGetLastError());
 }
 
 }  // namespace evoc
