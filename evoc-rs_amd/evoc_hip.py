@@ -189,17 +189,49 @@ class NearestNeighbourParams:
                          self.device_id)
 
 
-class EvocResult:
-    """Mirrors ``EvocResult<T>`` (src/lib.rs:106-161)."""
+class _Owner:
+    """Frees a library result once no array viewing its buffers is left."""
 
-    def __init__(self, r):
+    def __init__(self, ptr):
+        self.ptr = ptr
+        self._free = lib().evoc_hip_result_free
+
+    def __del__(self):
+        if self.ptr:
+            self._free(self.ptr)
+            self.ptr = None
+
+
+class _View:
+    """Array interface over one buffer of a library result; keeps the result alive."""
+
+    def __init__(self, owner, addr, shape, typestr):
+        self._owner = owner
+        self.__array_interface__ = {"data": (addr, False), "shape": shape, "typestr": typestr,
+                                    "version": 3}
+
+
+class EvocResult:
+    """Mirrors ``EvocResult<T>`` (src/lib.rs:106-161). The arrays view the library's
+    result buffers (no copy); the buffers are freed with the last array."""
+
+    def __init__(self, ptr):
+        r = ptr.contents
         n, k, nl, dim = r.n, r.k, r.n_layers, r.dim
-        self.cluster_layers = np.ctypeslib.as_array(r.cluster_layers, (nl, n)).copy() if nl else np.zeros((0, n), np.int64)
-        self.membership_strengths = np.ctypeslib.as_array(r.membership_strengths, (nl, n)).copy() if nl else np.zeros((0, n), np.float32)
+        own = _Owner(ptr)
+
+        def view(field, shape, typestr, empty_dtype):
+            addr = C.cast(getattr(r, field), C.c_void_p).value
+            if not addr or 0 in shape:
+                return np.zeros(shape, empty_dtype)
+            return np.asarray(_View(own, addr, shape, typestr))
+
+        self.cluster_layers = view("cluster_layers", (nl, n), "<i8", np.int64)
+        self.membership_strengths = view("membership_strengths", (nl, n), "<f4", np.float32)
         self.persistence_scores = np.ctypeslib.as_array(r.persistence_scores, (nl,)).copy() if nl else np.zeros(0)
-        self.nn_indices = np.ctypeslib.as_array(r.nn_indices, (n, k)).copy() if n and k else np.zeros((n, k), np.int32)
-        self.nn_distances = np.ctypeslib.as_array(r.nn_distances, (n, k)).copy() if n and k else np.zeros((n, k), np.float32)
-        self.embedding = np.ctypeslib.as_array(r.embedding, (n, dim)).copy() if n and dim else np.zeros((n, dim), np.float32)
+        self.nn_indices = view("nn_indices", (n, k), "<i4", np.int32)
+        self.nn_distances = view("nn_distances", (n, k), "<f4", np.float32)
+        self.embedding = view("embedding", (n, dim), "<f4", np.float32)
         self.best_layer = r.best_layer
         self.stage_ms = dict(zip(["knn", "fuzzy", "init", "embed", "mst", "layers", "total",
                                   "upload"], list(r.stage_ms)))
@@ -245,10 +277,7 @@ def evoc(data, params=None, nn_params=None, seed=42, verbose=0, precomputed_knn=
                               None if pi is None else pi.ctypes.data_as(vp),
                               None if pd is None else pd.ctypes.data_as(vp), pk,
                               C.byref(p), C.byref(q), seed, verbose, C.byref(out)))
-    try:
-        return EvocResult(out.contents)
-    finally:
-        lib().evoc_hip_result_free(out)
+    return EvocResult(out)
 
 
 def evoc_device(dev_ptr, n, d, ld=None, params=None, nn_params=None, seed=42, verbose=0):
@@ -258,10 +287,7 @@ def evoc_device(dev_ptr, n, d, ld=None, params=None, nn_params=None, seed=42, ve
     out = C.POINTER(_Result)()
     _check(lib().evoc_hip_run_device(C.c_void_p(dev_ptr), n, d, ld or d, C.byref(p), C.byref(q),
                                      seed, verbose, C.byref(out)))
-    try:
-        return EvocResult(out.contents)
-    finally:
-        lib().evoc_hip_result_free(out)
+    return EvocResult(out)
 
 
 # ---------------------------------------------------------------- stage entry points
