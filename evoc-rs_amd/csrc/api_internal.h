@@ -15,6 +15,7 @@ struct NoDevice : std::runtime_error {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t aux = nullptr;  // copies of finished results back to the host, beside stream
   DevPool pool;
 };
 

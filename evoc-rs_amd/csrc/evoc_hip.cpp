@@ -36,6 +36,7 @@ Ctx& context(int device) {
   auto c = std::make_unique<Ctx>();
   c->device = device;
   EVOC_HIP(hipStreamCreate(&c->stream));
+  EVOC_HIP(hipStreamCreate(&c->aux));
   Ctx& ref = *c;
   g_ctx[device] = std::move(c);
   return ref;
@@ -46,6 +47,7 @@ void release_all() {
   for (auto& kv : g_ctx) {
     (void)hipSetDevice(kv.first);
     (void)hipStreamSynchronize(kv.second->stream);
+    (void)hipStreamSynchronize(kv.second->aux);
     kv.second->pool.release();
   }
 }

@@ -4,6 +4,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <unordered_set>
 
 #include "api_internal.h"
@@ -89,7 +90,51 @@ void validate_params(const evoc_params_t& p) {
   EVOC_CHECK(p.max_layers >= 0, "max_layers must be >= 0");
 }
 
-evoc_result_t* alloc_result(int64_t n, int k, int dim, int nl) {
+void free_result(evoc_result_t* r);
+
+// Device-to-host copy of finished results on the context's second stream, driven by a
+// helper thread (a copy into pageable memory holds its calling thread until it is done),
+// so that the later stages run meanwhile. The destructor waits for it.
+struct BackCopy {
+  std::thread t;
+  hipError_t err = hipSuccess;
+  void start(Ctx& c, void* d1, const void* s1, size_t b1, void* d2, const void* s2, size_t b2) {
+    t = std::thread([this, &c, d1, s1, b1, d2, s2, b2]() {
+      hipError_t e = hipSetDevice(c.device);
+      if (e == hipSuccess && b1) e = hipMemcpyAsync(d1, s1, b1, hipMemcpyDeviceToHost, c.aux);
+      if (e == hipSuccess && b2) e = hipMemcpyAsync(d2, s2, b2, hipMemcpyDeviceToHost, c.aux);
+      if (e == hipSuccess) e = hipStreamSynchronize(c.aux);
+      err = e;
+    });
+  }
+  void finish() {
+    if (t.joinable()) t.join();
+    EVOC_HIP(err);
+  }
+  ~BackCopy() {
+    if (t.joinable()) t.join();
+  }
+};
+
+// host buffers of a result, freed unless handed over to it
+struct HostOut {
+  int32_t* idx = nullptr;
+  float* dist = nullptr;
+  float* emb = nullptr;
+  ~HostOut() {
+    std::free(idx);
+    std::free(dist);
+    std::free(emb);
+  }
+};
+template <typename T>
+T* host_alloc(size_t count) {
+  void* p = std::malloc(count ? count * sizeof(T) : 1);
+  if (!p) throw std::bad_alloc();
+  return static_cast<T*>(p);
+}
+
+evoc_result_t* alloc_result(int64_t n, int k, int dim, int nl, HostOut& take) {
   auto* r = static_cast<evoc_result_t*>(std::calloc(1, sizeof(evoc_result_t)));
   if (!r) throw std::bad_alloc();
   r->n = n;
@@ -101,12 +146,18 @@ evoc_result_t* alloc_result(int64_t n, int k, int dim, int nl) {
     if (!p) throw std::bad_alloc();
     return p;
   };
-  r->cluster_layers = static_cast<int64_t*>(al((size_t)nl * n * sizeof(int64_t)));
-  r->membership_strengths = static_cast<float*>(al((size_t)nl * n * sizeof(float)));
-  r->persistence_scores = static_cast<double*>(al((size_t)nl * sizeof(double)));
-  r->nn_indices = static_cast<int32_t*>(al((size_t)n * k * sizeof(int32_t)));
-  r->nn_distances = static_cast<float*>(al((size_t)n * k * sizeof(float)));
-  r->embedding = static_cast<float*>(al((size_t)n * dim * sizeof(float)));
+  // the kNN graph and the embedding: buffers already filled, handed over
+  std::swap(r->nn_indices, take.idx);
+  std::swap(r->nn_distances, take.dist);
+  std::swap(r->embedding, take.emb);
+  try {
+    r->cluster_layers = static_cast<int64_t*>(al((size_t)nl * n * sizeof(int64_t)));
+    r->membership_strengths = static_cast<float*>(al((size_t)nl * n * sizeof(float)));
+    r->persistence_scores = static_cast<double*>(al((size_t)nl * sizeof(double)));
+  } catch (...) {
+    free_result(r);
+    throw;
+  }
   return r;
 }
 
@@ -198,6 +249,12 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
   EVOC_HIP(hipStreamSynchronize(s));
   stage[0] = ms_since(t0);
   if (verbose) std::printf("kNN search done in %.2f ms.\n", stage[0]);
+  HostOut ho;
+  ho.idx = host_alloc<int32_t>((size_t)n * k);
+  ho.dist = host_alloc<float>((size_t)n * k);
+  BackCopy knn_back;  // joined before didx / ddist are released
+  knn_back.start(c, ho.idx, didx.p, (size_t)n * k * sizeof(int32_t), ho.dist, ddist.p,
+                 (size_t)n * k * sizeof(float));
   // 2. fuzzy simplicial set
   t0 = clk::now();
   if (verbose) std::printf("Constructing fuzzy simplicial set...\n");
@@ -287,25 +344,21 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
   }
   EVOC_HIP(hipStreamSynchronize(s));
   stage[3] = ms_since(t0);
+  ho.emb = host_alloc<float>((size_t)n * dim);
+  BackCopy emb_back;  // joined before demb is released
+  emb_back.start(c, ho.emb, demb.p, (size_t)n * dim * sizeof(float), nullptr, nullptr, 0);
   if (verbose) std::printf(" ... embedding done in %.2f ms.\n", stage[3]);
   // 5./6. clustering
   if (verbose) std::printf("Running density-based clustering...\n");
   Layers lay = cluster_device(demb.p, n, dim, p, s, pool, &stage[4], &stage[5], verbose);
   const int nl = (int)lay.labels.size();
-  evoc_result_t* r = alloc_result(n, k, dim, nl);
-  try {
-    for (int l = 0; l < nl; ++l) {
-      std::memcpy(r->cluster_layers + (size_t)l * n, lay.labels[l].data(), n * sizeof(int64_t));
-      std::memcpy(r->membership_strengths + (size_t)l * n, lay.strengths[l].data(), n * sizeof(float));
-      r->persistence_scores[l] = lay.persistence[l];
-    }
-    EVOC_HIP(hipMemcpyAsync(r->nn_indices, didx.p, n * k * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    EVOC_HIP(hipMemcpyAsync(r->nn_distances, ddist.p, n * k * sizeof(float), hipMemcpyDeviceToHost, s));
-    EVOC_HIP(hipMemcpyAsync(r->embedding, demb.p, n * dim * sizeof(float), hipMemcpyDeviceToHost, s));
-    EVOC_HIP(hipStreamSynchronize(s));
-  } catch (...) {
-    free_result(r);
-    throw;
+  knn_back.finish();
+  emb_back.finish();
+  evoc_result_t* r = alloc_result(n, k, dim, nl, ho);
+  for (int l = 0; l < nl; ++l) {
+    std::memcpy(r->cluster_layers + (size_t)l * n, lay.labels[l].data(), n * sizeof(int64_t));
+    std::memcpy(r->membership_strengths + (size_t)l * n, lay.strengths[l].data(), n * sizeof(float));
+    r->persistence_scores[l] = lay.persistence[l];
   }
   r->best_layer = best_layer_index(lay.persistence);
   r->knn_fallback_rows = fallback;
