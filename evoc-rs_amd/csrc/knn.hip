@@ -1381,7 +1381,8 @@ __global__ __launch_bounds__(256) void rb_merge_kernel(
 constexpr int MAXT = 16;  // d <= 1024 handled with the query row held in registers
 
 __global__ __launch_bounds__(256) void knn_rerank_kernel(
-    const float* __restrict__ X, int64_t n, int d, int64_t ld, const float* __restrict__ nrm,
+    const float* __restrict__ X, const uint16_t* __restrict__ Xh, int dpad, int64_t n, int d,
+    int64_t ld, const float* __restrict__ nrm,
     const float* __restrict__ sqn, const float* __restrict__ colscale,
     const unsigned* __restrict__ flags, const int32_t* __restrict__ cand_idx,
     const float* __restrict__ cand_key, int C, int halves, const float* __restrict__ row_bound,
@@ -1422,33 +1423,68 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
   }
   float my_d = FINF, my_sq = FINF;
   int my_j = -1;
+  // Candidate rows are read from the bf16 copy when every input is bf16-exact (Xh set):
+  // the same values in half the bytes. Two candidates are evaluated together so that
+  // their loads are in flight at once; each one's sum is formed exactly as alone.
+  auto load = [&](int j, int cc) -> float {
+    return Xh ? __uint_as_float((uint32_t)Xh[(int64_t)j * dpad + cc] << 16)
+              : X[(int64_t)j * ld + cc];
+  };
+  auto term = [&](float acc, float qt, float v) {
+    if (metric == 1) {
+      const float p = qt * v;
+      return acc + p;
+    }
+    const float df = qt - v;
+    const float p = df * df;
+    return acc + p;
+  };
+  auto finish = [&](float acc, int c, int j) {
+    acc = tree64(acc);
+    acc = __shfl(acc, 0, 64);
+    const float dist = metric == 1 ? cos_dist(acc, nrm[row], nrm[j]) : sqrtf(acc);
+    if (lane == c) {
+      my_d = dist;
+      my_sq = acc;
+      my_j = j;
+    }
+  };
   auto evaluate = [&](unsigned long long mask) {
-    for (; mask; mask &= mask - 1) {
-      const int c = __ffsll((long long)mask) - 1;
-      const int j = __shfl(cj, c, 64);
-      float acc = 0.0f;
+    while (mask) {
+      const int c1 = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      const int j1 = __shfl(cj, c1, 64);
+      if (mask) {
+        const int c2 = __ffsll((long long)mask) - 1;
+        mask &= mask - 1;
+        const int j2 = __shfl(cj, c2, 64);
+        float v1[MAXT], v2[MAXT];
 #pragma unroll
-      for (int t = 0; t < MAXT; ++t) {
-        const int cc = t * 64 + lane;
-        if (t < nt && cc < d) {
-          const float v = X[(int64_t)j * ld + cc];
-          if (metric == 1) {
-            const float p = q[t] * v;
-            acc = acc + p;
-          } else {
-            const float df = q[t] - v;
-            const float p = df * df;
-            acc = acc + p;
+        for (int t = 0; t < MAXT; ++t) {
+          const int cc = t * 64 + lane;
+          const bool in = t < nt && cc < d;
+          v1[t] = in ? load(j1, cc) : 0.0f;
+          v2[t] = in ? load(j2, cc) : 0.0f;
+        }
+        float a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t) {
+          const int cc = t * 64 + lane;
+          if (t < nt && cc < d) {
+            a1 = term(a1, q[t], v1[t]);
+            a2 = term(a2, q[t], v2[t]);
           }
         }
-      }
-      acc = tree64(acc);
-      acc = __shfl(acc, 0, 64);
-      const float dist = metric == 1 ? cos_dist(acc, nrm[row], nrm[j]) : sqrtf(acc);
-      if (lane == c) {
-        my_d = dist;
-        my_sq = acc;
-        my_j = j;
+        finish(a1, c1, j1);
+        finish(a2, c2, j2);
+      } else {
+        float a1 = 0.0f;
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t) {
+          const int cc = t * 64 + lane;
+          if (t < nt && cc < d) a1 = term(a1, q[t], load(j1, cc));
+        }
+        finish(a1, c1, j1);
       }
     }
   };
@@ -1923,7 +1959,9 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     }
   }
   knn_rerank_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(
-      dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, sym ? midx.p : cidx.p,
+      dX, (ra && hflags[0] == 0) ? Xh.p : nullptr, dpad, n, d, ld, nrm.p, sqn.p, colscale.p,
+      flags.p,
+      sym ? midx.p : cidx.p,
       sym ? mkey.p : ckey.p, C, (ra && dpad < 512) ? 1 : 2, sym ? bnd.p : nullptr, k, metric,
       gamma, d_idx, d_dist, fcount.p, frows.p);
   EVOC_HIP(hipGetLastError());

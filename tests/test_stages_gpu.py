@@ -296,3 +296,26 @@ def test_strided_input_matches_contiguous():
     a = H.evoc(x.astype(np.float32), seed=3)
     b = H.evoc(xf, seed=3)
     assert np.array_equal(a.cluster_layers, b.cluster_layers)
+
+
+def test_result_arrays_outlive_the_result():
+    """The arrays of an EvocResult view the library's result buffers: one array kept alone
+    keeps them, and later calls (which allocate and free results of their own) leave it
+    intact."""
+    import gc
+
+    x, _ = datagen.gaussian_mixture(3000, 32, 8, 5.0, 1.0, seed=11)
+    r = H.evoc(x, seed=3)
+    emb, lab, nn = r.embedding, r.cluster_layers, r.nn_indices
+    keep = (emb.copy(), lab.copy(), nn.copy())
+    del r
+    gc.collect()
+    for seed in (4, 5):
+        r2 = H.evoc(x, seed=seed)
+        assert r2.embedding.shape == emb.shape
+        del r2
+        gc.collect()
+    assert np.array_equal(emb.view(np.uint32), keep[0].view(np.uint32))
+    assert np.array_equal(lab, keep[1])
+    assert np.array_equal(nn, keep[2])
+    emb[0, 0] = 1.0  # writable views
