@@ -6,6 +6,9 @@
 // reference's sparse algebra (src/utils/sparse.rs) so the layout is reproducible:
 //   * coarsening  R^T_norm G R   (per-partition dense accumulator, columns ascending),
 //   * expansion   ((G o G^T) R_norm)_l1 L  and  (R_norm)_l1 L, averaged, centred, x0.1.
+#include <omp.h>
+
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <numeric>
@@ -468,6 +471,64 @@ std::vector<float> level(const Sp& g, int dim, const Rows& data, int64_t d, int 
     lap("expansion");
     return res;
   }
+  // mate[p]: the position of G(j, i) for the entry p = (i, j), or -1. The transpose
+  // (entries by column, rows ascending within one: per-thread counts and cursors) is
+  // merged with each row, both in ascending order: O(nnz) in all.
+  const int64_t nnz = g.ptr[n];
+  std::vector<int64_t> mate(nnz, -1);
+  {
+    const int T = std::max(1, omp_get_max_threads());
+    const size_t W = (size_t)n + 1;
+    std::vector<int64_t> cur((size_t)T * W, 0);
+#pragma omp parallel num_threads(T)
+    {
+      const int t = omp_get_thread_num();
+      int64_t* c = &cur[(size_t)t * W];
+      for (int64_t i = n * t / T; i < n * (t + 1) / T; ++i)
+        for (int64_t p = g.ptr[i]; p < g.ptr[i + 1]; ++p) ++c[g.col[p]];
+    }
+    std::vector<int64_t> tptr(n + 1);
+    int64_t run = 0;
+    for (int64_t j = 0; j < n; ++j) {
+      tptr[j] = run;
+      for (int t = 0; t < T; ++t) {
+        const int64_t k = cur[(size_t)t * W + j];
+        cur[(size_t)t * W + j] = run;
+        run += k;
+      }
+    }
+    tptr[n] = run;
+    std::vector<int32_t> trow(std::max<int64_t>(nnz, 1));
+    std::vector<int64_t> tpos(std::max<int64_t>(nnz, 1));
+#pragma omp parallel num_threads(T)
+    {
+      const int t = omp_get_thread_num();
+      int64_t* c = &cur[(size_t)t * W];
+      for (int64_t i = n * t / T; i < n * (t + 1) / T; ++i)
+        for (int64_t p = g.ptr[i]; p < g.ptr[i + 1]; ++p) {
+          const int64_t q = c[g.col[p]]++;
+          trow[q] = (int32_t)i;
+          tpos[q] = p;
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t j = 0; j < n; ++j) {
+      int64_t a = g.ptr[j], b = tptr[j];
+      const int64_t ae = g.ptr[j + 1], be = tptr[j + 1];
+      while (a < ae && b < be) {
+        const int32_t c = g.col[a], r = trow[b];
+        if (c < r) {
+          ++a;
+        } else if (r < c) {
+          ++b;
+        } else {
+          mate[tpos[b]] = a;  // entry (r, j) meets G(j, r)
+          ++a;
+          ++b;
+        }
+      }
+    }
+  }
 #pragma omp parallel
   {
     std::vector<float> acc(np, 0.0f);
@@ -479,12 +540,10 @@ std::vector<float> level(const Sp& g, int dim, const Rows& data, int64_t d, int 
       touched.clear();
       for (int64_t p = g.ptr[i]; p < g.ptr[i + 1]; ++p) {
         const int32_t j = g.col[p];
-        // G^T(i, j) = G(j, i): binary search in row j (columns sorted)
-        const int32_t* b = g.col.data() + g.ptr[j];
-        const int32_t* e = g.col.data() + g.ptr[j + 1];
-        const int32_t* f = std::lower_bound(b, e, (int32_t)i);
-        if (f == e || *f != (int32_t)i) continue;
-        const float sym = g.val[p] * g.val[g.ptr[j] + (f - b)];
+        // G^T(i, j) = G(j, i)
+        const int64_t q = mate[p];
+        if (q < 0) continue;
+        const float sym = g.val[p] * g.val[q];
         const int32_t a = part[j];
         const float pr = sym * sa[a];
         if (!hit[a]) {
