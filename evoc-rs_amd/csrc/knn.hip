@@ -1236,7 +1236,7 @@ __global__ void rb_sample_kernel(const uint16_t* __restrict__ Xh, const float* _
 // A row's reverse threshold: the RB_THR_RANK-th smallest key of its two sorted half-lists
 // against the sample (any threshold is correct; this one queues about RB_THR_RANK / S of
 // the transposed keys, S the sample size).
-constexpr int RB_THR_RANK = 16;
+constexpr int RB_THR_RANK = 8;
 __global__ void rb_thr0_kernel(const float* __restrict__ ckey, int64_t n, int64_t npad,
                                float* __restrict__ thr0) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
