@@ -1236,7 +1236,7 @@ __global__ void rb_sample_kernel(const uint16_t* __restrict__ Xh, const float* _
 // A row's reverse threshold: the RB_THR_RANK-th smallest key of its two sorted half-lists
 // against the sample (any threshold is correct; this one queues about RB_THR_RANK / S of
 // the transposed keys, S the sample size).
-constexpr int RB_THR_RANK = 8;
+constexpr int RB_THR_RANK = 6;
 __global__ void rb_thr0_kernel(const float* __restrict__ ckey, int64_t n, int64_t npad,
                                float* __restrict__ thr0) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1813,7 +1813,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   Buf<float> ss, thr0, qk;
   Buf<int32_t> qj, qr, qcount;
   if (sym) {
-    sym_S = std::min<int64_t>(65536, std::max<int64_t>(256, n / 32 / 256 * 256));
+    sym_S = std::min<int64_t>(65536, std::max<int64_t>(256, n / 64 / 256 * 256));
     sym_S = std::min<int64_t>(sym_S, n);
     sym_Spad = ceil_div(sym_S, RA_BN) * RA_BN;
     Xs.alloc(pool, sym_Spad * dpad);
