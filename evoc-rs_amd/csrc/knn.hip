@@ -801,15 +801,16 @@ struct RbRing {
 // symmetric sweep: each wavefront stages its reverse candidates (column, row, dot) in LDS
 // and copies them to its global queue in batches, with no vector-memory operation of the
 // ring in flight (stores mixed with the ring's loads would void its counted waits)
+constexpr int RB_PASS_C = 8;  // candidates per (row, half) kept by the threshold pass
 constexpr int RB_QL = 192;                                      // LDS queue entries per wavefront
 constexpr size_t RB_QBYTES = (size_t)RB_WAVES * RB_QL * 12;     // after the ring
 
-template <int NKC, int KC, bool COS, bool SYM>
+// C: candidates kept per (row, column half); the threshold pass needs only RB_THR_RANK
+template <int NKC, int KC, bool COS, bool SYM, int C = 16>
 __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     const uint16_t* __restrict__ Xh, int64_t n, int64_t npad, int dpad,
     const float* __restrict__ colscale, int32_t* __restrict__ cand_idx,
     float* __restrict__ cand_key, const RbCols cols) {
-  constexpr int C = 16;
   using R = RbRing<KC>;
   constexpr int NST = R::NST;
   constexpr int LOADS = R::LOADS;
@@ -1203,6 +1204,10 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   }
 }
 
+template __global__ void knn_topc_rb_kernel<2, 256, true, false, RB_PASS_C>(
+    const uint16_t*, int64_t, int64_t, int, const float*, int32_t*, float*, const RbCols);
+template __global__ void knn_topc_rb_kernel<2, 256, false, false, RB_PASS_C>(
+    const uint16_t*, int64_t, int64_t, int, const float*, int32_t*, float*, const RbCols);
 template __global__ void knn_topc_rb_kernel<2, 256, true, false>(const uint16_t*, int64_t, int64_t,
                                                                  int, const float*, int32_t*,
                                                                  float*, const RbCols);
@@ -1237,20 +1242,22 @@ __global__ void rb_sample_kernel(const uint16_t* __restrict__ Xh, const float* _
 // against the sample (any threshold is correct; this one queues about RB_THR_RANK / S of
 // the transposed keys, S the sample size).
 constexpr int RB_THR_RANK = 6;
+static_assert(RB_THR_RANK <= RB_PASS_C, "the pass lists must hold the threshold rank");
 __global__ void rb_thr0_kernel(const float* __restrict__ ckey, int64_t n, int64_t npad,
                                float* __restrict__ thr0) {
+  constexpr int L = RB_PASS_C;
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= npad) return;
   if (r >= n) {
     thr0[r] = -FINF;
     return;
   }
-  const float* a = ckey + (r * 2) * 16;
-  const float* b = ckey + (r * 2 + 1) * 16;
+  const float* a = ckey + (r * 2) * L;
+  const float* b = ckey + (r * 2 + 1) * L;
   int ia = 0, ib = 0;
   float v = FINF;
   for (int t = 0; t < RB_THR_RANK; ++t) {
-    const float x = ia < 16 ? a[ia] : FINF, y = ib < 16 ? b[ib] : FINF;
+    const float x = ia < L ? a[ia] : FINF, y = ib < L ? b[ib] : FINF;
     if (x <= y) {
       v = x;
       ++ia;
@@ -1735,6 +1742,10 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<256>::SMEM));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, true, false>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<256>::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, true, false, RB_PASS_C>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<256>::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, false, false, RB_PASS_C>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<256>::SMEM));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, false, false>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<256>::SMEM));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, true, true>,
@@ -1857,10 +1868,10 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
         RbCols sc{Xs.p, ss.p, nullptr, sym_S, sym_Spad / RA_BN, 0, nullptr, nullptr, nullptr, 0,
                   nullptr};
         if (metric == 1)
-          knn_topc_rb_kernel<2, 256, true, false><<<grid, 512, RbRing<256>::SMEM, s>>>(
+          knn_topc_rb_kernel<2, 256, true, false, RB_PASS_C><<<grid, 512, RbRing<256>::SMEM, s>>>(
               Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, sc);
         else
-          knn_topc_rb_kernel<2, 256, false, false><<<grid, 512, RbRing<256>::SMEM, s>>>(
+          knn_topc_rb_kernel<2, 256, false, false, RB_PASS_C><<<grid, 512, RbRing<256>::SMEM, s>>>(
               Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, sc);
         rb_thr0_kernel<<<(unsigned)ceil_div(npad, 256), 256, 0, s>>>(ckey.p, n, npad, thr0.p);
         EVOC_HIP(hipEventRecord(e0, s));
