@@ -271,7 +271,8 @@ __global__ __launch_bounds__(256) void exp_node_kernel(
 // (the reference's iterator sum). The workgroup streams 64 KB chunks of rows into LDS,
 // transposed, double-buffered; one lane per column then runs the dependent adds out of
 // LDS while the next chunk's loads are in flight.
-constexpr int CS_THREADS = 256;
+constexpr int CS_THREADS = 512;
+constexpr int CS_LOADERS = CS_THREADS - 64;  // wavefront 0 sums, the others stage chunks
 constexpr int CS_FLOATS = 16384;  // floats per chunk buffer (64 KB)
 
 // LDS pitch of one column: 4 mod 64 words, so that the summing lanes' 16-byte reads start
@@ -287,22 +288,28 @@ __global__ __launch_bounds__(CS_THREADS) void exp_colsum_kernel(const float* __r
   const int pitch = cs_pitch(dim);
   const int t = threadIdx.x;
   const int64_t nchunks = (n + rows - 1) / rows;
-  constexpr int PER = CS_FLOATS / CS_THREADS;  // floats per thread per chunk
+  // wavefront 0 only sums, so that its dependent adds never wait on its own share of the
+  // staging; the other wavefronts load the next chunk and store it transposed
+  constexpr int PER = (CS_FLOATS + CS_LOADERS - 1) / CS_LOADERS;  // floats per loader
+  const bool loader = t >= 64;
+  const int lt = t - 64;
   float stage[PER];
   auto load = [&](int64_t k) {
+    if (!loader) return;
     const int64_t base = k * rows * dim;
     const int64_t lim = n * dim - base;
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      const int q = u * CS_THREADS + t;
+      const int q = u * CS_LOADERS + lt;
       stage[u] = (q < rows * dim && q < lim) ? res[base + q] : 0.0f;
     }
   };
   auto store = [&](int buf) {
+    if (!loader) return;
     float* B = cs_buf + buf * dim * pitch;
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      const int q = u * CS_THREADS + t;
+      const int q = u * CS_LOADERS + lt;
       if (q < rows * dim) B[(q % dim) * pitch + q / dim] = stage[u];
     }
   };
