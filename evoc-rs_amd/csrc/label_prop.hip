@@ -14,7 +14,7 @@ namespace evoc {
 namespace {
 
 constexpr int PM_THREADS = 256;
-constexpr int PM_AHEAD = 16;
+constexpr int PM_AHEAD = 32;  // member rows loaded ahead of their in-order adds
 
 // one workgroup per (partition, stripe of 256 columns)
 __global__ __launch_bounds__(PM_THREADS) void partition_means_kernel(
