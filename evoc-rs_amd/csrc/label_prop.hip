@@ -493,6 +493,48 @@ __global__ void lp_idx_to_mem_kernel(int64_t n, const uint32_t* __restrict__ idx
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) mem[i] = (int32_t)idx[i];
 }
+// x added cnt times to 0.0f in f32, one rounding per add, for a positive normal x whose
+// partial sums stay finite. Within one binade [2^(e-1), 2^e) of the running sum every add
+// moves it by the same multiple of the binade's ulp u (x/u rounded to nearest; on a tie the
+// first add makes the sum's last bit even and later adds keep it so), so whole runs of adds
+// are taken in one double-precision step and only the add that leaves a binade is done in f32.
+__device__ float repeated_sum(float x, int64_t cnt) {
+  if (cnt <= 0) return 0.0f;
+  float s = x;  // 0 + x is exact
+  int64_t k = 1;
+  while (k < cnt) {
+    int e;
+    frexp((double)s, &e);  // s in [2^(e-1), 2^e)
+    const double u = ldexp(1.0, e - 24), top = ldexp(1.0, e);
+    const double q = (double)x / u, f = floor(q);
+    double step;
+    if (q - f == 0.5) {
+      s = s + x;
+      ++k;
+      int e2;
+      frexp((double)s, &e2);
+      if (e2 != e || k >= cnt) continue;
+      step = (fmod(f, 2.0) == 0.0 ? f : f + 1.0) * u;
+    } else {
+      step = (q - f < 0.5 ? f : f + 1.0) * u;
+    }
+    if (step <= 0.0) break;  // x below half an ulp: the sum no longer moves
+    // adds j = 1..m start below top - x, so each lands on s + j*step (at most top)
+    const double room = top - (double)x - (double)s;
+    int64_t m = room > 0.0 ? (int64_t)ceil(room / step) : 0;
+    while (m > 0 && (double)(m - 1) * step >= room) --m;
+    while ((double)m * step < room) ++m;
+    if (m > cnt - k) m = cnt - k;
+    s = (float)((double)s + (double)m * step);
+    k += m;
+    if (k < cnt) {
+      s = s + x;
+      ++k;
+    }
+  }
+  return s;
+}
+
 // column norm 1/sqrt(|a|) of the partition map (1.0f added |a| times, as the host sums
 // it) and the L1 row normalisation of R^T_norm (|sa| added |a| times in order)
 __global__ void lp_norms_kernel(int64_t np, const int64_t* __restrict__ mptr,
@@ -503,9 +545,7 @@ __global__ void lp_norms_kernel(int64_t np, const int64_t* __restrict__ mptr,
   const float sq = (float)min<int64_t>(cnt, 16777216);  // f32 count saturates at 2^24
   const float nrm = sqrtf(sq);
   const float s = nrm > 0.0f ? 1.0f / nrm : 1.0f;
-  float l1 = 0.0f;
-  const float as = fabsf(s);
-  for (int64_t m = 0; m < cnt; ++m) l1 += as;
+  const float l1 = repeated_sum(fabsf(s), cnt);  // s is in [2^-12, 1]
   sa[a] = s;
   coef[a] = l1 > 0.0f ? s * (1.0f / l1) : s;
 }
