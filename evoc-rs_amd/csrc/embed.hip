@@ -20,7 +20,6 @@ namespace evoc {
 namespace {
 
 constexpr int MAXD = 16;
-constexpr int WIDE_DEG = 64;   // nodes with at least this many edges take the wide kernel
 constexpr int WIDE_CAP = 256;  // gradient terms staged per wavefront at a time
 
 __global__ void max_weight_kernel(const float* __restrict__ w, int64_t nnz,
@@ -331,12 +330,12 @@ __global__ __launch_bounds__(256) void epoch_wide_kernel(
   }
 }
 
-// nodes with at least WIDE_DEG edges, in any order
-__global__ void wide_nodes_kernel(const int64_t* __restrict__ indptr, int64_t n,
+// nodes with at least wide_deg edges, in any order
+__global__ void wide_nodes_kernel(const int64_t* __restrict__ indptr, int64_t n, int64_t wide_deg,
                                   int32_t* __restrict__ list, int* __restrict__ count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (indptr[i + 1] - indptr[i] >= WIDE_DEG) list[atomicAdd(count, 1)] = (int32_t)i;
+  if (indptr[i + 1] - indptr[i] >= wide_deg) list[atomicAdd(count, 1)] = (int32_t)i;
 }
 
 template <int D>
@@ -344,10 +343,10 @@ void launch_epoch(unsigned grid, hipStream_t s, int64_t n, int dim, const int64_
                   const int32_t* other, float* ns, const float* es, float* nn, float nf,
                   const float* eo, float* en, float* m, float* v, float et, uint32_t ep, float la,
                   float epsc, float gamma, float noise, float b1, float b2, uint64_t seed,
-                  const int32_t* wide, int64_t nwide) {
+                  const int32_t* wide, int64_t nwide, int64_t wide_deg) {
   epoch_kernel<D><<<grid, 256, 0, s>>>(n, dim, indptr, other, ns, es, nn, nf, eo, en, m, v, et, ep,
                                        la, epsc, gamma, noise, b1, b2, seed,
-                                       nwide > 0 ? (int64_t)WIDE_DEG : INT64_MAX);
+                                       nwide > 0 ? wide_deg : INT64_MAX);
   if (nwide > 0)
     epoch_wide_kernel<D><<<(unsigned)ceil_div(nwide, 4), 256, 0, s>>>(
         n, dim, indptr, other, ns, es, nn, nf, eo, en, m, v, et, ep, la, epsc, gamma, noise, b1,
@@ -388,7 +387,8 @@ void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, b
   Buf<int32_t> wide(pool, n);
   Buf<int> nwide_d(pool, 1);
   EVOC_HIP(hipMemsetAsync(nwide_d.p, 0, sizeof(int), s));
-  wide_nodes_kernel<<<grid, 256, 0, s>>>(g.indptr.p, n, wide.p, nwide_d.p);
+  EVOC_CHECK(p.wide_deg >= 1, "embedding: wide_deg must be >= 1");
+  wide_nodes_kernel<<<grid, 256, 0, s>>>(g.indptr.p, n, p.wide_deg, wide.p, nwide_d.p);
   int nwide_h = 0;
   EVOC_HIP(hipMemcpyAsync(&nwide_h, nwide_d.p, sizeof(int), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
@@ -412,7 +412,7 @@ void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, b
 #define EVOC_EPOCH(D)                                                                         \
   launch_epoch<D>(grid, s, n, dim, g.indptr.p, g.indices.p, next_s.p, eps_s.p, next_n.p,    \
                   neg_factor, cur, nxt, m.p, v.p, epoch_t, (uint32_t)epoch, lr_ad, epsc, gamma, \
-                  p.noise_level, one_m_b1, one_m_b2, seed, wide.p, nwide)
+                  p.noise_level, one_m_b1, one_m_b2, seed, wide.p, nwide, p.wide_deg)
     switch (dim) {
       case 2: EVOC_EPOCH(2); break;
       case 4: EVOC_EPOCH(4); break;

@@ -128,6 +128,9 @@ struct EmbedParams {
   float negative_sample_rate = 1.0f;
   float initial_alpha = 0.1f;
   float beta1 = 0.5f, beta2 = 0.9f, eps = 1e-7f;
+  // nodes with at least this many edges take the wavefront-per-node epoch kernel (same
+  // arithmetic, so any value gives the same layout); small coarse graphs use 1
+  int64_t wide_deg = 64;
 };
 // d_emb holds the initial layout on entry (or is filled from Philox when
 // random_init) and the final layout on exit.

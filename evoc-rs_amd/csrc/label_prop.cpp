@@ -345,6 +345,7 @@ void coarse_embedding(const Sp& g, int dim, int epochs, float alpha, std::vector
   EmbedParams ep;
   ep.n_epochs = epochs;
   ep.initial_alpha = alpha;
+  ep.wide_deg = 1;  // a few thousand nodes: one wavefront each keeps their gathers in parallel
   embed_device(dg, dim, ep, de.p, false, seed, s, pool);
   EVOC_HIP(hipMemcpyAsync(layout.data(), de.p, n * dim * sizeof(float), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
@@ -361,6 +362,7 @@ void coarse_embedding_dev(const Graph& g, int dim, int epochs, float alpha,
   EmbedParams ep;
   ep.n_epochs = epochs;
   ep.initial_alpha = alpha;
+  ep.wide_deg = 1;  // a few thousand nodes: one wavefront each keeps their gathers in parallel
   embed_device(dg, dim, ep, de.p, false, seed, s, pool);
   EVOC_HIP(hipMemcpyAsync(layout.data(), de.p, n * dim * sizeof(float), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
