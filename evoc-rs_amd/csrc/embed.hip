@@ -183,21 +183,64 @@ __global__ __launch_bounds__(256) void epoch_wide_kernel(
   uint32_t draw_carry = 0;
   const int64_t s0 = indptr[i], s1 = indptr[i + 1];
   float* vb = vbuf[wave];
+  // Loads run ahead of the 64-slot chunks: a chunk's slot inputs two chunks ahead, its
+  // positive endpoint row one chunk ahead, so inside a chunk only the negative samples'
+  // rows are waited for. Values are read for lanes past the row end or not due this epoch
+  // only where the slot exists, and used only for due slots.
+  float ns_a = 0.0f, ps_a = 0.0f, nn_a = 0.0f, ns_b = 0.0f, ps_b = 0.0f, nn_b = 0.0f;
+  int32_t j_a = 0, j_b = 0;  // _a: the current chunk, _b: the next one
+  float eo_a[DD];
+  if (s0 + lane < s1) {
+    ns_a = next_s[s0 + lane];
+    j_a = other[s0 + lane];
+    ps_a = eps_s[s0 + lane];
+    nn_a = next_n[s0 + lane];
+  }
+  if (s0 + 64 + lane < s1) {
+    ns_b = next_s[s0 + 64 + lane];
+    j_b = other[s0 + 64 + lane];
+    ps_b = eps_s[s0 + 64 + lane];
+    nn_b = next_n[s0 + 64 + lane];
+  }
+  {
+    const bool due = s0 + lane < s1 && !(ns_a > epoch_t);
+#pragma unroll
+    for (int c = 0; c < DD; ++c) eo_a[c] = due && c < nd ? e_old[(int64_t)j_a * nd + c] : 0.0f;
+  }
   for (int64_t sb = s0; sb < s1; sb += 64) {
     const int64_t s = sb + lane;
     const bool valid = s < s1;
-    const float ns = valid ? next_s[s] : 0.0f;
+    const float ns = valid ? ns_a : 0.0f;
     const bool active = valid && !(ns > epoch_t);
     int32_t j = 0;
     float ps = 0.0f, pn = 0.0f, nn = 0.0f;
     uint64_t cnt = 0;
     if (active) {
-      j = other[s];
-      ps = eps_s[s];
+      j = j_a;
+      ps = ps_a;
       pn = ps * neg_factor;
-      nn = next_n[s];
+      nn = nn_a;
       const float f = floorf((epoch_t - nn) / pn);
       cnt = (f >= 0.0f && f < 1.8e19f) ? (uint64_t)f : 0;
+    }
+    float eo[DD];
+#pragma unroll
+    for (int c = 0; c < DD; ++c) eo[c] = eo_a[c];
+    // the next chunk's positive rows and the slot inputs of the one after it
+    {
+      const bool due = s + 64 < s1 && !(ns_b > epoch_t);
+#pragma unroll
+      for (int c = 0; c < DD; ++c) eo_a[c] = due && c < nd ? e_old[(int64_t)j_b * nd + c] : 0.0f;
+      ns_a = ns_b;
+      j_a = j_b;
+      ps_a = ps_b;
+      nn_a = nn_b;
+      if (s + 128 < s1) {
+        ns_b = next_s[s + 128];
+        j_b = other[s + 128];
+        ps_b = eps_s[s + 128];
+        nn_b = next_n[s + 128];
+      }
     }
     touched |= __ballot(active) != 0;
     const uint32_t nneg = (active && (int64_t)j > i) ? (uint32_t)cnt : 0u;
@@ -217,12 +260,9 @@ __global__ __launch_bounds__(256) void epoch_wide_kernel(
     const uint32_t total = __shfl(vinc, 63, 64);
     draw_carry += __shfl(dinc, 63, 64);
     // this lane's positive term
-    float eo[DD];
     float coeff = 0.0f;
     bool pos_ok = false;
     if (active) {
-#pragma unroll
-      for (int c = 0; c < DD; ++c) eo[c] = c < nd ? e_old[(int64_t)j * nd + c] : 0.0f;
       float dsq = 0.0f;
 #pragma unroll
       for (int c = 0; c < DD; ++c)
