@@ -21,6 +21,9 @@ namespace {
 
 constexpr int MAXD = 16;
 constexpr int WIDE_CAP = 256;  // gradient terms staged per wavefront at a time
+constexpr int HUB_DEG = 1024;  // nodes with at least this many edges take a whole workgroup
+constexpr int HUB_WAVES = 16;  // wavefronts per hub node: one 64-slot chunk each at a time
+constexpr int HUB_CAP = 1024;  // gradient terms staged per hub workgroup at a time
 
 __global__ void max_weight_kernel(const float* __restrict__ w, int64_t nnz,
                                   unsigned* __restrict__ out) {
@@ -370,12 +373,205 @@ __global__ __launch_bounds__(256) void epoch_wide_kernel(
   }
 }
 
-// nodes with at least wide_deg edges, in any order
+// One workgroup per node with at least HUB_DEG edges (a coarse graph's largest partition
+// can touch every other one, thousands of 64-slot chunks in one row); identical
+// arithmetic to epoch_kernel. The row is walked in groups of HUB_WAVES chunks: wavefront w
+// takes the w-th chunk of a group, a scan over the wavefronts' totals gives each slot its
+// draw numbers and the position of its terms, the terms of the group are staged in LDS
+// (HUB_CAP at a time) and wavefront 0 adds them in slot order.
+template <int D>
+__global__ __launch_bounds__(1024) void epoch_hub_kernel(
+    int64_t n, int dim, const int64_t* __restrict__ indptr, const int32_t* __restrict__ other,
+    float* __restrict__ next_s, const float* __restrict__ eps_s, float* __restrict__ next_n,
+    float neg_factor, const float* __restrict__ e_old, float* __restrict__ e_new,
+    float* __restrict__ m, float* __restrict__ v, float epoch_t, uint32_t epoch, float lr_ad,
+    float epsc, float gamma, float noise, float one_m_b1, float one_m_b2, uint64_t seed,
+    const int32_t* __restrict__ hub_nodes) {
+  constexpr int DD = D > 0 ? D : MAXD;
+  __shared__ float tbuf[HUB_CAP * DD];
+  __shared__ uint32_t wdraw[HUB_WAVES], wterm[HUB_WAVES];
+  __shared__ int wtouch[HUB_WAVES];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t i = hub_nodes[blockIdx.x];  // the grid has one workgroup per listed node
+  const int nd = D > 0 ? D : dim;
+  float ei[DD];
+#pragma unroll
+  for (int c = 0; c < DD; ++c) ei[c] = c < nd ? e_old[i * nd + c] : 0.0f;
+  float gl = 0.0f;  // wavefront 0, lane c < nd: gradient component c
+  bool touched = false;
+  uint32_t draw_carry = 0;
+  const int64_t s0 = indptr[i], s1 = indptr[i + 1];
+  for (int64_t gb = s0; gb < s1; gb += 64 * HUB_WAVES) {
+    const int64_t s = gb + (int64_t)wave * 64 + lane;
+    const bool valid = s < s1;
+    const float ns = valid ? next_s[s] : 0.0f;
+    const bool active = valid && !(ns > epoch_t);
+    int32_t j = 0;
+    float ps = 0.0f, pn = 0.0f, nn = 0.0f;
+    uint64_t cnt = 0;
+    if (active) {
+      j = other[s];
+      ps = eps_s[s];
+      pn = ps * neg_factor;
+      nn = next_n[s];
+      const float f = floorf((epoch_t - nn) / pn);
+      cnt = (f >= 0.0f && f < 1.8e19f) ? (uint64_t)f : 0;
+    }
+    touched |= __ballot(active) != 0;
+    const uint32_t nneg = (active && (int64_t)j > i) ? (uint32_t)cnt : 0u;
+    const uint32_t nvec = active ? 1u + nneg : 0u;
+    // inclusive prefix sums over the lanes (slot order) of draws and of staged terms
+    uint32_t dinc = nneg, vinc = nvec;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t td = __shfl_up(dinc, o, 64), tv = __shfl_up(vinc, o, 64);
+      if (lane >= o) {
+        dinc += td;
+        vinc += tv;
+      }
+    }
+    if (lane == 63) {
+      wdraw[wave] = dinc;
+      wterm[wave] = vinc;
+    }
+    __syncthreads();
+    uint32_t dbase = draw_carry, vbase = 0, dtot = 0, vtot = 0;
+    for (int w = 0; w < HUB_WAVES; ++w) {  // earlier chunks of the group, and the group totals
+      const uint32_t a = wdraw[w], b = wterm[w];
+      if (w < wave) {
+        dbase += a;
+        vbase += b;
+      }
+      dtot += a;
+      vtot += b;
+    }
+    __syncthreads();  // every wavefront has read the totals before the next group's are written
+    const uint32_t draw0 = dbase + dinc - nneg;
+    const uint32_t voff = vbase + vinc - nvec;
+    draw_carry += dtot;
+    // this lane's positive term
+    float eo[DD];
+    float coeff = 0.0f;
+    bool pos_ok = false;
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < DD; ++c) eo[c] = c < nd ? e_old[(int64_t)j * nd + c] : 0.0f;
+      float dsq = 0.0f;
+#pragma unroll
+      for (int c = 0; c < DD; ++c)
+        if (c < nd) {
+          const float df = ei[c] - eo[c];
+          const float p = df * df;
+          dsq = dsq + p;
+        }
+      if (dsq >= 1e-8f) {
+        const float dist = sqrtf(dsq);
+        const float num = -((2.0f * noise) * dist) - 2.0f;
+        const float den = ((2.0f * dsq) - (0.5f * dist)) + 1.0f;
+        coeff = num / den;
+        pos_ok = true;
+      }
+    }
+    for (uint32_t w0 = 0; w0 < vtot; w0 += HUB_CAP) {
+      // terms of this lane that fall in the window [w0, w0 + HUB_CAP)
+      const uint32_t lo = voff > w0 ? voff : w0;
+      const uint32_t hi = min(voff + nvec, w0 + (uint32_t)HUB_CAP);
+      for (uint32_t t = lo; t < hi; ++t) {
+        const uint32_t q = t - voff;
+        float* tb = tbuf + (t - w0) * DD;
+        bool good = false;
+        if (q == 0) {
+          good = pos_ok;
+          if (good) {
+#pragma unroll
+            for (int c = 0; c < DD; ++c)
+              if (c < nd) tb[c] = coeff * (ei[c] - eo[c]);
+          }
+        } else {
+          const uint32_t r = Philox::first(seed, draw0 + (q - 1), epoch, (uint32_t)i, TAG_NEG);
+          const int64_t kk = Philox::range(r, (uint64_t)n);
+          if (kk != i) {
+            float ek[DD];
+#pragma unroll
+            for (int c = 0; c < DD; ++c) ek[c] = c < nd ? e_old[kk * nd + c] : 0.0f;
+            float dk = 0.0f;
+#pragma unroll
+            for (int c = 0; c < DD; ++c)
+              if (c < nd) {
+                const float df = ei[c] - ek[c];
+                const float p = df * df;
+                dk = dk + p;
+              }
+            if (dk > 0.01f) {
+              good = true;
+              const float gc = (gamma * 4.0f) / ((1.0f + 0.25f * dk) * dk);
+#pragma unroll
+              for (int c = 0; c < DD; ++c)
+                if (c < nd) {
+                  const float diff = ei[c] - ek[c];
+                  tb[c] = fminf(fmaxf(gc * diff, -4.0f), 4.0f);
+                }
+            }
+          }
+        }
+        // a skipped term is staged as +0, which leaves the running sum's bits unchanged
+        if (!good) {
+#pragma unroll
+          for (int c = 0; c < DD; ++c)
+            if (c < nd) tb[c] = 0.0f;
+        }
+      }
+      __syncthreads();
+      if (wave == 0 && lane < nd) {
+        const uint32_t cntw = min(vtot - w0, (uint32_t)HUB_CAP);
+        uint32_t t = 0;
+        for (; t + 8 <= cntw; t += 8) {  // reads ahead of the dependent adds
+          float x[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) x[u] = tbuf[(t + u) * DD + lane];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) gl = gl + x[u];
+        }
+        for (; t < cntw; ++t) gl = gl + tbuf[t * DD + lane];
+      }
+      __syncthreads();
+    }
+    if (active) {
+      next_s[s] = ns + ps;
+      next_n[s] = nn + (float)cnt * pn;
+    }
+  }
+  if (lane == 0) wtouch[wave] = touched ? 1 : 0;
+  __syncthreads();
+  if (wave == 0 && lane < nd) {
+    bool any = false;
+    for (int w = 0; w < HUB_WAVES; ++w) any |= wtouch[w] != 0;
+    const int64_t qd = i * nd + lane;
+    const float eic = e_old[qd];
+    if (any) {
+      float mq = m[qd], vq = v[qd];
+      mq = mq + one_m_b1 * (gl - mq);
+      vq = vq + one_m_b2 * (gl * gl - vq);
+      m[qd] = mq;
+      v[qd] = vq;
+      const float step = (lr_ad * mq) / (sqrtf(vq) + epsc);
+      e_new[qd] = eic + step;
+    } else {
+      e_new[qd] = eic;
+    }
+  }
+}
+
+// nodes with at least wide_deg edges, in any order: those below hub_deg into list, the
+// others into hubs
 __global__ void wide_nodes_kernel(const int64_t* __restrict__ indptr, int64_t n, int64_t wide_deg,
-                                  int32_t* __restrict__ list, int* __restrict__ count) {
+                                  int64_t hub_deg, int32_t* __restrict__ list,
+                                  int32_t* __restrict__ hubs, int* __restrict__ count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (indptr[i + 1] - indptr[i] >= wide_deg) list[atomicAdd(count, 1)] = (int32_t)i;
+  const int64_t deg = indptr[i + 1] - indptr[i];
+  if (deg >= hub_deg) hubs[atomicAdd(count + 1, 1)] = (int32_t)i;
+  else if (deg >= wide_deg) list[atomicAdd(count, 1)] = (int32_t)i;
 }
 
 template <int D>
@@ -383,14 +579,19 @@ void launch_epoch(unsigned grid, hipStream_t s, int64_t n, int dim, const int64_
                   const int32_t* other, float* ns, const float* es, float* nn, float nf,
                   const float* eo, float* en, float* m, float* v, float et, uint32_t ep, float la,
                   float epsc, float gamma, float noise, float b1, float b2, uint64_t seed,
-                  const int32_t* wide, int64_t nwide, int64_t wide_deg) {
+                  const int32_t* wide, int64_t nwide, int64_t wide_deg, const int32_t* hubs,
+                  int64_t nhub) {
   epoch_kernel<D><<<grid, 256, 0, s>>>(n, dim, indptr, other, ns, es, nn, nf, eo, en, m, v, et, ep,
                                        la, epsc, gamma, noise, b1, b2, seed,
-                                       nwide > 0 ? wide_deg : INT64_MAX);
+                                       nwide + nhub > 0 ? wide_deg : INT64_MAX);
   if (nwide > 0)
     epoch_wide_kernel<D><<<(unsigned)ceil_div(nwide, 4), 256, 0, s>>>(
         n, dim, indptr, other, ns, es, nn, nf, eo, en, m, v, et, ep, la, epsc, gamma, noise, b1,
         b2, seed, wide, nwide);
+  if (nhub > 0)
+    epoch_hub_kernel<D><<<(unsigned)nhub, 64 * HUB_WAVES, 0, s>>>(
+        n, dim, indptr, other, ns, es, nn, nf, eo, en, m, v, et, ep, la, epsc, gamma, noise, b1,
+        b2, seed, hubs);
 }
 
 }  // namespace
@@ -424,15 +625,17 @@ void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, b
   const float n_epochs_f = (float)p.n_epochs;
   const unsigned grid = (unsigned)ceil_div(n, 256);
   // high-degree nodes for the wavefront-per-node kernel
-  Buf<int32_t> wide(pool, n);
-  Buf<int> nwide_d(pool, 1);
-  EVOC_HIP(hipMemsetAsync(nwide_d.p, 0, sizeof(int), s));
+  Buf<int32_t> wide(pool, n), hubs(pool, n);
+  Buf<int> nwide_d(pool, 2);
+  EVOC_HIP(hipMemsetAsync(nwide_d.p, 0, 2 * sizeof(int), s));
   EVOC_CHECK(p.wide_deg >= 1, "embedding: wide_deg must be >= 1");
-  wide_nodes_kernel<<<grid, 256, 0, s>>>(g.indptr.p, n, p.wide_deg, wide.p, nwide_d.p);
-  int nwide_h = 0;
-  EVOC_HIP(hipMemcpyAsync(&nwide_h, nwide_d.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  const int64_t hub_deg = std::max<int64_t>(p.wide_deg, HUB_DEG);
+  wide_nodes_kernel<<<grid, 256, 0, s>>>(g.indptr.p, n, p.wide_deg, hub_deg, wide.p, hubs.p,
+                                         nwide_d.p);
+  int nwide_h[2] = {0, 0};
+  EVOC_HIP(hipMemcpyAsync(nwide_h, nwide_d.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
-  const int64_t nwide = nwide_h;
+  const int64_t nwide = nwide_h[0], nhub = nwide_h[1];
   hipEvent_t e0, e1;
   EVOC_HIP(hipEventCreate(&e0));
   EVOC_HIP(hipEventCreate(&e1));
@@ -452,7 +655,8 @@ void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, b
 #define EVOC_EPOCH(D)                                                                         \
   launch_epoch<D>(grid, s, n, dim, g.indptr.p, g.indices.p, next_s.p, eps_s.p, next_n.p,    \
                   neg_factor, cur, nxt, m.p, v.p, epoch_t, (uint32_t)epoch, lr_ad, epsc, gamma, \
-                  p.noise_level, one_m_b1, one_m_b2, seed, wide.p, nwide, p.wide_deg)
+                  p.noise_level, one_m_b1, one_m_b2, seed, wide.p, nwide, p.wide_deg, \
+                  hubs.p, nhub)
     switch (dim) {
       case 2: EVOC_EPOCH(2); break;
       case 4: EVOC_EPOCH(4); break;

@@ -72,9 +72,9 @@ def test_embedding_asymmetric_graph_parity():
     assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
 
 
-def _hub_graph(seed=9, n=400):
-    """Symmetric column-sorted CSR: 4 random edges per node plus three hubs with ~200
-    edges each (several 64-slot chunks), symmetric weights in (0.05, 1)."""
+def _hub_graph(seed=9, n=400, hub_size=200):
+    """Symmetric column-sorted CSR: 4 random edges per node plus three hubs with about
+    hub_size edges each (several 64-slot chunks), symmetric weights in (0.05, 1)."""
     rng = np.random.default_rng(seed)
     adj = [set() for _ in range(n)]
     for i in range(n):
@@ -83,7 +83,7 @@ def _hub_graph(seed=9, n=400):
                 adj[i].add(int(j))
                 adj[int(j)].add(i)
     for h in (0, 7, 123):
-        for j in rng.choice(n, size=200, replace=False):
+        for j in rng.choice(n, size=hub_size, replace=False):
             if int(j) != h:
                 adj[h].add(int(j))
                 adj[int(j)].add(h)
@@ -112,6 +112,20 @@ def test_embedding_hub_nodes_parity():
         g = H.embedding(ip, ind, w, dim, n_epochs=25, negative_sample_rate=rate, seed=13)
         o = O.embedding(ip, ind, w, dim, n_epochs=25, negative_sample_rate=rate, seed=13)
         assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), (dim, rate)
+
+
+def test_embedding_workgroup_hub_nodes_parity():
+    """Nodes with 1024 or more edges take one workgroup each: 16 wavefronts stage the
+    terms of 16 chunks at once and one adds them in slot order. Hubs of about 1100 and
+    2900 edges end inside the first and the third group of 1024 slots; a negative-sample
+    rate of 5 stages more than 1024 terms per group. Bit for bit equal to the oracle."""
+    for n, hub, dims in ((1200, 1100, ((8, 1.0), (5, 5.0))), (3000, 2900, ((8, 5.0), (16, 1.0)))):
+        ip, ind, w = _hub_graph(seed=21, n=n, hub_size=hub)
+        assert np.diff(ip).max() >= 1024
+        for dim, rate in dims:
+            g = H.embedding(ip, ind, w, dim, n_epochs=12, negative_sample_rate=rate, seed=17)
+            o = O.embedding(ip, ind, w, dim, n_epochs=12, negative_sample_rate=rate, seed=17)
+            assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), (n, dim, rate)
 
 
 def test_label_prop_init_hub_graph_parity():
