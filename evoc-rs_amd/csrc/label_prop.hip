@@ -7,7 +7,7 @@
 // member by member in ascending node order, each term rounded as coef * x before the
 // add -- the order the sparse product walks a CSR row -- so the f32 values equal the
 // host formulation bit for bit. One workgroup per (partition, 256 columns), one thread
-// per column; sixteen member rows are loaded ahead of the dependent adds.
+// per column; member rows are loaded up to two batches of 32 ahead of the dependent adds.
 #include "kernels.h"
 
 namespace evoc {
@@ -28,14 +28,28 @@ __global__ __launch_bounds__(PM_THREADS) void partition_means_kernel(
     if (c >= d) return;
     float acc = 0.0f;
     int64_t m = m0;
-    for (; m + PM_AHEAD <= m1; m += PM_AHEAD) {
-      float v[PM_AHEAD];
+    // batches of PM_AHEAD member rows; the next batch's loads are issued before this
+    // batch's adds, so two batches are in flight
+    float v[PM_AHEAD];
+    if (m + PM_AHEAD <= m1) {
 #pragma unroll
       for (int u = 0; u < PM_AHEAD; ++u) v[u] = X[(int64_t)mem[m + u] * ld + c];
+    }
+    for (; m + PM_AHEAD <= m1; m += PM_AHEAD) {
+      const bool more = m + 2 * PM_AHEAD <= m1;
+      float w[PM_AHEAD];
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < PM_AHEAD; ++u) w[u] = X[(int64_t)mem[m + PM_AHEAD + u] * ld + c];
+      }
 #pragma unroll
       for (int u = 0; u < PM_AHEAD; ++u) {
         const float t = cf * v[u];
         acc = acc + t;
+      }
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < PM_AHEAD; ++u) v[u] = w[u];
       }
     }
     for (; m < m1; ++m) {
