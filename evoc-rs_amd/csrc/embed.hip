@@ -13,7 +13,8 @@
 // run one wavefront per node instead: lanes load the neighbours and compute every
 // gradient term (the pull of each sampled edge and each negative sample, with the same
 // per-node draw numbering), stage them in LDS, and the lanes of the layout dimensions
-// add them in the original order, so both kernels give bit-identical layouts.
+// add them in the original order, so both kernels give bit-identical layouts. Nodes with
+// 1024 or more edges take a workgroup of 16 wavefronts, which stage 16 chunks at once.
 #include "kernels.h"
 
 namespace evoc {
