@@ -1,26 +1,28 @@
 // EVoC node embedding (evoc_embedding, src/graph/embedding.rs:48-298) on gfx950.
 //
-// One thread per node and epoch, reading the pre-epoch layout (Jacobi update, as the
-// reference's parallel gradient pass does) and writing the next layout into a second
-// buffer. The per-edge schedule (next sample / next negative epoch) is stored once per
-// CSR slot: the two slots of an edge start equal and are advanced by the same f32
-// operations by their two owners, so they stay identical without a shared write and
-// without a separate schedule pass (embedding.rs:275-286). Negative samples are drawn
-// from Philox keyed by (seed, node, epoch, draw). Adam moments live in HBM next to the
-// layout. All arithmetic is f32 in the reference's operation order.
+// Every node and epoch reads the pre-epoch layout (Jacobi update, as the reference's
+// parallel gradient pass does) and writes the next layout into a second buffer. The
+// per-edge schedule (next sample / next negative epoch) is stored once per CSR slot: the
+// two slots of an edge start equal and are advanced by the same f32 operations by their
+// two owners, so they stay identical without a shared write and without a separate
+// schedule pass (embedding.rs:275-286). Negative samples are drawn from Philox keyed by
+// (seed, node, epoch, draw). Adam moments live in HBM next to the layout. All arithmetic
+// is f32 in the reference's operation order.
 //
-// Nodes with many edges (coarse graphs of the initial layout have thousands per node)
-// run one wavefront per node instead: lanes load the neighbours and compute every
-// gradient term (the pull of each sampled edge and each negative sample, with the same
-// per-node draw numbering), stage them in LDS, and the lanes of the layout dimensions
-// add them in the original order, so both kernels give bit-identical layouts. Nodes with
-// 1024 or more edges take a workgroup of 16 wavefronts, which stage 16 chunks at once.
+// A node's lanes load a chunk of its row's slots at a time and compute every gradient
+// term of the chunk (the pull of each sampled edge and each negative sample, with the
+// per-node draw numbering) in parallel, stage the terms in LDS, and the lanes of the
+// layout dimensions add them in slot order, the order of the reference's per-node loop.
+// A node takes 8 lanes (at least dim; chunks of 8 slots), a wavefront with 64 or more
+// edges (chunks of 64), or a workgroup of 16 wavefronts with 1024 or more (16 chunks of
+// 64 staged at once); the three kernels give bit-identical layouts.
 #include "kernels.h"
 
 namespace evoc {
 namespace {
 
 constexpr int MAXD = 16;
+constexpr int GSEG = 8;  // lanes per node of the grouped epoch kernel (at least dim)
 constexpr int WIDE_CAP = 256;  // gradient terms staged per wavefront at a time
 constexpr int HUB_DEG = 1024;  // nodes with at least this many edges take a whole workgroup
 constexpr int HUB_WAVES = 16;  // wavefronts per hub node: one 64-slot chunk each at a time
@@ -60,110 +62,7 @@ __global__ void random_init_kernel(float* __restrict__ e, int64_t n, int dim, ui
   e[q] = (float)(-0.25 + 0.5 * Philox::unit(seed, (uint32_t)i, (uint32_t)c, 0, TAG_EINI));
 }
 
-template <int D>
-__global__ __launch_bounds__(256) void epoch_kernel(
-    int64_t n, int dim, const int64_t* __restrict__ indptr, const int32_t* __restrict__ other,
-    float* __restrict__ next_s, const float* __restrict__ eps_s, float* __restrict__ next_n,
-    float neg_factor, const float* __restrict__ e_old, float* __restrict__ e_new,
-    float* __restrict__ m, float* __restrict__ v, float epoch_t, uint32_t epoch, float lr_ad,
-    float epsc, float gamma, float noise, float one_m_b1, float one_m_b2, uint64_t seed,
-    int64_t wide_deg) {
-  constexpr int DD = D > 0 ? D : MAXD;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (indptr[i + 1] - indptr[i] >= wide_deg) return;  // done by epoch_wide_kernel
-  const int nd = D > 0 ? D : dim;
-  float ei[DD], g[DD];
-#pragma unroll
-  for (int c = 0; c < DD; ++c) {
-    ei[c] = c < nd ? e_old[i * nd + c] : 0.0f;
-    g[c] = 0.0f;
-  }
-  bool touched = false;
-  uint32_t draw = 0;
-  const int64_t s0 = indptr[i], s1 = indptr[i + 1];
-  for (int64_t s = s0; s < s1; ++s) {
-    const float ns = next_s[s];
-    if (ns > epoch_t) continue;
-    touched = true;
-    const int32_t j = other[s];
-    float eo[DD];
-#pragma unroll
-    for (int c = 0; c < DD; ++c) eo[c] = c < nd ? e_old[(int64_t)j * nd + c] : 0.0f;
-    float dsq = 0.0f;
-#pragma unroll
-    for (int c = 0; c < DD; ++c)
-      if (c < nd) {
-        const float df = ei[c] - eo[c];
-        const float p = df * df;
-        dsq = dsq + p;
-      }
-    if (dsq >= 1e-8f) {
-      const float dist = sqrtf(dsq);
-      const float num = -((2.0f * noise) * dist) - 2.0f;
-      const float den = ((2.0f * dsq) - (0.5f * dist)) + 1.0f;
-      const float coeff = num / den;
-#pragma unroll
-      for (int c = 0; c < DD; ++c)
-        if (c < nd) g[c] = g[c] + coeff * (ei[c] - eo[c]);
-    }
-    const float ps = eps_s[s];
-    const float pn = ps * neg_factor;
-    const float nn = next_n[s];
-    const float f = floorf((epoch_t - nn) / pn);
-    const uint64_t cnt = (f >= 0.0f && f < 1.8e19f) ? (uint64_t)f : 0;
-    if ((int64_t)j > i) {  // smaller endpoint draws the negatives
-      for (uint64_t q = 0; q < cnt; ++q) {
-        const uint32_t r = Philox::first(seed, draw++, epoch, (uint32_t)i, TAG_NEG);
-        const int64_t kk = Philox::range(r, (uint64_t)n);
-        if (kk == i) continue;
-        float ek[DD];
-#pragma unroll
-        for (int c = 0; c < DD; ++c) ek[c] = c < nd ? e_old[kk * nd + c] : 0.0f;
-        float dk = 0.0f;
-#pragma unroll
-        for (int c = 0; c < DD; ++c)
-          if (c < nd) {
-            const float df = ei[c] - ek[c];
-            const float p = df * df;
-            dk = dk + p;
-          }
-        if (dk > 0.01f) {
-          const float gc = (gamma * 4.0f) / ((1.0f + 0.25f * dk) * dk);
-#pragma unroll
-          for (int c = 0; c < DD; ++c)
-            if (c < nd) {
-              const float diff = ei[c] - ek[c];
-              g[c] = g[c] + fminf(fmaxf(gc * diff, -4.0f), 4.0f);
-            }
-        }
-      }
-    }
-    next_s[s] = ns + ps;
-    next_n[s] = nn + (float)cnt * pn;
-  }
-  if (touched) {
-#pragma unroll
-    for (int c = 0; c < DD; ++c)
-      if (c < nd) {
-        const int64_t q = i * nd + c;
-        const float gd = g[c];
-        float mq = m[q], vq = v[q];
-        mq = mq + one_m_b1 * (gd - mq);
-        vq = vq + one_m_b2 * (gd * gd - vq);
-        m[q] = mq;
-        v[q] = vq;
-        const float step = (lr_ad * mq) / (sqrtf(vq) + epsc);
-        e_new[q] = ei[c] + step;
-      }
-  } else {
-#pragma unroll
-    for (int c = 0; c < DD; ++c)
-      if (c < nd) e_new[i * nd + c] = ei[c];
-  }
-}
-
-// One wavefront per high-degree node; identical arithmetic to epoch_kernel.
+// One wavefront per node with at least wide_deg edges, 64 slots at a time.
 template <int D>
 __global__ __launch_bounds__(256) void epoch_wide_kernel(
     int64_t n, int dim, const int64_t* __restrict__ indptr, const int32_t* __restrict__ other,
@@ -374,9 +273,188 @@ __global__ __launch_bounds__(256) void epoch_wide_kernel(
   }
 }
 
+// SEG lanes per node (SEG >= dim), 64 / SEG nodes per wavefront, for nodes with fewer than
+// wide_deg edges: the wavefront-per-node scheme on SEG-lane segments. A segment walks its
+// node's row SEG slots at a time (slot loads coalesced, the neighbour and negative-sample rows
+// gathered by all lanes at once), stages the chunk's gradient terms in LDS and its lanes
+// c < dim add them in slot order.
+template <int D, int SEG>
+__global__ __launch_bounds__(256) void epoch_group_kernel(
+    int64_t n, int dim, const int64_t* __restrict__ indptr, const int32_t* __restrict__ other,
+    float* __restrict__ next_s, const float* __restrict__ eps_s, float* __restrict__ next_n,
+    float neg_factor, const float* __restrict__ e_old, float* __restrict__ e_new,
+    float* __restrict__ m, float* __restrict__ v, float epoch_t, uint32_t epoch, float lr_ad,
+    float epsc, float gamma, float noise, float one_m_b1, float one_m_b2, uint64_t seed,
+    int64_t wide_deg) {
+  constexpr int DD = D > 0 ? D : MAXD;
+  constexpr int NSEG = 64 / SEG;
+  constexpr int GCAP = 32 * SEG / DD;  // terms staged per node at a time
+  static_assert(SEG >= DD && 64 % SEG == 0, "a segment holds one lane per dimension");
+  __shared__ float gbuf[4][NSEG][GCAP * DD];  // [wavefront][segment]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int seg = lane / SEG, sl = lane % SEG;
+  const int64_t i = ((int64_t)blockIdx.x * 4 + wave) * NSEG + seg;
+  const int nd = D > 0 ? D : dim;
+  int64_t s0 = 0, s1 = 0;
+  bool own = false;
+  if (i < n) {
+    s0 = indptr[i];
+    s1 = indptr[i + 1];
+    own = s1 - s0 < wide_deg;  // the others are done by the wide and hub kernels
+    if (!own) s1 = s0;
+  }
+  float ei[DD];
+#pragma unroll
+  for (int c = 0; c < DD; ++c) ei[c] = own && c < nd ? e_old[i * nd + c] : 0.0f;
+  float gl = 0.0f;  // lane sl < nd: gradient component sl
+  bool touched = false;
+  uint32_t draw_carry = 0;
+  float* vb = gbuf[wave][seg];
+  const uint64_t segmask = (SEG == 64 ? ~0ull : ((1ull << SEG) - 1)) << (seg * SEG);
+  // every segment runs the wavefront's chunk count; finished ones have no valid slots
+  for (int64_t sb = s0; __ballot(sb < s1) != 0; sb += SEG) {
+    const int64_t s = sb + sl;
+    const bool valid = s < s1;
+    const float ns = valid ? next_s[s] : 0.0f;
+    const bool active = valid && !(ns > epoch_t);
+    int32_t j = 0;
+    float ps = 0.0f, pn = 0.0f, nn = 0.0f;
+    uint64_t cnt = 0;
+    if (active) {
+      j = other[s];
+      ps = eps_s[s];
+      pn = ps * neg_factor;
+      nn = next_n[s];
+      const float f = floorf((epoch_t - nn) / pn);
+      cnt = (f >= 0.0f && f < 1.8e19f) ? (uint64_t)f : 0;
+    }
+    touched |= (__ballot(active) & segmask) != 0;
+    const uint32_t nneg = (active && (int64_t)j > i) ? (uint32_t)cnt : 0u;
+    const uint32_t nvec = active ? 1u + nneg : 0u;
+    // inclusive prefix sums over the segment's lanes (slot order) of draws and terms
+    uint32_t dinc = nneg, vinc = nvec;
+#pragma unroll
+    for (int o = 1; o < SEG; o <<= 1) {
+      const uint32_t td = __shfl_up(dinc, o, SEG), tv = __shfl_up(vinc, o, SEG);
+      if (sl >= o) {
+        dinc += td;
+        vinc += tv;
+      }
+    }
+    const uint32_t draw0 = draw_carry + dinc - nneg;
+    const uint32_t voff = vinc - nvec;
+    const uint32_t total = __shfl(vinc, SEG - 1, SEG);
+    draw_carry += __shfl(dinc, SEG - 1, SEG);
+    uint32_t tmax = total;  // windows run to the largest segment total
+#pragma unroll
+    for (int o = SEG; o < 64; o <<= 1) tmax = max(tmax, (uint32_t)__shfl_xor((int)tmax, o, 64));
+    // this lane's positive term
+    float eo[DD];
+    float coeff = 0.0f;
+    bool pos_ok = false;
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < DD; ++c) eo[c] = c < nd ? e_old[(int64_t)j * nd + c] : 0.0f;
+      float dsq = 0.0f;
+#pragma unroll
+      for (int c = 0; c < DD; ++c)
+        if (c < nd) {
+          const float df = ei[c] - eo[c];
+          const float p = df * df;
+          dsq = dsq + p;
+        }
+      if (dsq >= 1e-8f) {
+        const float dist = sqrtf(dsq);
+        const float num = -((2.0f * noise) * dist) - 2.0f;
+        const float den = ((2.0f * dsq) - (0.5f * dist)) + 1.0f;
+        coeff = num / den;
+        pos_ok = true;
+      }
+    }
+    for (uint32_t w0 = 0; w0 < tmax; w0 += GCAP) {
+      // terms of this lane that fall in the window [w0, w0 + GCAP)
+      const uint32_t lo = voff > w0 ? voff : w0;
+      const uint32_t hi = min(voff + nvec, w0 + (uint32_t)GCAP);
+      for (uint32_t t = lo; t < hi; ++t) {
+        const uint32_t q = t - voff;
+        float* tb = vb + (t - w0) * DD;
+        bool good = false;
+        if (q == 0) {
+          good = pos_ok;
+          if (good) {
+#pragma unroll
+            for (int c = 0; c < DD; ++c)
+              if (c < nd) tb[c] = coeff * (ei[c] - eo[c]);
+          }
+        } else {
+          const uint32_t r = Philox::first(seed, draw0 + (q - 1), epoch, (uint32_t)i, TAG_NEG);
+          const int64_t kk = Philox::range(r, (uint64_t)n);
+          if (kk != i) {
+            float ek[DD];
+#pragma unroll
+            for (int c = 0; c < DD; ++c) ek[c] = c < nd ? e_old[kk * nd + c] : 0.0f;
+            float dk = 0.0f;
+#pragma unroll
+            for (int c = 0; c < DD; ++c)
+              if (c < nd) {
+                const float df = ei[c] - ek[c];
+                const float p = df * df;
+                dk = dk + p;
+              }
+            if (dk > 0.01f) {
+              good = true;
+              const float gc = (gamma * 4.0f) / ((1.0f + 0.25f * dk) * dk);
+#pragma unroll
+              for (int c = 0; c < DD; ++c)
+                if (c < nd) {
+                  const float diff = ei[c] - ek[c];
+                  tb[c] = fminf(fmaxf(gc * diff, -4.0f), 4.0f);
+                }
+            }
+          }
+        }
+        // a skipped term is staged as +0, which leaves the running sum's bits unchanged
+        if (!good) {
+#pragma unroll
+          for (int c = 0; c < DD; ++c)
+            if (c < nd) tb[c] = 0.0f;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (sl < nd && total > w0) {
+        const uint32_t cntw = min(total - w0, (uint32_t)GCAP);
+        for (uint32_t t = 0; t < cntw; ++t) gl = gl + vb[t * DD + sl];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (active) {
+      next_s[s] = ns + ps;
+      next_n[s] = nn + (float)cnt * pn;
+    }
+  }
+  if (own && sl < nd) {
+    const int64_t qd = i * nd + sl;
+    const float eic = e_old[qd];
+    if (touched) {
+      float mq = m[qd], vq = v[qd];
+      mq = mq + one_m_b1 * (gl - mq);
+      vq = vq + one_m_b2 * (gl * gl - vq);
+      m[qd] = mq;
+      v[qd] = vq;
+      const float step = (lr_ad * mq) / (sqrtf(vq) + epsc);
+      e_new[qd] = eic + step;
+    } else {
+      e_new[qd] = eic;
+    }
+  }
+}
+
 // One workgroup per node with at least HUB_DEG edges (a coarse graph's largest partition
-// can touch every other one, thousands of 64-slot chunks in one row); identical
-// arithmetic to epoch_kernel. The row is walked in groups of HUB_WAVES chunks: wavefront w
+// can touch every other one, thousands of 64-slot chunks in one row). The row is walked in groups of HUB_WAVES chunks: wavefront w
 // takes the w-th chunk of a group, a scan over the wavefronts' totals gives each slot its
 // draw numbers and the position of its terms, the terms of the group are staged in LDS
 // (HUB_CAP at a time) and wavefront 0 adds them in slot order.
@@ -576,15 +654,16 @@ __global__ void wide_nodes_kernel(const int64_t* __restrict__ indptr, int64_t n,
 }
 
 template <int D>
-void launch_epoch(unsigned grid, hipStream_t s, int64_t n, int dim, const int64_t* indptr,
+void launch_epoch(hipStream_t s, int64_t n, int dim, const int64_t* indptr,
                   const int32_t* other, float* ns, const float* es, float* nn, float nf,
                   const float* eo, float* en, float* m, float* v, float et, uint32_t ep, float la,
                   float epsc, float gamma, float noise, float b1, float b2, uint64_t seed,
                   const int32_t* wide, int64_t nwide, int64_t wide_deg, const int32_t* hubs,
                   int64_t nhub) {
-  epoch_kernel<D><<<grid, 256, 0, s>>>(n, dim, indptr, other, ns, es, nn, nf, eo, en, m, v, et, ep,
-                                       la, epsc, gamma, noise, b1, b2, seed,
-                                       nwide + nhub > 0 ? wide_deg : INT64_MAX);
+  constexpr int SEG = (D > 0 ? D : MAXD) > GSEG ? (D > 0 ? D : MAXD) : GSEG;
+  epoch_group_kernel<D, SEG><<<(unsigned)ceil_div(n, 4 * (64 / SEG)), 256, 0, s>>>(
+      n, dim, indptr, other, ns, es, nn, nf, eo, en, m, v, et, ep, la, epsc, gamma, noise, b1, b2,
+      seed, nwide + nhub > 0 ? wide_deg : INT64_MAX);
   if (nwide > 0)
     epoch_wide_kernel<D><<<(unsigned)ceil_div(nwide, 4), 256, 0, s>>>(
         n, dim, indptr, other, ns, es, nn, nf, eo, en, m, v, et, ep, la, epsc, gamma, noise, b1,
@@ -654,7 +733,7 @@ void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, b
     const float gamma = 0.5f + (float)epoch / denom;
     const float lr_ad = lr * ad_scale;
 #define EVOC_EPOCH(D)                                                                         \
-  launch_epoch<D>(grid, s, n, dim, g.indptr.p, g.indices.p, next_s.p, eps_s.p, next_n.p,    \
+  launch_epoch<D>(s, n, dim, g.indptr.p, g.indices.p, next_s.p, eps_s.p, next_n.p,    \
                   neg_factor, cur, nxt, m.p, v.p, epoch_t, (uint32_t)epoch, lr_ad, epsc, gamma, \
                   p.noise_level, one_m_b1, one_m_b2, seed, wide.p, nwide, p.wide_deg, \
                   hubs.p, nhub)
