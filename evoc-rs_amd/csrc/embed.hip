@@ -312,21 +312,48 @@ __global__ __launch_bounds__(256) void epoch_group_kernel(
   float* vb = gbuf[wave][seg];
   const uint64_t segmask = (SEG == 64 ? ~0ull : ((1ull << SEG) - 1)) << (seg * SEG);
   // every segment runs the wavefront's chunk count; finished ones have no valid slots
+  // the next chunk's slot inputs and positive endpoint rows are loaded one chunk ahead
+  float ns_a = 0.0f, ps_a = 0.0f, nn_a = 0.0f;
+  int32_t j_a = 0;
+  float eo_a[DD];
+  if (s0 + sl < s1) {
+    ns_a = next_s[s0 + sl];
+    j_a = other[s0 + sl];
+    ps_a = eps_s[s0 + sl];
+    nn_a = next_n[s0 + sl];
+  }
+  {
+    const bool due = s0 + sl < s1 && !(ns_a > epoch_t);
+#pragma unroll
+    for (int c = 0; c < DD; ++c) eo_a[c] = due && c < nd ? e_old[(int64_t)j_a * nd + c] : 0.0f;
+  }
   for (int64_t sb = s0; __ballot(sb < s1) != 0; sb += SEG) {
     const int64_t s = sb + sl;
     const bool valid = s < s1;
-    const float ns = valid ? next_s[s] : 0.0f;
+    const float ns = valid ? ns_a : 0.0f;
     const bool active = valid && !(ns > epoch_t);
     int32_t j = 0;
     float ps = 0.0f, pn = 0.0f, nn = 0.0f;
     uint64_t cnt = 0;
     if (active) {
-      j = other[s];
-      ps = eps_s[s];
+      j = j_a;
+      ps = ps_a;
       pn = ps * neg_factor;
-      nn = next_n[s];
+      nn = nn_a;
       const float f = floorf((epoch_t - nn) / pn);
       cnt = (f >= 0.0f && f < 1.8e19f) ? (uint64_t)f : 0;
+    }
+    float eo[DD];
+#pragma unroll
+    for (int c = 0; c < DD; ++c) eo[c] = eo_a[c];
+    if (s + SEG < s1) {
+      ns_a = next_s[s + SEG];
+      j_a = other[s + SEG];
+      ps_a = eps_s[s + SEG];
+      nn_a = next_n[s + SEG];
+      const bool due = !(ns_a > epoch_t);
+#pragma unroll
+      for (int c = 0; c < DD; ++c) eo_a[c] = due && c < nd ? e_old[(int64_t)j_a * nd + c] : 0.0f;
     }
     touched |= (__ballot(active) & segmask) != 0;
     const uint32_t nneg = (active && (int64_t)j > i) ? (uint32_t)cnt : 0u;
@@ -349,12 +376,9 @@ __global__ __launch_bounds__(256) void epoch_group_kernel(
 #pragma unroll
     for (int o = SEG; o < 64; o <<= 1) tmax = max(tmax, (uint32_t)__shfl_xor((int)tmax, o, 64));
     // this lane's positive term
-    float eo[DD];
     float coeff = 0.0f;
     bool pos_ok = false;
     if (active) {
-#pragma unroll
-      for (int c = 0; c < DD; ++c) eo[c] = c < nd ? e_old[(int64_t)j * nd + c] : 0.0f;
       float dsq = 0.0f;
 #pragma unroll
       for (int c = 0; c < DD; ++c)
