@@ -113,22 +113,38 @@ __global__ __launch_bounds__(64 * LP_WAVES) void lp_step_kernel(
   const int64_t me = i0 + lane;
   const int64_t lme = me < n ? lab_in[me] : 0;
   if (me < n && lme >= 0) lab_out[me] = lme;
-  for (unsigned long long todo = __ballot(me < n && lme < 0); todo != 0; todo &= todo - 1) {
+  // the row bounds and first 64 slots (neighbour labels, weights) of the next unlabelled
+  // node are loaded while the current one is voted
+  int64_t ns0 = 0, ns1 = 0, nlab = -1;
+  float nval = 0.0f;
+  auto fetch = [&](int64_t j) {
+    ns0 = indptr[j];
+    ns1 = indptr[j + 1];
+    const int64_t p = ns0 + lane;
+    nlab = p < ns1 ? lab_in[col[p]] : -1;
+    nval = p < ns1 ? val[p] : 0.0f;
+  };
+  unsigned long long todo = __ballot(me < n && lme < 0);
+  if (todo) fetch(i0 + (__ffsll((long long)todo) - 1));
+  for (; todo != 0; todo &= todo - 1) {
     const int64_t i = i0 + (__ffsll((long long)todo) - 1);
-    const int64_t s0 = indptr[i], s1 = indptr[i + 1];
+    const int64_t s0 = ns0, s1 = ns1, lab0 = nlab;
+    const float val0 = nval;
+    const unsigned long long rest = todo & (todo - 1);
+    if (rest) fetch(i0 + (__ffsll((long long)rest) - 1));
     float top = 1.0f;
     int64_t best = -1;
     for (int64_t pb = s0; pb < s1; pb += 64) {
       const int64_t p = pb + lane;
       const bool pv = p < s1;
-      const int64_t lp = pv ? lab_in[col[p]] : -1;
+      const int64_t lp = pb == s0 ? lab0 : (pv ? lab_in[col[p]] : -1);
       if (__ballot(lp >= 0) == 0) continue;  // no label in this chunk: nothing to vote
       bool first = lp >= 0;
-      float vw = pv ? val[p] : 0.0f;
+      float vw = pb == s0 ? val0 : (pv ? val[p] : 0.0f);
       for (int64_t qb = s0; qb < s1; qb += 64) {
         const int64_t q = qb + lane;
-        const int64_t lq = q < s1 ? lab_in[col[q]] : -1;
-        const float wq = q < s1 ? val[q] : 0.0f;
+        const int64_t lq = qb == s0 ? lab0 : (q < s1 ? lab_in[col[q]] : -1);
+        const float wq = qb == s0 ? val0 : (q < s1 ? val[q] : 0.0f);
         // only labelled slots can match a lane's label (an unlabelled lane's sums are
         // never read), so the walk visits those alone, in slot order
         for (unsigned long long lm = __ballot(lq >= 0); lm != 0; lm &= lm - 1) {
