@@ -193,6 +193,18 @@ void persistence_bins_device(const int64_t* h_bpos, const int64_t* h_dpos, const
 void radix_sort_pairs(uint64_t* d_key, uint32_t* d_val, int64_t n, int key_bits, hipStream_t s,
                       DevPool& pool);
 
+// ------------------------------------------------------------------ input checks
+// throws std::invalid_argument("<what> contains a non-finite value") if any of the n x d
+// row-major values (leading dimension ld) is NaN or +-inf
+void check_finite_device(const float* dX, int64_t n, int d, int64_t ld, const char* what,
+                         hipStream_t s, DevPool& pool);
+inline void check_finite_host(const float* p, int64_t count, const char* what) {
+  bool bad = false;
+#pragma omp parallel for reduction(|| : bad) schedule(static) if (count > (1 << 20))
+  for (int64_t i = 0; i < count; ++i) bad = bad || !std::isfinite(p[i]);
+  EVOC_CHECK(!bad, std::string(what) + " contains a non-finite value");
+}
+
 // ------------------------------------------------------------------ scans
 void exclusive_scan_i64(const int64_t* d_in, int64_t* d_out, int64_t n, hipStream_t s,
                         DevPool& pool);

@@ -190,7 +190,12 @@ Layers cluster_device(const float* d_emb, int64_t n, int dim, const evoc_params_
   mst_device(d_emb, n, dim, p.min_samples, u.data(), v.data(), w.data(), s, pool);
   lap("mst");
   int64_t ne = std::max<int64_t>(n - 1, 0);
-  while (ne > 0 && u[ne - 1] < 0) --ne;  // disconnected (non-finite input): fewer edges
+  while (ne > 0 && u[ne - 1] < 0) --ne;
+  // A spanning tree is guaranteed for finite embeddings whose squared distances do not
+  // overflow (every entry point rejects non-finite input); the condensed tree is rooted
+  // at the last linkage node, so a forest must not reach it.
+  EVOC_CHECK(ne == std::max<int64_t>(n - 1, 0),
+             "the minimum spanning tree is disconnected (embedding distances overflow f32)");
   const Linkage L = linkage_from_sorted(u.data(), v.data(), w.data(), ne, n);
   lap("linkage");
   if (mst_ms) *mst_ms = ms_since(t0);
@@ -218,6 +223,7 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
   hipStream_t s = c.stream;
   DevPool& pool = c.pool;
   const auto t_all = clk::now();
+  check_finite_device(dX, n, (int)d, ld, "input", s, pool);
   double stage[8] = {0};
   const int k = pre_idx ? pre_k : p.n_neighbours;
   // 1. kNN
@@ -233,8 +239,11 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
       std::unordered_set<int32_t> seen;
       for (int t = 0; t < k; ++t) {
         const int32_t j = pre_idx[i * k + t];
+        const float dj = pre_dist[i * k + t];
         EVOC_CHECK(j >= -1 && j < n, "precomputed kNN index out of range");
         EVOC_CHECK(j < 0 || seen.insert(j).second, "precomputed kNN row repeats an index");
+        EVOC_CHECK(j < 0 || (std::isfinite(dj) && dj >= 0.0f),
+                   "precomputed kNN distance is negative or non-finite");
       }
     }
     EVOC_HIP(hipMemcpyAsync(didx.p, pre_idx, n * k * sizeof(int32_t), hipMemcpyHostToDevice, s));
@@ -406,8 +415,6 @@ int evoc_hip_run(const float* data, int64_t n, int64_t d, int64_t row_stride, in
         for (int64_t j = 0; j < d; ++j) packed[i * d + j] = data[i * row_stride + j * col_stride];
       xr = packed.data();
     }
-    for (int64_t i = 0; i < n * d; ++i)
-      EVOC_CHECK(std::isfinite(xr[i]), "input contains a non-finite value");
     Buf<float> dx(c.pool, n * d);
     EVOC_HIP(hipMemcpyAsync(dx.p, xr, n * d * sizeof(float), hipMemcpyHostToDevice, c.stream));
     EVOC_HIP(hipStreamSynchronize(c.stream));
@@ -443,6 +450,7 @@ int evoc_hip_fuzzy(const int32_t* knn_idx, const float* knn_dist, int64_t n, int
   return guarded([&] {
     EVOC_CHECK(knn_idx && knn_dist && indptr && indices && weights && nnz, "fuzzy: NULL buffer");
     EVOC_CHECK(n >= 1 && k >= 1 && k <= 32, "fuzzy: need n >= 1 and k in [1, 32]");
+    check_finite_host(knn_dist, n * k, "knn_dist");
     Ctx& c = context(0);
     Buf<int32_t> di;
     Buf<float> dd, ds(c.pool, n), dr(c.pool, n);
@@ -470,6 +478,8 @@ int evoc_hip_embed(const int64_t* indptr, const int32_t* indices, const float* w
     EVOC_CHECK(indptr && out, "embed: NULL buffer");
     EVOC_CHECK(n >= 0 && dim >= 1 && dim <= 16, "embed: dim must be in [1, 16]");
     if (n == 0) return;
+    check_finite_host(weights, indptr[n], "weights");
+    if (init) check_finite_host(init, n * dim, "init");
     Ctx& c = context(0);
     std::vector<int64_t> ptr(indptr, indptr + n + 1);
     std::vector<int32_t> col(indices, indices + ptr[n]);
@@ -506,6 +516,8 @@ int evoc_hip_label_prop_init(const int64_t* indptr, const int32_t* indices, cons
   return guarded([&] {
     EVOC_CHECK(indptr && out, "label_prop_init: NULL buffer");
     EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "label_prop_init: dim must be in [1, 16]");
+    check_finite_host(weights, indptr[n], "weights");
+    if (data) check_finite_host(data, n * d, "data");
     Ctx& c = context(0);
     Buf<float> dx;
     if (data) to_dev(dx, data, n * d, c.pool, c.stream);
@@ -520,6 +532,7 @@ int evoc_hip_core_sq(const float* emb, int64_t n, int32_t dim, int32_t min_sampl
     EVOC_CHECK(emb && core_sq, "core_sq: NULL buffer");
     EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "core_sq: dim must be in [1, 16]");
     EVOC_CHECK(min_samples >= 0 && min_samples <= 32, "min_samples must be in [0, 32]");
+    check_finite_host(emb, n * dim, "embedding");
     Ctx& c = context(0);
     Buf<float> de, dc(c.pool, n);
     to_dev(de, emb, n * dim, c.pool, c.stream);
@@ -535,6 +548,7 @@ int evoc_hip_mst(const float* emb, int64_t n, int32_t dim, int32_t min_samples, 
     EVOC_CHECK(emb && u && v && w, "mst: NULL buffer");
     EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "mst: dim must be in [1, 16]");
     EVOC_CHECK(min_samples >= 0 && min_samples <= 32, "min_samples must be in [0, 32]");
+    check_finite_host(emb, n * dim, "embedding");
     Ctx& c = context(0);
     Buf<float> de;
     to_dev(de, emb, n * dim, c.pool, c.stream);
@@ -548,6 +562,8 @@ int evoc_hip_cluster_layers(const float* emb, int64_t n, int32_t dim, int32_t mi
   return guarded([&] {
     EVOC_CHECK(emb && labels && strengths && persistence && n_layers, "layers: NULL buffer");
     EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "layers: dim must be in [1, 16]");
+    EVOC_CHECK(min_samples >= 0 && min_samples <= 32, "min_samples must be in [0, 32]");
+    check_finite_host(emb, n * dim, "embedding");
     Ctx& c = context(0);
     Buf<float> de;
     to_dev(de, emb, n * dim, c.pool, c.stream);
@@ -573,6 +589,8 @@ int evoc_hip_search_n_clusters(const float* emb, int64_t n, int32_t dim, int32_t
   return guarded([&] {
     EVOC_CHECK(emb && labels && strengths, "search: NULL buffer");
     EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "search: dim must be in [1, 16]");
+    EVOC_CHECK(min_samples >= 0 && min_samples <= 32, "min_samples must be in [0, 32]");
+    check_finite_host(emb, n * dim, "embedding");
     Ctx& c = context(0);
     Buf<float> de;
     to_dev(de, emb, n * dim, c.pool, c.stream);

@@ -250,6 +250,32 @@ class EvocResult:
         return int(max(lab.max(initial=-1) + 1, 0))
 
 
+def _check_precomputed(precomputed_knn, n):
+    """PreComputedKnn (src/prelude.rs:85): an (n, k) index array (-1 pads a short row) and
+    an (n, k) distance array. The library reads n * k entries of each, so the shapes are
+    checked here; real entries need finite, non-negative distances."""
+    if len(precomputed_knn) != 2:
+        raise ValueError("precomputed_knn must be a pair (indices, distances)")
+    pi = np.asarray(precomputed_knn[0])
+    pd = np.asarray(precomputed_knn[1])
+    if pi.ndim != 2 or pi.shape[0] != n or pi.shape[1] < 1:
+        raise ValueError(f"precomputed kNN indices must have shape ({n}, k), got {pi.shape}")
+    if pd.shape != pi.shape:
+        raise ValueError(f"precomputed kNN distances have shape {pd.shape}, indices {pi.shape}")
+    if pi.shape[1] > 32:
+        raise ValueError("precomputed kNN: at most 32 neighbours per row")
+    if not np.issubdtype(pi.dtype, np.integer):
+        raise ValueError("precomputed kNN indices must be integers")
+    if pi.size and (pi.min() < -1 or pi.max() >= n):
+        raise ValueError("precomputed kNN index out of range")
+    pi = np.ascontiguousarray(pi, np.int32)
+    pd = np.ascontiguousarray(pd, np.float32)
+    real = pi >= 0
+    if not np.all(np.isfinite(pd[real])) or np.any(pd[real] < 0):
+        raise ValueError("precomputed kNN distances must be finite and non-negative")
+    return pi, pd
+
+
 def evoc(data, params=None, nn_params=None, seed=42, verbose=0, precomputed_knn=None):
     """evoc_rs::evoc (src/lib.rs:199-352) on the GPU."""
     x = np.asarray(data)
@@ -268,8 +294,7 @@ def evoc(data, params=None, nn_params=None, seed=42, verbose=0, precomputed_knn=
     pi = pd = None
     pk = 0
     if precomputed_knn is not None:
-        pi = np.ascontiguousarray(precomputed_knn[0], np.int32)
-        pd = np.ascontiguousarray(precomputed_knn[1], np.float32)
+        pi, pd = _check_precomputed(precomputed_knn, n)
         pk = pi.shape[1]
     out = C.POINTER(_Result)()
     flat = base.ravel(order="K")  # memory order; strides describe the layout

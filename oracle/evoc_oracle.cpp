@@ -236,6 +236,10 @@ static void smooth_knn_dist(const float* dists, int64_t n, int k, size_t k_targe
   }
 }
 
+static int64_t symmetrise_coo(const std::vector<int64_t>& rr, const std::vector<int32_t>& cc,
+                              const std::vector<float>& ww, int64_t n, int64_t* indptr,
+                              int32_t* indices, float* weights);
+
 // build_fuzzy_simplicial_set: src/graph/fuzzy_graph.rs:275-295, knn_to_coo :114-159,
 // symmetrise_graph :174-228 (rows emitted sorted by column).
 extern "C" int64_t orc_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n,
@@ -278,6 +282,15 @@ extern "C" int64_t orc_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist
     }
     return p;
   }
+  return symmetrise_coo(rr, cc, ww, n, indptr, indices, weights);
+}
+
+// symmetrise_graph (src/graph/fuzzy_graph.rs:174-228) of a directed COO list: fuzzy union
+// w_ij + w_ji - w_ij * w_ji, entries <= 0 dropped, both directions stored, rows sorted by
+// column. A repeated (i, j) keeps its last weight.
+static int64_t symmetrise_coo(const std::vector<int64_t>& rr, const std::vector<int32_t>& cc,
+                              const std::vector<float>& ww, int64_t n, int64_t* indptr,
+                              int32_t* indices, float* weights) {
   // forward[i][j] = w(i->j), backward[j][i] = w(i->j); last insert wins
   std::vector<std::map<int32_t, float>> fwd(n), bwd(n);
   for (size_t e = 0; e < rr.size(); ++e) {
@@ -307,6 +320,16 @@ extern "C" int64_t orc_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist
     indptr[i + 1] = p;
   }
   return p;
+}
+
+// symmetrise_coo on explicit directed weights (for the reference's union tests)
+extern "C" int64_t orc_symmetrise(const int64_t* rows, const int32_t* cols, const float* vals,
+                                  int64_t nnz, int64_t n, int64_t* indptr, int32_t* indices,
+                                  float* weights) {
+  std::vector<int64_t> rr(rows, rows + nnz);
+  std::vector<int32_t> cc(cols, cols + nnz);
+  std::vector<float> ww(vals, vals + nnz);
+  return symmetrise_coo(rr, cc, ww, n, indptr, indices, weights);
 }
 
 // ================================================================ A7: node embedding
@@ -619,6 +642,40 @@ std::vector<float> matmul_dense(const Csr& a, const std::vector<float>& rhs, int
   return out;
 }
 
+// Csr::clip_values (src/utils/sparse.rs:502-510)
+void clip_values(Csr& a, float lo, float hi) {
+  for (auto& v : a.data) {
+    if (v < lo) v = lo;
+    else if (v > hi) v = hi;
+  }
+}
+// Csr::from_coo (src/utils/sparse.rs:194-235): sort by (row, col), fold duplicates by
+// summing in sorted order (a stable sort, so duplicates add in input order)
+Csr from_coo(const int64_t* rows, const int64_t* cols, const float* vals, int64_t nnz, int64_t n) {
+  Csr c;
+  c.nrows = c.ncols = n;
+  c.indptr.assign(n + 1, 0);
+  std::vector<int64_t> ord(nnz);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
+    return rows[a] != rows[b] ? rows[a] < rows[b] : cols[a] < cols[b];
+  });
+  int64_t pr = -1, pc = -1;
+  for (int64_t q : ord) {
+    if (rows[q] == pr && cols[q] == pc) {
+      c.data.back() = c.data.back() + vals[q];
+    } else {
+      c.data.push_back(vals[q]);
+      c.indices.push_back(cols[q]);
+      c.indptr[rows[q] + 1]++;
+      pr = rows[q];
+      pc = cols[q];
+    }
+  }
+  for (int64_t i = 0; i < n; ++i) c.indptr[i + 1] += c.indptr[i];
+  return c;
+}
+
 // label_prop_iteration: src/graph/label_prop.rs:45-86
 std::vector<int64_t> lp_iteration(const Csr& g, const std::vector<int64_t>& labels) {
   std::vector<int64_t> out(g.nrows);
@@ -836,39 +893,11 @@ std::vector<float> random_init(int64_t n, int dim, uint64_t seed) {
   return out;
 }
 
-// label_prop_init_inner: src/graph/label_prop.rs:403-517
-std::vector<float> lp_init_inner(const Csr& g, int dim, const std::vector<float>* data,
-                                 int64_t d, int n_emb_epochs, int64_t approx_parts, int n_lp_iter,
-                                 int64_t base_thresh, double scaling, uint64_t seed) {
+// The expansion of label_prop_init_inner (src/graph/label_prop.rs:487-516): the mean of
+// rows_l1((G .* G^T) R_norm) L and rows_l1(R_norm) L, columns centred, times scaling.
+std::vector<float> expand_layout(const Csr& g, const Csr& nrm, const std::vector<float>& layout,
+                                 int dim, double scaling) {
   const int64_t n = g.nrows;
-  if (n < base_thresh) {
-    if (data && (int64_t)data->size() == n * d && dim <= d) return pca_init(*data, n, d, dim);
-    return random_init(n, dim, seed);
-  }
-  int64_t n_parts = 0;
-  std::vector<int64_t> part = lp_loop(g, approx_parts, n_lp_iter, seed, &n_parts);
-  Csr base = from_partition(part, n_parts);
-  Csr nrm = norm_cols_l2(base);
-  Csr nrt = transpose(nrm);
-  Csr reducer = norm_rows_l1(nrt);
-  Csr reduced = matmul(matmul(nrt, g), base);
-  for (auto& v : reduced.data) {
-    if (v < 0.0f) v = 0.0f;
-    else if (v > 1.0f) v = 1.0f;
-  }
-  std::vector<float> rdata;
-  if (data) rdata = matmul_dense(reducer, *data, d);
-  int ep = std::min(n_emb_epochs, 255);
-  std::vector<float> rinit = lp_init_inner(reduced, dim, data ? &rdata : nullptr, d, ep,
-                                           approx_parts / 4, n_lp_iter, base_thresh, scaling,
-                                           seed + 3000);
-  std::vector<std::vector<std::pair<int32_t, float>>> adj(reduced.nrows);
-  for (int64_t i = 0; i < reduced.nrows; ++i)
-    for (int64_t p = reduced.indptr[i]; p < reduced.indptr[i + 1]; ++p)
-      adj[i].push_back({(int32_t)reduced.indices[p], reduced.data[p]});
-  std::vector<float> layout(reduced.nrows * dim);
-  embedding_impl(adj, dim, ep, 0.5f, 1.0f, (float)(0.001 * (double)n_emb_epochs), 0.5f, 0.9f,
-                 1e-7f, rinit.data(), seed + 4000, layout.data());
   Csr gt = transpose(g);
   Csr sym = hadamard(g, gt);
   Csr expander = norm_rows_l1(matmul(sym, nrm));
@@ -887,6 +916,39 @@ std::vector<float> lp_init_inner(const Csr& g, int dim, const std::vector<float>
   const float sc = (float)scaling;
   for (auto& v : res) v = sc * v;
   return res;
+}
+
+// label_prop_init_inner: src/graph/label_prop.rs:403-517
+std::vector<float> lp_init_inner(const Csr& g, int dim, const std::vector<float>* data,
+                                 int64_t d, int n_emb_epochs, int64_t approx_parts, int n_lp_iter,
+                                 int64_t base_thresh, double scaling, uint64_t seed) {
+  const int64_t n = g.nrows;
+  if (n < base_thresh) {
+    if (data && (int64_t)data->size() == n * d && dim <= d) return pca_init(*data, n, d, dim);
+    return random_init(n, dim, seed);
+  }
+  int64_t n_parts = 0;
+  std::vector<int64_t> part = lp_loop(g, approx_parts, n_lp_iter, seed, &n_parts);
+  Csr base = from_partition(part, n_parts);
+  Csr nrm = norm_cols_l2(base);
+  Csr nrt = transpose(nrm);
+  Csr reducer = norm_rows_l1(nrt);
+  Csr reduced = matmul(matmul(nrt, g), base);
+  clip_values(reduced, 0.0f, 1.0f);
+  std::vector<float> rdata;
+  if (data) rdata = matmul_dense(reducer, *data, d);
+  int ep = std::min(n_emb_epochs, 255);
+  std::vector<float> rinit = lp_init_inner(reduced, dim, data ? &rdata : nullptr, d, ep,
+                                           approx_parts / 4, n_lp_iter, base_thresh, scaling,
+                                           seed + 3000);
+  std::vector<std::vector<std::pair<int32_t, float>>> adj(reduced.nrows);
+  for (int64_t i = 0; i < reduced.nrows; ++i)
+    for (int64_t p = reduced.indptr[i]; p < reduced.indptr[i + 1]; ++p)
+      adj[i].push_back({(int32_t)reduced.indices[p], reduced.data[p]});
+  std::vector<float> layout(reduced.nrows * dim);
+  embedding_impl(adj, dim, ep, 0.5f, 1.0f, (float)(0.001 * (double)n_emb_epochs), 0.5f, 0.9f,
+                 1e-7f, rinit.data(), seed + 4000, layout.data());
+  return expand_layout(g, nrm, layout, dim, scaling);
 }
 }  // namespace
 
@@ -908,6 +970,136 @@ extern "C" int orc_label_prop_init(const int64_t* indptr, const int32_t* indices
   std::vector<float> r =
       lp_init_inner(g, dim, data ? &dv : nullptr, d, 50, approx, 20, 64, 0.1, seed);
   std::memcpy(out, r.data(), n * dim * sizeof(float));
+  return 0;
+}
+
+// ---------------------------------------------------------------- test entry points
+// The label-prop pieces and the CSR algebra one by one, so that the reference's own unit
+// tests (src/graph/label_prop.rs:574-738, src/utils/sparse.rs:656-951) pin them.
+namespace {
+Csr csr_in(const int64_t* ip, const int64_t* ix, const float* dv, int64_t nr, int64_t nc) {
+  Csr c;
+  c.nrows = nr;
+  c.ncols = nc;
+  c.indptr.assign(ip, ip + nr + 1);
+  c.indices.assign(ix, ix + ip[nr]);
+  c.data.assign(dv, dv + ip[nr]);
+  return c;
+}
+int64_t csr_out(const Csr& c, int64_t* ip, int64_t* ix, float* dv, int64_t cap, int64_t* shape) {
+  shape[0] = c.nrows;
+  shape[1] = c.ncols;
+  const int64_t nnz = (int64_t)c.data.size();
+  if (nnz > cap) return -1;
+  std::copy(c.indptr.begin(), c.indptr.end(), ip);
+  std::copy(c.indices.begin(), c.indices.end(), ix);
+  std::copy(c.data.begin(), c.data.end(), dv);
+  return nnz;
+}
+}  // namespace
+
+// op: 0 transpose(A), 1 A*B, 2 A .* B, 3 normalise_cols_l2(A), 4 normalise_rows_l1(A),
+// 5 clip_values(A, 0, 1). Returns the stored count, or -1 if it exceeds cap.
+extern "C" int64_t orc_csr_op(int op, const int64_t* aip, const int64_t* aix, const float* adv,
+                              int64_t anr, int64_t anc, const int64_t* bip, const int64_t* bix,
+                              const float* bdv, int64_t bnr, int64_t bnc, int64_t* oip,
+                              int64_t* oix, float* odv, int64_t cap, int64_t* shape) {
+  Csr a = csr_in(aip, aix, adv, anr, anc);
+  Csr r;
+  switch (op) {
+    case 0: r = transpose(a); break;
+    case 1: r = matmul(a, csr_in(bip, bix, bdv, bnr, bnc)); break;
+    case 2: r = hadamard(a, csr_in(bip, bix, bdv, bnr, bnc)); break;
+    case 3: r = norm_cols_l2(a); break;
+    case 4: r = norm_rows_l1(a); break;
+    case 5: r = a; clip_values(r, 0.0f, 1.0f); break;
+    default: return -1;
+  }
+  return csr_out(r, oip, oix, odv, cap, shape);
+}
+
+extern "C" int64_t orc_csr_from_coo(const int64_t* rows, const int64_t* cols, const float* vals,
+                                    int64_t nnz, int64_t n, int64_t* oip, int64_t* oix,
+                                    float* odv) {
+  int64_t shape[2];
+  return csr_out(from_coo(rows, cols, vals, nnz, n), oip, oix, odv, nnz, shape);
+}
+
+extern "C" int orc_csr_from_partition(const int64_t* part, int64_t n, int64_t n_parts,
+                                      int64_t* oip, int64_t* oix, float* odv) {
+  int64_t shape[2];
+  return csr_out(from_partition(std::vector<int64_t>(part, part + n), n_parts), oip, oix, odv, n,
+                 shape) < 0 ? -1 : 0;
+}
+
+extern "C" int orc_csr_matmul_dense(const int64_t* ip, const int64_t* ix, const float* dv,
+                                    int64_t nr, int64_t nc, const float* rhs, int64_t d,
+                                    float* out) {
+  std::vector<float> r(rhs, rhs + nc * d);
+  std::vector<float> o = matmul_dense(csr_in(ip, ix, dv, nr, nc), r, d);
+  std::copy(o.begin(), o.end(), out);
+  return 0;
+}
+
+extern "C" int orc_lp_iteration(const int64_t* ip, const int64_t* ix, const float* dv, int64_t n,
+                                const int64_t* labels, int64_t* out) {
+  std::vector<int64_t> l(labels, labels + n);
+  std::vector<int64_t> r = lp_iteration(csr_in(ip, ix, dv, n, n), l);
+  std::copy(r.begin(), r.end(), out);
+  return 0;
+}
+
+extern "C" int orc_lp_outliers(const int64_t* ip, const int64_t* ix, const float* dv, int64_t n,
+                               int64_t* labels, uint64_t seed) {
+  std::vector<int64_t> l(labels, labels + n);
+  lp_outliers(csr_in(ip, ix, dv, n, n), l, seed);
+  std::copy(l.begin(), l.end(), labels);
+  return 0;
+}
+
+extern "C" int64_t orc_lp_remap(int64_t* labels, int64_t n) {
+  std::vector<int64_t> l(labels, labels + n);
+  const int64_t m = lp_remap(l);
+  std::copy(l.begin(), l.end(), labels);
+  return m;
+}
+
+extern "C" int64_t orc_lp_loop(const int64_t* ip, const int64_t* ix, const float* dv, int64_t n,
+                               int64_t approx_parts, int n_iter, uint64_t seed, int64_t* part) {
+  int64_t np = 0;
+  std::vector<int64_t> p = lp_loop(csr_in(ip, ix, dv, n, n), approx_parts, n_iter, seed, &np);
+  std::copy(p.begin(), p.end(), part);
+  return np;
+}
+
+extern "C" int orc_unit_range(float* m, int64_t n, int64_t d) {
+  std::vector<float> v(m, m + n * d);
+  unit_range(v, n, d);
+  std::copy(v.begin(), v.end(), m);
+  return 0;
+}
+
+extern "C" int orc_pca_init(const float* data, int64_t n, int64_t d, int dim, float* out) {
+  std::vector<float> v(data, data + n * d);
+  std::vector<float> r = pca_init(v, n, d, dim);
+  std::copy(r.begin(), r.end(), out);
+  return 0;
+}
+
+extern "C" int orc_random_init(int64_t n, int dim, uint64_t seed, float* out) {
+  std::vector<float> r = random_init(n, dim, seed);
+  std::copy(r.begin(), r.end(), out);
+  return 0;
+}
+
+// expand_layout on its own, the partition given as labels
+extern "C" int orc_lp_expand(const int64_t* ip, const int64_t* ix, const float* dv, int64_t n,
+                             const int64_t* part, int64_t n_parts, const float* layout, int dim,
+                             double scaling, float* out) {
+  Csr nrm = norm_cols_l2(from_partition(std::vector<int64_t>(part, part + n), n_parts));
+  std::vector<float> lay(layout, layout + n_parts * dim);
+  std::vector<float> r = expand_layout(csr_in(ip, ix, dv, n, n), nrm, lay, dim, scaling);
+  std::copy(r.begin(), r.end(), out);
   return 0;
 }
 

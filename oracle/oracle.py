@@ -104,6 +104,27 @@ def lib():
         L.orc_philox_u32.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
         L.orc_philox_u32.restype = C.c_uint32
         L.orc_num_threads.restype = C.c_int
+        L.orc_symmetrise.argtypes = [i64p, i32p, f32p, C.c_int64, C.c_int64, i64p, i32p, f32p]
+        L.orc_symmetrise.restype = C.c_int64
+        csr3 = [i64p, i64p, f32p, C.c_int64, C.c_int64]
+        L.orc_csr_op.argtypes = [C.c_int] + csr3 + csr3 + [i64p, i64p, f32p, C.c_int64, i64p]
+        L.orc_csr_op.restype = C.c_int64
+        L.orc_csr_from_coo.argtypes = [i64p, i64p, f32p, C.c_int64, C.c_int64, i64p, i64p, f32p]
+        L.orc_csr_from_coo.restype = C.c_int64
+        L.orc_csr_from_partition.argtypes = [i64p, C.c_int64, C.c_int64, i64p, i64p, f32p]
+        L.orc_csr_matmul_dense.argtypes = csr3 + [f32p, C.c_int64, f32p]
+        L.orc_lp_iteration.argtypes = [i64p, i64p, f32p, C.c_int64, i64p, i64p]
+        L.orc_lp_outliers.argtypes = [i64p, i64p, f32p, C.c_int64, i64p, C.c_uint64]
+        L.orc_lp_remap.argtypes = [i64p, C.c_int64]
+        L.orc_lp_remap.restype = C.c_int64
+        L.orc_lp_loop.argtypes = [i64p, i64p, f32p, C.c_int64, C.c_int64, C.c_int, C.c_uint64,
+                                  i64p]
+        L.orc_lp_loop.restype = C.c_int64
+        L.orc_unit_range.argtypes = [f32p, C.c_int64, C.c_int64]
+        L.orc_pca_init.argtypes = [f32p, C.c_int64, C.c_int64, C.c_int, f32p]
+        L.orc_random_init.argtypes = [C.c_int64, C.c_int, C.c_uint64, f32p]
+        L.orc_lp_expand.argtypes = [i64p, i64p, f32p, C.c_int64, i64p, C.c_int64, f32p, C.c_int,
+                                    C.c_double, f32p]
         _lib = L
     return _lib
 
@@ -156,6 +177,19 @@ def fuzzy_graph(idx, dist, effective_k=None, symmetrise=True):
     w = np.empty(2 * n * k, np.float32)
     nnz = lib().orc_fuzzy_graph(idx, dist, n, k, eff, int(symmetrise), sig, rho, indptr, ind, w)
     return indptr, ind[:nnz].copy(), w[:nnz].copy(), sig, rho
+
+
+def symmetrise(rows, cols, vals, n):
+    """symmetrise_graph on an explicit directed COO list -> (indptr, indices, weights)."""
+    rows = np.ascontiguousarray(rows, np.int64)
+    cols = np.ascontiguousarray(cols, np.int32)
+    vals = _f32(vals)
+    m = 2 * max(len(vals), 1)
+    ip = np.empty(n + 1, np.int64)
+    ix = np.empty(m, np.int32)
+    w = np.empty(m, np.float32)
+    nnz = lib().orc_symmetrise(rows, cols, vals, len(vals), n, ip, ix, w)
+    return ip, ix[:nnz].copy(), w[:nnz].copy()
 
 
 def embedding(indptr, ind, w, dim, n_epochs=50, noise_level=0.5, negative_sample_rate=1.0,
@@ -371,3 +405,138 @@ def evoc(x, p=None, seed=42, pre_idx=None, pre_dist=None, want_intermediates=Fal
                         _ptr(ki), _ptr(kd), _ptr(emb), _ptr(init), _ptr(tim))
     return dict(labels=labels[:nl], strengths=st[:nl], persistence=pers[:nl], knn_idx=ki,
                 knn_dist=kd, embedding=emb, init=init, timings=tim)
+
+
+# ---------------------------------------------------------------- CSR algebra and label-prop
+class Csr:
+    """A CSR matrix as the oracle stores it (int64 indices, f32 values)."""
+
+    def __init__(self, indptr, indices, data, nrows, ncols):
+        self.indptr = np.ascontiguousarray(indptr, np.int64)
+        self.indices = np.ascontiguousarray(indices, np.int64)
+        self.data = _f32(data)
+        self.nrows, self.ncols = int(nrows), int(ncols)
+        if len(self.indices) == 0:
+            self.indices = np.zeros(1, np.int64)
+            self.data = np.zeros(1, np.float32)
+            self._nnz = 0
+        else:
+            self._nnz = int(self.indptr[-1])
+
+    def nnz(self):
+        return self._nnz
+
+    def args(self):
+        return (self.indptr, self.indices, self.data, self.nrows, self.ncols)
+
+    def to_dense(self):
+        d = np.zeros((self.nrows, self.ncols), np.float32)
+        for i in range(self.nrows):
+            for p in range(self.indptr[i], self.indptr[i + 1]):
+                d[i, self.indices[p]] += self.data[p]
+        return d
+
+    def rows(self):
+        return [[(int(self.indices[p]), float(self.data[p]))
+                 for p in range(self.indptr[i], self.indptr[i + 1])] for i in range(self.nrows)]
+
+
+_CSR_OPS = {"transpose": 0, "matmul": 1, "elementwise_mul": 2, "normalise_cols_l2": 3,
+            "normalise_rows_l1": 4, "clip_values": 5}
+
+
+def csr_op(op, a, b=None):
+    cap = max(a.nrows * (b.ncols if b is not None else a.ncols), a.nnz(), 1) + 1
+    bb = b if b is not None else a
+    ip = np.empty(max(a.nrows, a.ncols, bb.ncols) + 2, np.int64)
+    ix = np.empty(cap, np.int64)
+    dv = np.empty(cap, np.float32)
+    shape = np.zeros(2, np.int64)
+    nnz = lib().orc_csr_op(_CSR_OPS[op], *a.args(), *bb.args(), ip, ix, dv, cap, shape)
+    assert nnz >= 0
+    return Csr(ip[:shape[0] + 1].copy(), ix[:nnz].copy(), dv[:nnz].copy(), shape[0], shape[1])
+
+
+def csr_from_coo(rows, cols, vals, n):
+    rows = np.ascontiguousarray(rows, np.int64)
+    cols = np.ascontiguousarray(cols, np.int64)
+    vals = _f32(vals)
+    m = max(len(vals), 1)
+    ip = np.empty(n + 1, np.int64)
+    ix = np.empty(m, np.int64)
+    dv = np.empty(m, np.float32)
+    nnz = lib().orc_csr_from_coo(rows if len(rows) else np.zeros(1, np.int64),
+                                 cols if len(cols) else np.zeros(1, np.int64),
+                                 vals if len(vals) else np.zeros(1, np.float32), len(vals), n,
+                                 ip, ix, dv)
+    return Csr(ip, ix[:nnz].copy(), dv[:nnz].copy(), n, n)
+
+
+def csr_from_partition(part, n_parts):
+    part = np.ascontiguousarray(part, np.int64)
+    n = len(part)
+    ip = np.empty(n + 1, np.int64)
+    ix = np.empty(max(n, 1), np.int64)
+    dv = np.empty(max(n, 1), np.float32)
+    lib().orc_csr_from_partition(part, n, n_parts, ip, ix, dv)
+    return Csr(ip, ix[:n], dv[:n], n, n_parts)
+
+
+def csr_matmul_dense(a, rhs):
+    rhs = _f32(rhs)
+    out = np.empty((a.nrows, rhs.shape[1]), np.float32)
+    lib().orc_csr_matmul_dense(*a.args(), rhs, rhs.shape[1], out)
+    return out
+
+
+def lp_iteration(g, labels):
+    labels = np.ascontiguousarray(labels, np.int64)
+    out = np.empty_like(labels)
+    lib().orc_lp_iteration(g.indptr, g.indices, g.data, g.nrows, labels, out)
+    return out
+
+
+def lp_outliers(g, labels, seed):
+    labels = np.array(labels, np.int64)
+    lib().orc_lp_outliers(g.indptr, g.indices, g.data, g.nrows, labels, seed)
+    return labels
+
+
+def lp_remap(labels):
+    labels = np.array(labels, np.int64)
+    m = lib().orc_lp_remap(labels, len(labels))
+    return m, labels
+
+
+def lp_loop(g, approx_parts, n_iter, seed):
+    part = np.empty(g.nrows, np.int64)
+    m = lib().orc_lp_loop(g.indptr, g.indices, g.data, g.nrows, approx_parts, n_iter, seed, part)
+    return part, m
+
+
+def unit_range(m):
+    m = np.array(m, np.float32)
+    lib().orc_unit_range(m, m.shape[0], m.shape[1])
+    return m
+
+
+def pca_init(data, dim):
+    data = _f32(data)
+    out = np.empty((data.shape[0], dim), np.float32)
+    lib().orc_pca_init(data, data.shape[0], data.shape[1], dim, out)
+    return out
+
+
+def random_init(n, dim, seed):
+    out = np.empty((n, dim), np.float32)
+    lib().orc_random_init(n, dim, seed, out)
+    return out
+
+
+def lp_expand(g, part, n_parts, layout, scaling=0.1):
+    layout = _f32(layout)
+    dim = layout.shape[1]
+    out = np.empty((g.nrows, dim), np.float32)
+    lib().orc_lp_expand(g.indptr, g.indices, g.data, g.nrows, np.ascontiguousarray(part, np.int64),
+                        n_parts, layout, dim, scaling, out)
+    return out

@@ -61,3 +61,55 @@ def test_invalid_arguments_are_status_codes():
 def test_device_count_without_gpu_is_not_an_error():
     n = H.device_count()
     assert n >= 0
+
+
+def test_non_finite_embeddings_rejected_before_any_device_call():
+    """Every entry point that takes an embedding or distances rejects NaN / inf with
+    EVOC_ERR_INVALID_ARGUMENT before it touches a device (so this runs without a GPU)."""
+    L = H.lib()
+    e = np.zeros((6, 2), np.float32)
+    e[3, 1] = np.nan
+    u = np.zeros(5, np.int32)
+    w = np.zeros(5, np.float32)
+    assert L.evoc_hip_mst(e, 6, 2, 2, u, u.copy(), w) == 1
+    assert b"non-finite" in L.evoc_hip_last_error()
+    assert L.evoc_hip_core_sq(e, 6, 2, 2, np.zeros(6, np.float32)) == 1
+    lab = np.zeros((2, 6), np.int64)
+    st = np.zeros((2, 6), np.float32)
+    nl = C.c_int64(0)
+    assert L.evoc_hip_cluster_layers(e, 6, 2, 2, 2, 0.2, 2, lab, st, np.zeros(2), C.byref(nl)) == 1
+    e[3, 1] = np.inf
+    assert L.evoc_hip_search_n_clusters(e, 6, 2, 2, 2, lab[0].copy(), st[0].copy()) == 1
+    assert b"non-finite" in L.evoc_hip_last_error()
+    x = np.ones((4, 3), np.float32)
+    x[2, 2] = -np.inf
+    idx = np.zeros((4, 2), np.int32)
+    dist = np.zeros((4, 2), np.float32)
+    assert L.evoc_hip_knn(x, 4, 3, 0, 2, 0, idx, dist, None, None) == 1
+    dist[1, 1] = np.nan
+    assert L.evoc_hip_fuzzy(idx, dist, 4, 2, 2.0, 1, np.zeros(4, np.float32),
+                            np.zeros(4, np.float32), np.zeros(5, np.int64),
+                            np.zeros(8, np.int32), np.zeros(8, np.float32),
+                            C.byref(C.c_int64(0))) == 1
+
+
+def test_precomputed_knn_shapes_and_values_checked():
+    import pytest
+    x = np.zeros((5, 3), np.float32)
+    good_i = np.array([[1, 2], [0, 2], [0, 1], [4, -1], [3, -1]])
+    good_d = np.array([[0.1, 0.2], [0.1, 0.3], [0.2, 0.3], [0.5, np.inf], [0.5, np.nan]],
+                      np.float32)
+    pi, pd = H._check_precomputed((good_i, good_d), 5)   # -1 pads: their distance is free
+    assert pi.dtype == np.int32 and pd.dtype == np.float32 and pi.shape == (5, 2)
+    bad = [
+        (good_i[:4], good_d[:4]),                 # too few rows
+        (good_i, good_d[:, :1]),                  # distance shape differs
+        (good_i.ravel(), good_d.ravel()),         # not 2-d
+        (good_i + 10, good_d),                    # index out of range
+        (good_i.astype(np.float32), good_d),      # not integers
+        (good_i, np.where(good_i >= 0, -1.0, 0.0).astype(np.float32)),  # negative distance
+        (good_i, np.where(good_i >= 0, np.nan, 0.0).astype(np.float32)),  # NaN distance
+    ]
+    for pi, pd in bad:
+        with pytest.raises(ValueError):
+            H.evoc(x, precomputed_knn=(pi, pd))

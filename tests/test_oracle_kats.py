@@ -430,3 +430,91 @@ def test_end_to_end_blobs_recovered():
     pers = r["persistence"]
     best = max(i for i in range(len(pers)) if pers[i] == pers.max()) if len(pers) > 1 else 0
     assert _acc(r["labels"][best], gt) > 0.95
+
+
+def test_fp64_persistence_selects_the_reference_f32_layers():
+    """The library sums barcode persistence in fp64; the reference sums in T = f32
+    (src/clustering/persistence.rs:135-173) and find_peaks / select_diverse_peaks compare
+    those sums exactly. Over 40 seeded embeddings of varied size and cluster count the two
+    modes of the oracle select identical layers (labels, order and best layer); the
+    library is bit-exact against the fp64 mode (tests/test_stages_gpu.py)."""
+    agree = 0
+    total = 40
+    for seed in range(total):
+        rng = np.random.default_rng(seed)
+        n = int(rng.integers(500, 2500))
+        c = int(rng.integers(3, 40))
+        e, _ = datagen.gaussian_mixture(n, 8, c, float(rng.uniform(1.5, 5.0)), 1.0, seed=seed)
+        l64, s64, p64 = O.cluster_layers(e, 5, 5, 0.2, 10, fp64=True)
+        l32, s32, p32 = O.cluster_layers(e, 5, 5, 0.2, 10, fp64=False)
+        same = l64.shape == l32.shape and np.array_equal(l64, l32) and \
+            np.array_equal(s64.view(np.uint32), s32.view(np.uint32))
+        if same:
+            b64 = max(i for i in range(len(p64)) if p64[i] == p64.max()) if len(p64) > 1 else 0
+            b32 = max(i for i in range(len(p32)) if p32[i] == p32.max()) if len(p32) > 1 else 0
+            same = b64 == b32
+        agree += same
+    print(f"fp64 vs f32 persistence: identical layer selection on {agree}/{total} embeddings")
+    assert agree == total
+
+
+# ------------------------------------------------------------------ embedding trajectories
+# Two nodes joined by one edge of weight 1 (src/graph/embedding.rs:48-298, defaults
+# noise 0.5, alpha 0.1, beta1 0.5, beta2 0.9, eps 1e-7). Expected positions are literals
+# derived by hand in float64 from the closed forms written next to them; f32 arithmetic
+# agrees to a few ulps. Schedule: eps = max_w / w = 1, eps_neg = 1.5 / rate; both "next"
+# times start at their period, so epoch 0 samples nothing.
+_IP2 = np.array([0, 1, 2])
+_IND2 = np.array([1, 0], np.int32)
+_W2 = np.ones(2, np.float32)
+
+
+def test_embedding_epoch_zero_is_a_no_op():
+    init = np.array([[0.0, 0.0], [1.0, 0.0]], np.float32)
+    out = O.embedding(_IP2, _IND2, _W2, 2, n_epochs=1, init=init, seed=3)
+    assert np.array_equal(out, init)
+
+
+def test_embedding_first_step_bias_terms_and_lr():
+    """Epoch 1 (t = 2): lr = 0.1 (1 - 1/2) = 0.05. Attraction at d = 1:
+    coeff = (-2*0.5*1 - 2) / (2 - 0.5 + 1) = -1.2, gradient of node 0 = 1.2 along x.
+    Adam from zero moments: m = 0.5 * 1.2, v = 0.1 * 1.44, step =
+    lr * sqrt(1 - 0.81) / (1 - 0.25) * m / (sqrt(v) + eps') = 0.0459468239.
+    A t = 1 bias correction would give 0.05, lr without the decay 0.0918936,
+    a sampled epoch 0 two steps."""
+    init = np.array([[0.0, 0.0], [1.0, 0.0]], np.float32)
+    out = O.embedding(_IP2, _IND2, _W2, 2, n_epochs=2, init=init, seed=3)
+    np.testing.assert_allclose(out, [[0.0459468239, 0.0], [0.9540531761, 0.0]], rtol=0, atol=2e-7)
+
+
+def test_embedding_second_step_attractive_coefficient():
+    """Three epochs. Epoch 1 (lr = 0.0666667) moves each node 0.0612624319 inward; epoch 2
+    (t = 3, lr = 0.0333333) sees d2 = 1 - 2 * 0.0612624319 and the gradient
+    g2 = -coeff(d2) d2 = 1.2016598677 with coeff(d) = (-d - 2) / (2 d^2 - 0.5 d + 1);
+    m2 = 0.6 + 0.5 (g2 - 0.6), v2 = 0.144 + 0.1 (g2^2 - 0.144), and the step
+    lr * sqrt(1 - 0.729) / 0.875 * m2 / sqrt(v2) = 0.0341290262. The second step depends on
+    g2 / g1, so it pins the attractive coefficient (a clipped or differently scaled one
+    changes it)."""
+    init = np.array([[0.0, 0.0], [1.0, 0.0]], np.float32)
+    out = O.embedding(_IP2, _IND2, _W2, 2, n_epochs=3, init=init, seed=3)
+    np.testing.assert_allclose(out, [[0.0953914581, 0.0], [0.9046085419, 0.0]], rtol=0, atol=2e-7)
+
+
+def test_embedding_negative_counts_and_gamma_schedule():
+    """d = 2, negative_sample_rate 3 (eps_neg = 0.5, next_neg = 0.5), three epochs, seed 12:
+    the pinned Philox draws of node 0 (the smaller endpoint) all land on node 1.
+    Epoch 1: n_neg = floor((1 - 0.5) / 0.5) = 1 and gamma = 0.5 + 1/2 = 1.0. Attraction
+    -coeff(2) * 2 = +1.0 and repulsion gamma*4 / ((1 + 0.25*4) * 4) * (-2) = -1.0 cancel:
+    node 0 has gradient 0 and does not move (any other gamma moves it). Node 1 (no
+    negatives) steps 0.0666667 * 0.9189365 inward. next_neg becomes 0.5 + 1 * 0.5 = 1.0.
+    Epoch 2: n_neg = floor((2 - 1) / 0.5) = 2 and gamma = 1.5: node 0's gradient is
+    -coeff(d) d + 2 * clip(-gamma*4 d / ((1 + d^2/4) d^2)) = -2.1793687 at d = 1.9387376,
+    a step of -0.0313563360; node 1 ends at 1.9045601471."""
+    for dr, ep in ((0, 1), (0, 2), (1, 2)):
+        r = O.lib().orc_philox_u32(12, dr, ep, 0, 0x454E4547)
+        assert (r * 2) >> 32 == 1   # draw lands on node 1, not on node 0 itself
+    init = np.array([[0.0, 0.0], [2.0, 0.0]], np.float32)
+    out = O.embedding(_IP2, _IND2, _W2, 2, n_epochs=3, negative_sample_rate=3.0, init=init,
+                      seed=12)
+    np.testing.assert_allclose(out, [[-0.0313563360, 0.0], [1.9045601471, 0.0]], rtol=0,
+                               atol=4e-7)

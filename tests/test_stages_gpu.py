@@ -333,3 +333,33 @@ def test_result_arrays_outlive_the_result():
     assert np.array_equal(lab, keep[1])
     assert np.array_equal(nn, keep[2])
     emb[0, 0] = 1.0  # writable views
+
+
+def test_cluster_layers_agree_with_reference_f32_persistence():
+    """The GPU's fp64 persistence sums against the oracle's reference-order f32 mode
+    (src/clustering/persistence.rs:135-173): identical layers and labels."""
+    agree = 0
+    for seed in range(6):
+        e, _ = _embedding(3000 + 500 * seed, seed=40 + seed)
+        gl, gs, gp = H.cluster_layers(e, 5, 5, 0.2, 10)
+        ol, os_, op = O.cluster_layers(e, 5, 5, 0.2, 10, fp64=False)
+        agree += gl.shape == ol.shape and np.array_equal(gl, ol)
+    print(f"GPU fp64 vs oracle f32 persistence: identical layers on {agree}/6 embeddings")
+    assert agree == 6
+
+
+def test_non_finite_input_rejected_on_device_paths():
+    """evoc_hip_run and evoc_hip_run_device check the matrix on the device before any stage
+    reads it; a NaN or inf anywhere is EVOC_ERR_INVALID_ARGUMENT."""
+    import torch
+    x, _ = datagen.gaussian_mixture(600, 16, 4, 4.0, 1.0, seed=2)
+    for bad in (np.nan, np.inf):
+        y = x.copy()
+        y[417, 9] = bad
+        with pytest.raises(H.EvocError, match="non-finite"):
+            H.evoc(y, seed=1)
+        dev = torch.from_numpy(y).cuda()
+        with pytest.raises(H.EvocError, match="non-finite"):
+            H.evoc_device(dev.data_ptr(), 600, 16, seed=1)
+    r = H.evoc(x, seed=1)   # the library is still usable afterwards
+    assert r.cluster_layers.shape[1] == 600
