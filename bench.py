@@ -28,31 +28,12 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 HBM_PEAK_GBS = 8000.0
 
 
-def make_workload(n, d, n_centres=200, zipf=1.1, sd=0.15, noise_frac=0.05, seed=42):
-    """Seeded synthetic matrix generated directly in device memory (torch is only the
-    allocator here). Values are rounded to bf16-representable float32."""
-    import torch
-    g = torch.Generator(device="cuda").manual_seed(seed)
-    c = torch.randn(n_centres, d, device="cuda", generator=g)
-    c = c / c.norm(dim=1, keepdim=True)
-    n_noise = int(round(noise_frac * n))
-    n_cl = n - n_noise
-    p = 1.0 / torch.arange(1, n_centres + 1, device="cuda", dtype=torch.float64) ** zipf
-    lab = torch.multinomial(p / p.sum(), n_cl, replacement=True, generator=g)
-    x = torch.empty(n, d, device="cuda", dtype=torch.float32)
-    chunk = 1 << 18
-    for s in range(0, n_cl, chunk):
-        e = min(n_cl, s + chunk)
-        pts = c[lab[s:e]] + sd * torch.randn(e - s, d, device="cuda", generator=g)
-        x[s:e] = pts / pts.norm(dim=1, keepdim=True)
-    if n_noise:
-        z = torch.randn(n_noise, d, device="cuda", generator=g)
-        x[n_cl:] = z / z.norm(dim=1, keepdim=True)
-    perm = torch.randperm(n, device="cuda", generator=g)
-    x = x[perm].to(torch.bfloat16).to(torch.float32).contiguous()
-    labels = torch.cat([lab, torch.full((n_noise,), -1, device="cuda", dtype=lab.dtype)])[perm]
-    torch.cuda.synchronize()
-    return x, labels
+def make_workload(n, d, seed=42):
+    """The seeded 200-centre Zipf(1.1) unit-norm mixture with 5% noise, generated directly
+    in device memory (tests/datagen.py: unit_mixture_device) and rounded to
+    bf16-representable float32."""
+    import datagen
+    return datagen.unit_mixture_device(n, d, seed=seed)
 
 
 def read_traffic(symmetric):

@@ -394,4 +394,29 @@ int best_layer_index(const std::vector<double>& p) {
   return best;
 }
 
+// Adjusted Rand index (Hubert and Arabie, 1985) of two labellings; every label value,
+// -1 included, is a cluster of its own. Two labellings with one cluster each score 1.
+double adjusted_rand_index(const int64_t* a, const int64_t* b, int64_t n) {
+  if (n <= 1) return 1.0;
+  std::unordered_map<int64_t, int64_t> ca, cb;
+  std::unordered_map<uint64_t, int64_t> cab;
+  std::unordered_map<int64_t, int64_t> ia, ib;
+  for (int64_t i = 0; i < n; ++i) {
+    auto ra = ia.emplace(a[i], (int64_t)ia.size()).first->second;
+    auto rb = ib.emplace(b[i], (int64_t)ib.size()).first->second;
+    ca[ra]++;
+    cb[rb]++;
+    cab[((uint64_t)ra << 32) | (uint64_t)rb]++;
+  }
+  auto c2 = [](int64_t m) { return 0.5 * (double)m * (double)(m - 1); };
+  double sab = 0, sa = 0, sb = 0;
+  for (auto& kv : cab) sab += c2(kv.second);
+  for (auto& kv : ca) sa += c2(kv.second);
+  for (auto& kv : cb) sb += c2(kv.second);
+  const double expected = sa * sb / c2(n);
+  const double mx = 0.5 * (sa + sb);
+  if (mx == expected) return 1.0;
+  return (sab - expected) / (mx - expected);
+}
+
 }  // namespace evoc

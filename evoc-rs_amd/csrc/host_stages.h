@@ -50,5 +50,7 @@ Layers cluster_layers_from_linkage(const Linkage& L, int64_t base_mcs, double mi
 void search_n_clusters_from_linkage(const Linkage& L, int64_t target_k, int64_t* labels,
                                     float* strengths);
 int best_layer_index(const std::vector<double>& persistence);
+// adjusted Rand index of two labellings of n points (-1 counts as a label)
+double adjusted_rand_index(const int64_t* a, const int64_t* b, int64_t n);
 
 }  // namespace evoc

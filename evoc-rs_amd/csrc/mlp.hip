@@ -340,10 +340,10 @@ static void mlp_adam(MlpModel& M, float lr, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ warm start
-void warm_start_init(const float* dX, int64_t n, int d, int64_t ld, const Graph& g,
-                     const evoc_params_t& p, const evoc_hip_nn_params_t& nn, int dim,
-                     uint64_t seed, MlpWarmStart& ws, float* d_init, hipStream_t s,
-                     DevPool& pool, int verbose) {
+bool warm_start_init(const float* dX, int64_t n, int d, int64_t ld, const evoc_params_t& p,
+                     const evoc_hip_nn_params_t& nn, int dim, uint64_t seed, MlpWarmStart& ws,
+                     const WarmJudge& judge, float* d_init, hipStream_t s, DevPool& pool,
+                     int verbose) {
   const int64_t S = std::min<int64_t>(ws.subsample, n);
   // 1. subsample (sorted, Philox-keyed) and its plain layout
   std::vector<int64_t> order(n);
@@ -359,6 +359,7 @@ void warm_start_init(const float* dX, int64_t n, int d, int64_t ld, const Graph&
   Buf<int64_t> dsub(pool, S);
   EVOC_HIP(hipMemcpyAsync(dsub.p, sub.data(), S * sizeof(int64_t), hipMemcpyHostToDevice, s));
   Buf<float> xs(pool, (size_t)S * d);
+  Graph sg_keep;
   {
     gather_rows_kernel<<<(unsigned)ceil_div(S * d, 256), 256, 0, s>>>(dX, ld, d, dsub.p, S, xs.p);
     EVOC_HIP(hipGetLastError());
@@ -366,7 +367,7 @@ void warm_start_init(const float* dX, int64_t n, int d, int64_t ld, const Graph&
     Buf<int32_t> si(pool, S * k);
     Buf<float> sd(pool, S * k);
     knn_device(xs.p, S, d, d, nn.metric, k, nn.knn_precision, si.p, sd.p, nullptr, s, pool);
-    Graph sg;
+    Graph& sg = sg_keep;
     fuzzy_device(si.p, sd.p, S, k, p.neighbour_scale * (float)k, true, nullptr, nullptr, sg, s, pool);
     std::vector<int64_t> ptr(S + 1);
     std::vector<int32_t> col(sg.nnz);
@@ -467,24 +468,36 @@ void warm_start_init(const float* dX, int64_t n, int d, int64_t ld, const Graph&
     ws.train_epochs = epochs;
     if (verbose) std::printf("  MLP trained on %lld points: %d epochs, mse %.4f, %.1f ms\n",
                              (long long)S, epochs, last_loss, tms);
-    // 3. predict the layout of every point
+    // 3. predict: the subsample first (for the judge), then, if accepted, every point
     const int chunk = 16384;
     MlpBatch P;
     P.ensure(M, chunk, pool);
-    for (int64_t r0 = 0; r0 < n; r0 += chunk) {
-      const int rows = (int)std::min<int64_t>(chunk, n - r0);
-      const int R = (int)pad32(rows);
-      gather_bf16_kernel<<<(unsigned)ceil_div((int64_t)R * M.Dp, 256), 256, 0, s>>>(
-          dX, ld, d, nullptr, r0, rows, R, M.Dp, P.X.p);
-      const int save = P.rows_pad;
-      P.rows_pad = R;
-      mlp_forward(M, P, s);
-      P.rows_pad = save;
-      unstandardise_kernel<<<(unsigned)ceil_div((int64_t)rows * dim, 256), 256, 0, s>>>(
-          P.Y.p, M.Op, dim, rows, dmu.p, dsd.p, d_init + r0 * dim);
-    }
-    EVOC_HIP(hipGetLastError());
+    auto predict = [&](const float* src, int64_t lds, int64_t rows_total, float* dst) {
+      for (int64_t r0 = 0; r0 < rows_total; r0 += chunk) {
+        const int rows = (int)std::min<int64_t>(chunk, rows_total - r0);
+        const int R = (int)pad32(rows);
+        gather_bf16_kernel<<<(unsigned)ceil_div((int64_t)R * M.Dp, 256), 256, 0, s>>>(
+            src, lds, d, nullptr, r0, rows, R, M.Dp, P.X.p);
+        const int save = P.rows_pad;
+        P.rows_pad = R;
+        mlp_forward(M, P, s);
+        P.rows_pad = save;
+        unstandardise_kernel<<<(unsigned)ceil_div((int64_t)rows * dim, 256), 256, 0, s>>>(
+            P.Y.p, M.Op, dim, rows, dmu.p, dsd.p, dst + r0 * dim);
+      }
+      EVOC_HIP(hipGetLastError());
+    };
+    Buf<float> pred(pool, (size_t)S * dim);
+    predict(xs.p, d, S, pred.p);
     EVOC_HIP(hipStreamSynchronize(s));
+    ws.accepted = judge(sg_keep, lay.p, pred.p, S, ws);
+    if (verbose)
+      std::printf("  MLP layout on the subsample: ARI %.4f, MST ratio %.4f -> %s\n", ws.judge_ari,
+                  ws.judge_mst_ratio, ws.accepted ? "accepted" : "rejected (plain init)");
+    if (!ws.accepted) return false;
+    predict(dX, ld, n, d_init);
+    EVOC_HIP(hipStreamSynchronize(s));
+    return true;
   }
 }
 

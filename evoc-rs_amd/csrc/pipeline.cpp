@@ -175,7 +175,7 @@ void free_result(evoc_result_t* r) {
 // The clustering stage on a device embedding: MST -> linkage -> layers or search.
 Layers cluster_device(const float* d_emb, int64_t n, int dim, const evoc_params_t& p,
                       hipStream_t s, DevPool& pool, double* mst_ms, double* layer_ms,
-                      int verbose = 0) {
+                      int verbose = 0, double* mst_total = nullptr) {
   Layers out;
   auto t0 = clk::now();
   auto tl = t0;
@@ -196,6 +196,11 @@ Layers cluster_device(const float* d_emb, int64_t n, int dim, const evoc_params_
   // at the last linkage node, so a forest must not reach it.
   EVOC_CHECK(ne == std::max<int64_t>(n - 1, 0),
              "the minimum spanning tree is disconnected (embedding distances overflow f32)");
+  if (mst_total) {
+    double t = 0.0;
+    for (int64_t e = 0; e < ne; ++e) t += (double)w[e];
+    *mst_total = t;
+  }
   const Linkage L = linkage_from_sorted(u.data(), v.data(), w.data(), ne, n);
   lap("linkage");
   if (mst_ms) *mst_ms = ms_since(t0);
@@ -282,13 +287,41 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
   t0 = clk::now();
   Buf<float> demb(pool, n * dim);
   int epochs = p.n_epochs;
+  MlpWarmStart ws;
+  bool warm_used = false;
   if (nn.warm_start) {
     if (verbose) std::printf("Computing learned warm-start initialisation...\n");
-    MlpWarmStart ws;
     ws.subsample = nn.mlp_subsample > 0 ? nn.mlp_subsample : 50000;
-    warm_start_init(dX, n, (int)d, ld, g, p, nn, dim, seed, ws, demb.p, s, pool, verbose);
-    epochs = nn.warm_epochs > 0 ? nn.warm_epochs : std::max(1, (int)(0.3 * p.n_epochs));
-  } else {
+    const int warm_epochs =
+        nn.warm_epochs > 0 ? nn.warm_epochs : std::max(1, (int)(0.3 * p.n_epochs));
+    // The learned layout is accepted only if, on the training subsample, the warm-started
+    // short run reproduces the plain run: best-layer labels with ARI >= 0.98 and the MST
+    // total weight within 1%. Otherwise the plain initial layout and epoch count are used.
+    WarmJudge judge = [&](const Graph& sg, const float* d_plain, const float* d_pred, int64_t S,
+                          MlpWarmStart& w) {
+      Buf<float> dw(pool, (size_t)S * dim);
+      EVOC_HIP(hipMemcpyAsync(dw.p, d_pred, (size_t)S * dim * sizeof(float),
+                              hipMemcpyDeviceToDevice, s));
+      EmbedParams wp;
+      wp.n_epochs = warm_epochs;
+      wp.noise_level = p.noise_level;
+      wp.initial_alpha = 0.1f;
+      embed_device(sg, dim, wp, dw.p, false, seed, s, pool);
+      EVOC_HIP(hipStreamSynchronize(s));
+      double mst_plain = 0.0, mst_warm = 0.0;
+      Layers lp = cluster_device(d_plain, S, dim, p, s, pool, nullptr, nullptr, 0, &mst_plain);
+      Layers lw = cluster_device(dw.p, S, dim, p, s, pool, nullptr, nullptr, 0, &mst_warm);
+      const auto& bp = lp.labels[best_layer_index(lp.persistence)];
+      const auto& bw = lw.labels[best_layer_index(lw.persistence)];
+      w.judge_ari = adjusted_rand_index(bp.data(), bw.data(), S);
+      w.judge_mst_ratio = mst_plain > 0.0 ? mst_warm / mst_plain : 1.0;
+      return w.judge_ari >= 0.98 && std::fabs(w.judge_mst_ratio - 1.0) <= 0.01;
+    };
+    warm_used = warm_start_init(dX, n, (int)d, ld, p, nn, dim, seed, ws, judge, demb.p, s, pool,
+                                verbose);
+    if (warm_used) epochs = warm_epochs;
+  }
+  if (!warm_used) {
     if (verbose) std::printf("Computing label propagation initialisation...\n");
     if (p.symmetrise) {  // the device CSR is the column-sorted graph
       label_prop_init_device(g.indptr.p, g.indices.p, g.weights.p, g.nnz, n, dim, dX, d, ld, seed,
@@ -371,6 +404,9 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
   }
   r->best_layer = best_layer_index(lay.persistence);
   r->knn_fallback_rows = fallback;
+  r->warm_start_used = warm_used ? 1 : 0;
+  r->warm_start_ari = ws.judge_ari;
+  r->warm_start_mst_ratio = ws.judge_mst_ratio;
   r->knn_kernel_ms = knn_kernel_ms;
   r->embed_kernel_ms = embed_kernel_ms;
   stage[6] = ms_since(t_all);

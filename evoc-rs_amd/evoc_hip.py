@@ -74,6 +74,9 @@ class _Result(C.Structure):
         ("knn_fallback_rows", C.c_int64),
         ("knn_kernel_ms", C.c_double),
         ("embed_kernel_ms", C.c_double),
+        ("warm_start_used", C.c_int32),
+        ("warm_start_ari", C.c_double),
+        ("warm_start_mst_ratio", C.c_double),
     ]
 
 
@@ -238,6 +241,9 @@ class EvocResult:
         self.knn_fallback_rows = r.knn_fallback_rows
         self.knn_kernel_ms = r.knn_kernel_ms
         self.embed_kernel_ms = r.embed_kernel_ms
+        self.warm_start_used = bool(r.warm_start_used)
+        self.warm_start_ari = r.warm_start_ari
+        self.warm_start_mst_ratio = r.warm_start_mst_ratio
 
     def best_labels(self):
         return self.cluster_layers[self.best_layer]

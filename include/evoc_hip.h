@@ -57,7 +57,9 @@ typedef struct {
 typedef struct {
   int32_t metric;          /* EVOC_METRIC_* */
   int32_t knn_precision;   /* EVOC_KNN_* */
-  int32_t warm_start;      /* 0 = label-propagation init (plain), 1 = learned MLP init */
+  int32_t warm_start;      /* 0 = label-propagation init (plain), 1 = learned MLP init,
+                            * accepted only if it passes the check on its training
+                            * subsample, else the plain init runs */
   int32_t warm_epochs;     /* embedding epochs of the warm-started run; 0 = 30% of n_epochs */
   int32_t mlp_subsample;   /* points in the MLP training subsample; 0 = 50000 */
   int32_t device_id;       /* HIP device ordinal */
@@ -81,6 +83,12 @@ typedef struct {
   int64_t knn_fallback_rows;       /* rows re-searched exactly after a failed certificate */
   double knn_kernel_ms;            /* MFMA top-candidate kernel, hipEvent-timed on its stream */
   double embed_kernel_ms;          /* all embedding epochs, hipEvent-timed */
+  /* learned warm start (warm_start = 1): whether its layout was accepted, and the check
+   * on the training subsample that decided it (warm-started short run against the plain
+   * run: best-layer ARI >= 0.98 and MST total weight ratio within 1%) */
+  int32_t warm_start_used;
+  double warm_start_ari;
+  double warm_start_mst_ratio;
 } evoc_result_t;
 
 void evoc_hip_default_params(evoc_params_t* p);

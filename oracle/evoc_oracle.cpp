@@ -57,14 +57,21 @@ inline double rng_unit(const uint32_t r[4]) {  // [0,1) with 53 bits
 // ---------------------------------------------------------------- distances
 // High-dimensional f32 dot / squared distance: 64 strided partial sums, then a
 // halving tree (s = 32..1: acc[j] += acc[j+s]).
+// (Full blocks of 64 first, the ragged tail after: the same order, and the lane loop of a
+// full block vectorises.)
 float canon_dot(const float* a, const float* b, int64_t d) {
   float acc[64];
   for (int j = 0; j < 64; ++j) acc[j] = 0.0f;
-  for (int64_t t = 0; t < d; t += 64)
-    for (int j = 0; j < 64 && t + j < d; ++j) {
+  const int64_t full = d / 64 * 64;
+  for (int64_t t = 0; t < full; t += 64)
+    for (int j = 0; j < 64; ++j) {
       float p = a[t + j] * b[t + j];
       acc[j] = acc[j] + p;
     }
+  for (int j = 0; full + j < d; ++j) {
+    float p = a[full + j] * b[full + j];
+    acc[j] = acc[j] + p;
+  }
   for (int s = 32; s >= 1; s >>= 1)
     for (int j = 0; j < s; ++j) acc[j] = acc[j] + acc[j + s];
   return acc[0];
@@ -72,12 +79,18 @@ float canon_dot(const float* a, const float* b, int64_t d) {
 float canon_sq(const float* a, const float* b, int64_t d) {
   float acc[64];
   for (int j = 0; j < 64; ++j) acc[j] = 0.0f;
-  for (int64_t t = 0; t < d; t += 64)
-    for (int j = 0; j < 64 && t + j < d; ++j) {
+  const int64_t full = d / 64 * 64;
+  for (int64_t t = 0; t < full; t += 64)
+    for (int j = 0; j < 64; ++j) {
       float df = a[t + j] - b[t + j];
       float p = df * df;
       acc[j] = acc[j] + p;
     }
+  for (int j = 0; full + j < d; ++j) {
+    float df = a[full + j] - b[full + j];
+    float p = df * df;
+    acc[j] = acc[j] + p;
+  }
   for (int s = 32; s >= 1; s >>= 1)
     for (int j = 0; j < s; ++j) acc[j] = acc[j] + acc[j + s];
   return acc[0];
@@ -291,35 +304,53 @@ extern "C" int64_t orc_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist
 static int64_t symmetrise_coo(const std::vector<int64_t>& rr, const std::vector<int32_t>& cc,
                               const std::vector<float>& ww, int64_t n, int64_t* indptr,
                               int32_t* indices, float* weights) {
-  // forward[i][j] = w(i->j), backward[j][i] = w(i->j); last insert wins
-  std::vector<std::map<int32_t, float>> fwd(n), bwd(n);
-  for (size_t e = 0; e < rr.size(); ++e) {
-    fwd[rr[e]][cc[e]] = ww[e];
-    bwd[cc[e]][(int32_t)rr[e]] = ww[e];
+  // Every directed entry (i -> j, w) is filed under row i as forward and under row j as
+  // backward; records keep their input order within a row (counting sort), then each row is
+  // sorted stably by column, so for a repeated pair the last record of a direction wins.
+  struct Rec { int32_t col; int32_t dir; float w; };
+  const int64_t m = (int64_t)rr.size();
+  std::vector<int64_t> off(n + 1, 0);
+  for (int64_t e = 0; e < m; ++e) {
+    off[rr[e] + 1]++;
+    off[cc[e] + 1]++;
   }
-  int64_t p = 0;
-  indptr[0] = 0;
+  for (int64_t i = 0; i < n; ++i) off[i + 1] += off[i];
+  std::vector<Rec> rec(2 * m);
+  std::vector<int64_t> cur(off.begin(), off.end() - 1);
+  for (int64_t e = 0; e < m; ++e) {
+    rec[cur[rr[e]]++] = {cc[e], 0, ww[e]};
+    rec[cur[cc[e]]++] = {(int32_t)rr[e], 1, ww[e]};
+  }
+  std::vector<int64_t> cnt(n, 0);
+#pragma omp parallel for schedule(dynamic, 1024)
   for (int64_t i = 0; i < n; ++i) {
-    std::set<int32_t> keys;
-    for (auto& kv : fwd[i]) keys.insert(kv.first);
-    for (auto& kv : bwd[i]) keys.insert(kv.first);
-    for (int32_t j : keys) {
-      auto a = fwd[i].find(j);
-      auto b = bwd[i].find(j);
-      float wij = a == fwd[i].end() ? 0.0f : a->second;
-      float wji = b == bwd[i].end() ? 0.0f : b->second;
+    std::stable_sort(rec.begin() + off[i], rec.begin() + off[i + 1],
+                     [](const Rec& a, const Rec& b) { return a.col < b.col; });
+    // compact in place: one record per column with (w_ij, w_ji) folded
+    int64_t out = off[i];
+    for (int64_t q = off[i]; q < off[i + 1];) {
+      const int32_t j = rec[q].col;
+      float wij = 0.0f, wji = 0.0f;
+      for (; q < off[i + 1] && rec[q].col == j; ++q) {
+        if (rec[q].dir == 0) wij = rec[q].w;
+        else wji = rec[q].w;
+      }
       float s = wij + wji;
       float pr = wij * wji;
       float wsym = s - pr;
-      if (wsym > 0.0f) {
-        indices[p] = j;
-        weights[p] = wsym;
-        ++p;
-      }
+      if (wsym > 0.0f) rec[out++] = {j, 0, wsym};
     }
-    indptr[i + 1] = p;
+    cnt[i] = out - off[i];
   }
-  return p;
+  indptr[0] = 0;
+  for (int64_t i = 0; i < n; ++i) indptr[i + 1] = indptr[i] + cnt[i];
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t q = 0; q < cnt[i]; ++q) {
+      indices[indptr[i] + q] = rec[off[i] + q].col;
+      weights[indptr[i] + q] = rec[off[i] + q].w;
+    }
+  return indptr[n];
 }
 
 // symmetrise_coo on explicit directed weights (for the reference's union tests)

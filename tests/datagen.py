@@ -58,3 +58,49 @@ def to_bf16_values(x):
     b = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
     r = (b + 0x7FFF + ((b >> 16) & 1)) >> 16
     return (r.astype(np.uint32) << 16).view(np.float32)
+
+
+def unit_mixture_device(n, d, n_centres=200, zipf=1.1, sd=0.15, noise_frac=0.05, seed=42,
+                        hierarchical=None):
+    """The same kind of mixture as `unit_mixture`, generated on the GPU with torch (seeded;
+    torch is only the allocator and RNG here). Values are rounded to bf16-representable
+    float32. Returns (x, labels) as device tensors; noise points are labelled -1.
+
+    hierarchical=(n_parents, children_per_parent, child_spread, leaf_sd) instead draws
+    parents ~ N(0, I), children = parent + child_spread * N(0, I), points =
+    child + leaf_sd * N(0, I), all L2-normalised, equal-size leaves, no noise."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    x = torch.empty(n, d, device="cuda", dtype=torch.float32)
+    chunk = 1 << 18
+    if hierarchical is not None:
+        n_par, n_child, spread, leaf_sd = hierarchical
+        par = torch.randn(n_par, d, device="cuda", generator=g)
+        c = par.repeat_interleave(n_child, 0) + spread * torch.randn(n_par * n_child, d,
+                                                                     device="cuda", generator=g)
+        lab = torch.arange(n, device="cuda") % (n_par * n_child)
+        for s in range(0, n, chunk):
+            e = min(n, s + chunk)
+            pts = c[lab[s:e]] + leaf_sd * torch.randn(e - s, d, device="cuda", generator=g)
+            x[s:e] = pts / pts.norm(dim=1, keepdim=True)
+        labels = lab
+    else:
+        c = torch.randn(n_centres, d, device="cuda", generator=g)
+        c = c / c.norm(dim=1, keepdim=True)
+        n_noise = int(round(noise_frac * n))
+        n_cl = n - n_noise
+        p = 1.0 / torch.arange(1, n_centres + 1, device="cuda", dtype=torch.float64) ** zipf
+        lab = torch.multinomial(p / p.sum(), n_cl, replacement=True, generator=g)
+        for s in range(0, n_cl, chunk):
+            e = min(n_cl, s + chunk)
+            pts = c[lab[s:e]] + sd * torch.randn(e - s, d, device="cuda", generator=g)
+            x[s:e] = pts / pts.norm(dim=1, keepdim=True)
+        if n_noise:
+            z = torch.randn(n_noise, d, device="cuda", generator=g)
+            x[n_cl:] = z / z.norm(dim=1, keepdim=True)
+        labels = torch.cat([lab, torch.full((n_noise,), -1, device="cuda", dtype=lab.dtype)])
+    perm = torch.randperm(n, device="cuda", generator=g)
+    x = x[perm].to(torch.bfloat16).to(torch.float32).contiguous()
+    labels = labels[perm]
+    torch.cuda.synchronize()
+    return x, labels

@@ -351,15 +351,15 @@ def test_cluster_layers_agree_with_reference_f32_persistence():
 def test_non_finite_input_rejected_on_device_paths():
     """evoc_hip_run and evoc_hip_run_device check the matrix on the device before any stage
     reads it; a NaN or inf anywhere is EVOC_ERR_INVALID_ARGUMENT."""
-    import torch
+    from devbuf import DeviceArray
     x, _ = datagen.gaussian_mixture(600, 16, 4, 4.0, 1.0, seed=2)
     for bad in (np.nan, np.inf):
         y = x.copy()
         y[417, 9] = bad
         with pytest.raises(H.EvocError, match="non-finite"):
             H.evoc(y, seed=1)
-        dev = torch.from_numpy(y).cuda()
+        dev = DeviceArray(y)
         with pytest.raises(H.EvocError, match="non-finite"):
-            H.evoc_device(dev.data_ptr(), 600, 16, seed=1)
+            H.evoc_device(dev.ptr.value, 600, 16, seed=1)
     r = H.evoc(x, seed=1)   # the library is still usable afterwards
     assert r.cluster_layers.shape[1] == 600
