@@ -113,3 +113,26 @@ def test_precomputed_knn_shapes_and_values_checked():
     for pi, pd in bad:
         with pytest.raises(ValueError):
             H.evoc(x, precomputed_knn=(pi, pd))
+
+
+def _struct_fields(src, name):
+    """Field names of `typedef struct { ... } name;` in the C header."""
+    body = re.search(r"typedef struct \{([^{}]*)\}\s*" + name + ";", src).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    return [re.findall(r"(\w+)(?:\[\d+\])?\s*;", line)[0] for line in body.split("\n")
+            if re.search(r"\w+(?:\[\d+\])?\s*;", line)]
+
+
+def test_rust_and_python_bindings_match_the_header():
+    hdr = open(os.path.join(REPO, "include", "evoc_hip.h")).read()
+    rs = open(os.path.join(REPO, "rust", "evoc-hip-sys", "src", "lib.rs")).read()
+    # every function the header declares is bound in the -sys crate
+    rust_fns = set(re.findall(r"pub fn (evoc_hip_\w+)\s*\(", rs))
+    assert set(_declared()) <= rust_fns, set(_declared()) - rust_fns
+    for cname, pyname in (("evoc_params_t", "_Params"), ("evoc_hip_nn_params_t", "_NnParams"),
+                          ("evoc_result_t", "_Result")):
+        cf = _struct_fields(hdr, cname)
+        body = re.search(r"pub struct " + cname + r" \{(.*?)\n\}", rs, re.S).group(1)
+        rf = re.findall(r"pub (\w+):", body)
+        assert rf == cf, (cname, rf, cf)
+        assert [f[0] for f in getattr(H, pyname)._fields_] == cf, cname
