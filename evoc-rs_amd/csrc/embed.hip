@@ -16,6 +16,7 @@
 // A node takes 8 lanes (at least dim; chunks of 8 slots), a wavefront with 64 or more
 // edges (chunks of 64), or a workgroup of 16 wavefronts with 1024 or more (16 chunks of
 // 64 staged at once); the three kernels give bit-identical layouts.
+#include "api_internal.h"
 #include "kernels.h"
 
 namespace evoc {
@@ -700,28 +701,51 @@ void launch_epoch(hipStream_t s, int64_t n, int dim, const int64_t* indptr,
 
 }  // namespace
 
-void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, bool random_init,
-                  uint64_t seed, hipStream_t s, DevPool& pool, float* kernel_ms) {
-  const int64_t n = g.n;
-  EVOC_CHECK(dim >= 1 && dim <= MAXD, "embedding: dim must be in [1, 16]");
+namespace {
+
+__global__ void zero_words_kernel(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x)
+    p[q] = 0u;
+}
+void zero_words(void* p, int64_t words, hipStream_t s) {
+  if (words <= 0) return;
+  zero_words_kernel<<<(unsigned)std::min<int64_t>(ceil_div(words, 256), 4096), 256, 0, s>>>(
+      static_cast<uint32_t*>(p), words);
+}
+
+// Device memory the epochs use besides the graph and the layout: the per-slot schedule,
+// the Adam moments, the second layout buffer of the Jacobi update and the high-degree
+// node lists (nnz, n * dim and n entries).
+struct EmbedScratch {
+  unsigned* maxw;  // 1
+  float *eps_s, *next_s, *next_n;  // nnz each
+  float *m, *v, *e2;  // n * dim each
+  int32_t *wide, *hubs;  // n each
+  int* nwide;  // 2
+};
+
+// The epochs of embed_device on a CSR of embedding slots (indptr/indices/weights),
+// scratch given by the caller. No device allocation; kernel_ms (if not null) adds two
+// events.
+void embed_core(const int64_t* indptr, const int32_t* indices, const float* weights, int64_t n,
+                int64_t nnz, int dim, const EmbedParams& p, float* d_emb, bool random_init,
+                uint64_t seed, hipStream_t s, const EmbedScratch& w, float* kernel_ms) {
   if (n == 0) return;
   const unsigned gn = (unsigned)ceil_div(n * dim, 256);
   if (random_init) random_init_kernel<<<gn, 256, 0, s>>>(d_emb, n, dim, seed);
-  if (g.nnz == 0 || p.n_epochs <= 0) return;
-  const int64_t nnz = g.nnz;
-  Buf<unsigned> maxw(pool, 1);
-  Buf<float> eps_s(pool, nnz), next_s(pool, nnz), next_n(pool, nnz);
-  Buf<float> m(pool, n * dim), v(pool, n * dim), e2(pool, n * dim);
-  EVOC_HIP(hipMemsetAsync(maxw.p, 0, sizeof(unsigned), s));
-  EVOC_HIP(hipMemsetAsync(m.p, 0, n * dim * sizeof(float), s));
-  EVOC_HIP(hipMemsetAsync(v.p, 0, n * dim * sizeof(float), s));
+  if (nnz == 0 || p.n_epochs <= 0) return;
+  zero_words(w.maxw, 1, s);
+  zero_words(w.m, n * dim, s);
+  zero_words(w.v, n * dim, s);
+  zero_words(w.nwide, 2, s);
   max_weight_kernel<<<(unsigned)std::min<int64_t>(ceil_div(nnz, 256), 4096), 256, 0, s>>>(
-      g.weights.p, nnz, maxw.p);
+      weights, nnz, w.maxw);
   const float neg_factor = 1.5f / p.negative_sample_rate;
   schedule_init_kernel<<<(unsigned)ceil_div(nnz, 256), 256, 0, s>>>(
-      g.weights.p, nnz, maxw.p, neg_factor, eps_s.p, next_s.p, next_n.p);
+      weights, nnz, w.maxw, neg_factor, w.eps_s, w.next_s, w.next_n);
   unsigned hmax = 0;
-  EVOC_HIP(hipMemcpyAsync(&hmax, maxw.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipMemcpyAsync(&hmax, w.maxw, sizeof(unsigned), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
   if (hmax == 0) return;  // every weight is zero: epochs_per_sample = 1/(0/0) -> reference NaN path skipped
   const float one_m_b1 = 1.0f - p.beta1, one_m_b2 = 1.0f - p.beta2;
@@ -729,23 +753,22 @@ void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, b
   const float n_epochs_f = (float)p.n_epochs;
   const unsigned grid = (unsigned)ceil_div(n, 256);
   // high-degree nodes for the wavefront-per-node kernel
-  Buf<int32_t> wide(pool, n), hubs(pool, n);
-  Buf<int> nwide_d(pool, 2);
-  EVOC_HIP(hipMemsetAsync(nwide_d.p, 0, 2 * sizeof(int), s));
   EVOC_CHECK(p.wide_deg >= 1, "embedding: wide_deg must be >= 1");
   const int64_t hub_deg = std::max<int64_t>(p.wide_deg, HUB_DEG);
-  wide_nodes_kernel<<<grid, 256, 0, s>>>(g.indptr.p, n, p.wide_deg, hub_deg, wide.p, hubs.p,
-                                         nwide_d.p);
+  wide_nodes_kernel<<<grid, 256, 0, s>>>(indptr, n, p.wide_deg, hub_deg, w.wide, w.hubs,
+                                         w.nwide);
   int nwide_h[2] = {0, 0};
-  EVOC_HIP(hipMemcpyAsync(nwide_h, nwide_d.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipMemcpyAsync(nwide_h, w.nwide, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
   const int64_t nwide = nwide_h[0], nhub = nwide_h[1];
-  hipEvent_t e0, e1;
-  EVOC_HIP(hipEventCreate(&e0));
-  EVOC_HIP(hipEventCreate(&e1));
-  EVOC_HIP(hipEventRecord(e0, s));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (kernel_ms) {
+    EVOC_HIP(hipEventCreate(&e0));
+    EVOC_HIP(hipEventCreate(&e1));
+    EVOC_HIP(hipEventRecord(e0, s));
+  }
   float* cur = d_emb;
-  float* nxt = e2.p;
+  float* nxt = w.e2;
   for (int epoch = 0; epoch < p.n_epochs; ++epoch) {
     const float lr = p.initial_alpha * (1.0f - (float)epoch / n_epochs_f);
     const float epoch_t = (float)epoch;
@@ -757,10 +780,9 @@ void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, b
     const float gamma = 0.5f + (float)epoch / denom;
     const float lr_ad = lr * ad_scale;
 #define EVOC_EPOCH(D)                                                                         \
-  launch_epoch<D>(s, n, dim, g.indptr.p, g.indices.p, next_s.p, eps_s.p, next_n.p,    \
-                  neg_factor, cur, nxt, m.p, v.p, epoch_t, (uint32_t)epoch, lr_ad, epsc, gamma, \
-                  p.noise_level, one_m_b1, one_m_b2, seed, wide.p, nwide, p.wide_deg, \
-                  hubs.p, nhub)
+  launch_epoch<D>(s, n, dim, indptr, indices, w.next_s, w.eps_s, w.next_n, neg_factor, cur, nxt, \
+                  w.m, w.v, epoch_t, (uint32_t)epoch, lr_ad, epsc, gamma, p.noise_level,      \
+                  one_m_b1, one_m_b2, seed, w.wide, nwide, p.wide_deg, w.hubs, nhub)
     switch (dim) {
       case 2: EVOC_EPOCH(2); break;
       case 4: EVOC_EPOCH(4); break;
@@ -772,15 +794,125 @@ void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, b
     std::swap(cur, nxt);
   }
   EVOC_HIP(hipGetLastError());
-  EVOC_HIP(hipEventRecord(e1, s));
+  if (kernel_ms) EVOC_HIP(hipEventRecord(e1, s));
   if (cur != d_emb)
     EVOC_HIP(hipMemcpyAsync(d_emb, cur, n * dim * sizeof(float), hipMemcpyDeviceToDevice, s));
   EVOC_HIP(hipStreamSynchronize(s));
-  float ms = 0.0f;
-  EVOC_HIP(hipEventElapsedTime(&ms, e0, e1));
-  if (kernel_ms) *kernel_ms = ms;
-  EVOC_HIP(hipEventDestroy(e0));
-  EVOC_HIP(hipEventDestroy(e1));
+  if (kernel_ms) {
+    float ms = 0.0f;
+    EVOC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *kernel_ms = ms;
+    EVOC_HIP(hipEventDestroy(e0));
+    EVOC_HIP(hipEventDestroy(e1));
+  }
+}
+
+}  // namespace
+
+void embed_device(const Graph& g, int dim, const EmbedParams& p, float* d_emb, bool random_init,
+                  uint64_t seed, hipStream_t s, DevPool& pool, float* kernel_ms) {
+  const int64_t n = g.n;
+  EVOC_CHECK(dim >= 1 && dim <= MAXD, "embedding: dim must be in [1, 16]");
+  if (n == 0) return;
+  const int64_t nnz = g.nnz;
+  const bool epochs = nnz > 0 && p.n_epochs > 0;
+  Buf<unsigned> maxw(pool, 1);
+  Buf<float> eps_s(pool, epochs ? nnz : 1), next_s(pool, epochs ? nnz : 1),
+      next_n(pool, epochs ? nnz : 1);
+  Buf<float> m(pool, epochs ? n * dim : 1), v(pool, epochs ? n * dim : 1),
+      e2(pool, epochs ? n * dim : 1);
+  Buf<int32_t> wide(pool, epochs ? n : 1), hubs(pool, epochs ? n : 1);
+  Buf<int> nwide(pool, 2);
+  EmbedScratch w{maxw.p, eps_s.p, next_s.p, next_n.p, m.p, v.p, e2.p, wide.p, hubs.p, nwide.p};
+  embed_core(g.indptr.p, g.indices.p, g.weights.p, n, nnz, dim, p, d_emb, random_init, seed, s, w,
+             kernel_ms);
 }
 
 }  // namespace evoc
+
+// ------------------------------------------------------------------ allocation-free variant
+// The epochs' scratch in one static device array per GPU, for callers that cannot allocate
+// (code under HIP graph capture, serving loops). Limits: n <= EMBED_NOALLOC_MAX_N nodes,
+// nnz <= EMBED_NOALLOC_MAX_NNZ slots, dim <= 16: 36.5 MB, held by each GPU the library
+// runs on until the process ends.
+namespace evoc {
+namespace {
+constexpr int64_t EMBED_NOALLOC_MAX_N = 65536;
+constexpr int64_t EMBED_NOALLOC_MAX_NNZ = 2097152;
+constexpr size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t SC_HEAD = 512;                                     // maxw, nwide
+constexpr size_t SC_SCHED = al256(EMBED_NOALLOC_MAX_NNZ * 4);       // eps_s, next_s, next_n
+constexpr size_t SC_STATE = al256(EMBED_NOALLOC_MAX_N * MAXD * 4);  // m, v, e2
+constexpr size_t SC_NODES = al256(EMBED_NOALLOC_MAX_N * 4);         // wide, hubs
+constexpr size_t SCRATCH_BYTES = SC_HEAD + 3 * SC_SCHED + 3 * SC_STATE + 2 * SC_NODES;
+constexpr int MAX_DEVICES = 64;
+}  // namespace
+}  // namespace evoc
+
+static __device__ __align__(256) unsigned char scratch[evoc::SCRATCH_BYTES];
+
+namespace evoc {
+namespace {
+// device address of `scratch` on each GPU, looked up once per device with it current
+unsigned char* g_scratch[MAX_DEVICES] = {};
+
+unsigned char* scratch_on(int device) {
+  EVOC_CHECK(device >= 0 && device < MAX_DEVICES, "device_id out of range");
+  if (!g_scratch[device]) {
+    void* ptr = nullptr;
+    EVOC_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[device] = static_cast<unsigned char*>(ptr);
+  }
+  return g_scratch[device];
+}
+}  // namespace
+}  // namespace evoc
+
+extern "C" int evoc_hip_embed_noalloc(const int64_t* d_indptr, const int32_t* d_indices,
+                                      const float* d_weights, int64_t n, int64_t nnz, int32_t dim,
+                                      int32_t n_epochs, float noise_level,
+                                      float negative_sample_rate, float initial_alpha,
+                                      float beta1, float beta2, float eps, int32_t random_init,
+                                      uint64_t seed, int32_t device_id, float* d_emb) {
+  using namespace evoc;
+  return guarded([&] {
+    EVOC_CHECK(d_indptr && d_emb && (nnz == 0 || (d_indices && d_weights)),
+               "embed_noalloc: NULL buffer");
+    EVOC_CHECK(n >= 0 && nnz >= 0 && dim >= 1 && dim <= MAXD,
+               "embed_noalloc: need n >= 0, nnz >= 0 and dim in [1, 16]");
+    EVOC_CHECK(n <= EMBED_NOALLOC_MAX_N, "embed_noalloc: n exceeds 65536 (the static scratch)");
+    EVOC_CHECK(nnz <= EMBED_NOALLOC_MAX_NNZ,
+               "embed_noalloc: nnz exceeds 2097152 (the static scratch)");
+    EVOC_CHECK(n_epochs >= 0, "embed_noalloc: n_epochs must be >= 0");
+    Ctx& c = context(device_id);
+    unsigned char* base = scratch_on(device_id);
+    EmbedScratch w;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+      unsigned char* q = base + off;
+      off += al256(bytes);
+      return q;
+    };
+    w.maxw = reinterpret_cast<unsigned*>(take(256));
+    w.nwide = reinterpret_cast<int*>(take(256));
+    w.eps_s = reinterpret_cast<float*>(take(SC_SCHED));
+    w.next_s = reinterpret_cast<float*>(take(SC_SCHED));
+    w.next_n = reinterpret_cast<float*>(take(SC_SCHED));
+    w.m = reinterpret_cast<float*>(take(SC_STATE));
+    w.v = reinterpret_cast<float*>(take(SC_STATE));
+    w.e2 = reinterpret_cast<float*>(take(SC_STATE));
+    w.wide = reinterpret_cast<int32_t*>(take(SC_NODES));
+    w.hubs = reinterpret_cast<int32_t*>(take(SC_NODES));
+    EVOC_CHECK(off <= SCRATCH_BYTES, "embed_noalloc: scratch layout");
+    EmbedParams ep;
+    ep.n_epochs = n_epochs;
+    ep.noise_level = noise_level;
+    ep.negative_sample_rate = negative_sample_rate;
+    ep.initial_alpha = initial_alpha;
+    ep.beta1 = beta1;
+    ep.beta2 = beta2;
+    ep.eps = eps;
+    embed_core(d_indptr, d_indices, d_weights, n, nnz, dim, ep, d_emb, random_init != 0, seed,
+               c.stream, w, nullptr);
+  });
+}

@@ -105,6 +105,10 @@ _SIGS = {
     "evoc_hip_embed": (C.c_int, [i64p, i32p, f32p, C.c_int64, C.c_int32, C.c_int32, C.c_float,
                                  C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, vp,
                                  C.c_uint64, f32p]),
+    "evoc_hip_embed_noalloc": (C.c_int, [vp, vp, vp, C.c_int64, C.c_int64, C.c_int32,
+                                         C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float,
+                                         C.c_float, C.c_float, C.c_int32, C.c_uint64, C.c_int32,
+                                         vp]),
     "evoc_hip_label_prop_init": (C.c_int, [i64p, i32p, f32p, C.c_int64, C.c_int32, vp,
                                            C.c_int64, C.c_uint64, f32p]),
     "evoc_hip_core_sq": (C.c_int, [f32p, C.c_int64, C.c_int32, C.c_int32, f32p]),
@@ -372,6 +376,16 @@ def embedding(indptr, ind, w, dim, n_epochs=50, noise_level=0.5, negative_sample
                                 noise_level, negative_sample_rate, initial_alpha, beta1, beta2,
                                 eps, None if ia is None else ia.ctypes.data_as(vp), seed, out))
     return out
+
+
+def embedding_noalloc(d_indptr, d_indices, d_weights, n, nnz, dim, d_emb, n_epochs=50,
+                      noise_level=0.5, negative_sample_rate=1.0, initial_alpha=0.1, beta1=0.5,
+                      beta2=0.9, eps=1e-7, random_init=False, seed=42, device_id=0):
+    """evoc_hip_embed_noalloc: the epochs on a symmetric column-sorted CSR already in device
+    memory (integer addresses), d_emb updated in place; allocates no device memory."""
+    _check(lib().evoc_hip_embed_noalloc(d_indptr, d_indices, d_weights, n, nnz, dim, n_epochs,
+                                        noise_level, negative_sample_rate, initial_alpha, beta1,
+                                        beta2, eps, int(random_init), seed, device_id, d_emb))
 
 
 def label_prop_init(indptr, ind, w, dim, data=None, seed=42):

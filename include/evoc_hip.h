@@ -139,6 +139,22 @@ int evoc_hip_embed(const int64_t* indptr, const int32_t* indices, const float* w
                    float negative_sample_rate, float initial_alpha, float beta1, float beta2,
                    float eps, const float* init, uint64_t seed, float* out);
 
+/* Allocation-free variant of evoc_hip_embed, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops). No counterpart in the
+ * reference. Every pointer is device memory: the graph must be symmetric with each row
+ * sorted by column (the fuzzy set's CSR; its rows are then the embedding's slot lists),
+ * with finite weights; d_emb holds the initial layout (or is filled from the seed when
+ * random_init != 0) and receives the final one, n x dim row-major. The epochs' scratch is
+ * a static per-GPU device array: n <= 65536, nnz <= 2097152, dim <= 16 (larger sizes
+ * return EVOC_ERR_INVALID_ARGUMENT). Runs on the library's stream for device_id and
+ * returns after it finishes. Calls that overlap on one GPU share the scratch and race;
+ * the library's call lock serialises calls made through it. */
+int evoc_hip_embed_noalloc(const int64_t* d_indptr, const int32_t* d_indices,
+                           const float* d_weights, int64_t n, int64_t nnz, int32_t dim,
+                           int32_t n_epochs, float noise_level, float negative_sample_rate,
+                           float initial_alpha, float beta1, float beta2, float eps,
+                           int32_t random_init, uint64_t seed, int32_t device_id, float* d_emb);
+
 /* label_propagation_init (src/graph/label_prop.rs:537-564) on a column-sorted CSR.
  * data (n x d, row-major) may be NULL. */
 int evoc_hip_label_prop_init(const int64_t* indptr, const int32_t* indices,

@@ -163,6 +163,28 @@ extern "C" {
         out: *mut c_float,
     ) -> c_int;
 
+    /// Allocation-free `evoc_hip_embed` on device buffers (static per-GPU scratch:
+    /// n <= 65536, nnz <= 2097152, dim <= 16).
+    pub fn evoc_hip_embed_noalloc(
+        d_indptr: *const i64,
+        d_indices: *const i32,
+        d_weights: *const c_float,
+        n: i64,
+        nnz: i64,
+        dim: i32,
+        n_epochs: i32,
+        noise_level: c_float,
+        negative_sample_rate: c_float,
+        initial_alpha: c_float,
+        beta1: c_float,
+        beta2: c_float,
+        eps: c_float,
+        random_init: i32,
+        seed: u64,
+        device_id: i32,
+        d_emb: *mut c_float,
+    ) -> c_int;
+
     pub fn evoc_hip_label_prop_init(
         indptr: *const i64,
         indices: *const i32,

@@ -20,14 +20,22 @@ def hip():
 
 
 class DeviceArray:
-    """A float32 array copied into device memory; freed with the object."""
+    """A host array copied into device memory (any dtype); freed with the object."""
 
-    def __init__(self, a):
-        a = np.ascontiguousarray(a, np.float32)
-        self.shape = a.shape
+    def __init__(self, a, dtype=np.float32):
+        a = np.ascontiguousarray(a, dtype)
+        self.shape, self.dtype = a.shape, a.dtype
+        self.nbytes = a.nbytes
         self.ptr = C.c_void_p()
         assert hip().hipMalloc(C.byref(self.ptr), max(a.nbytes, 4)) == 0
-        assert hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+        if a.nbytes:
+            assert hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+
+    def to_host(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            assert hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2) == 0  # to host
+        return out
 
     def __del__(self):
         if self.ptr:
