@@ -4,6 +4,8 @@
 #include "evoc_oracle.h"
 
 #include <algorithm>
+#include <atomic>
+#include <memory>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -214,6 +216,166 @@ extern "C" int orc_knn_rows(const float* x, int64_t n, int64_t d, int metric, in
       dist_out[r * k + t] = t < kk ? best[t].first : INFINITY;
     }
   }
+  return 0;
+}
+
+// ---------------------------------------------------------------- NNDescent
+// Textbook NN-Descent (Dong, Charikar and Li, WWW 2011): random initial lists, then local
+// joins of each node's sampled new and old neighbours (forward and reverse, rho * K of
+// each) until fewer than delta * n * K list entries change. A stand-in for the reference's
+// default backend (ann-search-rs nndescent, not vendored), used only to time the CPU
+// baseline. Distances are the oracle's canonical ones, so a found neighbour carries the
+// exact search's distance bits.
+namespace {
+struct NndEntry { float d; int32_t j; uint8_t fresh; };
+struct SpinLock {
+  std::atomic<int> f{0};
+  void lock() { while (f.exchange(1, std::memory_order_acquire)) {} }
+  void unlock() { f.store(0, std::memory_order_release); }
+};
+}  // namespace
+
+extern "C" int orc_knn_nndescent(const float* x, int64_t n, int64_t d, int metric, int k,
+                                 uint64_t seed, int max_iters, float delta, float rho,
+                                 int32_t* idx_out, float* dist_out, int* iters_done) {
+  const int K = (int)std::min<int64_t>(k, n - 1);
+  if (K <= 0) {
+    for (int64_t q = 0; q < n * k; ++q) { idx_out[q] = -1; dist_out[q] = INFINITY; }
+    if (iters_done) *iters_done = 0;
+    return 0;
+  }
+  std::vector<float> nrm(n, 0.0f);
+  if (metric == 1) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) nrm[i] = std::sqrt(canon_dot(x + i * d, x + i * d, d));
+  }
+  auto dist = [&](int64_t a, int64_t b) {
+    return metric == 1 ? cos_dist(canon_dot(x + a * d, x + b * d, d), nrm[a], nrm[b])
+                       : std::sqrt(canon_sq(x + a * d, x + b * d, d));
+  };
+  std::vector<NndEntry> L(n * K);
+  std::unique_ptr<SpinLock[]> locks(new SpinLock[n]);
+  // insert (dd, j) into i's sorted list unless present or not better than the last
+  auto push = [&](int64_t i, float dd, int32_t j) -> int {
+    NndEntry* r = &L[i * K];
+    if (!(dd < r[K - 1].d || (dd == r[K - 1].d && j < r[K - 1].j))) return 0;
+    locks[i].lock();
+    int res = 0;
+    if (dd < r[K - 1].d || (dd == r[K - 1].d && j < r[K - 1].j)) {
+      bool dup = false;
+      for (int q = 0; q < K; ++q) dup |= r[q].j == j;
+      if (!dup) {
+        int q = K - 1;
+        while (q > 0 && (r[q - 1].d > dd || (r[q - 1].d == dd && r[q - 1].j > j))) {
+          r[q] = r[q - 1];
+          --q;
+        }
+        r[q] = {dd, j, 1};
+        res = 1;
+      }
+    }
+    locks[i].unlock();
+    return res;
+  };
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < n; ++i) {
+    for (int q = 0; q < K; ++q) L[i * K + q] = {INFINITY, INT32_MAX, 0};
+    int got = 0;
+    for (uint32_t t = 0; got < K; ++t) {
+      uint32_t r[4];
+      philox(seed, (uint32_t)i, (uint32_t)(i >> 32), t, 0x4E4E4449u, r);
+      const int64_t j = rng_range(r[0], (uint64_t)n);
+      if (j == i) continue;
+      bool dup = false;
+      for (int q = 0; q < K; ++q) dup |= L[i * K + q].j == (int32_t)j;
+      if (dup) continue;
+      NndEntry* row = &L[i * K];
+      const float dd = dist(i, j);
+      int q = K - 1;
+      while (q > 0 && (row[q - 1].d > dd || (row[q - 1].d == dd && row[q - 1].j > (int32_t)j))) {
+        row[q] = row[q - 1];
+        --q;
+      }
+      row[q] = {dd, (int32_t)j, 1};
+      ++got;
+    }
+  }
+  const int S = std::max(1, (int)(rho * (float)K));
+  std::vector<int32_t> nf(n * S), of(n * K);
+  std::vector<int32_t> nfc(n), ofc(n);
+  std::vector<int32_t> nr(n * S), orv(n * S);
+  std::vector<std::atomic<int32_t>> nrc(n), orc(n);
+  int it = 0;
+  for (; it < max_iters; ++it) {
+    // forward samples: up to S new entries per node (marked old), all old entries
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      int a = 0, b = 0;
+      for (int q = 0; q < K; ++q) {
+        NndEntry& e = L[i * K + q];
+        if (e.j == INT32_MAX) continue;
+        if (e.fresh && a < S) {
+          nf[i * S + a++] = e.j;
+          e.fresh = 0;
+        } else if (!e.fresh) {
+          of[i * K + b++] = e.j;
+        }
+      }
+      nfc[i] = a;
+      ofc[i] = b;
+      nrc[i].store(0, std::memory_order_relaxed);
+      orc[i].store(0, std::memory_order_relaxed);
+    }
+    // reverse samples (the first S arrivals of each kind)
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      for (int a = 0; a < nfc[i]; ++a) {
+        const int32_t j = nf[i * S + a];
+        const int32_t pos = nrc[j].fetch_add(1, std::memory_order_relaxed);
+        if (pos < S) nr[(int64_t)j * S + pos] = (int32_t)i;
+      }
+      for (int b = 0; b < std::min(ofc[i], S); ++b) {
+        const int32_t j = of[i * K + b];
+        const int32_t pos = orc[j].fetch_add(1, std::memory_order_relaxed);
+        if (pos < S) orv[(int64_t)j * S + pos] = (int32_t)i;
+      }
+    }
+    int64_t updates = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : updates)
+    for (int64_t i = 0; i < n; ++i) {
+      std::vector<int32_t> nw, od;
+      for (int a = 0; a < nfc[i]; ++a) nw.push_back(nf[i * S + a]);
+      for (int a = 0; a < std::min<int32_t>(nrc[i].load(), S); ++a) nw.push_back(nr[i * S + a]);
+      for (int b = 0; b < ofc[i]; ++b) od.push_back(of[i * K + b]);
+      for (int b = 0; b < std::min<int32_t>(orc[i].load(), S); ++b) od.push_back(orv[i * S + b]);
+      std::sort(nw.begin(), nw.end());
+      nw.erase(std::unique(nw.begin(), nw.end()), nw.end());
+      std::sort(od.begin(), od.end());
+      od.erase(std::unique(od.begin(), od.end()), od.end());
+      for (size_t a = 0; a < nw.size(); ++a) {
+        for (size_t b = a + 1; b < nw.size(); ++b) {
+          const float dd = dist(nw[a], nw[b]);
+          updates += push(nw[a], dd, nw[b]) + push(nw[b], dd, nw[a]);
+        }
+        for (size_t b = 0; b < od.size(); ++b) {
+          if (od[b] == nw[a]) continue;
+          const float dd = dist(nw[a], od[b]);
+          updates += push(nw[a], dd, od[b]) + push(od[b], dd, nw[a]);
+        }
+      }
+    }
+    if ((double)updates < (double)delta * (double)n * (double)K) {
+      ++it;
+      break;
+    }
+  }
+  for (int64_t i = 0; i < n; ++i)
+    for (int q = 0; q < k; ++q) {
+      const bool ok = q < K && L[i * K + q].j != INT32_MAX;
+      idx_out[i * k + q] = ok ? L[i * K + q].j : -1;
+      dist_out[i * k + q] = ok ? L[i * K + q].d : INFINITY;
+    }
+  if (iters_done) *iters_done = it;
   return 0;
 }
 
@@ -1215,6 +1377,247 @@ int64_t mst_impl(const T* e, int64_t n, int dim, int min_samples, int32_t* uo, i
   return n - 1;
 }
 
+// ---------------------------------------------------------------- reference complexity
+// The same core distances and the same (unique) minimum spanning tree in O(n log n) work
+// per round, in the shape of the reference's own algorithm: a kd-tree over the embedding
+// (src/clustering/kd_tree.rs:93-214), k-nearest-neighbour queries for the core distances
+// (kd_tree.rs:471-486) and Boruvka rounds of nearest-other-component queries with
+// component pruning (kd_tree.rs:272-419, src/clustering/mst.rs:107-165). Used by the CPU
+// baseline; the dense Prim above stays the parity reference, and the two must agree.
+template <typename T>
+struct KdTree {
+  int dim = 0;
+  int64_t n = 0;
+  std::vector<int64_t> perm;      // position -> point
+  std::vector<T> pts;             // coordinates by position
+  struct Node { int64_t lo, hi; int32_t left, right; };
+  std::vector<Node> nodes;
+  std::vector<T> bmin, bmax;      // node boxes
+  static constexpr int LEAF = 16;
+  void build(const T* e, int64_t n_, int dim_) {
+    n = n_;
+    dim = dim_;
+    perm.resize(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    nodes.clear();
+    bmin.clear();
+    bmax.clear();
+    std::vector<int32_t> stack{make(e, 0, n)};
+    while (!stack.empty()) {
+      const int32_t id = stack.back();
+      stack.pop_back();
+      const int64_t lo = nodes[id].lo, hi = nodes[id].hi;
+      if (hi - lo <= LEAF) continue;
+      int c = 0;
+      T wmax = -1;
+      for (int q = 0; q < dim; ++q) {
+        T w = bmax[id * dim + q] - bmin[id * dim + q];
+        if (w > wmax) { wmax = w; c = q; }
+      }
+      const int64_t mid = (lo + hi) / 2;
+      std::nth_element(perm.begin() + lo, perm.begin() + mid, perm.begin() + hi,
+                       [&](int64_t a, int64_t b) {
+                         const T va = e[a * dim + c], vb = e[b * dim + c];
+                         return va != vb ? va < vb : a < b;
+                       });
+      const int32_t l = make(e, lo, mid), r = make(e, mid, hi);
+      nodes[id].left = l;
+      nodes[id].right = r;
+      stack.push_back(l);
+      stack.push_back(r);
+    }
+    pts.resize(n * dim);
+    for (int64_t q = 0; q < n; ++q)
+      for (int c = 0; c < dim; ++c) pts[q * dim + c] = e[perm[q] * dim + c];
+  }
+  int32_t make(const T* e, int64_t lo, int64_t hi) {
+    const int32_t id = (int32_t)nodes.size();
+    nodes.push_back({lo, hi, -1, -1});
+    bmin.resize((size_t)(id + 1) * dim, std::numeric_limits<T>::infinity());
+    bmax.resize((size_t)(id + 1) * dim, -std::numeric_limits<T>::infinity());
+    for (int64_t q = lo; q < hi; ++q)
+      for (int c = 0; c < dim; ++c) {
+        const T v = e[perm[q] * dim + c];
+        bmin[id * dim + c] = std::min(bmin[id * dim + c], v);
+        bmax[id * dim + c] = std::max(bmax[id * dim + c], v);
+      }
+    return id;
+  }
+  // a lower bound of dsq_seq(x, y) for every y in the box: each term's difference rounds
+  // no larger than the point's own, and the sequential sum is monotone in every term
+  T box_lb(const T* x, int32_t id) const {
+    T s = T(0);
+    for (int c = 0; c < dim; ++c) {
+      T g = T(0);
+      if (x[c] < bmin[id * dim + c]) g = bmin[id * dim + c] - x[c];
+      else if (x[c] > bmax[id * dim + c]) g = x[c] - bmax[id * dim + c];
+      T p = g * g;
+      s = s + p;
+    }
+    return s;
+  }
+};
+
+template <typename T>
+void core_sq_kd(const T* e, int64_t n, int dim, int min_samples, T* core, const KdTree<T>& t,
+                std::vector<int32_t>* nbrs) {
+  const int64_t k = std::min<int64_t>(min_samples, n - 1);
+  if (nbrs) nbrs->assign(n * std::max<int64_t>(k, 1), -1);
+  if (k <= 0) {
+    for (int64_t i = 0; i < n; ++i) core[i] = T(0);
+    return;
+  }
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < n; ++i) {
+    const T* x = e + i * dim;
+    std::vector<std::pair<T, int64_t>> best;  // ascending (dsq, point)
+    best.reserve(k + 1);
+    std::vector<int32_t> st{0};
+    while (!st.empty()) {
+      const int32_t id = st.back();
+      st.pop_back();
+      if ((int64_t)best.size() == k && t.box_lb(x, id) > best.back().first) continue;
+      const auto& nd = t.nodes[id];
+      if (nd.left < 0) {
+        for (int64_t q = nd.lo; q < nd.hi; ++q) {
+          const int64_t j = t.perm[q];
+          if (j == i) continue;
+          const T dd = dsq_seq(x, &t.pts[q * dim], dim);
+          std::pair<T, int64_t> c(dd, j);
+          if ((int64_t)best.size() == k && !(c < best.back())) continue;
+          best.insert(std::upper_bound(best.begin(), best.end(), c), c);
+          if ((int64_t)best.size() > k) best.pop_back();
+        }
+        continue;
+      }
+      const T ll = t.box_lb(x, nd.left), lr = t.box_lb(x, nd.right);
+      if (ll <= lr) { st.push_back(nd.right); st.push_back(nd.left); }
+      else { st.push_back(nd.left); st.push_back(nd.right); }
+    }
+    core[i] = best.back().first;
+    if (nbrs)
+      for (int64_t q = 0; q < k; ++q) (*nbrs)[i * k + q] = (int32_t)best[q].second;
+  }
+}
+
+template <typename T>
+int64_t mst_boruvka_impl(const T* e, int64_t n, int dim, int min_samples, int32_t* uo,
+                         int32_t* vo, T* wo) {
+  if (n <= 1) return 0;
+  KdTree<T> t;
+  t.build(e, n, dim);
+  std::vector<T> core(n);
+  std::vector<int32_t> nb;
+  core_sq_kd(e, n, dim, min_samples, core.data(), t, &nb);
+  const int64_t kn = std::max<int64_t>(std::min<int64_t>(min_samples, n - 1), 1);
+  const int32_t nn = (int32_t)t.nodes.size();
+  // smallest core distance in each node (children after parents in `nodes`)
+  std::vector<T> ncore(nn);
+  for (int32_t id = nn - 1; id >= 0; --id) {
+    const auto& nd = t.nodes[id];
+    if (nd.left < 0) {
+      T m = std::numeric_limits<T>::infinity();
+      for (int64_t q = nd.lo; q < nd.hi; ++q) m = std::min(m, core[t.perm[q]]);
+      ncore[id] = m;
+    } else {
+      ncore[id] = std::min(ncore[nd.left], ncore[nd.right]);
+    }
+  }
+  std::vector<int64_t> parent(n);
+  std::iota(parent.begin(), parent.end(), 0);
+  auto find = [&](int64_t a) {
+    while (parent[a] != a) { parent[a] = parent[parent[a]]; a = parent[a]; }
+    return a;
+  };
+  std::vector<int64_t> comp(n), ncomp(nn);
+  std::vector<EKey<T>> edges;
+  edges.reserve(n - 1);
+  const T inf = std::numeric_limits<T>::infinity();
+  const EKey<T> none{inf, INT32_MAX, INT32_MAX};
+  while ((int64_t)edges.size() < n - 1) {
+    for (int64_t i = 0; i < n; ++i) comp[i] = find(i);
+    for (int32_t id = nn - 1; id >= 0; --id) {
+      const auto& nd = t.nodes[id];
+      if (nd.left < 0) {
+        int64_t c = comp[t.perm[nd.lo]];
+        for (int64_t q = nd.lo + 1; q < nd.hi && c >= 0; ++q)
+          if (comp[t.perm[q]] != c) c = -1;
+        ncomp[id] = c;
+      } else {
+        ncomp[id] = ncomp[nd.left] == ncomp[nd.right] ? ncomp[nd.left] : -1;
+      }
+    }
+    std::vector<EKey<T>> best(n, none);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t ci = comp[i];
+      const T* x = e + i * dim;
+      EKey<T> b = none;
+      // the core-distance neighbours give a first bound
+      for (int64_t q = 0; q < kn; ++q) {
+        const int64_t j = nb[i * kn + q];
+        if (j < 0 || comp[j] == ci) continue;
+        const T mr = std::max(std::max(dsq_seq(x, e + j * dim, dim), core[i]), core[j]);
+        EKey<T> key{mr, (int32_t)std::min(i, j), (int32_t)std::max(i, j)};
+        if (key < b) b = key;
+      }
+      std::vector<int32_t> st{0};
+      while (!st.empty()) {
+        const int32_t id = st.back();
+        st.pop_back();
+        if (ncomp[id] == ci) continue;
+        const T lb = std::max(std::max(t.box_lb(x, id), core[i]), ncore[id]);
+        if (lb > b.w) continue;
+        const auto& nd = t.nodes[id];
+        if (nd.left < 0) {
+          for (int64_t q = nd.lo; q < nd.hi; ++q) {
+            const int64_t j = t.perm[q];
+            if (comp[j] == ci) continue;
+            const T mr = std::max(std::max(dsq_seq(x, &t.pts[q * dim], dim), core[i]), core[j]);
+            EKey<T> key{mr, (int32_t)std::min(i, j), (int32_t)std::max(i, j)};
+            if (key < b) b = key;
+          }
+          continue;
+        }
+        const T ll = t.box_lb(x, nd.left), lr = t.box_lb(x, nd.right);
+        if (ll <= lr) { st.push_back(nd.right); st.push_back(nd.left); }
+        else { st.push_back(nd.left); st.push_back(nd.right); }
+      }
+      best[i] = b;
+    }
+    std::vector<EKey<T>> cbest(n, none);
+    for (int64_t i = 0; i < n; ++i)
+      if (best[i] < cbest[comp[i]]) cbest[comp[i]] = best[i];
+    int64_t added = 0;
+    for (int64_t c = 0; c < n; ++c) {
+      if (comp[c] != c || cbest[c].a == INT32_MAX) continue;
+      const int64_t ra = find(cbest[c].a), rb = find(cbest[c].b);
+      if (ra == rb) continue;  // the other component chose the same edge
+      parent[ra] = rb;
+      edges.push_back(cbest[c]);
+      ++added;
+    }
+    if (added == 0) break;  // disconnected (not for finite input)
+  }
+  std::sort(edges.begin(), edges.end());
+  const int64_t ne = (int64_t)edges.size();
+  for (int64_t i = 0; i < ne; ++i) {
+    uo[i] = edges[i].a;
+    vo[i] = edges[i].b;
+    wo[i] = std::sqrt(edges[i].w);
+  }
+  return ne;
+}
+
+// which MST the layer extraction uses: 0 the dense Prim (parity reference), 1 the kd-tree
+// Boruvka (same tree, reference complexity)
+int g_mst_algo = 0;
+template <typename T>
+int64_t mst_any(const T* e, int64_t n, int dim, int ms, int32_t* u, int32_t* v, T* w) {
+  return g_mst_algo == 1 ? mst_boruvka_impl<T>(e, n, dim, ms, u, v, w)
+                         : mst_impl<T>(e, n, dim, ms, u, v, w);
+}
+
 // mst_to_linkage_tree: src/clustering/linkage.rs:323-367. The stable sort by weight is
 // made total with (w, min(u,v), max(u,v)); the parent walk is a union-find whose
 // roots carry the current cluster label (same result, no O(n^2) walk).
@@ -1584,7 +1987,7 @@ int64_t layers_impl(const T* emb, int64_t n, int dim, int min_samples, int64_t b
   if (n == 0) return 0;
   std::vector<int32_t> u(n), v(n);
   std::vector<T> w(n);
-  int64_t ne = mst_impl<T>(emb, n, dim, min_samples, u.data(), v.data(), w.data());
+  int64_t ne = mst_any<T>(emb, n, dim, min_samples, u.data(), v.data(), w.data());
   std::vector<int64_t> L(ne), R(ne), S(ne);
   std::vector<T> D(ne);
   linkage_impl<T>(u.data(), v.data(), w.data(), ne, n, L.data(), R.data(), D.data(), S.data());
@@ -1637,7 +2040,7 @@ int search_impl(const T* emb, int64_t n, int dim, int min_samples, int64_t targe
   if (n == 0) return 0;
   std::vector<int32_t> u(n), v(n);
   std::vector<T> w(n);
-  int64_t ne = mst_impl<T>(emb, n, dim, min_samples, u.data(), v.data(), w.data());
+  int64_t ne = mst_any<T>(emb, n, dim, min_samples, u.data(), v.data(), w.data());
   std::vector<int64_t> L(ne), R(ne), S(ne);
   std::vector<T> D(ne);
   linkage_impl<T>(u.data(), v.data(), w.data(), ne, n, L.data(), R.data(), D.data(), S.data());
@@ -1698,6 +2101,23 @@ extern "C" int64_t orc_mst_f64(const double* emb, int64_t n, int dim, int ms, in
                                int32_t* v, double* w) {
   return mst_impl<double>(emb, n, dim, ms, u, v, w);
 }
+extern "C" int64_t orc_mst_boruvka(const float* emb, int64_t n, int dim, int ms, int32_t* u,
+                                   int32_t* v, float* w) {
+  return mst_boruvka_impl<float>(emb, n, dim, ms, u, v, w);
+}
+extern "C" int orc_core_sq_kd(const float* emb, int64_t n, int dim, int ms, float* core) {
+  if (n == 0) return 0;
+  KdTree<float> t;
+  t.build(emb, n, dim);
+  core_sq_kd<float>(emb, n, dim, ms, core, t, nullptr);
+  return 0;
+}
+extern "C" int orc_set_mst_algo(int algo) {
+  const int old = g_mst_algo;
+  g_mst_algo = algo;
+  return old;
+}
+
 extern "C" int orc_linkage(int32_t* u, int32_t* v, float* w, int64_t ne, int64_t n,
                            int64_t* left, int64_t* right, float* dist, int64_t* size) {
   return linkage_impl<float>(u, v, w, ne, n, left, right, dist, size);

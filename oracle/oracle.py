@@ -106,6 +106,14 @@ def lib():
         L.orc_num_threads.restype = C.c_int
         L.orc_symmetrise.argtypes = [i64p, i32p, f32p, C.c_int64, C.c_int64, i64p, i32p, f32p]
         L.orc_symmetrise.restype = C.c_int64
+        L.orc_mst_boruvka.argtypes = [f32p, C.c_int64, C.c_int, C.c_int, i32p, i32p, f32p]
+        L.orc_mst_boruvka.restype = C.c_int64
+        L.orc_core_sq_kd.argtypes = [f32p, C.c_int64, C.c_int, C.c_int, f32p]
+        L.orc_set_mst_algo.argtypes = [C.c_int]
+        L.orc_set_mst_algo.restype = C.c_int
+        L.orc_knn_nndescent.argtypes = [f32p, C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_uint64,
+                                        C.c_int, C.c_float, C.c_float, i32p, f32p,
+                                        C.POINTER(C.c_int)]
         csr3 = [i64p, i64p, f32p, C.c_int64, C.c_int64]
         L.orc_csr_op.argtypes = [C.c_int] + csr3 + csr3 + [i64p, i64p, f32p, C.c_int64, i64p]
         L.orc_csr_op.restype = C.c_int64
@@ -540,3 +548,41 @@ def lp_expand(g, part, n_parts, layout, scaling=0.1):
     lib().orc_lp_expand(g.indptr, g.indices, g.data, g.nrows, np.ascontiguousarray(part, np.int64),
                         n_parts, layout, dim, scaling, out)
     return out
+
+
+# ---------------------------------------------------------------- reference-complexity baselines
+def mst_boruvka(emb, min_samples):
+    """kd-tree Boruvka: the same unique MST as mst() (canonical tie order), O(n log n) rounds."""
+    emb = _f32(emb)
+    n, dim = emb.shape
+    m = max(n - 1, 1)
+    u = np.empty(m, np.int32)
+    v = np.empty(m, np.int32)
+    w = np.empty(m, np.float32)
+    ne = lib().orc_mst_boruvka(emb, n, dim, min_samples, u, v, w)
+    return u[:ne].copy(), v[:ne].copy(), w[:ne].copy()
+
+
+def core_sq_kd(emb, min_samples):
+    emb = _f32(emb)
+    n, dim = emb.shape
+    out = np.empty(n, np.float32)
+    lib().orc_core_sq_kd(emb, n, dim, min_samples, out)
+    return out
+
+
+def set_mst_algo(algo):
+    """0: dense Prim (the parity reference), 1: kd-tree Boruvka. Returns the previous."""
+    return lib().orc_set_mst_algo(int(algo))
+
+
+def knn_nndescent(x, k, metric=0, seed=42, max_iters=12, delta=0.001, rho=1.0):
+    """Textbook NN-Descent; returns (idx, dist, iterations)."""
+    x = _f32(x)
+    n, d = x.shape
+    idx = np.empty((n, k), np.int32)
+    dist = np.empty((n, k), np.float32)
+    it = C.c_int(0)
+    lib().orc_knn_nndescent(x, n, d, metric, k, seed, max_iters, delta, rho, idx, dist,
+                            C.byref(it))
+    return idx, dist, it.value
