@@ -151,12 +151,14 @@ int evoc_hip_knn(const float* x, int64_t n, int64_t d, int32_t metric, int32_t k
   });
 }
 
-int evoc_hip_knn_last_stats(double* kernel_flop, double* kernel_ms, int32_t* symmetric) {
+int evoc_hip_knn_last_stats(double* kernel_flop, double* kernel_ms, int32_t* symmetric,
+                            int64_t* escalated_rows) {
   return guarded([&] {
     const KnnStats& st = last_knn_stats();
     if (kernel_flop) *kernel_flop = st.flops;
     if (kernel_ms) *kernel_ms = st.mfma_ms;
     if (symmetric) *symmetric = st.symmetric;
+    if (escalated_rows) *escalated_rows = st.escalated_rows;
   });
 }
 

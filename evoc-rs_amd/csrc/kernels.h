@@ -104,6 +104,7 @@ struct KnnStats {
   float mfma_ms = 0.0f;
   double flops = 0.0;
   int symmetric = 0;  // the symmetric sweep ran
+  int escalated_rows = 0;  // rows searched again with 64 candidates after a failed certificate
 };
 // the calling thread's last knn_device call
 KnnStats& last_knn_stats();

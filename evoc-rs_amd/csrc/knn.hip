@@ -134,10 +134,14 @@ __device__ __forceinline__ void ring_barrier() {
 }
 
 template <bool SPLIT, int C>  // C = list length per column half
+// Query rows are the rows of Xh / Xl themselves (self-join), or, with qmap, the rows of
+// Qh / Ql: nq gathered rows whose global indices qmap holds in ascending order (the
+// re-search of rows whose certificate failed, with longer lists).
 __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
     const uint16_t* __restrict__ Xh, const uint16_t* __restrict__ Xl, int64_t n, int64_t npad,
     int dpad, const float* __restrict__ colscale, int metric, int32_t* __restrict__ cand_idx,
-    float* __restrict__ cand_key) {
+    float* __restrict__ cand_key, const uint16_t* __restrict__ Qh,
+    const uint16_t* __restrict__ Ql, const int32_t* __restrict__ qmap, int64_t nq) {
   constexpr int NOPS = SPLIT ? 4 : 2;           // A hi, B hi, (A lo, B lo)
   constexpr int NST = SPLIT ? 2 : 4;            // stages in the LDS ring
   constexpr int OPB = BM * 4 * 16;              // bytes per staged operand chunk (8 KB)
@@ -155,6 +159,14 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
   const int wm = wave >> 1, wn = wave & 1;
   const int64_t rb = blockIdx.x;
   const int64_t row0 = rb * BM;
+  // global index of each query row of the block (-1 past the last one), for the self mask
+  __shared__ int32_t qg[BM];
+  const uint16_t* const QH = qmap ? Qh : Xh;
+  const uint16_t* const QL = qmap ? Ql : Xl;
+  const int64_t nrows = qmap ? nq : n;
+  if (tid < BM) qg[tid] = row0 + tid < nrows ? (qmap ? qmap[row0 + tid] : (int32_t)(row0 + tid)) : -1;
+  const int64_t qlo = qmap ? qmap[row0] : row0;
+  const int64_t qhi = qmap ? qmap[std::min<int64_t>(row0 + BM, nq) - 1] : row0 + BM - 1;
   const int nck = dpad / KC;
   const int64_t ntile = npad / BN;
   const int64_t total = ntile * nck;
@@ -177,7 +189,7 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
     char* st = ring + (int)(g % NST) * STB;
 #pragma unroll
     for (int op = 0; op < NOPS; ++op) {
-      const uint16_t* src = (op < 2) ? Xh : Xl;
+      const uint16_t* src = (op & 1) ? ((op < 2) ? Xh : Xl) : ((op < 2) ? QH : QL);
       const int64_t rbase = (op & 1) ? ct * BN : row0;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -257,7 +269,7 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
 
     // ---------------- epilogue of column tile ct: filter keys against the row thresholds
     const int par = (int)(ct & 1);
-    const bool diag = ct == rb;
+    const bool diag = ct * BN <= qhi && ct * BN + BN > qlo;
     const bool tail = (ct + 1) * BN > n;
 #define EVOC_KEY(TM, TN, J, I, OUT)                                                    \
   do {                                                                                   \
@@ -266,7 +278,7 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
     if (diag || tail) {                                                                  \
       const int rl_ = wm * 64 + (TM) * 32 + (J) * 8 + (lane >> 5) * 4 + (I);             \
       const int64_t gc_ = ct * BN + wn * 64 + (TN) * 32 + (lane & 31);                   \
-      if (gc_ >= n || gc_ == row0 + rl_) k_ = FINF;                                      \
+      if (gc_ >= n || gc_ == qg[rl_]) k_ = FINF;                                         \
     }                                                                                    \
     OUT = k_;                                                                            \
   } while (0)
@@ -371,7 +383,7 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
   }
   // each (row, column-half) list is written as is: candidates per row = 2*C
   const int64_t grow = row0 + orow;
-  if (grow < n) {
+  if (grow < nrows) {
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       cand_key[(grow * 2 + ohalf) * C + q] = lk[q];
@@ -1393,10 +1405,13 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     const unsigned* __restrict__ flags, const int32_t* __restrict__ cand_idx,
     const float* __restrict__ cand_key, int C, int halves, const float* __restrict__ row_bound,
     int k, int metric, float gamma, int32_t* __restrict__ out_idx, float* __restrict__ out_dist,
-    int* __restrict__ fail_count, int32_t* __restrict__ fail_rows) {
+    int* __restrict__ fail_count, int32_t* __restrict__ fail_rows,
+    const int32_t* __restrict__ rowmap, int64_t nrows) {
+  // the lists of list row `lr` belong to data row `row` (rowmap, or the same row)
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t row = (int64_t)blockIdx.x * 4 + wave;
-  if (row >= n) return;
+  const int64_t lr = (int64_t)blockIdx.x * 4 + wave;
+  if (lr >= (rowmap ? nrows : n)) return;
+  const int64_t row = rowmap ? rowmap[lr] : lr;
   const int nt = (d + 63) / 64;
   float q[MAXT];
 #pragma unroll
@@ -1417,8 +1432,8 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     const float nmax = __uint_as_float(flags[1]);
     E = 1.5f * (dt * u * (sqn[row] + nmax * nmax) + 2.0f * (gamma + 2.0f * u) * nrm[row] * nmax);
   }
-  const int cj = lane < C ? cand_idx[row * C + lane] : -1;
-  const float ckey = cj >= 0 ? cand_key[row * C + lane] : FINF;
+  const int cj = lane < C ? cand_idx[lr * C + lane] : -1;
+  const float ckey = cj >= 0 ? cand_key[lr * C + lane] : FINF;
   const bool valid = cj >= 0;
   const int nvalid = __popcll(__ballot(valid));
   int krank = 0;
@@ -1537,9 +1552,9 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
     const float dk2 = __shfl(my_sq, src, 64);
     // one sorted list of C keys, or two column halves that each kept their C/2 best, or
     // (symmetric sweep) the bound its merge computed
-    const float kc = row_bound ? row_bound[row]
-                     : halves == 2 ? fminf(cand_key[row * C + (C / 2 - 1)], cand_key[row * C + (C - 1)])
-                                   : cand_key[row * C + (C - 1)];
+    const float kc = row_bound ? row_bound[lr]
+                     : halves == 2 ? fminf(cand_key[lr * C + (C / 2 - 1)], cand_key[lr * C + (C - 1)])
+                                   : cand_key[lr * C + (C - 1)];
     if (metric == 1) {
       const float lb = 1.0f + kc * colscale[row];
       pass = (lb - eps) > dk;
@@ -1713,6 +1728,31 @@ static int64_t cu_count() {
   return cus > 0 ? cus : 1;
 }
 
+// Rows whose certificate failed, gathered (16-byte pieces; rows past nq zero) for the
+// re-search with longer lists.
+__global__ void knn_gather_rows_kernel(const uint16_t* __restrict__ Xh,
+                                       const uint16_t* __restrict__ Xl,
+                                       const int32_t* __restrict__ qmap, int64_t nq,
+                                       int64_t nqpad, int dpad, uint16_t* __restrict__ Qh,
+                                       uint16_t* __restrict__ Ql) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = dpad / 8;
+  if (t >= nqpad * per) return;
+  const int64_t r = t / per;
+  const int c = (int)(t - r * per);
+  int4 h = make_int4(0, 0, 0, 0), l = make_int4(0, 0, 0, 0);
+  if (r < nq) {
+    const int64_t src = qmap[r];
+    h = reinterpret_cast<const int4*>(Xh + src * dpad)[c];
+    if (Xl) l = reinterpret_cast<const int4*>(Xl + src * dpad)[c];
+  }
+  reinterpret_cast<int4*>(Qh + r * dpad)[c] = h;
+  if (Ql) reinterpret_cast<int4*>(Ql + r * dpad)[c] = l;
+}
+// at least this many failed certificates: re-search them with 64 candidates per row
+// (fewer go straight to the exact scan, which is cheaper for a handful of rows)
+constexpr int KNN_ESCALATE_MIN = 1024;
+
 void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k, int precision,
                 int32_t* d_idx, float* d_dist, KnnStats* stats, hipStream_t s, DevPool& pool) {
   EVOC_CHECK(n >= 1 && d >= 1, "knn: empty input");
@@ -1720,14 +1760,16 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   EVOC_CHECK(d <= MAXT * 64, "knn: d must be <= 1024");
   static bool attr_set = false;
   if (!attr_set) {
+    // the streaming kernel: at most 134 KB of ring, key tile and thresholds (beside its
+    // 512-byte static array of query indices)
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<true, 16>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<true, 32>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<false, 16>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<false, 32>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 1, 128, false>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<128>::SMEM));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 1, 128, true>,
@@ -1907,17 +1949,21 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     if (split) {
       if (CH == 16)
         knn_topc_kernel<true, 16><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
-                                                          metric, cidx.p, ckey.p);
+                                                          metric, cidx.p, ckey.p, nullptr,
+                                                          nullptr, nullptr, 0);
       else
         knn_topc_kernel<true, 32><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
-                                                          metric, cidx.p, ckey.p);
+                                                          metric, cidx.p, ckey.p, nullptr,
+                                                          nullptr, nullptr, 0);
     } else {
       if (CH == 16)
         knn_topc_kernel<false, 16><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad,
-                                                           colscale.p, metric, cidx.p, ckey.p);
+                                                           colscale.p, metric, cidx.p, ckey.p,
+                                                           nullptr, nullptr, nullptr, 0);
       else
         knn_topc_kernel<false, 32><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad,
-                                                           colscale.p, metric, cidx.p, ckey.p);
+                                                           colscale.p, metric, cidx.p, ckey.p,
+                                                           nullptr, nullptr, nullptr, 0);
     }
   }
   EVOC_HIP(hipGetLastError());
@@ -1969,11 +2015,55 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   knn_rerank_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(
       dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, sym ? midx.p : cidx.p,
       sym ? mkey.p : ckey.p, C, (ra && dpad < 512) ? 1 : 2, sym ? bnd.p : nullptr, k, metric,
-      gamma, d_idx, d_dist, fcount.p, frows.p);
+      gamma, d_idx, d_dist, fcount.p, frows.p, nullptr, 0);
   EVOC_HIP(hipGetLastError());
   int nfail = 0;
   EVOC_HIP(hipMemcpyAsync(&nfail, fcount.p, sizeof(int), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
+  int escalated = 0;
+  if (nfail >= KNN_ESCALATE_MIN && C < 64) {
+    // Many certificates failed (neighbour distances concentrated, as in a dense leaf of a
+    // hierarchical mixture): search those rows again, gathered, with 32 candidates per
+    // column half, and re-rank them; only rows that fail again take the exact scan.
+    escalated = nfail;
+    std::vector<int32_t> hf(nfail);
+    EVOC_HIP(hipMemcpyAsync(hf.data(), frows.p, nfail * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipStreamSynchronize(s));
+    std::sort(hf.begin(), hf.end());
+    const int64_t nq = nfail, nqpad = ceil_div(nq, BM) * BM;
+    Buf<int32_t> qmap(pool, nq), ecidx(pool, nq * 64), frows2(pool, nq);
+    Buf<float> eckey(pool, nq * 64);
+    Buf<uint16_t> Qh(pool, nqpad * dpad), Ql;
+    if (split) Ql.alloc(pool, nqpad * dpad);
+    EVOC_HIP(hipMemcpyAsync(qmap.p, hf.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    const int64_t pieces = nqpad * (dpad / 8);
+    knn_gather_rows_kernel<<<(unsigned)ceil_div(pieces, 256), 256, 0, s>>>(
+        Xh.p, split ? Xl.p : nullptr, qmap.p, nq, nqpad, dpad, Qh.p, split ? Ql.p : nullptr);
+    const int nops = split ? 4 : 2;
+    const int nst = split ? 2 : 4;
+    const size_t smem = (size_t)nst * nops * BM * 4 * 16 + BM * SCS * sizeof(float) +
+                        2 * BM * sizeof(float) + 4 * sizeof(int);
+    const unsigned grid = (unsigned)(nqpad / BM);
+    if (split)
+      knn_topc_kernel<true, 32><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
+                                                        metric, ecidx.p, eckey.p, Qh.p, Ql.p,
+                                                        qmap.p, nq);
+    else
+      knn_topc_kernel<false, 32><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad,
+                                                         colscale.p, metric, ecidx.p, eckey.p,
+                                                         Qh.p, nullptr, qmap.p, nq);
+    EVOC_HIP(hipGetLastError());
+    EVOC_HIP(hipMemsetAsync(fcount.p, 0, sizeof(int), s));
+    knn_rerank_kernel<<<(unsigned)ceil_div(nq, 4), 256, 0, s>>>(
+        dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, ecidx.p, eckey.p, 64, 2, nullptr, k,
+        metric, gamma, d_idx, d_dist, fcount.p, frows2.p, qmap.p, nq);
+    EVOC_HIP(hipGetLastError());
+    EVOC_HIP(hipMemcpyAsync(&nfail, fcount.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipStreamSynchronize(s));
+    if (nfail > 0)
+      EVOC_HIP(hipMemcpyAsync(frows.p, frows2.p, nfail * sizeof(int32_t),
+                              hipMemcpyDeviceToDevice, s));
+  }
   if (nfail > 0) {
     // the columns are split over `parts` workgroups of four wavefronts, EX_ROWS failing
     // rows per launch; the per-wavefront lists of each row are then merged
@@ -1999,6 +2089,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   KnnStats st;
   st.split = split ? 1 : 0;
   st.fallback_rows = nfail;
+  st.escalated_rows = escalated;
   st.candidates = C;
   st.mfma_ms = mfma_ms;
   double pairs = (double)npad * (double)npad;

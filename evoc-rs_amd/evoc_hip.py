@@ -99,7 +99,7 @@ _SIGS = {
     "evoc_hip_knn": (C.c_int, [f32p, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                i32p, f32p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "evoc_hip_knn_last_stats": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                          C.POINTER(C.c_int32)]),
+                                          C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
     "evoc_hip_fuzzy": (C.c_int, [i32p, f32p, C.c_int64, C.c_int32, C.c_float, C.c_int32, f32p,
                                  f32p, i64p, i32p, f32p, C.POINTER(C.c_int64)]),
     "evoc_hip_embed": (C.c_int, [i64p, i32p, f32p, C.c_int64, C.c_int32, C.c_int32, C.c_float,
@@ -338,15 +338,18 @@ def knn(x, k, metric="euclidean", precision="auto", return_stats=False):
     if return_stats:
         st = knn_last_stats()
         return idx, dist, dict(fallback_rows=fb.value, kernel_ms=ms.value,
-                               kernel_flop=st["kernel_flop"], symmetric=st["symmetric"])
+                               kernel_flop=st["kernel_flop"], symmetric=st["symmetric"],
+                               escalated_rows=st["escalated_rows"])
     return idx, dist
 
 
 def knn_last_stats():
-    """The calling thread's last kNN search: tile-kernel flop and time, symmetric sweep."""
-    fl, ms, sym = C.c_double(0), C.c_double(0), C.c_int32(0)
-    _check(lib().evoc_hip_knn_last_stats(C.byref(fl), C.byref(ms), C.byref(sym)))
-    return dict(kernel_flop=fl.value, kernel_ms=ms.value, symmetric=bool(sym.value))
+    """The calling thread's last kNN search: tile-kernel flop and time, symmetric sweep,
+    rows searched again with 64 candidates after a failed certificate."""
+    fl, ms, sym, esc = C.c_double(0), C.c_double(0), C.c_int32(0), C.c_int64(0)
+    _check(lib().evoc_hip_knn_last_stats(C.byref(fl), C.byref(ms), C.byref(sym), C.byref(esc)))
+    return dict(kernel_flop=fl.value, kernel_ms=ms.value, symmetric=bool(sym.value),
+                escalated_rows=esc.value)
 
 
 def fuzzy_graph(idx, dist, effective_k=None, symmetrise=True):

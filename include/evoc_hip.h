@@ -121,10 +121,13 @@ int evoc_hip_knn(const float* x, int64_t n, int64_t d, int32_t metric, int32_t k
 
 /* Diagnostics of the calling thread's last kNN search (evoc_hip_knn or the kNN stage of
  * evoc_hip_run / evoc_hip_run_device): the MFMA flop the tile kernel executed, its
- * hipEvent time in ms, and whether the symmetric sweep (each pair of 256-row blocks once,
- * transposed keys queued for the other block) ran. No counterpart in the reference: it
- * serves roofline accounting and tests. */
-int evoc_hip_knn_last_stats(double* kernel_flop, double* kernel_ms, int32_t* symmetric);
+ * hipEvent time in ms, whether the symmetric sweep (each pair of 256-row blocks once,
+ * transposed keys queued for the other block) ran, and how many rows whose certificate
+ * failed were searched again with 64 candidates (escalated) before the exact scan took
+ * the rest. No counterpart in the reference: it serves roofline accounting and tests.
+ * Any pointer may be NULL. */
+int evoc_hip_knn_last_stats(double* kernel_flop, double* kernel_ms, int32_t* symmetric,
+                            int64_t* escalated_rows);
 
 /* build_fuzzy_simplicial_set (src/graph/fuzzy_graph.rs:275-295) as CSR. indices and
  * weights need room for 2*n*k entries; *nnz receives the stored count. */

@@ -128,6 +128,7 @@ extern "C" {
         kernel_flop: *mut c_double,
         kernel_ms: *mut c_double,
         symmetric: *mut i32,
+        escalated_rows: *mut i64,
     ) -> c_int;
 
     pub fn evoc_hip_fuzzy(

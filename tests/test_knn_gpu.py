@@ -144,3 +144,21 @@ def test_knn_symmetric_equals_plain_at_300k(monkeypatch):
     assert ss["kernel_flop"] < 0.51 * ps["kernel_flop"]
     assert np.array_equal(pi, si)
     assert np.array_equal(pd.view(np.uint32), sd.view(np.uint32))
+
+
+def test_knn_escalation_of_concentrated_rows():
+    """Leaves of a hierarchical mixture whose neighbour distances concentrate: the 32
+    candidates of the first pass fail many certificates, those rows are searched again with
+    64 candidates and re-ranked, and only the rest take the exact scan. Bit-exact either
+    way; d = 768 takes the streaming kernel, d = 512 the resident-query one."""
+    rng = np.random.default_rng(31)
+    for d in (768, 512):
+        par = rng.normal(size=(6, d))
+        child = np.repeat(par, 5, 0) + 0.3 * rng.normal(size=(30, d))
+        lab = np.arange(12000) % 30
+        x = child[lab] + 0.1 * rng.normal(size=(12000, d))
+        x = datagen.to_bf16_values((x / np.linalg.norm(x, axis=1, keepdims=True)).astype(np.float32))
+        st = _check(x, 15, "cosine")
+        esc = H.knn_last_stats()["escalated_rows"]
+        print(d, "escalated", esc, "exact", st["fallback_rows"])
+        assert esc >= 1024 and st["fallback_rows"] < esc
