@@ -108,6 +108,8 @@ inline float cos_dist(float dot, float na, float nb) {
 // reference calls f32::exp (libm expf, within ~0.5 ulp of this); the correctly rounded
 // value is the platform-independent choice both sides share.
 inline float cexp(float x) { return (float)std::exp((double)x); }
+// fp64 instantiations of the fuzzy set (error statements against fp64) use exp itself
+inline double cexp(double x) { return std::exp(x); }
 
 // Low-dimensional squared distance (embedding space): sequential sum.
 template <typename T>
@@ -380,30 +382,33 @@ extern "C" int orc_knn_nndescent(const float* x, int64_t n, int64_t d, int metri
 }
 
 // ================================================================ A2-A5: fuzzy set
-// smooth_knn_dist: src/graph/fuzzy_graph.rs:33-98 (local_connectivity = 1, n_iter = 64)
-static void smooth_knn_dist(const float* dists, int64_t n, int k, size_t k_target,
-                            float* sigma, float* rho) {
+// smooth_knn_dist: src/graph/fuzzy_graph.rs:33-98 (local_connectivity = 1, n_iter = 64).
+// T = float is the reference's arithmetic; T = double is the fp64 restatement the error
+// statements compare against.
+template <typename T>
+static void smooth_knn_dist(const T* dists, int64_t n, int k, size_t k_target, T* sigma,
+                            T* rho) {
   const double target = std::log2((double)k_target);
   const double tol = 1e-5;
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; ++i) {
-    const float* d = dists + i * k;
-    float r = d[0];  // local_connectivity = 1: index 0, no interpolation
-    float lo = 0.0f, hi = std::numeric_limits<float>::max(), mid = 1.0f;
+    const T* d = dists + i * k;
+    T r = d[0];  // local_connectivity = 1: index 0, no interpolation
+    T lo = T(0), hi = std::numeric_limits<T>::max(), mid = T(1);
     for (int it = 0; it < 64; ++it) {
-      float val = 0.0f;
+      T val = T(0);
       for (int t = 0; t < k; ++t) {
-        float adjusted = std::fmax(d[t] - r, 0.0f);
+        T adjusted = std::fmax(d[t] - r, T(0));
         val += cexp(-(adjusted / mid));
       }
       if (std::fabs((double)val - target) < tol) break;
       if ((double)val > target) {
         hi = mid;
-        mid = (lo + hi) / 2.0f;
+        mid = (lo + hi) / T(2);
       } else {
         lo = mid;
-        if (hi == std::numeric_limits<float>::max()) mid = mid * 2.0f;
-        else mid = (lo + hi) / 2.0f;
+        if (hi == std::numeric_limits<T>::max()) mid = mid * T(2);
+        else mid = (lo + hi) / T(2);
       }
     }
     sigma[i] = mid;
@@ -411,35 +416,36 @@ static void smooth_knn_dist(const float* dists, int64_t n, int k, size_t k_targe
   }
 }
 
+template <typename T>
 static int64_t symmetrise_coo(const std::vector<int64_t>& rr, const std::vector<int32_t>& cc,
-                              const std::vector<float>& ww, int64_t n, int64_t* indptr,
-                              int32_t* indices, float* weights);
+                              const std::vector<T>& ww, int64_t n, int64_t* indptr,
+                              int32_t* indices, T* weights);
 
 // build_fuzzy_simplicial_set: src/graph/fuzzy_graph.rs:275-295, knn_to_coo :114-159,
 // symmetrise_graph :174-228 (rows emitted sorted by column).
-extern "C" int64_t orc_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n,
-                                   int k, float effective_k, int symmetrise, float* sigma_out,
-                                   float* rho_out, int64_t* indptr, int32_t* indices,
-                                   float* weights) {
+template <typename T>
+static int64_t fuzzy_graph_impl(const int32_t* knn_idx, const T* knn_dist, int64_t n, int k,
+                                float effective_k, int symmetrise, T* sigma_out, T* rho_out,
+                                int64_t* indptr, int32_t* indices, T* weights) {
   // effective_n_neighbors.to_usize().unwrap_or(k): truncation when > -1
   size_t kt = (effective_k > -1.0f && std::isfinite(effective_k)) ? (size_t)effective_k : (size_t)k;
-  std::vector<float> sigma(n), rho(n);
+  std::vector<T> sigma(n), rho(n);
   smooth_knn_dist(knn_dist, n, k, kt, sigma.data(), rho.data());
-  if (sigma_out) std::memcpy(sigma_out, sigma.data(), n * sizeof(float));
-  if (rho_out) std::memcpy(rho_out, rho.data(), n * sizeof(float));
+  if (sigma_out) std::memcpy(sigma_out, sigma.data(), n * sizeof(T));
+  if (rho_out) std::memcpy(rho_out, rho.data(), n * sizeof(T));
   // knn_to_coo
   std::vector<int64_t> rr;
   std::vector<int32_t> cc;
-  std::vector<float> ww;
+  std::vector<T> ww;
   for (int64_t i = 0; i < n; ++i)
     for (int t = 0; t < k; ++t) {
       int64_t j = knn_idx[i * k + t];
       if (j == i || j < 0) continue;
-      float adjusted = std::fmax(knn_dist[i * k + t] - rho[i], 0.0f);
-      float w;
-      if (sigma[i] > 0.0f) w = cexp(-(adjusted / sigma[i]));
-      else if (adjusted > 0.0f) w = 0.0f;
-      else w = 1.0f;
+      T adjusted = std::fmax(knn_dist[i * k + t] - rho[i], T(0));
+      T w;
+      if (sigma[i] > T(0)) w = cexp(-(adjusted / sigma[i]));
+      else if (adjusted > T(0)) w = T(0);
+      else w = T(1);
       rr.push_back(i);
       cc.push_back((int32_t)j);
       ww.push_back(w);
@@ -460,16 +466,34 @@ extern "C" int64_t orc_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist
   return symmetrise_coo(rr, cc, ww, n, indptr, indices, weights);
 }
 
+extern "C" int64_t orc_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n,
+                                   int k, float effective_k, int symmetrise, float* sigma_out,
+                                   float* rho_out, int64_t* indptr, int32_t* indices,
+                                   float* weights) {
+  return fuzzy_graph_impl<float>(knn_idx, knn_dist, n, k, effective_k, symmetrise, sigma_out,
+                                 rho_out, indptr, indices, weights);
+}
+
+// the same steps in fp64 (distances given in fp64)
+extern "C" int64_t orc_fuzzy_graph_f64(const int32_t* knn_idx, const double* knn_dist,
+                                       int64_t n, int k, float effective_k, int symmetrise,
+                                       double* sigma_out, double* rho_out, int64_t* indptr,
+                                       int32_t* indices, double* weights) {
+  return fuzzy_graph_impl<double>(knn_idx, knn_dist, n, k, effective_k, symmetrise, sigma_out,
+                                  rho_out, indptr, indices, weights);
+}
+
 // symmetrise_graph (src/graph/fuzzy_graph.rs:174-228) of a directed COO list: fuzzy union
 // w_ij + w_ji - w_ij * w_ji, entries <= 0 dropped, both directions stored, rows sorted by
 // column. A repeated (i, j) keeps its last weight.
+template <typename T>
 static int64_t symmetrise_coo(const std::vector<int64_t>& rr, const std::vector<int32_t>& cc,
-                              const std::vector<float>& ww, int64_t n, int64_t* indptr,
-                              int32_t* indices, float* weights) {
+                              const std::vector<T>& ww, int64_t n, int64_t* indptr,
+                              int32_t* indices, T* weights) {
   // Every directed entry (i -> j, w) is filed under row i as forward and under row j as
   // backward; records keep their input order within a row (counting sort), then each row is
   // sorted stably by column, so for a repeated pair the last record of a direction wins.
-  struct Rec { int32_t col; int32_t dir; float w; };
+  struct Rec { int32_t col; int32_t dir; T w; };
   const int64_t m = (int64_t)rr.size();
   std::vector<int64_t> off(n + 1, 0);
   for (int64_t e = 0; e < m; ++e) {
@@ -492,15 +516,15 @@ static int64_t symmetrise_coo(const std::vector<int64_t>& rr, const std::vector<
     int64_t out = off[i];
     for (int64_t q = off[i]; q < off[i + 1];) {
       const int32_t j = rec[q].col;
-      float wij = 0.0f, wji = 0.0f;
+      T wij = T(0), wji = T(0);
       for (; q < off[i + 1] && rec[q].col == j; ++q) {
         if (rec[q].dir == 0) wij = rec[q].w;
         else wji = rec[q].w;
       }
-      float s = wij + wji;
-      float pr = wij * wji;
-      float wsym = s - pr;
-      if (wsym > 0.0f) rec[out++] = {j, 0, wsym};
+      T s = wij + wji;
+      T pr = wij * wji;
+      T wsym = s - pr;
+      if (wsym > T(0)) rec[out++] = {j, 0, wsym};
     }
     cnt[i] = out - off[i];
   }

@@ -56,6 +56,9 @@ def lib():
         L.orc_fuzzy_graph.argtypes = [i32p, f32p, C.c_int64, C.c_int, C.c_float, C.c_int,
                                       f32p, f32p, i64p, i32p, f32p]
         L.orc_fuzzy_graph.restype = C.c_int64
+        L.orc_fuzzy_graph_f64.argtypes = [i32p, f64p, C.c_int64, C.c_int, C.c_float, C.c_int,
+                                          f64p, f64p, i64p, i32p, f64p]
+        L.orc_fuzzy_graph_f64.restype = C.c_int64
         L.orc_embedding.argtypes = [i64p, i32p, f32p, C.c_int64, C.c_int, C.c_int, C.c_float,
                                     C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                     vp, C.c_uint64, f32p]
@@ -173,17 +176,21 @@ def canon_dist(a, b, metric=0):
     return lib().orc_canon_dist(a, b, a.shape[0], metric)
 
 
-def fuzzy_graph(idx, dist, effective_k=None, symmetrise=True):
+def fuzzy_graph(idx, dist, effective_k=None, symmetrise=True, fp64=False):
+    """build_fuzzy_simplicial_set; fp64=True runs the same steps in fp64 (distances taken
+    as fp64) for error statements."""
     idx = np.ascontiguousarray(idx, np.int32)
-    dist = _f32(dist)
+    ft = np.float64 if fp64 else np.float32
+    dist = _f64(dist) if fp64 else _f32(dist)
     n, k = idx.shape
     eff = float(k) if effective_k is None else float(effective_k)
-    sig = np.empty(n, np.float32)
-    rho = np.empty(n, np.float32)
+    sig = np.empty(n, ft)
+    rho = np.empty(n, ft)
     indptr = np.empty(n + 1, np.int64)
     ind = np.empty(2 * n * k, np.int32)
-    w = np.empty(2 * n * k, np.float32)
-    nnz = lib().orc_fuzzy_graph(idx, dist, n, k, eff, int(symmetrise), sig, rho, indptr, ind, w)
+    w = np.empty(2 * n * k, ft)
+    fn = lib().orc_fuzzy_graph_f64 if fp64 else lib().orc_fuzzy_graph
+    nnz = fn(idx, dist, n, k, eff, int(symmetrise), sig, rho, indptr, ind, w)
     return indptr, ind[:nnz].copy(), w[:nnz].copy(), sig, rho
 
 
