@@ -1,7 +1,7 @@
 // Cluster hierarchy on the host: single linkage, condensed tree, leaves, labels,
 // membership strengths, min-cluster-size barcodes, layer selection and the
-// approx_n_clusters search. Each step is O(n) (or O(n log n)); the persistence bin
-// sums run in fp64 on the GPU (persistence.hip).
+// approx_n_clusters search. Each step is O(n) (or O(n log n)), on arrays that are
+// already in host memory.
 //
 //  linkage      src/clustering/linkage.rs:323-367   (union-find relabel, no parent walk)
 //  condense     src/clustering/condensed_tree.rs:127-248 (explicit-stack right-first DFS)
@@ -13,6 +13,8 @@
 //  persistence  src/clustering/persistence.rs:135-173 (fp64, barcode order per bin)
 //  layers       src/clustering/persistence.rs:188-463
 //  search       src/lib.rs:377-447
+#include <omp.h>
+
 #include <numeric>
 #include <unordered_map>
 
@@ -237,8 +239,11 @@ Bar barcodes(const CTree& t, int64_t min_size) {
   return b;
 }
 
-void total_persistence(const Bar& b, std::vector<double>& sizes, std::vector<double>& pers,
-                       hipStream_t s, DevPool& pool) {
+// Total persistence per size bin in fp64: a bin's value is the sum, in barcode order, of
+// the contributions of the barcodes that cover it. Walking the barcodes in order and
+// adding each contribution to the bins it covers performs exactly those adds, in that
+// order, for every bin; threads own disjoint ranges of bins, so the order is kept.
+void total_persistence(const Bar& b, std::vector<double>& sizes, std::vector<double>& pers) {
   sizes.assign(b.birth.begin(), b.birth.end());
   std::sort(sizes.begin(), sizes.end());
   sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
@@ -251,8 +256,20 @@ void total_persistence(const Bar& b, std::vector<double>& sizes, std::vector<dou
     contrib.push_back((b.death[i] - b.birth[i]) * b.lam[i]);
   }
   pers.assign(sizes.size(), 0.0);
-  persistence_bins_device(bp.data(), dp.data(), contrib.data(), (int64_t)bp.size(),
-                          (int64_t)sizes.size(), pers.data(), s, pool);
+  const int64_t nbins = (int64_t)sizes.size(), nbc = (int64_t)bp.size();
+  int64_t cover = 0;
+  for (int64_t q = 0; q < nbc; ++q) cover += dp[q] - bp[q];
+#pragma omp parallel if (cover > (1 << 14))
+  {
+    const int64_t T = omp_get_num_threads(), t = omp_get_thread_num();
+    const int64_t lo = nbins * t / T, hi = nbins * (t + 1) / T;
+    double* out = pers.data();
+    for (int64_t q = 0; q < nbc; ++q) {
+      const int64_t a = std::max(bp[q], lo), e = std::min(dp[q], hi);
+      const double c = contrib[q];
+      for (int64_t k = a; k < e; ++k) out[k] = out[k] + c;
+    }
+  }
 }
 
 std::vector<int64_t> find_peaks(const std::vector<double>& x) {
@@ -295,7 +312,7 @@ int64_t n_clusters(const std::vector<int64_t>& l) {
 }  // namespace
 
 Layers cluster_layers_from_linkage(const Linkage& L, int64_t base_mcs, double min_sim,
-                                   int64_t max_layers, hipStream_t s, DevPool& pool) {
+                                   int64_t max_layers) {
   const int64_t n = L.n;
   Layers out;
   const CTree t = condense(L, base_mcs);
@@ -306,7 +323,7 @@ Layers cluster_layers_from_linkage(const Linkage& L, int64_t base_mcs, double mi
   const Bar b = barcodes(t, base_mcs);
   if (!b.birth.empty()) {
     std::vector<double> sizes, curve;
-    total_persistence(b, sizes, curve, s, pool);
+    total_persistence(b, sizes, curve);
     const std::vector<int64_t> pk = find_peaks(curve);
     std::vector<std::pair<int64_t, double>> idx;
     for (auto p : pk) idx.push_back({p, curve[p]});

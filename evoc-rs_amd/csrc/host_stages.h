@@ -46,7 +46,7 @@ struct Layers {
   std::vector<double> persistence;
 };
 Layers cluster_layers_from_linkage(const Linkage& L, int64_t base_mcs, double min_sim,
-                                   int64_t max_layers, hipStream_t s, DevPool& pool);
+                                   int64_t max_layers);
 void search_n_clusters_from_linkage(const Linkage& L, int64_t target_k, int64_t* labels,
                                     float* strengths);
 int best_layer_index(const std::vector<double>& persistence);

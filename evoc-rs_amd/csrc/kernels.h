@@ -182,13 +182,6 @@ void core_sq_device(const float* d_emb, int64_t n, int dim, int min_samples, flo
 void mst_device(const float* d_emb, int64_t n, int dim, int min_samples, int32_t* h_u,
                 int32_t* h_v, float* h_w, hipStream_t s, DevPool& pool, int* rounds = nullptr);
 
-// ------------------------------------------------------------------ persistence (fp64)
-// persistence[k] = sum over barcodes b (in order, root skipped) with
-// bpos[b] <= k < dpos[b] of contrib[b]
-void persistence_bins_device(const int64_t* h_bpos, const int64_t* h_dpos, const double* h_contrib,
-                             int64_t nbc, int64_t nbins, double* h_out, hipStream_t s,
-                             DevPool& pool);
-
 // ------------------------------------------------------------------ sort
 // stable LSD radix sort of (key, value) pairs on the low key_bits bits of key
 void radix_sort_pairs(uint64_t* d_key, uint32_t* d_val, int64_t n, int key_bits, hipStream_t s,

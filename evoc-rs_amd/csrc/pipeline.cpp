@@ -213,7 +213,7 @@ Layers cluster_device(const float* d_emb, int64_t n, int dim, const evoc_params_
     out.persistence.assign(1, 0.0);
   } else {
     out = cluster_layers_from_linkage(L, p.base_min_cluster_size, p.min_similarity_threshold,
-                                      p.max_layers, s, pool);
+                                      p.max_layers);
   }
   lap("layers");
   if (layer_ms) *layer_ms = ms_since(t0);

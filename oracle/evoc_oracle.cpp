@@ -1661,9 +1661,11 @@ int linkage_impl(int32_t* u, int32_t* v, T* w, int64_t ne, int64_t n, int64_t* l
     sv[i] = v[order[i]];
     sw[i] = w[order[i]];
   }
-  std::memcpy(u, su.data(), ne * sizeof(int32_t));
-  std::memcpy(v, sv.data(), ne * sizeof(int32_t));
-  std::memcpy(w, sw.data(), ne * sizeof(T));
+  if (ne > 0) {  // an empty vector's data() may be null, which memcpy does not accept
+    std::memcpy(u, su.data(), ne * sizeof(int32_t));
+    std::memcpy(v, sv.data(), ne * sizeof(int32_t));
+    std::memcpy(w, sw.data(), ne * sizeof(T));
+  }
   std::vector<int64_t> par(n), lab(n), sz(2 * n, 0);
   std::iota(par.begin(), par.end(), 0);
   std::iota(lab.begin(), lab.end(), 0);

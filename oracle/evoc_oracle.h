@@ -84,7 +84,9 @@ int orc_strengths(const int64_t* parent, const int64_t* child, const float* lamb
 int64_t orc_find_peaks(const double* x, int64_t len, int64_t* peaks);
 
 // persistence curve for a condensed tree: fp64 = 1 uses double throughout
-// (the HIP library's precision), 0 = f32 as in the reference.
+// (the HIP library's precision), 0 = f32 as in the reference. Returns
+// (number of barcodes << 32) | number of size bins; the bins go to sizes_out and
+// persistence_out, the barcodes to birth_out, death_out and lambda_death_out.
 int64_t orc_persistence(const int64_t* parent, const int64_t* child, const float* lambda_val,
                         const int64_t* child_size, int64_t n_nodes, int64_t n,
                         int64_t base_mcs, int fp64, double* sizes_out,
@@ -138,6 +140,49 @@ int64_t orc_evoc(const float* x, int64_t n, int64_t d, const int32_t* pre_idx,
 
 uint32_t orc_philox_u32(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3);
 int orc_num_threads(void);
+
+// ---- pieces exported for the known-answer tests and the CPU baselines
+int orc_knn_nndescent(const float* x, int64_t n, int64_t d, int metric, int k, uint64_t seed,
+                      int max_iters, float delta, float rho, int32_t* idx_out, float* dist_out,
+                      int* iters_done);
+int64_t orc_fuzzy_graph_f64(const int32_t* knn_idx, const double* knn_dist, int64_t n, int k,
+                            float effective_k, int symmetrise, double* sigma_out,
+                            double* rho_out, int64_t* indptr, int32_t* indices,
+                            double* weights);
+int64_t orc_symmetrise(const int64_t* rows, const int32_t* cols, const float* vals, int64_t nnz,
+                       int64_t n, int64_t* indptr, int32_t* indices, float* weights);
+// op: 0 transpose, 1 product a*b, 2 element-wise product, 3 L2-normalised columns,
+// 4 L1-normalised rows, 5 values clipped to [0, 1] (b unused except for 1 and 2);
+// returns nnz; -1 for an unknown op, or when cap is too small (shape then holds the
+// result's shape)
+int64_t orc_csr_op(int op, const int64_t* aip, const int64_t* aix, const float* adv, int64_t anr,
+                   int64_t anc, const int64_t* bip, const int64_t* bix, const float* bdv,
+                   int64_t bnr, int64_t bnc, int64_t* oip, int64_t* oix, float* odv,
+                   int64_t cap, int64_t* shape);
+int64_t orc_csr_from_coo(const int64_t* rows, const int64_t* cols, const float* vals,
+                         int64_t nnz, int64_t n, int64_t* oip, int64_t* oix, float* odv);
+int orc_csr_from_partition(const int64_t* part, int64_t n, int64_t n_parts, int64_t* oip,
+                           int64_t* oix, float* odv);
+int orc_csr_matmul_dense(const int64_t* ip, const int64_t* ix, const float* dv, int64_t nr,
+                         int64_t nc, const float* rhs, int64_t d, float* out);
+int orc_lp_iteration(const int64_t* ip, const int64_t* ix, const float* dv, int64_t n,
+                     const int64_t* labels, int64_t* out);
+int orc_lp_outliers(const int64_t* ip, const int64_t* ix, const float* dv, int64_t n,
+                    int64_t* labels, uint64_t seed);
+int64_t orc_lp_remap(int64_t* labels, int64_t n);
+int64_t orc_lp_loop(const int64_t* ip, const int64_t* ix, const float* dv, int64_t n,
+                    int64_t approx_parts, int n_iter, uint64_t seed, int64_t* part);
+int orc_unit_range(float* m, int64_t n, int64_t d);
+int orc_pca_init(const float* data, int64_t n, int64_t d, int dim, float* out);
+int orc_random_init(int64_t n, int dim, uint64_t seed, float* out);
+int orc_lp_expand(const int64_t* ip, const int64_t* ix, const float* dv, int64_t n,
+                  const int64_t* part, int64_t n_parts, const float* layout, int dim,
+                  double scaling, float* out);
+int64_t orc_mst_boruvka(const float* emb, int64_t n, int dim, int ms, int32_t* u, int32_t* v,
+                        float* w);
+int orc_core_sq_kd(const float* emb, int64_t n, int dim, int ms, float* core);
+// 0: dense Prim (the parity reference), 1: kd-tree Boruvka; returns the previous choice
+int orc_set_mst_algo(int algo);
 
 #ifdef __cplusplus
 }
