@@ -108,8 +108,10 @@ struct KnnStats {
 };
 // the calling thread's last knn_device call
 KnnStats& last_knn_stats();
+// verbose > 0 prints the tile pass, the escalation and the exact search as they finish
 void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k, int precision,
-                int32_t* d_idx, float* d_dist, KnnStats* stats, hipStream_t s, DevPool& pool);
+                int32_t* d_idx, float* d_dist, KnnStats* stats, hipStream_t s, DevPool& pool,
+                int verbose = 0);
 
 // ------------------------------------------------------------------ fuzzy set
 struct Graph {  // CSR, rows sorted by column when symmetrised

@@ -10,6 +10,7 @@
 //                 (64 strided lane sums + halving tree), sort by (dist, idx), and a
 //                 per-row certificate that no non-candidate can enter the top k.
 //  4. knn_exact:  brute-force exact rows for the (rare) rows whose certificate fails.
+#include <cstdio>
 #include <cstdlib>
 
 #include "kernels.h"
@@ -141,7 +142,9 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
     const uint16_t* __restrict__ Xh, const uint16_t* __restrict__ Xl, int64_t n, int64_t npad,
     int dpad, const float* __restrict__ colscale, int metric, int32_t* __restrict__ cand_idx,
     float* __restrict__ cand_key, const uint16_t* __restrict__ Qh,
-    const uint16_t* __restrict__ Ql, const int32_t* __restrict__ qmap, int64_t nq) {
+    const uint16_t* __restrict__ Ql, const int32_t* __restrict__ qmap, int64_t nq, int64_t ct0,
+    int64_t ct1) {
+  // column tiles [ct0, ct1) of BN columns are swept (a shard of the columns, or all of them)
   constexpr int NOPS = SPLIT ? 4 : 2;           // A hi, B hi, (A lo, B lo)
   constexpr int NST = SPLIT ? 2 : 4;            // stages in the LDS ring
   constexpr int OPB = BM * 4 * 16;              // bytes per staged operand chunk (8 KB)
@@ -168,8 +171,8 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
   const int64_t qlo = qmap ? qmap[row0] : row0;
   const int64_t qhi = qmap ? qmap[std::min<int64_t>(row0 + BM, nq) - 1] : row0 + BM - 1;
   const int nck = dpad / KC;
-  const int64_t ntile = npad / BN;
-  const int64_t total = ntile * nck;
+  const int64_t g0 = ct0 * nck;
+  const int64_t total = ct1 * nck;
   const int orow = tid & (BM - 1), ohalf = tid >> 7;  // list owned by this thread
 
   float lk[C];
@@ -206,8 +209,8 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
   f32x16 acc[2][2];
   float cs[2] = {0.0f, 0.0f};
   constexpr int PER_CHUNK = NOPS * 2;  // load instructions per thread per chunk
-  for (int64_t g = 0; g < NST - 1 && g < total; ++g) issue(g);
-  for (int64_t g = 0; g < total; ++g) {
+  for (int64_t g = g0; g < g0 + NST - 1 && g < total; ++g) issue(g);
+  for (int64_t g = g0; g < total; ++g) {
     const int64_t ahead = std::min<int64_t>(NST - 2, total - 1 - g);
     if (ahead >= 2) wait_vm<2 * PER_CHUNK>();
     else if (ahead == 1) wait_vm<PER_CHUNK>();
@@ -1569,6 +1572,158 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
   }
 }
 
+// ------------------------------------------------------------------ 3b. escalated rows
+// Re-rank of a row searched again over S column shards, each of which kept two column
+// halves of 32 candidates (list row lr of shard t at cand[(t * nrows + lr) * 64]). Every
+// column a shard left out has a key at or above that shard's smaller last key, so the
+// bound of the row is the smallest of those over the shards, which sits near rank 64 S of
+// the row rather than 64. Every valid candidate (S per lane) gets its exact distance, the
+// best k by (distance, index) are written, and the certificate is the one of
+// knn_rerank_kernel against that bound; a failing row is listed for the exact scan.
+template <int S>
+__global__ __launch_bounds__(256) void knn_rerank_all_kernel(
+    const float* __restrict__ X, int64_t n, int d, int64_t ld, const float* __restrict__ nrm,
+    const float* __restrict__ sqn, const float* __restrict__ colscale,
+    const unsigned* __restrict__ flags, const int32_t* __restrict__ cand_idx,
+    const float* __restrict__ cand_key, int k, int metric, float gamma,
+    int32_t* __restrict__ out_idx, float* __restrict__ out_dist, int* __restrict__ fail_count,
+    int32_t* __restrict__ fail_rows, const int32_t* __restrict__ rowmap, int64_t nrows) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t lr = (int64_t)blockIdx.x * 4 + wave;
+  if (lr >= nrows) return;
+  const int64_t row = rowmap[lr];
+  const int nt = (d + 63) / 64;
+  float q[MAXT];
+#pragma unroll
+  for (int t = 0; t < MAXT; ++t) {
+    const int c = t * 64 + lane;
+    q[t] = (t < nt && c < d) ? X[row * ld + c] : 0.0f;
+  }
+  const float u = 5.9604645e-08f;  // 2^-24
+  const float dt = (float)(d / 64 + 12);
+  float E = 0.0f, eps = 0.0f;
+  if (metric == 1) {
+    eps = 1.5f * gamma + 64.0f * u;
+  } else {
+    const float nmax = __uint_as_float(flags[1]);
+    E = 1.5f * (dt * u * (sqn[row] + nmax * nmax) + 2.0f * (gamma + 2.0f * u) * nrm[row] * nmax);
+  }
+  int cj[S];
+  float bound = FINF;
+  int nvalid = 0;
+#pragma unroll
+  for (int t = 0; t < S; ++t) {
+    const int64_t base = ((int64_t)t * nrows + lr) * 64;
+    cj[t] = cand_idx[base + lane];
+    const float ck = cand_key[base + lane];
+    bound = fminf(bound, fminf(__shfl(ck, 31, 64), __shfl(ck, 63, 64)));
+    nvalid += __popcll(__ballot(cj[t] >= 0));
+  }
+  // exact canonical distances of every valid candidate, two rows' loads in flight at once
+  float my_d[S], my_sq[S];
+#pragma unroll
+  for (int t = 0; t < S; ++t) {
+    my_d[t] = FINF;
+    my_sq[t] = FINF;
+  }
+  auto finish = [&](float acc, int c, int j, int t) {
+    acc = tree64(acc);
+    acc = __shfl(acc, 0, 64);
+    const float dist = metric == 1 ? cos_dist(acc, nrm[row], nrm[j]) : sqrtf(acc);
+    if (lane == c) {
+      my_d[t] = dist;
+      my_sq[t] = acc;
+    }
+  };
+#pragma unroll
+  for (int t = 0; t < S; ++t) {
+    unsigned long long mask = __ballot(cj[t] >= 0);
+    while (mask) {
+      const int c1 = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      const int j1 = __shfl(cj[t], c1, 64);
+      const int c2 = mask ? __ffsll((long long)mask) - 1 : -1;
+      if (mask) mask &= mask - 1;
+      const int j2 = c2 >= 0 ? __shfl(cj[t], c2, 64) : -1;
+      float v1[MAXT], v2[MAXT];
+#pragma unroll
+      for (int tt = 0; tt < MAXT; ++tt) {
+        const int cc = tt * 64 + lane;
+        const bool in = tt < nt && cc < d;
+        v1[tt] = in ? X[(int64_t)j1 * ld + cc] : 0.0f;
+        v2[tt] = (in && j2 >= 0) ? X[(int64_t)j2 * ld + cc] : 0.0f;
+      }
+      float a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+      for (int tt = 0; tt < MAXT; ++tt) {
+        const int cc = tt * 64 + lane;
+        if (tt < nt && cc < d) {
+          if (metric == 1) {
+            const float p1 = q[tt] * v1[tt], p2 = q[tt] * v2[tt];
+            a1 = a1 + p1;
+            a2 = a2 + p2;
+          } else {
+            const float f1 = q[tt] - v1[tt], f2 = q[tt] - v2[tt];
+            const float p1 = f1 * f1, p2 = f2 * f2;
+            a1 = a1 + p1;
+            a2 = a2 + p2;
+          }
+        }
+      }
+      finish(a1, c1, j1, t);
+      if (c2 >= 0) finish(a2, c2, j2, t);
+    }
+  }
+  // rank of each candidate among all of them by (distance, index)
+  int rank[S];
+#pragma unroll
+  for (int t = 0; t < S; ++t) rank[t] = 0;
+#pragma unroll
+  for (int t2 = 0; t2 < S; ++t2)
+    for (int c = 0; c < 64; ++c) {
+      const float od = __shfl(my_d[t2], c, 64);
+      const int oj = __shfl(cj[t2], c, 64);
+      if (oj < 0) continue;
+#pragma unroll
+      for (int t = 0; t < S; ++t)
+        rank[t] += (od < my_d[t] || (od == my_d[t] && oj < cj[t])) ? 1 : 0;
+    }
+  float dk = FINF, dk2 = FINF;
+#pragma unroll
+  for (int t = 0; t < S; ++t) {
+    if (cj[t] >= 0 && rank[t] < k) {
+      out_idx[row * k + rank[t]] = cj[t];
+      out_dist[row * k + rank[t]] = my_d[t];
+    }
+    const bool kth = cj[t] >= 0 && rank[t] == k - 1;
+    const unsigned long long kb = __ballot(kth);
+    if (kb) {
+      const int src = __ffsll((long long)kb) - 1;
+      dk = __shfl(my_d[t], src, 64);
+      dk2 = __shfl(my_sq[t], src, 64);
+    }
+  }
+  for (int s = lane; s < k; s += 64)
+    if (s >= nvalid) {
+      out_idx[row * k + s] = -1;
+      out_dist[row * k + s] = FINF;
+    }
+  bool pass;
+  if ((int64_t)nvalid >= n - 1) {
+    pass = true;
+  } else if (nvalid < k) {
+    pass = false;
+  } else if (metric == 1) {
+    pass = (1.0f + bound * colscale[row]) - eps > dk;
+  } else {
+    pass = sqn[row] + bound - E > dk2 * (1.0f + (dt + 4.0f) * u) + 1e-30f;
+  }
+  if (!pass && lane == 0) {
+    const int pos = atomicAdd(fail_count, 1);
+    fail_rows[pos] = (int32_t)row;
+  }
+}
+
 // ------------------------------------------------------------------ 4. exact rows
 constexpr int KMAX = 32;
 
@@ -1752,9 +1907,11 @@ __global__ void knn_gather_rows_kernel(const uint16_t* __restrict__ Xh,
 // at least this many failed certificates: re-search them with 64 candidates per row
 // (fewer go straight to the exact scan, which is cheaper for a handful of rows)
 constexpr int KNN_ESCALATE_MIN = 1024;
+constexpr int KNN_ESC_SHARDS = 4;  // column shards of the escalation search
 
 void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k, int precision,
-                int32_t* d_idx, float* d_dist, KnnStats* stats, hipStream_t s, DevPool& pool) {
+                int32_t* d_idx, float* d_dist, KnnStats* stats, hipStream_t s, DevPool& pool,
+                int verbose) {
   EVOC_CHECK(n >= 1 && d >= 1, "knn: empty input");
   EVOC_CHECK(k >= 1 && k <= KMAX, "knn: k must be in [1, 32]");
   EVOC_CHECK(d <= MAXT * 64, "knn: d must be <= 1024");
@@ -1950,20 +2107,20 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       if (CH == 16)
         knn_topc_kernel<true, 16><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
                                                           metric, cidx.p, ckey.p, nullptr,
-                                                          nullptr, nullptr, 0);
+                                                          nullptr, nullptr, 0, 0, npad / BN);
       else
         knn_topc_kernel<true, 32><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
                                                           metric, cidx.p, ckey.p, nullptr,
-                                                          nullptr, nullptr, 0);
+                                                          nullptr, nullptr, 0, 0, npad / BN);
     } else {
       if (CH == 16)
         knn_topc_kernel<false, 16><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad,
                                                            colscale.p, metric, cidx.p, ckey.p,
-                                                           nullptr, nullptr, nullptr, 0);
+                                                           nullptr, nullptr, nullptr, 0, 0, npad / BN);
       else
         knn_topc_kernel<false, 32><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad,
                                                            colscale.p, metric, cidx.p, ckey.p,
-                                                           nullptr, nullptr, nullptr, 0);
+                                                           nullptr, nullptr, nullptr, 0, 0, npad / BN);
     }
   }
   EVOC_HIP(hipGetLastError());
@@ -2020,19 +2177,31 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   int nfail = 0;
   EVOC_HIP(hipMemcpyAsync(&nfail, fcount.p, sizeof(int), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
+  if (verbose) {
+    float ms = 0.0f;
+    EVOC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    std::printf("  kNN tile pass %.0f ms (%d candidates per row%s); %d of %lld rows failed the "
+                "certificate\n", ms, C, sym ? ", symmetric sweep" : "", nfail, (long long)n);
+    std::fflush(stdout);
+  }
   int escalated = 0;
-  if (nfail >= KNN_ESCALATE_MIN && C < 64) {
+  // EVOC_HIP_KNN_ESCALATE=1 escalates any number of failing rows (tests of small inputs)
+  const char* esc_env = std::getenv("EVOC_HIP_KNN_ESCALATE");
+  const int esc_min = (esc_env && esc_env[0] == '1') ? 1 : KNN_ESCALATE_MIN;
+  if (nfail >= esc_min && C < 64) {
     // Many certificates failed (neighbour distances concentrated, as in a dense leaf of a
-    // hierarchical mixture): search those rows again, gathered, with 32 candidates per
-    // column half, and re-rank them; only rows that fail again take the exact scan.
+    // hierarchical mixture): search those rows again, gathered, over KNN_ESC_SHARDS column
+    // shards with 32 candidates per column half each, and re-rank every candidate; the
+    // row's bound is then the smallest of the shards' bounds, near rank 64 * shards. Only
+    // rows that fail again take the exact scan.
     escalated = nfail;
     std::vector<int32_t> hf(nfail);
     EVOC_HIP(hipMemcpyAsync(hf.data(), frows.p, nfail * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     EVOC_HIP(hipStreamSynchronize(s));
     std::sort(hf.begin(), hf.end());
     const int64_t nq = nfail, nqpad = ceil_div(nq, BM) * BM;
-    Buf<int32_t> qmap(pool, nq), ecidx(pool, nq * 64), frows2(pool, nq);
-    Buf<float> eckey(pool, nq * 64);
+    Buf<int32_t> qmap(pool, nq), ecidx(pool, KNN_ESC_SHARDS * nq * 64), frows2(pool, nq);
+    Buf<float> eckey(pool, KNN_ESC_SHARDS * nq * 64);
     Buf<uint16_t> Qh(pool, nqpad * dpad), Ql;
     if (split) Ql.alloc(pool, nqpad * dpad);
     EVOC_HIP(hipMemcpyAsync(qmap.p, hf.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice, s));
@@ -2044,25 +2213,36 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     const size_t smem = (size_t)nst * nops * BM * 4 * 16 + BM * SCS * sizeof(float) +
                         2 * BM * sizeof(float) + 4 * sizeof(int);
     const unsigned grid = (unsigned)(nqpad / BM);
-    if (split)
-      knn_topc_kernel<true, 32><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
-                                                        metric, ecidx.p, eckey.p, Qh.p, Ql.p,
-                                                        qmap.p, nq);
-    else
-      knn_topc_kernel<false, 32><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad,
-                                                         colscale.p, metric, ecidx.p, eckey.p,
-                                                         Qh.p, nullptr, qmap.p, nq);
+    const int64_t ntile = npad / BN;
+    for (int sh = 0; sh < KNN_ESC_SHARDS; ++sh) {
+      const int64_t ct0 = ntile * sh / KNN_ESC_SHARDS, ct1 = ntile * (sh + 1) / KNN_ESC_SHARDS;
+      int32_t* oi = ecidx.p + (int64_t)sh * nq * 64;
+      float* ok = eckey.p + (int64_t)sh * nq * 64;
+      if (split)
+        knn_topc_kernel<true, 32><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
+                                                          metric, oi, ok, Qh.p, Ql.p, qmap.p, nq,
+                                                          ct0, ct1);
+      else
+        knn_topc_kernel<false, 32><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad,
+                                                           colscale.p, metric, oi, ok, Qh.p,
+                                                           nullptr, qmap.p, nq, ct0, ct1);
+    }
     EVOC_HIP(hipGetLastError());
     EVOC_HIP(hipMemsetAsync(fcount.p, 0, sizeof(int), s));
-    knn_rerank_kernel<<<(unsigned)ceil_div(nq, 4), 256, 0, s>>>(
-        dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, ecidx.p, eckey.p, 64, 2, nullptr, k,
-        metric, gamma, d_idx, d_dist, fcount.p, frows2.p, qmap.p, nq);
+    knn_rerank_all_kernel<KNN_ESC_SHARDS><<<(unsigned)ceil_div(nq, 4), 256, 0, s>>>(
+        dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, ecidx.p, eckey.p, k, metric, gamma,
+        d_idx, d_dist, fcount.p, frows2.p, qmap.p, nq);
     EVOC_HIP(hipGetLastError());
     EVOC_HIP(hipMemcpyAsync(&nfail, fcount.p, sizeof(int), hipMemcpyDeviceToHost, s));
     EVOC_HIP(hipStreamSynchronize(s));
     if (nfail > 0)
       EVOC_HIP(hipMemcpyAsync(frows.p, frows2.p, nfail * sizeof(int32_t),
                               hipMemcpyDeviceToDevice, s));
+    if (verbose) {
+      std::printf("  kNN escalation of %d rows over %d column shards done; %d rows go to the "
+                  "exact search\n", escalated, KNN_ESC_SHARDS, nfail);
+      std::fflush(stdout);
+    }
   }
   if (nfail > 0) {
     // the columns are split over `parts` workgroups of four wavefronts, EX_ROWS failing
@@ -2079,6 +2259,11 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
                                                         pi.p);
       knn_exact_merge_kernel<<<(unsigned)rows, 256, 256 * KMAX * 8, s>>>(
           k, frows.p + f0, parts * 4, pk.p, pi.p, d_idx, d_dist);
+      if (verbose && f0 + rows < nfail && ((f0 / EX_ROWS) + 1) % 4096 == 0) {
+        EVOC_HIP(hipStreamSynchronize(s));
+        std::printf("  kNN exact search: %d of %d rows done\n", f0 + rows, nfail);
+        std::fflush(stdout);
+      }
     }
     EVOC_HIP(hipGetLastError());
   }

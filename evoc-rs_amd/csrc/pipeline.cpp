@@ -256,7 +256,8 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
   } else {
     if (verbose) std::printf("Running exhaustive nearest neighbour search on the GPU...\n");
     KnnStats st;
-    knn_device(dX, n, (int)d, ld, nn.metric, k, nn.knn_precision, didx.p, ddist.p, &st, s, pool);
+    knn_device(dX, n, (int)d, ld, nn.metric, k, nn.knn_precision, didx.p, ddist.p, &st, s, pool,
+               verbose);
     fallback = st.fallback_rows;
     knn_kernel_ms = st.mfma_ms;
   }

@@ -10,7 +10,13 @@
   cosine: sampled kNN rows against the oracle (the symmetric sweep runs at this size), the
   full fuzzy CSR against the oracle's on the GPU's kNN graph, and properties of the whole
   run (finite layout, layer structure, recovery of the planted clusters).
+- 4,000,000 x 768 hierarchical mixture (100 parents ~ N(0, I), 10 children each at 0.3 x
+  the parent spread, leaf spread 0.1, L2-normalised), cosine, approx_n_clusters = 500,
+  learned warm start: sampled kNN rows against the oracle, the full fuzzy CSR against the
+  oracle's, and properties of the whole run. It takes minutes, so it runs only with
+  EVOC_HIP_FULL_SIZE=1.
 """
+import os
 import time
 
 import numpy as np
@@ -129,3 +135,50 @@ def test_config_1M_x512_bf16_cosine():
           f"{[round(v, 4) for v in scores]}, best layer {score:.4f}")
     assert r.knn_fallback_rows >= 0
     assert max(scores) > 0.25
+
+
+@pytest.mark.skipif(os.environ.get("EVOC_HIP_FULL_SIZE") != "1",
+                    reason="4,000,000 x 768 takes minutes; set EVOC_HIP_FULL_SIZE=1")
+def test_config_4M_x768_hierarchical_approx_500():
+    import torch
+    n, d = 4_000_000, 768
+    xd, ld = datagen.unit_mixture_device(n, d, seed=5, hierarchical=(100, 10, 0.3, 0.1))
+    leaves = ld.cpu().numpy()
+    nn = H.NearestNeighbourParams(dist_metric="cosine", warm_start=True)
+    p = H.EvocParams(embedding_dim=8, approx_n_clusters=500)
+    t0 = time.perf_counter()
+    r = H.evoc_device(xd.data_ptr(), n, d, params=p, nn_params=nn, seed=42, verbose=1)
+    wall = time.perf_counter() - t0
+    st = H.knn_last_stats()
+    print(f"4M x 768: wall {wall:.1f} s, stages {r.stage_ms}, kNN tile kernel "
+          f"{st['kernel_ms']:.0f} ms ({st['kernel_flop'] / st['kernel_ms'] / 1e9:.0f} TFLOP/s), "
+          f"escalated rows {st['escalated_rows']}, exact fallback rows {r.knn_fallback_rows}, "
+          f"warm start used {r.warm_start_used} (judge ARI {r.warm_start_ari:.4f}, "
+          f"MST ratio {r.warm_start_mst_ratio:.4f})", flush=True)
+    x = xd.cpu().numpy()
+    del xd, ld
+    torch.cuda.empty_cache()
+    rows = _sample_rows(n, 256, 5)
+    oi, od = O.knn_rows(x, rows, 15, metric=1)
+    assert np.array_equal(r.nn_indices[rows], oi), \
+        f"rows differ: {rows[(r.nn_indices[rows] != oi).any(1)][:10]}"
+    assert np.array_equal(r.nn_distances[rows].view(np.uint32), od.view(np.uint32))
+    nd, ni = r.nn_distances, r.nn_indices
+    assert (ni >= 0).all() and (ni < n).all()
+    assert not (ni == np.arange(n)[:, None]).any()
+    assert ((nd[:, 1:] > nd[:, :-1]) | ((nd[:, 1:] == nd[:, :-1]) & (ni[:, 1:] > ni[:, :-1]))).all()
+    del x
+    gp, gi, gw, _, _ = H.fuzzy_graph(ni, nd, 15.0)
+    op, oi2, ow, _, _ = O.fuzzy_graph(ni, nd, 15.0)
+    assert np.array_equal(gp, op) and np.array_equal(gi, oi2)
+    np.testing.assert_allclose(gw, ow, rtol=1e-5, atol=0)
+    del gp, gi, gw, op, oi2, ow
+    assert np.isfinite(r.embedding).all() and r.embedding.shape == (n, 8)
+    assert len(r.cluster_layers) == 1
+    lab = r.cluster_layers[0]
+    k = int(lab.max()) + 1
+    parents = leaves // 10
+    print(f"approx_n_clusters=500: {k} clusters, noise fraction {(lab < 0).mean():.4f}, "
+          f"ARI vs the 1,000 leaves {ari(lab, leaves):.4f}, vs the 100 parents "
+          f"{ari(lab, parents):.4f}", flush=True)
+    assert k >= 1 and lab.min() >= -1

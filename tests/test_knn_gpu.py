@@ -148,9 +148,10 @@ def test_knn_symmetric_equals_plain_at_300k(monkeypatch):
 
 def test_knn_escalation_of_concentrated_rows():
     """Leaves of a hierarchical mixture whose neighbour distances concentrate: the 32
-    candidates of the first pass fail many certificates, those rows are searched again with
-    64 candidates and re-ranked, and only the rest take the exact scan. Bit-exact either
-    way; d = 768 takes the streaming kernel, d = 512 the resident-query one."""
+    candidates of the first pass fail many certificates, those rows are searched again over
+    four column shards (32 candidates per column half each), every candidate is re-ranked,
+    and only the rest take the exact scan. Bit-exact either way; d = 768 takes the
+    streaming kernel, d = 512 the resident-query one."""
     rng = np.random.default_rng(31)
     for d in (768, 512):
         par = rng.normal(size=(6, d))
@@ -162,3 +163,23 @@ def test_knn_escalation_of_concentrated_rows():
         esc = H.knn_last_stats()["escalated_rows"]
         print(d, "escalated", esc, "exact", st["fallback_rows"])
         assert esc >= 1024 and st["fallback_rows"] < esc
+
+
+@pytest.mark.parametrize("n,d,metric", [(300, 300, "cosine"), (1500, 768, "cosine"),
+                                        (2049, 64, "euclidean"), (700, 520, "euclidean")])
+def test_knn_forced_escalation_small_inputs(n, d, metric, monkeypatch):
+    """EVOC_HIP_KNN_ESCALATE=1 sends every failing row through the sharded search, also at
+    sizes where a shard holds one column tile or none (300 rows: 3 tiles of 128 columns
+    for 4 shards)."""
+    monkeypatch.setenv("EVOC_HIP_KNN_ESCALATE", "1")
+    # one tight leaf: the 15th and 33rd keys of a row lie closer than the certificate's
+    # error term, so first-pass certificates fail
+    rng = np.random.default_rng(n + d)
+    x = rng.normal(size=d) + 0.02 * rng.normal(size=(n, d))
+    if metric == "cosine":
+        x = x / np.linalg.norm(x, axis=1, keepdims=True)
+    x = datagen.to_bf16_values(x.astype(np.float32))
+    st = _check(x, 15, metric)
+    esc = H.knn_last_stats()["escalated_rows"]
+    print(n, d, metric, "escalated", esc, "exact", st["fallback_rows"])
+    assert esc > 0
