@@ -777,7 +777,9 @@ constexpr int RB_ROWS = 256;
 constexpr int RB_WAVES = 8;
 constexpr int RB_BKT = 16;     // bucket slots per (row, half)
 constexpr int RB_DRAIN = 8;    // drain once some (row, half) holds this many
-constexpr int RB_BGRP = 4;     // B fragments read from LDS ahead of their MFMAs
+#ifndef RB_LA
+#define RB_LA 4        // B fragments read from LDS ahead of their MFMAs
+#endif
 constexpr int RB_WREG = 64 * RB_BKT * 8;                // per-wavefront buckets / key tile
 static_assert(RB_WREG >= 32 * RA_SCS * 4, "the key-tile dump must fit the bucket region");
 constexpr int RB_EPI_BYTES = RB_WAVES * RB_WREG + RB_WAVES * 64 * 4 + RB_WAVES * 32 * 4;
@@ -791,7 +793,8 @@ constexpr int RB_EPI_BYTES = RB_WAVES * RB_WREG + RB_WAVES * 64 * 4 + RB_WAVES *
 struct RbCols {
   const uint16_t* X;    // column rows, bf16, dpad wide, ntile * 32 of them (zero padded)
   const float* scale;   // their key scales
-  float* thr0;          // symmetric sweep: reverse-candidate threshold of every row
+  float* kt;            // symmetric sweep: (key scale, reverse-candidate threshold thr0)
+                        // of every row, interleaved; the sweep reads scales from here
   int64_t ncol;         // valid columns
   int64_t ntile;        // 32-column tiles
   int self;             // columns are the query rows: mask the diagonal
@@ -817,6 +820,21 @@ struct RbRing {
 // and copies them to its global queue in batches, with no vector-memory operation of the
 // ring in flight (stores mixed with the ring's loads would void its counted waits)
 constexpr int RB_PASS_C = 8;  // candidates per (row, half) kept by the threshold pass
+// buffer descriptor word 3 for the ring's raw loads (gfx9: 32-bit data format, no swizzle)
+constexpr int RB_RSRC_FLAGS = 0x00020000;
+// Raw buffer loads straight into LDS (each lane's 16 or 4 bytes land at lds + lane * size):
+// base is wave-uniform (the descriptor lives in scalar registers), bytes bounds the range.
+__device__ __forceinline__ void buffer_load_lds16(const void* base, int bytes, void* lds, int voff,
+                                                  int soff) {
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, RB_RSRC_FLAGS);
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void*)lds, 16, voff, soff, 0, 0);
+}
+__device__ __forceinline__ void buffer_load_lds4(const void* base, int bytes, void* lds, int voff) {
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, RB_RSRC_FLAGS);
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void*)lds, 4, voff, 0, 0, 0);
+}
 constexpr int RB_QL = 192;                                      // LDS queue entries per wavefront
 constexpr size_t RB_QBYTES = (size_t)RB_WAVES * RB_QL * 12;     // after the ring
 
@@ -842,32 +860,33 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   // The symmetric sweep takes its blocks in descending order: the blocks a workgroup
   // queues transposed keys for come after its own, so most of them have finished by then
   // and replaced their rows' thresholds with the tighter largest forward key.
-  const int64_t bid = SYM ? (int64_t)gridDim.x - 1 - (int64_t)blockIdx.x : (int64_t)blockIdx.x;
-  const int64_t row0 = bid * RB_ROWS;
-  const int64_t ntile = cols.ntile;
-  int64_t tile0 = 0, nt = ntile, tshift = 0;
+  // (tile and row indices fit 32 bits: n < 2^31; 64-bit products only for addresses)
+  const int bid = SYM ? (int)gridDim.x - 1 - (int)blockIdx.x : (int)blockIdx.x;
+  const int row0 = bid * RB_ROWS;
+  const int ntile = (int)cols.ntile;
+  int tile0 = 0, nt = ntile, tshift = 0;
   if (SYM) {
     // Block I owns the cyclic range I, I+1, ... of nblk blocks. The workgroups resident
     // together (consecutive blockIdx, `group` of them: blocks I0 - group + 1 .. I0) start
     // their walks at I0, so that they stream the shared part of their ranges in step and
     // the L2 serves most column reads; each walk then wraps to its own start.
-    const int64_t m = npad / RB_ROWS;  // 256-row blocks; ntile == 8 m
-    const int64_t I = bid;
-    const int64_t nblk = (m & 1) ? (m + 1) / 2 : m / 2 + (I < m / 2 ? 1 : 0);
+    const int m = (int)(npad / RB_ROWS);  // 256-row blocks; ntile == 8 m
+    const int I = bid;
+    const int nblk = (m & 1) ? (m + 1) / 2 : m / 2 + (I < m / 2 ? 1 : 0);
     tile0 = I * (RB_ROWS / RA_BN);
     nt = nblk * (RB_ROWS / RA_BN);
-    const int64_t g = cols.group > 0 ? cols.group : 1;
-    tshift = std::min<int64_t>((int64_t)blockIdx.x % g, nblk - 1) * (RB_ROWS / RA_BN);
+    const int g = cols.group > 0 ? (int)cols.group : 1;
+    tshift = std::min((int)blockIdx.x % g, nblk - 1) * (RB_ROWS / RA_BN);
   }
   // absolute column tile of the ct-th tile of this walk
-  auto tile_at = [&](int64_t ct) {
-    int64_t r = tshift + ct;
+  auto tile_at = [&](int ct) {
+    int r = tshift + ct;
     if (r >= nt) r -= nt;
-    int64_t a = tile0 + r;
+    int a = tile0 + r;
     if (a >= ntile) a -= ntile;
     return a;
   };
-  const int64_t total = nt * NKC;
+  const int total = nt * NKC;
   int qn = 0;  // this wavefront's reverse candidates in its global queue
   int lq = 0;  // and staged in LDS
   const int64_t qbase = SYM ? ((int64_t)blockIdx.x * RB_WAVES + wave) * cols.qcap : 0;
@@ -897,7 +916,7 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
 
   bf16x8 afr[NKC * KC / 16];
   {
-    const uint16_t* ar = Xh + (row0 + wave * 32 + (lane & 31)) * dpad + (lane >> 5) * 8;
+    const uint16_t* ar = Xh + (int64_t)(row0 + wave * 32 + (lane & 31)) * dpad + (lane >> 5) * 8;
 #pragma unroll
     for (int f = 0; f < NKC * KC / 16; ++f) afr[f] = *reinterpret_cast<const bf16x8*>(ar + f * 16);
   }
@@ -914,7 +933,7 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   // transposed candidates for most tiles before any per-row work
   float rs_min = 0.0f, rs_max = 0.0f;
   if (SYM) {
-    const int64_t r = row0 + wave * 32 + (lane & 31);
+    const int r = row0 + wave * 32 + (lane & 31);
     const float v = r < n ? colscale[r] : 0.0f;
     if (lane < 32) rsc[lane] = v;
     rs_min = v;
@@ -929,49 +948,41 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   }
   int bcnt = 0;
 
-  const uint16_t* pb[LOADS];
+  // Stage loads are buffer loads straight into LDS: a descriptor per column tile (its base
+  // is wave-uniform, so it lives in scalar registers) and fixed per-lane byte offsets,
+  // 32-bit since a tile spans 32 * dpad * 2 bytes. (The global-address form of the same
+  // load is counted by the compiler as a possible LDS access outstanding in lgkmcnt,
+  // which would turn every wait for a B fragment below into lgkmcnt(0).)
+  int voff[LOADS];
 #pragma unroll
   for (int h = 0; h < LOADS; ++h) {
     const int s = (h * RB_WAVES + wave) * 64 + lane;
     const int c = s / R::PIECES, qs = s % R::PIECES;
-    pb[h] = cols.X + (tile_at(0) * RA_BN + c) * dpad + (qs ^ (c & 15)) * 8;
+    voff[h] = (c * dpad + (qs ^ (c & 15)) * 8) * 2;
   }
-  // per tile, lanes 0-31 bring the 32 column scales and (symmetric sweep) lanes 32-63 the
-  // columns' reverse thresholds into the ring
-  const float* pc = ((SYM && lane >= 32) ? cols.thr0 : cols.scale) + tile_at(0) * RA_BN + (lane & 31);
-  int64_t gi = 0, tcur = 0;  // walk index of the tile being issued
+  // per tile, the even lanes bring the 32 column scales and the odd lanes (symmetric
+  // sweep) the columns' reverse thresholds into the ring, so column c's pair (scale,
+  // threshold) sits in 8 consecutive bytes; without thresholds both lanes load the scale
+  const int soff = SYM ? lane * 4 : (lane >> 1) * 4;
+  const int tile_bytes = RA_BN * dpad * 2;
+  int gi = 0, tcur = 0;  // walk index of the tile being issued
+  int tabs = tile_at(0);  // its absolute column tile
   int kci = 0, sloti = 0;
-  const int64_t tile_step = (int64_t)RA_BN * dpad - (int64_t)(NKC - 1) * KC;
   auto issue_next = [&]() {
     char* st = ring + sloti * R::STB;
 #pragma unroll
-    for (int h = 0; h < LOADS; ++h) {
-      const uint16_t* gp = pb[h];  // the builtin rejects the captured element itself
-      __builtin_amdgcn_global_load_lds(gp, (lds_void*)(st + (h * RB_WAVES + wave) * 1024), 16, 0, 0);
-    }
-    {
-      const float* cp = pc;
-      __builtin_amdgcn_global_load_lds(cp, (lds_void*)(st + R::STB_B + wave * 256), 4, 0, 0);
-    }
+    for (int h = 0; h < LOADS; ++h)
+      buffer_load_lds16(cols.X + (int64_t)tabs * (RA_BN * dpad), tile_bytes,
+                        st + (h * RB_WAVES + wave) * 1024, voff[h], kci * KC * 2);
+    buffer_load_lds4(SYM ? cols.kt + (int64_t)tabs * (2 * RA_BN) : cols.scale + (int64_t)tabs * RA_BN,
+                     SYM ? 8 * RA_BN : 4 * RA_BN, st + R::STB_B + wave * 256, soff);
+    // past the last stage the tile stays put: the tail reloads the last stage into a slot
+    // no later chunk reads, so every chunk issues the same loads and the waits are static
     if (gi + 1 < total) {
       if (++kci == NKC) {
         kci = 0;
-#pragma unroll
-        for (int h = 0; h < LOADS; ++h) pb[h] += tile_step;
-        pc += RA_BN;
-        if (SYM) {  // jumps of the walk: the range end back to its start, the last tile to 0
-          const int64_t from = tile_at(tcur);
-          ++tcur;
-          const int64_t jump = tile_at(tcur) - (from + 1);
-          if (jump != 0) {
-#pragma unroll
-            for (int h = 0; h < LOADS; ++h) pb[h] += jump * RA_BN * dpad;
-            pc += jump * RA_BN;
-          }
-        }
-      } else {
-#pragma unroll
-        for (int h = 0; h < LOADS; ++h) pb[h] += KC;
+        ++tcur;
+        tabs = SYM ? tile_at(tcur) : tcur;
       }
     }
     ++gi;
@@ -980,68 +991,99 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
 
   const int lcol = lane & 31, half = lcol >> 4, grp = lane >> 4;
   const int lrow = (lane >> 5) * 4;
+  // B piece q = 2*kk + (lane >> 5) of column lcol sits at slot lcol*PIECES + (q ^ (lcol &
+  // 15)) = lcol*PIECES + (2*kk ^ bm)
+  const int bcol = lcol * R::PIECES * 16, bm = (lane >> 5) ^ (lcol & 15);
+  // The ring's barrier sits after each stage's MFMAs, before the tile epilogue: a
+  // wavefront passes it once its own loads of the next stage have landed and every read
+  // of this stage has completed (each was waited for before the MFMA that used it). So
+  // past the barrier the next stage is whole, the slot of the stage just read may be
+  // refilled, and a wavefront still in its epilogue delays the others only when it is
+  // more than a stage behind.
   for (int g = 0; g < NST - 1; ++g) issue_next();
-  for (int64_t ct = 0; ct < nt; ++ct) {
+  wait_vm<(LOADS + 1) * (NST - 2)>();  // this wavefront's loads of stage 0 have landed
+  ring_barrier();
+  for (int ct = 0; ct < nt; ++ct) {
     f32x16 acc = (f32x16){};
+    float2 csv;  // the column's key scale and (symmetric sweep) reverse threshold
 #pragma unroll
     for (int kc = 0; kc < NKC; ++kc) {
-      const int64_t g = ct * NKC + kc;
-      wait_vm<(LOADS + 1) * (NST - 2)>();
-      ring_barrier();
-      // B piece q = 2*kk + (lane >> 5) of column lcol sits at slot lcol*PIECES + (q ^ (lcol &
-      // 15)) = lcol*PIECES + (2*kk ^ m); the per-lane parts are made opaque per chunk so the
-      // 16 addresses are formed in the loop rather than held in registers across it
-      int bcol = lcol * R::PIECES * 16, bm = (lane >> 5) ^ (lcol & 15);
-      asm volatile("" : "+v"(bcol), "+v"(bm));
+      const int g = ct * NKC + kc;
       const char* st = ring + (int)(g & (NST - 1)) * R::STB + bcol;
+      // RB_LA reads in flight ahead of the MFMAs: the first RB_LA fragments at once, then
+      // each next one as soon as an MFMA frees its register
+      bf16x8 bq[RB_LA];
 #pragma unroll
-      for (int k0 = 0; k0 < KC / 16; k0 += RB_BGRP) {
-        bf16x8 bq[RB_BGRP];
+      for (int u = 0; u < RB_LA; ++u) bq[u] = *reinterpret_cast<const bf16x8*>(st + ((2 * u) ^ bm) * 16);
 #pragma unroll
-        for (int u = 0; u < RB_BGRP; ++u)
-          bq[u] = *reinterpret_cast<const bf16x8*>(st + ((2 * (k0 + u)) ^ bm) * 16);
-#pragma unroll
-        for (int u = 0; u < RB_BGRP; ++u) {
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afr[kc * (KC / 16) + k0 + u], bq[u], acc,
-                                                        0, 0, 0);
-          if (k0 + u == 0) issue_next();
-        }
+      for (int kk = 0; kk < KC / 16; ++kk) {
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afr[kc * (KC / 16) + kk], bq[kk % RB_LA],
+                                                      acc, 0, 0, 0);
+        if (kk == 0) issue_next();  // stage g + NST - 1, into the slot of stage g - 1
+        if (kk + RB_LA < KC / 16)
+          bq[kk % RB_LA] = *reinterpret_cast<const bf16x8*>(st + ((2 * (kk + RB_LA)) ^ bm) * 16);
       }
+      // order: the first reads, then MFMA and read alternating, then the last MFMAs
+      __builtin_amdgcn_sched_group_barrier(0x100, RB_LA, 0);
+#pragma unroll
+      for (int kk = 0; kk + RB_LA < KC / 16; ++kk) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, RB_LA, 0);
+      if (kc == NKC - 1) {
+        // the column's scale and threshold, from the tile's last stage before its barrier
+        // (past it the slot may be refilled); written as asm with its own wait, since the
+        // compiler would wait for every load in flight before a plain read of the ring
+        const unsigned ca = lds_addr(ring + (int)(g & (NST - 1)) * R::STB + R::STB_B + wave * 256 +
+                                     lcol * 8);
+        asm volatile("ds_read_b64 %0, %1\ns_waitcnt lgkmcnt(0)" : "=v"(csv) : "v"(ca) : "memory");
+      }
+      wait_vm<(LOADS + 1) * (NST - 2)>();  // own loads of stage g + 1 landed
+      ring_barrier();
     }
 
     // ---------------- wavefront-private epilogue of column tile ct
-    const int64_t cta = SYM ? tile_at(ct) : ct;  // absolute column tile
+    const int cta = SYM ? tile_at(ct) : ct;  // absolute column tile
     const bool diag = cols.self && (cta * RA_BN < row0 + RB_ROWS) && (cta * RA_BN + RA_BN > row0);
     const bool tail = (cta + 1) * RA_BN > cols.ncol;
+    const float cs = csv.x;
+    float t0c = csv.y;
     if (SYM && (cta >> 3) != bid) {
+      // The largest of this lane's 16 dots (and of each group of four rows). Keys fall as
+      // dots rise (-(s c) for cosine with c >= 0, c - 2 s for squared Euclidean) and f32
+      // rounding is monotone, so every key formed from a dot is at least the key formed
+      // from the maximum.
+      float gm[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        gm[j] = fmaxf(fmaxf(acc[4 * j], acc[4 * j + 1]), fmaxf(acc[4 * j + 2], acc[4 * j + 3]));
+      const float smax = fmaxf(fmaxf(gm[0], gm[1]), fmaxf(gm[2], gm[3]));
       // Transposed keys: column cta*32 + lcol is a row too, and this wavefront's rows are
       // its candidates; a (row, column) pair is queued when its key, formed with the row's
-      // scale, beats the column's threshold. For cosine keys a bound filters first: for
-      // dots s <= ms and any row scale r in [rs_min, rs_max], -(s r) >= -max(ms rs_min,
-      // ms rs_max) (f32 rounding is monotone, so it holds as computed), and a group of four
-      // rows whose bound clears the threshold skips the exact test. The queue keeps the
-      // raw dot; the key is formed again, identically, when it is scattered.
-      float t0c;
-      {
-        const unsigned ca = lds_addr(ring + (int)((ct * NKC + NKC - 1) & (NST - 1)) * R::STB +
-                                     R::STB_B + wave * 256 + (32 + lcol) * 4);
-        asm volatile("ds_read_b32 %0, %1\ns_waitcnt lgkmcnt(0)" : "=v"(t0c) : "v"(ca) : "memory");
-      }
+      // scale, beats the column's threshold. Bounds filter first: for dots s <= ms and any
+      // row scale r in [rs_min, rs_max], -(s r) >= -max(ms rs_min, ms rs_max) and
+      // r - 2 s >= rs_min - 2 ms (f32 rounding is monotone, so they hold as computed): the
+      // whole tile with ms = smax, then, for cosine keys, each group of four rows. The
+      // queue keeps the raw dot; the key is formed again, identically, when it is
+      // scattered.
       const int cabs = (int)(cta * RA_BN) + lcol;
       if (!((int64_t)cabs < n)) t0c = -FINF;
-      unsigned m16 = 0xffffu;
-      if (COS) {  // per row group of 4 slots: the bound at the group's largest dot
-        m16 = 0;
+      const float rkmin = COS ? -fmaxf(smax * rs_min, smax * rs_max) : (rs_min - 2.0f * smax);
+      unsigned m16 = 0u;
+      if (__ballot(rkmin < t0c) != 0) {
+        m16 = 0xffffu;
+        if (COS) {  // per row group of 4 slots: the bound at the group's largest dot
+          m16 = 0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const float ms = fmaxf(fmaxf(acc[4 * j], acc[4 * j + 1]), fmaxf(acc[4 * j + 2], acc[4 * j + 3]));
-          m16 |= (-fmaxf(ms * rs_min, ms * rs_max) < t0c ? 15u : 0u) << (4 * j);
+          for (int j = 0; j < 4; ++j)
+            m16 |= (-fmaxf(gm[j] * rs_min, gm[j] * rs_max) < t0c ? 15u : 0u) << (4 * j);
         }
-      }
-      if (row0 + RB_ROWS > n) {  // the last block: rows past n are padding
+        if (row0 + RB_ROWS > n) {  // the last block: rows past n are padding
 #pragma unroll
-        for (int e = 0; e < 16; ++e)
-          if (!(row0 + wave * 32 + (e >> 2) * 8 + lrow + (e & 3) < n)) m16 &= ~(1u << e);
+          for (int e = 0; e < 16; ++e)
+            if (!(row0 + wave * 32 + (e >> 2) * 8 + lrow + (e & 3) < n)) m16 &= ~(1u << e);
+        }
       }
       if (__ballot(m16 != 0) != 0) {
         const int rbase = (int)(row0 + wave * 32) + lrow;
@@ -1078,21 +1120,20 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
         }
       }
     }
+    // Forward keys, each against its own (row, half) threshold (a bound over several rows
+    // lets almost every tile through on clustered data: some row of any 16 has a loose
+    // threshold while another row's cluster-mates give large dots).
     f32x4 tq[4];  // thresholds of this lane's 16 rows in its column half
-    float cs;
     {
       const unsigned ta = lds_addr(thr + 32 * half + lrow);
-      const unsigned ca = lds_addr(ring + (int)((ct * NKC + NKC - 1) & (NST - 1)) * R::STB +
-                                   R::STB_B + wave * 256 + lcol * 4);
       asm volatile(
-          "ds_read_b128 %0, %5\n"
-          "ds_read_b128 %1, %5 offset:32\n"
-          "ds_read_b128 %2, %5 offset:64\n"
-          "ds_read_b128 %3, %5 offset:96\n"
-          "ds_read_b32 %4, %6\n"
+          "ds_read_b128 %0, %4\n"
+          "ds_read_b128 %1, %4 offset:32\n"
+          "ds_read_b128 %2, %4 offset:64\n"
+          "ds_read_b128 %3, %4 offset:96\n"
           "s_waitcnt lgkmcnt(0)"
-          : "=v"(tq[0]), "=v"(tq[1]), "=v"(tq[2]), "=v"(tq[3]), "=v"(cs)
-          : "v"(ta), "v"(ca)
+          : "=v"(tq[0]), "=v"(tq[1]), "=v"(tq[2]), "=v"(tq[3])
+          : "v"(ta)
           : "memory");
     }
     int dv = (int)(cta * RA_BN - row0) - wave * 32 + lcol - lrow;
@@ -1108,6 +1149,12 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
 #pragma unroll
       for (int e = 0; e < 16; ++e)
         if (rem <= 0 || dv == (e >> 2) * 8 + (e & 3)) acc[e] = FINF;
+    }
+    {
+      unsigned long long anyp = 0;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) anyp |= __ballot(acc[e] < tq[e >> 2][e & 3]);
+      if (anyp == 0) continue;  // nothing in this tile enters a list
     }
     const int cbase = (int)(cta * RA_BN);
     bool touched = false, overflow = false;
@@ -1208,7 +1255,7 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     // still to come read the smaller value (or, from a stale cache line, the old one:
     // every value a threshold takes is an upper bound the merge accounts for).
     const float fwdmax = fmaxf(lk[C - 1], __shfl_xor(lk[C - 1], 32, 64));
-    if (lane < 32 && grow < n && fwdmax < cols.thr0[grow]) cols.thr0[grow] = fwdmax;
+    if (lane < 32 && grow < n && fwdmax < cols.kt[2 * grow + 1]) cols.kt[2 * grow + 1] = fwdmax;
   }
   if (grow < n) {
 #pragma unroll
@@ -1258,13 +1305,14 @@ __global__ void rb_sample_kernel(const uint16_t* __restrict__ Xh, const float* _
 // the transposed keys, S the sample size).
 constexpr int RB_THR_RANK = 6;
 static_assert(RB_THR_RANK <= RB_PASS_C, "the pass lists must hold the threshold rank");
-__global__ void rb_thr0_kernel(const float* __restrict__ ckey, int64_t n, int64_t npad,
-                               float* __restrict__ thr0) {
+__global__ void rb_thr0_kernel(const float* __restrict__ ckey, const float* __restrict__ colscale,
+                               int64_t n, int64_t npad, float* __restrict__ kt) {
   constexpr int L = RB_PASS_C;
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= npad) return;
+  kt[2 * r] = colscale[r];
   if (r >= n) {
-    thr0[r] = -FINF;
+    kt[2 * r + 1] = -FINF;
     return;
   }
   const float* a = ckey + (r * 2) * L;
@@ -1281,7 +1329,7 @@ __global__ void rb_thr0_kernel(const float* __restrict__ ckey, int64_t n, int64_
       ++ib;
     }
   }
-  thr0[r] = v;
+  kt[2 * r + 1] = v;
 }
 // Largest forward key of every row (+inf when its lists are not full): a reverse
 // candidate above it is beaten by all 32 forward candidates and cannot be kept.
@@ -1337,7 +1385,7 @@ __global__ __launch_bounds__(256) void rb_scatter_kernel(const int32_t* __restri
 constexpr int RB_RCAP = 768;
 __global__ __launch_bounds__(256) void rb_merge_kernel(
     const float* __restrict__ ckey, const int32_t* __restrict__ cidx, const int32_t* __restrict__ rcnt,
-    const float* __restrict__ rkey, const int32_t* __restrict__ ridx, const float* __restrict__ thr0,
+    const float* __restrict__ rkey, const int32_t* __restrict__ ridx, const float* __restrict__ kt,
     const int32_t* __restrict__ dropped, int64_t n, float* __restrict__ mkey,
     int32_t* __restrict__ midx, float* __restrict__ bound) {
   __shared__ float sk[4][32 + RB_RCAP];
@@ -1392,7 +1440,7 @@ __global__ __launch_bounds__(256) void rb_merge_kernel(
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) key31 = fminf(key31, __shfl_xor(key31, o, 64));
   if (lane == 0) {
-    float b = fminf(fminf(ckey[(row * 2) * 16 + 15], ckey[(row * 2 + 1) * 16 + 15]), thr0[row]);
+    float b = fminf(fminf(ckey[(row * 2) * 16 + 15], ckey[(row * 2 + 1) * 16 + 15]), kt[2 * row + 1]);
     if (nvalid > 32 || dropped[row]) b = fminf(b, key31);  // some candidate was ranked out
     if (cnt > RB_RCAP) b = -FINF;
     bound[row] = b;
@@ -2020,7 +2068,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   }
   int64_t sym_S = 0, sym_Spad = 0, sym_qcap = 0;
   Buf<uint16_t> Xs;
-  Buf<float> ss, thr0, qk;
+  Buf<float> ss, kt, qk;
   Buf<int32_t> qj, qr, qcount;
   if (sym) {
     sym_S = std::min<int64_t>(65536, std::max<int64_t>(256, n / 64 / 256 * 256));
@@ -2028,7 +2076,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     sym_Spad = ceil_div(sym_S, RA_BN) * RA_BN;
     Xs.alloc(pool, sym_Spad * dpad);
     ss.alloc(pool, sym_Spad);
-    thr0.alloc(pool, npad);
+    kt.alloc(pool, 2 * npad);
     rb_sample_kernel<<<(unsigned)ceil_div(sym_Spad * (dpad / 8), 256), 256, 0, s>>>(
         Xh.p, colscale.p, n, dpad, sym_S, sym_Spad, Xs.p, ss.p);
     // about 32 rows x n/2 columns x RB_THR_RANK/S reverse keys per wavefront; three times that
@@ -2072,9 +2120,10 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
         else
           knn_topc_rb_kernel<2, 256, false, false, RB_PASS_C><<<grid, 512, RbRing<256>::SMEM, s>>>(
               Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, sc);
-        rb_thr0_kernel<<<(unsigned)ceil_div(npad, 256), 256, 0, s>>>(ckey.p, n, npad, thr0.p);
+        rb_thr0_kernel<<<(unsigned)ceil_div(npad, 256), 256, 0, s>>>(ckey.p, colscale.p, n, npad,
+                                                                    kt.p);
         EVOC_HIP(hipEventRecord(e0, s));
-        cols.thr0 = thr0.p;
+        cols.kt = kt.p;
         cols.qj = qj.p;
         cols.qr = qr.p;
         cols.qk = qk.p;
@@ -2164,7 +2213,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       midx.alloc(pool, n * 32);
       bnd.alloc(pool, n);
       rb_merge_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(ckey.p, cidx.p, rcnt.p, rkey.p,
-                                                              ridx.p, thr0.p, dropped.p, n,
+                                                              ridx.p, kt.p, dropped.p, n,
                                                               mkey.p, midx.p, bnd.p);
       EVOC_HIP(hipGetLastError());
     }
