@@ -775,12 +775,13 @@ template __global__ void knn_topc_ra_kernel<32, 2, 256, true>(const uint16_t*, i
 // row has 32 candidates and its bound is the smaller of its two halves' last keys.
 constexpr int RB_ROWS = 256;
 constexpr int RB_WAVES = 8;
-constexpr int RB_BKT = 16;     // bucket slots per (row, half)
-constexpr int RB_DRAIN = 8;    // drain once some (row, half) holds this many
+constexpr int RB_BKT = 8;      // bucket slots per (row, half)
+constexpr int RB_DRAIN = 4;    // drain once some (row, half) holds this many
 #ifndef RB_LA
 #define RB_LA 4        // B fragments read from LDS ahead of their MFMAs
 #endif
-constexpr int RB_WREG = 64 * RB_BKT * 8;                // per-wavefront buckets / key tile
+// per-wavefront buckets, or the key tile dumped over them after a bucket overflow
+constexpr int RB_WREG = 64 * RB_BKT * 8 > 32 * RA_SCS * 4 ? 64 * RB_BKT * 8 : 32 * RA_SCS * 4;
 static_assert(RB_WREG >= 32 * RA_SCS * 4, "the key-tile dump must fit the bucket region");
 constexpr int RB_EPI_BYTES = RB_WAVES * RB_WREG + RB_WAVES * 64 * 4 + RB_WAVES * 32 * 4;
 
@@ -810,7 +811,7 @@ template <int KC>
 struct RbRing {
   static constexpr int STB_B = RA_BN * KC * 2;              // 16 KB at KC = 256
   static constexpr int STB = STB_B + RB_WAVES * 256;        // + column scales per wave
-  static constexpr int NST = 4;
+  static constexpr int NST = KC >= 512 ? 3 : 4;              // 32 KB stages: three fit
   static constexpr int PIECES = KC / 8;
   static constexpr int LOADS = STB_B / (64 * RB_WAVES * 16);  // 16-byte loads per thread
   static constexpr size_t SMEM = (size_t)NST * STB;
@@ -820,6 +821,7 @@ struct RbRing {
 // and copies them to its global queue in batches, with no vector-memory operation of the
 // ring in flight (stores mixed with the ring's loads would void its counted waits)
 constexpr int RB_PASS_C = 8;  // candidates per (row, half) kept by the threshold pass
+constexpr int RB_KC = 512, RB_NKC = 1;  // d <= 512: one stage holds all of d
 // buffer descriptor word 3 for the ring's raw loads (gfx9: 32-bit data format, no swizzle)
 constexpr int RB_RSRC_FLAGS = 0x00020000;
 // Raw buffer loads straight into LDS (each lane's 16 or 4 bytes land at lds + lane * size):
@@ -986,7 +988,7 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
       }
     }
     ++gi;
-    sloti = (sloti + 1) & (NST - 1);
+    sloti = sloti + 1 == NST ? 0 : sloti + 1;
   };
 
   const int lcol = lane & 31, half = lcol >> 4, grp = lane >> 4;
@@ -1003,13 +1005,13 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   for (int g = 0; g < NST - 1; ++g) issue_next();
   wait_vm<(LOADS + 1) * (NST - 2)>();  // this wavefront's loads of stage 0 have landed
   ring_barrier();
+  int slotc = 0;  // ring slot of the stage being read
   for (int ct = 0; ct < nt; ++ct) {
     f32x16 acc = (f32x16){};
     float2 csv;  // the column's key scale and (symmetric sweep) reverse threshold
 #pragma unroll
     for (int kc = 0; kc < NKC; ++kc) {
-      const int g = ct * NKC + kc;
-      const char* st = ring + (int)(g & (NST - 1)) * R::STB + bcol;
+      const char* st = ring + slotc * R::STB + bcol;
       // RB_LA reads in flight ahead of the MFMAs: the first RB_LA fragments at once, then
       // each next one as soon as an MFMA frees its register
       bf16x8 bq[RB_LA];
@@ -1035,12 +1037,12 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
         // the column's scale and threshold, from the tile's last stage before its barrier
         // (past it the slot may be refilled); written as asm with its own wait, since the
         // compiler would wait for every load in flight before a plain read of the ring
-        const unsigned ca = lds_addr(ring + (int)(g & (NST - 1)) * R::STB + R::STB_B + wave * 256 +
-                                     lcol * 8);
+        const unsigned ca = lds_addr(ring + slotc * R::STB + R::STB_B + wave * 256 + lcol * 8);
         asm volatile("ds_read_b64 %0, %1\ns_waitcnt lgkmcnt(0)" : "=v"(csv) : "v"(ca) : "memory");
       }
       wait_vm<(LOADS + 1) * (NST - 2)>();  // own loads of stage g + 1 landed
       ring_barrier();
+      slotc = slotc + 1 == NST ? 0 : slotc + 1;
     }
 
     // ---------------- wavefront-private epilogue of column tile ct
@@ -1050,15 +1052,17 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     const float cs = csv.x;
     float t0c = csv.y;
     if (SYM && (cta >> 3) != bid) {
-      // The largest of this lane's 16 dots (and of each group of four rows). Keys fall as
+      // The largest of this lane's 16 dots (and, below, of each group of four rows). Keys fall as
       // dots rise (-(s c) for cosine with c >= 0, c - 2 s for squared Euclidean) and f32
       // rounding is monotone, so every key formed from a dot is at least the key formed
       // from the maximum.
-      float gm[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        gm[j] = fmaxf(fmaxf(acc[4 * j], acc[4 * j + 1]), fmaxf(acc[4 * j + 2], acc[4 * j + 3]));
-      const float smax = fmaxf(fmaxf(gm[0], gm[1]), fmaxf(gm[2], gm[3]));
+      float smax;
+      {
+        const float m0 = fmaxf(fmaxf(acc[0], acc[1]), acc[2]), m1 = fmaxf(fmaxf(acc[3], acc[4]), acc[5]);
+        const float m2 = fmaxf(fmaxf(acc[6], acc[7]), acc[8]), m3 = fmaxf(fmaxf(acc[9], acc[10]), acc[11]);
+        const float m4 = fmaxf(fmaxf(acc[12], acc[13]), acc[14]);
+        smax = fmaxf(fmaxf(m0, m1), fmaxf(fmaxf(m2, m3), fmaxf(m4, acc[15])));
+      }
       // Transposed keys: column cta*32 + lcol is a row too, and this wavefront's rows are
       // its candidates; a (row, column) pair is queued when its key, formed with the row's
       // scale, beats the column's threshold. Bounds filter first: for dots s <= ms and any
@@ -1076,8 +1080,10 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
         if (COS) {  // per row group of 4 slots: the bound at the group's largest dot
           m16 = 0;
 #pragma unroll
-          for (int j = 0; j < 4; ++j)
-            m16 |= (-fmaxf(gm[j] * rs_min, gm[j] * rs_max) < t0c ? 15u : 0u) << (4 * j);
+          for (int j = 0; j < 4; ++j) {
+            const float gm = fmaxf(fmaxf(acc[4 * j], acc[4 * j + 1]), fmaxf(acc[4 * j + 2], acc[4 * j + 3]));
+            m16 |= (-fmaxf(gm * rs_min, gm * rs_max) < t0c ? 15u : 0u) << (4 * j);
+          }
         }
         if (row0 + RB_ROWS > n) {  // the last block: rows past n are padding
 #pragma unroll
@@ -1123,19 +1129,9 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     // Forward keys, each against its own (row, half) threshold (a bound over several rows
     // lets almost every tile through on clustered data: some row of any 16 has a loose
     // threshold while another row's cluster-mates give large dots).
-    f32x4 tq[4];  // thresholds of this lane's 16 rows in its column half
-    {
-      const unsigned ta = lds_addr(thr + 32 * half + lrow);
-      asm volatile(
-          "ds_read_b128 %0, %4\n"
-          "ds_read_b128 %1, %4 offset:32\n"
-          "ds_read_b128 %2, %4 offset:64\n"
-          "ds_read_b128 %3, %4 offset:96\n"
-          "s_waitcnt lgkmcnt(0)"
-          : "=v"(tq[0]), "=v"(tq[1]), "=v"(tq[2]), "=v"(tq[3])
-          : "v"(ta)
-          : "memory");
-    }
+    f32x4 tq[4];  // thresholds of this lane's 16 rows in its column half (changed only at drains)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) tq[j] = reinterpret_cast<const f32x4*>(thr + 32 * half + lrow)[2 * j];
     int dv = (int)(cta * RA_BN - row0) - wave * 32 + lcol - lrow;
     int rem = (int)std::min<int64_t>(cols.ncol - cta * RA_BN, RA_BN) - lcol;
     int sb = lrow * RA_SCS + lcol;
@@ -1266,20 +1262,20 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   }
 }
 
-template __global__ void knn_topc_rb_kernel<2, 256, true, false, RB_PASS_C>(
+template __global__ void knn_topc_rb_kernel<RB_NKC, RB_KC, true, false, RB_PASS_C>(
     const uint16_t*, int64_t, int64_t, int, const float*, int32_t*, float*, const RbCols);
-template __global__ void knn_topc_rb_kernel<2, 256, false, false, RB_PASS_C>(
+template __global__ void knn_topc_rb_kernel<RB_NKC, RB_KC, false, false, RB_PASS_C>(
     const uint16_t*, int64_t, int64_t, int, const float*, int32_t*, float*, const RbCols);
-template __global__ void knn_topc_rb_kernel<2, 256, true, false>(const uint16_t*, int64_t, int64_t,
+template __global__ void knn_topc_rb_kernel<RB_NKC, RB_KC, true, false>(const uint16_t*, int64_t, int64_t,
                                                                  int, const float*, int32_t*,
                                                                  float*, const RbCols);
-template __global__ void knn_topc_rb_kernel<2, 256, false, false>(const uint16_t*, int64_t, int64_t,
+template __global__ void knn_topc_rb_kernel<RB_NKC, RB_KC, false, false>(const uint16_t*, int64_t, int64_t,
                                                                   int, const float*, int32_t*,
                                                                   float*, const RbCols);
-template __global__ void knn_topc_rb_kernel<2, 256, true, true>(const uint16_t*, int64_t, int64_t,
+template __global__ void knn_topc_rb_kernel<RB_NKC, RB_KC, true, true>(const uint16_t*, int64_t, int64_t,
                                                                 int, const float*, int32_t*,
                                                                 float*, const RbCols);
-template __global__ void knn_topc_rb_kernel<2, 256, false, true>(const uint16_t*, int64_t, int64_t,
+template __global__ void knn_topc_rb_kernel<RB_NKC, RB_KC, false, true>(const uint16_t*, int64_t, int64_t,
                                                                  int, const float*, int32_t*,
                                                                  float*, const RbCols);
 
@@ -1987,20 +1983,20 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<256>::SMEM));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 2, 256, true>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<256>::SMEM));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, true, false>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<256>::SMEM));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, true, false, RB_PASS_C>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<256>::SMEM));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, false, false, RB_PASS_C>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<256>::SMEM));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, false, false>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<256>::SMEM));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, true, true>,
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<RB_NKC, RB_KC, true, false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<RB_KC>::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<RB_NKC, RB_KC, true, false, RB_PASS_C>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<RB_KC>::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<RB_NKC, RB_KC, false, false, RB_PASS_C>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<RB_KC>::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<RB_NKC, RB_KC, false, false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RbRing<RB_KC>::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<RB_NKC, RB_KC, true, true>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(RbRing<256>::SMEM + RB_QBYTES)));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<2, 256, false, true>,
+                                 (int)(RbRing<RB_KC>::SMEM + RB_QBYTES)));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<RB_NKC, RB_KC, false, true>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(RbRing<256>::SMEM + RB_QBYTES)));
+                                 (int)(RbRing<RB_KC>::SMEM + RB_QBYTES)));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_exact_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
     attr_set = true;
@@ -2115,10 +2111,10 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
         RbCols sc{Xs.p, ss.p, nullptr, sym_S, sym_Spad / RA_BN, 0, nullptr, nullptr, nullptr, 0,
                   nullptr};
         if (metric == 1)
-          knn_topc_rb_kernel<2, 256, true, false, RB_PASS_C><<<grid, 512, RbRing<256>::SMEM, s>>>(
+          knn_topc_rb_kernel<RB_NKC, RB_KC, true, false, RB_PASS_C><<<grid, 512, RbRing<RB_KC>::SMEM, s>>>(
               Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, sc);
         else
-          knn_topc_rb_kernel<2, 256, false, false, RB_PASS_C><<<grid, 512, RbRing<256>::SMEM, s>>>(
+          knn_topc_rb_kernel<RB_NKC, RB_KC, false, false, RB_PASS_C><<<grid, 512, RbRing<RB_KC>::SMEM, s>>>(
               Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, sc);
         rb_thr0_kernel<<<(unsigned)ceil_div(npad, 256), 256, 0, s>>>(ckey.p, colscale.p, n, npad,
                                                                     kt.p);
@@ -2131,17 +2127,17 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
         cols.qcount = qcount.p;
         cols.group = cu_count();
         if (metric == 1)
-          knn_topc_rb_kernel<2, 256, true, true><<<grid, 512, RbRing<256>::SMEM + RB_QBYTES, s>>>(
+          knn_topc_rb_kernel<RB_NKC, RB_KC, true, true><<<grid, 512, RbRing<RB_KC>::SMEM + RB_QBYTES, s>>>(
               Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
         else
-          knn_topc_rb_kernel<2, 256, false, true><<<grid, 512, RbRing<256>::SMEM + RB_QBYTES, s>>>(
+          knn_topc_rb_kernel<RB_NKC, RB_KC, false, true><<<grid, 512, RbRing<RB_KC>::SMEM + RB_QBYTES, s>>>(
               Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
       } else {
         if (metric == 1)
-          knn_topc_rb_kernel<2, 256, true, false><<<grid, 512, RbRing<256>::SMEM, s>>>(
+          knn_topc_rb_kernel<RB_NKC, RB_KC, true, false><<<grid, 512, RbRing<RB_KC>::SMEM, s>>>(
               Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
         else
-          knn_topc_rb_kernel<2, 256, false, false><<<grid, 512, RbRing<256>::SMEM, s>>>(
+          knn_topc_rb_kernel<RB_NKC, RB_KC, false, false><<<grid, 512, RbRing<RB_KC>::SMEM, s>>>(
               Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
       }
     }
@@ -2189,10 +2185,10 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
                   nullptr};
       const unsigned grid = (unsigned)(npad / RA_ROWS);
       if (metric == 1)
-        knn_topc_rb_kernel<2, 256, true, false><<<grid, 512, RbRing<256>::SMEM, s>>>(
+        knn_topc_rb_kernel<RB_NKC, RB_KC, true, false><<<grid, 512, RbRing<RB_KC>::SMEM, s>>>(
             Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
       else
-        knn_topc_rb_kernel<2, 256, false, false><<<grid, 512, RbRing<256>::SMEM, s>>>(
+        knn_topc_rb_kernel<RB_NKC, RB_KC, false, false><<<grid, 512, RbRing<RB_KC>::SMEM, s>>>(
             Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
       EVOC_HIP(hipGetLastError());
     } else {
