@@ -151,6 +151,35 @@ int evoc_hip_knn(const float* x, int64_t n, int64_t d, int32_t metric, int32_t k
   });
 }
 
+int evoc_hip_knn_tile_pass(const float* x, int64_t n, int64_t d, int32_t metric, int32_t k,
+                           int32_t precision, int32_t* cand_idx, float* cand_key,
+                           float* key_scale, int32_t* ncand, float* gamma) {
+  return guarded([&] {
+    EVOC_CHECK(x && cand_idx && cand_key && key_scale, "knn_tile_pass: NULL buffer");
+    EVOC_CHECK(n >= 1 && d >= 1, "knn_tile_pass: n and d must be >= 1");
+    EVOC_CHECK(metric == 0 || metric == 1, "knn_tile_pass: unknown metric");
+    EVOC_CHECK(k >= 1 && k <= 16, "knn_tile_pass: k must be in [1, 16]");
+    EVOC_CHECK(precision == EVOC_KNN_SPLIT || precision == EVOC_KNN_FP8,
+               "knn_tile_pass: precision must be EVOC_KNN_SPLIT or EVOC_KNN_FP8");
+    check_finite_host(x, n * d, "input");
+    Ctx& c = context(0);
+    Buf<float> dx(c.pool, n * d);
+    Buf<int32_t> di(c.pool, n * k);
+    Buf<float> dd(c.pool, n * k);
+    EVOC_HIP(hipMemcpyAsync(dx.p, x, n * d * sizeof(float), hipMemcpyHostToDevice, c.stream));
+    KnnStats st;
+    KnnCapture cap;
+    knn_device(dx.p, n, (int)d, d, metric, k, precision, di.p, dd.p, &st, c.stream, c.pool, 0,
+               &cap);
+    EVOC_CHECK(cap.C == 32 && (int64_t)cap.idx.size() == n * 32, "knn_tile_pass: no capture");
+    std::memcpy(cand_idx, cap.idx.data(), cap.idx.size() * sizeof(int32_t));
+    std::memcpy(cand_key, cap.key.data(), cap.key.size() * sizeof(float));
+    std::memcpy(key_scale, cap.scale.data(), n * sizeof(float));
+    if (ncand) *ncand = cap.C;
+    if (gamma) *gamma = cap.gamma;
+  });
+}
+
 int evoc_hip_knn_last_stats(double* kernel_flop, double* kernel_ms, int32_t* symmetric,
                             int64_t* escalated_rows) {
   return guarded([&] {

@@ -96,7 +96,7 @@ struct Buf {
 };
 
 // ------------------------------------------------------------------ kNN
-enum KnnPrecision { KNN_AUTO = 0, KNN_SPLIT = 1, KNN_BF16 = 2 };
+enum KnnPrecision { KNN_AUTO = 0, KNN_SPLIT = 1, KNN_BF16 = 2, KNN_FP8 = 3 };
 struct KnnStats {
   int split = 0;
   int fallback_rows = 0;
@@ -105,13 +105,24 @@ struct KnnStats {
   double flops = 0.0;
   int symmetric = 0;  // the symmetric sweep ran
   int escalated_rows = 0;  // rows searched again with 64 candidates after a failed certificate
+  int first_pass_failed = 0;  // rows whose first-pass certificate failed
+};
+// The first tile pass's raw candidate lists (two column halves of C/2 per row, keys in the
+// pass's own units) and the per-row key scales, copied to host memory when requested
+// (the stage entry point evoc_hip_knn_tile_pass, for tests of the tile kernels).
+struct KnnCapture {
+  std::vector<int32_t> idx;
+  std::vector<float> key;
+  std::vector<float> scale;
+  int C = 0;
+  float gamma = 0.0f;  // the certificate's relative dot error term of the pass
 };
 // the calling thread's last knn_device call
 KnnStats& last_knn_stats();
 // verbose > 0 prints the tile pass, the escalation and the exact search as they finish
 void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k, int precision,
                 int32_t* d_idx, float* d_dist, KnnStats* stats, hipStream_t s, DevPool& pool,
-                int verbose = 0);
+                int verbose = 0, KnnCapture* capture = nullptr);
 
 // ------------------------------------------------------------------ fuzzy set
 struct Graph {  // CSR, rows sorted by column when symmetrised

@@ -93,6 +93,58 @@ __global__ __launch_bounds__(256) void knn_prep_kernel(
   if (lane == 0 && maxbits) atomicMax(&flags[1], maxbits);
 }
 
+// FP8-e4m3 copy for the FP8 first pass (cosine): per row the scale s = max|x| / 448 maps
+// the largest magnitude onto e4m3's largest finite value; q = e4m3(x / s) rounded to
+// nearest even (hardware conversion, inputs clamped to +-448, subnormal results stored
+// as zero); the pass's key scale is
+// s / ||x||. The certificate needs the quantisation error: rho = ||x - s q|| / ||x|| per
+// row, formed in f64 from the decoded values (s q is exact in f64), rounded up and folded
+// into the maximum over rows (positive floats order as their bit patterns).
+__global__ __launch_bounds__(256) void knn_prep_fp8_kernel(
+    const float* __restrict__ X, int64_t n, int d, int64_t ld, int64_t npad, int dpad8,
+    const float* __restrict__ nrm, uint8_t* __restrict__ Xq, float* __restrict__ scale8,
+    unsigned* __restrict__ rho_bits) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * 4;
+  unsigned rmax = 0;
+  for (int64_t row = (int64_t)blockIdx.x * 4 + wave; row < npad; row += nwaves) {
+    float amax = 0.0f;
+    if (row < n)
+      for (int t = lane; t < d; t += 64) amax = fmaxf(amax, fabsf(X[row * ld + t]));
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
+    const float sc = amax > 0.0f ? amax / 448.0f : 0.0f;
+    const float inv = amax > 0.0f ? 448.0f / amax : 0.0f;
+    double e2 = 0.0, x2 = 0.0;
+    for (int t = lane; t < dpad8; t += 64) {
+      const float v = (row < n && t < d) ? X[row * ld + t] : 0.0f;
+      const float y = fminf(fmaxf(v * inv, -448.0f), 448.0f);
+      int pk = __builtin_amdgcn_cvt_pk_fp8_f32(y, 0.0f, 0, false);
+      float back = __builtin_amdgcn_cvt_f32_fp8(pk, 0);
+      if (fabsf(back) < 0.015625f) {  // subnormal e4m3 (below 2^-6) stored as zero, so the
+        pk = 0;                       // MFMA's reading of the bytes cannot differ from the
+        back = 0.0f;                  // values the error bound is formed from
+      }
+      const double r = (double)v - (double)sc * (double)back;
+      e2 += r * r;
+      x2 += (double)v * (double)v;
+      Xq[row * dpad8 + t] = (uint8_t)(pk & 0xff);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      e2 += __shfl_xor(e2, o, 64);
+      x2 += __shfl_xor(x2, o, 64);
+    }
+    if (lane == 0) {
+      float rho = 0.0f;
+      if (row < n && x2 > 0.0) rho = (float)(sqrt(e2 / x2) * (1.0 + 1e-6));
+      rmax = max(rmax, __float_as_uint(rho));
+      scale8[row] = (row < n && nrm[row] > 0.0f) ? sc / nrm[row] : 0.0f;
+    }
+  }
+  if (lane == 0 && rmax) atomicMax(rho_bits, rmax);
+}
+
 // ------------------------------------------------------------------ 2. MFMA top-C
 template <int C>
 __device__ __forceinline__ void list_insert(float (&lk)[C], int (&li)[C], float key, int idx) {
@@ -134,16 +186,22 @@ __device__ __forceinline__ void ring_barrier() {
   asm volatile("" ::: "memory");
 }
 
-template <bool SPLIT, int C>  // C = list length per column half
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+template <bool SPLIT, int C, bool F8 = false>  // C = list length per column half
 // Query rows are the rows of Xh / Xl themselves (self-join), or, with qmap, the rows of
 // Qh / Ql: nq gathered rows whose global indices qmap holds in ascending order (the
-// re-search of rows whose certificate failed, with longer lists).
+// re-search of rows whose certificate failed, with longer lists). F8: the rows are
+// FP8-e4m3 bytes (per-row scales folded into colscale), 64 per 64-byte chunk, multiplied
+// on v_mfma_scale_f32_32x32x64_f8f6f4 with unit scales and f32 accumulation.
 __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
     const uint16_t* __restrict__ Xh, const uint16_t* __restrict__ Xl, int64_t n, int64_t npad,
     int dpad, const float* __restrict__ colscale, int metric, int32_t* __restrict__ cand_idx,
     float* __restrict__ cand_key, const uint16_t* __restrict__ Qh,
     const uint16_t* __restrict__ Ql, const int32_t* __restrict__ qmap, int64_t nq, int64_t ct0,
     int64_t ct1) {
+  static_assert(!(F8 && SPLIT), "the FP8 pass has no split form");
   // column tiles [ct0, ct1) of BN columns are swept (a shard of the columns, or all of them)
   constexpr int NOPS = SPLIT ? 4 : 2;           // A hi, B hi, (A lo, B lo)
   constexpr int NST = SPLIT ? 2 : 4;            // stages in the LDS ring
@@ -170,7 +228,9 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
   if (tid < BM) qg[tid] = row0 + tid < nrows ? (qmap ? qmap[row0 + tid] : (int32_t)(row0 + tid)) : -1;
   const int64_t qlo = qmap ? qmap[row0] : row0;
   const int64_t qhi = qmap ? qmap[std::min<int64_t>(row0 + BM, nq) - 1] : row0 + BM - 1;
-  const int nck = dpad / KC;
+  // 64-byte chunks of a row: 32 bf16 or 64 fp8 values
+  const int64_t rowb = F8 ? (int64_t)dpad : 2 * (int64_t)dpad;  // bytes per row
+  const int nck = (int)(rowb / 64);
   const int64_t g0 = ct0 * nck;
   const int64_t total = ct1 * nck;
   const int orow = tid & (BM - 1), ohalf = tid >> 7;  // list owned by this thread
@@ -192,7 +252,8 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
     char* st = ring + (int)(g % NST) * STB;
 #pragma unroll
     for (int op = 0; op < NOPS; ++op) {
-      const uint16_t* src = (op & 1) ? ((op < 2) ? Xh : Xl) : ((op < 2) ? QH : QL);
+      const char* src = reinterpret_cast<const char*>((op & 1) ? ((op < 2) ? Xh : Xl)
+                                                              : ((op < 2) ? QH : QL));
       const int64_t rbase = (op & 1) ? ct * BN : row0;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -200,7 +261,7 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
         const int s = blk * 64 + lane;
         const int r = s >> 2, qs = s & 3;
         const int q = qs ^ ((r >> 2) & 3);
-        const uint16_t* gp = src + (rbase + r) * dpad + kc * KC + q * 8;
+        const char* gp = src + (rbase + r) * rowb + kc * 64 + q * 16;
         __builtin_amdgcn_global_load_lds(gp, (lds_void*)(st + op * OPB + blk * 1024), 16, 0, 0);
       }
     }
@@ -228,8 +289,35 @@ __global__ __launch_bounds__(256, 1) void knn_topc_kernel(
       for (int tn = 0; tn < 2; ++tn) cs[tn] = colscale[ct * BN + wn * 64 + tn * 32 + (lane & 31)];
     }
     const char* st = ring + (int)(g % NST) * STB;
+    if constexpr (F8) {
+      // lane l holds 32 of the chunk's 64 values (pieces 2h, 2h + 1, h = l >> 5) of row
+      // l & 31, for A and B alike, so each product pairs the same k
+      const int q0 = 2 * (lane >> 5);
+      v8i a[2], b[2];
 #pragma unroll
-    for (int kk = 0; kk < KC / 16; ++kk) {
+      for (int tm = 0; tm < 2; ++tm) {
+        const int r = wm * 64 + tm * 32 + (lane & 31);
+        const i32x4 lo = *reinterpret_cast<const i32x4*>(st + 0 * OPB + slot_of(r, q0) * 16);
+        const i32x4 hi = *reinterpret_cast<const i32x4*>(st + 0 * OPB + slot_of(r, q0 + 1) * 16);
+        a[tm] = (v8i){lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+      }
+#pragma unroll
+      for (int tn = 0; tn < 2; ++tn) {
+        const int r = wn * 64 + tn * 32 + (lane & 31);
+        const i32x4 lo = *reinterpret_cast<const i32x4*>(st + 1 * OPB + slot_of(r, q0) * 16);
+        const i32x4 hi = *reinterpret_cast<const i32x4*>(st + 1 * OPB + slot_of(r, q0 + 1) * 16);
+        b[tn] = (v8i){lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+      }
+      // formats 0/0: e4m3 for both operands; scales 127 (E8M0 for 2^0) leave them as is
+#pragma unroll
+      for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn)
+          acc[tm][tn] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[tm], b[tn], acc[tm][tn], 0, 0,
+                                                                       0, 127, 0, 127);
+    }
+#pragma unroll
+    for (int kk = 0; kk < (F8 ? 0 : KC / 16); ++kk) {
       const int q = kk * 2 + (lane >> 5);
       bf16x8 a[2], b[2];
 #pragma unroll
@@ -1955,8 +2043,11 @@ constexpr int KNN_ESC_SHARDS = 4;  // column shards of the escalation search
 
 void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k, int precision,
                 int32_t* d_idx, float* d_dist, KnnStats* stats, hipStream_t s, DevPool& pool,
-                int verbose) {
+                int verbose, KnnCapture* capture) {
   EVOC_CHECK(n >= 1 && d >= 1, "knn: empty input");
+  EVOC_CHECK(precision >= KNN_AUTO && precision <= KNN_FP8, "knn: unknown precision");
+  const bool fp8 = precision == KNN_FP8;
+  EVOC_CHECK(!fp8 || metric == 1, "knn: the fp8 precision supports the cosine metric only");
   EVOC_CHECK(k >= 1 && k <= KMAX, "knn: k must be in [1, 32]");
   EVOC_CHECK(d <= MAXT * 64, "knn: d must be <= 1024");
   static bool attr_set = false;
@@ -1970,6 +2061,10 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<false, 16>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<false, 32>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<false, 16, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_kernel<false, 32, true>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 144 * 1024));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_ra_kernel<32, 1, 128, false>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)RaRing<128>::SMEM));
@@ -2006,7 +2101,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   // halves of CH per row). The prep layout depends on the choice; an AUTO guess of
   // bf16-exact that the prep flags refute is prepared again in the split layout.
   const int CH = (k <= 16) ? 16 : 32;
-  const bool ra_possible = (precision != KNN_SPLIT) && d <= RA_MAXD && k <= 16;
+  const bool ra_possible = (precision != KNN_SPLIT) && !fp8 && d <= RA_MAXD && k <= 16;
   Buf<float> nrm, sqn, colscale;
   Buf<unsigned> flags(pool, 4);
   Buf<uint16_t> Xh, Xl;
@@ -2043,9 +2138,30 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       split = true;
     }
   } else {
+    // (the FP8 first pass is followed, for rows it cannot certify, by a bf16 search in
+    // the AUTO form: one product when the input is bf16-representable, else the split)
     const bool want_lo = precision != KNN_BF16;
     prep(false, want_lo);
-    split = want_lo && !(precision == KNN_AUTO && hflags[0] == 0);
+    split = want_lo && !((precision == KNN_AUTO || fp8) && hflags[0] == 0);
+  }
+  // FP8 copy, its key scales and the worst row's quantisation error
+  int dpad8 = 0;
+  Buf<uint8_t> Xq;
+  Buf<float> scale8;
+  float e_in8 = 0.0f;
+  if (fp8) {
+    dpad8 = (int)(ceil_div(d, 64) * 64);
+    Xq.alloc(pool, npad * dpad8);
+    scale8.alloc(pool, npad);
+    knn_prep_fp8_kernel<<<(unsigned)std::min<int64_t>(ceil_div(npad, 4), 4096), 256, 0, s>>>(
+        dX, n, d, ld, npad, dpad8, nrm.p, Xq.p, scale8.p, flags.p + 2);
+    EVOC_HIP(hipGetLastError());
+    unsigned rb = 0;
+    EVOC_HIP(hipMemcpyAsync(&rb, flags.p + 2, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipStreamSynchronize(s));
+    const float rho = __builtin_bit_cast(float, rb);
+    // |x.y - s_x s_y q_x.q_y| <= ||x|| ||e_y|| + ||e_x|| ||y|| + ||e_x|| ||e_y||
+    e_in8 = (2.0f * rho + rho * rho) * (1.0f + 1e-5f);
   }
   const int C = ra ? 32 : 2 * CH;
   Buf<int32_t> cidx(pool, n * C), frows(pool, n);
@@ -2055,6 +2171,10 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   const float u = 5.9604645e-08f;
   const float e_in = split ? 4.0f * 1.5258789e-05f : (hflags[0] ? 2.0f * 0.0078125f : 0.0f);
   const float gamma = 2.0f * (float)dpad * u + e_in;
+  // the FP8 pass's: its MFMA sums the 64 products of an instruction with an error that
+  // measured up to ~160 u of the sum of |products| (above the 2 d u of sequential f32
+  // adds), hence the 512 u allowance
+  const float gamma8 = (2.0f * (float)dpad8 + 512.0f) * u + e_in8;
   // The symmetric sweep (each pair of row blocks once) for the d in (256, 512] kernel at
   // large n; EVOC_HIP_KNN_SYMMETRIC=1 / 0 forces it on / off.
   bool sym = false;
@@ -2142,6 +2262,19 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       }
     }
 #undef EVOC_RA_LAUNCH
+  } else if (fp8) {
+    const size_t smem = (size_t)4 * 2 * BM * 4 * 16 + BM * SCS * sizeof(float) +
+                        2 * BM * sizeof(float) + 4 * sizeof(int);
+    const unsigned grid = (unsigned)(npad / BM);
+    const uint16_t* xq = reinterpret_cast<const uint16_t*>(Xq.p);  // bytes; the kernel reads bytes
+    if (CH == 16)
+      knn_topc_kernel<false, 16, true><<<grid, 256, smem, s>>>(xq, nullptr, n, npad, dpad8, scale8.p,
+                                                               metric, cidx.p, ckey.p, nullptr,
+                                                               nullptr, nullptr, 0, 0, npad / BN);
+    else
+      knn_topc_kernel<false, 32, true><<<grid, 256, smem, s>>>(xq, nullptr, n, npad, dpad8, scale8.p,
+                                                               metric, cidx.p, ckey.p, nullptr,
+                                                               nullptr, nullptr, 0, 0, npad / BN);
   } else {
     const int nops = split ? 4 : 2;
     const int nst = split ? 2 : 4;
@@ -2214,10 +2347,25 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       EVOC_HIP(hipGetLastError());
     }
   }
+  if (capture) {  // the first pass's raw lists, for tests of the tile kernels
+    capture->C = C;
+    capture->idx.resize((size_t)n * C);
+    capture->key.resize((size_t)n * C);
+    capture->scale.resize((size_t)n);
+    capture->gamma = fp8 ? gamma8 : gamma;
+    EVOC_HIP(hipMemcpyAsync(capture->idx.data(), cidx.p, (size_t)n * C * sizeof(int32_t),
+                            hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipMemcpyAsync(capture->key.data(), ckey.p, (size_t)n * C * sizeof(float),
+                            hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipMemcpyAsync(capture->scale.data(), fp8 ? scale8.p : colscale.p, n * sizeof(float),
+                            hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipStreamSynchronize(s));
+  }
+  // the FP8 pass's keys and error term are in its own units (row scales s / ||x||)
   knn_rerank_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(
-      dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, sym ? midx.p : cidx.p,
+      dX, n, d, ld, nrm.p, sqn.p, fp8 ? scale8.p : colscale.p, flags.p, sym ? midx.p : cidx.p,
       sym ? mkey.p : ckey.p, C, (ra && dpad < 512) ? 1 : 2, sym ? bnd.p : nullptr, k, metric,
-      gamma, d_idx, d_dist, fcount.p, frows.p, nullptr, 0);
+      fp8 ? gamma8 : gamma, d_idx, d_dist, fcount.p, frows.p, nullptr, 0);
   EVOC_HIP(hipGetLastError());
   int nfail = 0;
   EVOC_HIP(hipMemcpyAsync(&nfail, fcount.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -2230,6 +2378,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     std::fflush(stdout);
   }
   int escalated = 0;
+  const int first_failed = nfail;
   // EVOC_HIP_KNN_ESCALATE=1 escalates any number of failing rows (tests of small inputs)
   const char* esc_env = std::getenv("EVOC_HIP_KNN_ESCALATE");
   const int esc_min = (esc_env && esc_env[0] == '1') ? 1 : KNN_ESCALATE_MIN;
@@ -2320,6 +2469,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   st.split = split ? 1 : 0;
   st.fallback_rows = nfail;
   st.escalated_rows = escalated;
+  st.first_pass_failed = first_failed;
   st.candidates = C;
   st.mfma_ms = mfma_ms;
   double pairs = (double)npad * (double)npad;
@@ -2329,7 +2479,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     for (int64_t I = 0; I < m; ++I) blocks += (m & 1) ? (m + 1) / 2 : m / 2 + (I < m / 2 ? 1 : 0);
     pairs = (double)blocks * RB_ROWS * RB_ROWS;
   }
-  st.flops = 2.0 * pairs * (double)dpad * (split ? 3.0 : 1.0);
+  st.flops = fp8 ? 2.0 * pairs * (double)dpad8 : 2.0 * pairs * (double)dpad * (split ? 3.0 : 1.0);
   st.symmetric = sym ? 1 : 0;
   last_knn_stats() = st;
   if (stats) *stats = st;

@@ -23,7 +23,7 @@ f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 vp = C.c_void_p
 
 METRICS = {"euclidean": 0, "cosine": 1}
-PRECISIONS = {"auto": 0, "split": 1, "bf16": 2}
+PRECISIONS = {"auto": 0, "split": 1, "bf16": 2, "fp8": 3}
 
 
 class EvocError(RuntimeError):
@@ -100,6 +100,9 @@ _SIGS = {
                                i32p, f32p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "evoc_hip_knn_last_stats": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double),
                                           C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    "evoc_hip_knn_tile_pass": (C.c_int, [f32p, C.c_int64, C.c_int64, C.c_int32, C.c_int32,
+                                         C.c_int32, i32p, f32p, f32p, C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_float)]),
     "evoc_hip_fuzzy": (C.c_int, [i32p, f32p, C.c_int64, C.c_int32, C.c_float, C.c_int32, f32p,
                                  f32p, i64p, i32p, f32p, C.POINTER(C.c_int64)]),
     "evoc_hip_embed": (C.c_int, [i64p, i32p, f32p, C.c_int64, C.c_int32, C.c_int32, C.c_float,
@@ -341,6 +344,22 @@ def knn(x, k, metric="euclidean", precision="auto", return_stats=False):
                                kernel_flop=st["kernel_flop"], symmetric=st["symmetric"],
                                escalated_rows=st["escalated_rows"])
     return idx, dist
+
+
+def knn_tile_pass(x, k, metric="cosine", precision="fp8"):
+    """The first tile pass alone (evoc_hip_knn_tile_pass): raw candidate lists (n x 32: two
+    column halves of 16, ascending by key), their keys, the per-row key scales and the
+    certificate's relative error term of the pass."""
+    x = np.ascontiguousarray(x, np.float32)
+    n, d = x.shape
+    idx = np.empty((n, 32), np.int32)
+    key = np.empty((n, 32), np.float32)
+    scale = np.empty(n, np.float32)
+    nc = C.c_int32(0)
+    gam = C.c_float(0)
+    _check(lib().evoc_hip_knn_tile_pass(x, n, d, METRICS[metric], k, PRECISIONS[precision], idx,
+                                        key, scale, C.byref(nc), C.byref(gam)))
+    return idx, key, scale, gam.value
 
 
 def knn_last_stats():

@@ -34,8 +34,12 @@ enum {
 enum { EVOC_METRIC_EUCLIDEAN = 0, EVOC_METRIC_COSINE = 1 };
 /* kNN arithmetic: AUTO picks one bf16 MFMA per product when every input value is
  * bf16-representable, else the three-product hi/lo split; both are followed by the
- * exact f32 re-rank, so the returned neighbours are identical in every mode. */
-enum { EVOC_KNN_AUTO = 0, EVOC_KNN_SPLIT = 1, EVOC_KNN_BF16 = 2 };
+ * exact f32 re-rank, so the returned neighbours are identical in every mode. FP8
+ * (cosine metric only) runs the first pass on FP8-e4m3 values with per-row scales
+ * (max|x| / 448) and f32 accumulation; its certificate carries the quantisation error
+ * of the worst row, and rows it cannot certify are searched again in bf16 (then
+ * exactly), so the neighbours are again identical. */
+enum { EVOC_KNN_AUTO = 0, EVOC_KNN_SPLIT = 1, EVOC_KNN_BF16 = 2, EVOC_KNN_FP8 = 3 };
 
 /* Mirrors EvocParams (src/lib.rs:52-80); defaults src/lib.rs:83-99. */
 typedef struct {
@@ -128,6 +132,18 @@ int evoc_hip_knn(const float* x, int64_t n, int64_t d, int32_t metric, int32_t k
  * Any pointer may be NULL. */
 int evoc_hip_knn_last_stats(double* kernel_flop, double* kernel_ms, int32_t* symmetric,
                             int64_t* escalated_rows);
+
+/* The first tile pass of evoc_hip_knn alone, for tests of the tile kernels: per row the
+ * raw candidate lists (*ncand entries: two column halves, columns c with (c % 128) < 64
+ * and >= 64 of each 128-column tile, each ascending by the pass's key), the keys, the
+ * per-row key scale (cosine: 1/||x||, or s/||x|| with the FP8 row scale s = max|x|/448;
+ * the pass's key of a pair is -(dot * scale of the column), the dot of the stored values)
+ * and the certificate's relative error term of the pass. cand_idx and cand_key need room
+ * for n * 32 entries. No counterpart in the reference. Supported for the streaming
+ * kernel's precisions: EVOC_KNN_SPLIT and EVOC_KNN_FP8. */
+int evoc_hip_knn_tile_pass(const float* x, int64_t n, int64_t d, int32_t metric, int32_t k,
+                           int32_t precision, int32_t* cand_idx, float* cand_key,
+                           float* key_scale, int32_t* ncand, float* gamma);
 
 /* build_fuzzy_simplicial_set (src/graph/fuzzy_graph.rs:275-295) as CSR. indices and
  * weights need room for 2*n*k entries; *nnz receives the stored count. */

@@ -18,6 +18,7 @@ pub const EVOC_METRIC_COSINE: i32 = 1;
 pub const EVOC_KNN_AUTO: i32 = 0;
 pub const EVOC_KNN_SPLIT: i32 = 1;
 pub const EVOC_KNN_BF16: i32 = 2;
+pub const EVOC_KNN_FP8: i32 = 3;
 
 /// Mirrors `EvocParams` (src/lib.rs:52-80 of evoc-rs).
 #[repr(C)]
@@ -129,6 +130,20 @@ extern "C" {
         kernel_ms: *mut c_double,
         symmetric: *mut i32,
         escalated_rows: *mut i64,
+    ) -> c_int;
+
+    pub fn evoc_hip_knn_tile_pass(
+        x: *const c_float,
+        n: i64,
+        d: i64,
+        metric: i32,
+        k: i32,
+        precision: i32,
+        cand_idx: *mut i32,
+        cand_key: *mut c_float,
+        key_scale: *mut c_float,
+        ncand: *mut i32,
+        gamma: *mut c_float,
     ) -> c_int;
 
     pub fn evoc_hip_fuzzy(
