@@ -2040,6 +2040,14 @@ __global__ void knn_gather_rows_kernel(const uint16_t* __restrict__ Xh,
 // (fewer go straight to the exact scan, which is cheaper for a handful of rows)
 constexpr int KNN_ESCALATE_MIN = 1024;
 constexpr int KNN_ESC_SHARDS = 4;  // column shards of the escalation search
+// rows sampled to estimate the first pass's certificate rate (streaming kernel, n >= 2^18)
+constexpr int64_t KNN_SAMPLE_ROWS = 4096;
+constexpr int64_t KNN_SAMPLE_MIN_N = 262144;
+
+__global__ void knn_iota_kernel(int64_t n, int32_t* __restrict__ v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (int32_t)i;
+}
 
 void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k, int precision,
                 int32_t* d_idx, float* d_dist, KnnStats* stats, hipStream_t s, DevPool& pool,
@@ -2204,11 +2212,63 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     qk.alloc(pool, nwaves * sym_qcap);
     qcount.alloc(pool, nwaves);
   }
+  // The first pass's certificate rate, estimated on evenly spaced sample rows searched
+  // with the streaming kernel (bf16 or split, as the escalation would search them; the
+  // FP8 pass certifies fewer rows than that). Where more than half of them fail (neighbour
+  // distances concentrated below the error term, as in the dense leaves of a hierarchical
+  // mixture), the full first pass would certify little: it is skipped and every row goes
+  // straight to the sharded escalation. EVOC_HIP_KNN_SAMPLE=0 turns the test off.
+  bool skip_first = false;
+  if (!ra && !capture && n >= KNN_SAMPLE_MIN_N && C < 64) {
+    const char* senv = std::getenv("EVOC_HIP_KNN_SAMPLE");
+    if (!(senv && senv[0] == '0')) {
+      const int64_t S = KNN_SAMPLE_ROWS, Spad = ceil_div(S, BM) * BM;
+      std::vector<int32_t> hs(S);
+      for (int64_t i = 0; i < S; ++i) hs[i] = (int32_t)(i * n / S);  // increasing: n >= S
+      Buf<int32_t> sq(pool, S), scidx(pool, S * 32), sfr(pool, S);
+      Buf<float> sckey(pool, S * 32);
+      Buf<int> sfc(pool, 1);
+      Buf<uint16_t> SQh(pool, Spad * dpad), SQl;
+      if (split) SQl.alloc(pool, Spad * dpad);
+      EVOC_HIP(hipMemcpyAsync(sq.p, hs.data(), S * sizeof(int32_t), hipMemcpyHostToDevice, s));
+      EVOC_HIP(hipMemsetAsync(sfc.p, 0, sizeof(int), s));
+      knn_gather_rows_kernel<<<(unsigned)ceil_div(Spad * (dpad / 8), 256), 256, 0, s>>>(
+          Xh.p, split ? Xl.p : nullptr, sq.p, S, Spad, dpad, SQh.p, split ? SQl.p : nullptr);
+      const int nops = split ? 4 : 2;
+      const int nst = split ? 2 : 4;
+      const size_t smem = (size_t)nst * nops * BM * 4 * 16 + BM * SCS * sizeof(float) +
+                          2 * BM * sizeof(float) + 4 * sizeof(int);
+      if (split)
+        knn_topc_kernel<true, 16><<<(unsigned)(Spad / BM), 256, smem, s>>>(
+            Xh.p, Xl.p, n, npad, dpad, colscale.p, metric, scidx.p, sckey.p, SQh.p, SQl.p, sq.p, S,
+            0, npad / BN);
+      else
+        knn_topc_kernel<false, 16><<<(unsigned)(Spad / BM), 256, smem, s>>>(
+            Xh.p, nullptr, n, npad, dpad, colscale.p, metric, scidx.p, sckey.p, SQh.p, nullptr,
+            sq.p, S, 0, npad / BN);
+      EVOC_HIP(hipGetLastError());
+      // (the sampled rows' results land in the output and are rewritten by the search)
+      knn_rerank_kernel<<<(unsigned)ceil_div(S, 4), 256, 0, s>>>(
+          dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, scidx.p, sckey.p, 32, 2, nullptr, k,
+          metric, gamma, d_idx, d_dist, sfc.p, sfr.p, sq.p, S);
+      EVOC_HIP(hipGetLastError());
+      int sfail = 0;
+      EVOC_HIP(hipMemcpyAsync(&sfail, sfc.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      EVOC_HIP(hipStreamSynchronize(s));
+      skip_first = 2 * (int64_t)sfail > S;
+      if (verbose) {
+        std::printf("  kNN sample: %d of %lld rows failed the certificate%s\n", sfail,
+                    (long long)S, skip_first ? "; first pass skipped, every row escalated" : "");
+        std::fflush(stdout);
+      }
+    }
+  }
   hipEvent_t e0, e1;
   EVOC_HIP(hipEventCreate(&e0));
   EVOC_HIP(hipEventCreate(&e1));
   EVOC_HIP(hipEventRecord(e0, s));
-  if (ra) {
+  if (skip_first) {
+  } else if (ra) {
     const unsigned grid = (unsigned)(npad / RA_ROWS);
 #define EVOC_RA_LAUNCH(NKC_, KC_)                                                              \
   do {                                                                                         \
@@ -2361,20 +2421,27 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
                             hipMemcpyDeviceToHost, s));
     EVOC_HIP(hipStreamSynchronize(s));
   }
-  // the FP8 pass's keys and error term are in its own units (row scales s / ||x||)
-  knn_rerank_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(
-      dX, n, d, ld, nrm.p, sqn.p, fp8 ? scale8.p : colscale.p, flags.p, sym ? midx.p : cidx.p,
-      sym ? mkey.p : ckey.p, C, (ra && dpad < 512) ? 1 : 2, sym ? bnd.p : nullptr, k, metric,
-      fp8 ? gamma8 : gamma, d_idx, d_dist, fcount.p, frows.p, nullptr, 0);
-  EVOC_HIP(hipGetLastError());
   int nfail = 0;
-  EVOC_HIP(hipMemcpyAsync(&nfail, fcount.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  EVOC_HIP(hipStreamSynchronize(s));
-  if (verbose) {
+  if (skip_first) {  // every row is escalated
+    knn_iota_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(n, frows.p);
+    EVOC_HIP(hipGetLastError());
+    nfail = (int)n;
+  } else {
+    // the FP8 pass's keys and error term are in its own units (row scales s / ||x||)
+    knn_rerank_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(
+        dX, n, d, ld, nrm.p, sqn.p, fp8 ? scale8.p : colscale.p, flags.p, sym ? midx.p : cidx.p,
+        sym ? mkey.p : ckey.p, C, (ra && dpad < 512) ? 1 : 2, sym ? bnd.p : nullptr, k, metric,
+        fp8 ? gamma8 : gamma, d_idx, d_dist, fcount.p, frows.p, nullptr, 0);
+    EVOC_HIP(hipGetLastError());
+    EVOC_HIP(hipMemcpyAsync(&nfail, fcount.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipStreamSynchronize(s));
+  }
+  if (verbose && !skip_first) {
     float ms = 0.0f;
     EVOC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    std::printf("  kNN tile pass %.0f ms (%d candidates per row%s); %d of %lld rows failed the "
-                "certificate\n", ms, C, sym ? ", symmetric sweep" : "", nfail, (long long)n);
+    std::printf("  kNN tile pass %.0f ms (%d candidates per row%s%s); %d of %lld rows failed the "
+                "certificate\n", ms, C, sym ? ", symmetric sweep" : "", fp8 ? ", fp8" : "", nfail,
+                (long long)n);
     std::fflush(stdout);
   }
   int escalated = 0;
@@ -2408,6 +2475,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
                         2 * BM * sizeof(float) + 4 * sizeof(int);
     const unsigned grid = (unsigned)(nqpad / BM);
     const int64_t ntile = npad / BN;
+    if (skip_first) EVOC_HIP(hipEventRecord(e0, s));  // then the escalation is the tile pass
     for (int sh = 0; sh < KNN_ESC_SHARDS; ++sh) {
       const int64_t ct0 = ntile * sh / KNN_ESC_SHARDS, ct1 = ntile * (sh + 1) / KNN_ESC_SHARDS;
       int32_t* oi = ecidx.p + (int64_t)sh * nq * 64;
@@ -2422,6 +2490,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
                                                            nullptr, qmap.p, nq, ct0, ct1);
     }
     EVOC_HIP(hipGetLastError());
+    if (skip_first) EVOC_HIP(hipEventRecord(e1, s));
     EVOC_HIP(hipMemsetAsync(fcount.p, 0, sizeof(int), s));
     knn_rerank_all_kernel<KNN_ESC_SHARDS><<<(unsigned)ceil_div(nq, 4), 256, 0, s>>>(
         dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, ecidx.p, eckey.p, k, metric, gamma,
@@ -2480,6 +2549,8 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     pairs = (double)blocks * RB_ROWS * RB_ROWS;
   }
   st.flops = fp8 ? 2.0 * pairs * (double)dpad8 : 2.0 * pairs * (double)dpad * (split ? 3.0 : 1.0);
+  if (skip_first)  // the escalation of every row over all columns
+    st.flops = 2.0 * (double)ceil_div(n, BM) * BM * (double)npad * (double)dpad * (split ? 3.0 : 1.0);
   st.symmetric = sym ? 1 : 0;
   last_knn_stats() = st;
   if (stats) *stats = st;

@@ -183,3 +183,32 @@ def test_knn_forced_escalation_small_inputs(n, d, metric, monkeypatch):
     esc = H.knn_last_stats()["escalated_rows"]
     print(n, d, metric, "escalated", esc, "exact", st["fallback_rows"])
     assert esc > 0
+
+
+def _hierarchical_rows(n, d, seed):
+    import torch
+    x, _ = datagen.unit_mixture_device(n, d, seed=seed, hierarchical=(10, 10, 0.3, 0.1))
+    xh = x.cpu().numpy()
+    del x
+    torch.cuda.empty_cache()
+    return xh
+
+
+@pytest.mark.parametrize("sample", ["1", "0"])
+def test_knn_sampled_certificate_rate_skips_first_pass(sample, monkeypatch):
+    # 300,000 x 768 with 100 leaves of 3,000 points: neighbour distances inside a leaf are
+    # concentrated below the first pass's error term, so the 4,096-row sample fails and
+    # every row goes straight to the sharded escalation; with the sample test off the
+    # first pass runs and the same rows are escalated after it. Both equal the exact search.
+    monkeypatch.setenv("EVOC_HIP_KNN_SAMPLE", sample)
+    n, d = 300_000, 768
+    x = _hierarchical_rows(n, d, 7)
+    gi, gd, st = H.knn(x, 15, metric="cosine", return_stats=True)
+    rows = np.sort(np.random.default_rng(7).choice(n, size=512, replace=False))
+    oi, od = O.knn_rows(x, rows, 15, metric=1)
+    assert np.array_equal(gi[rows], oi)
+    assert np.array_equal(gd[rows].view(np.uint32), od.view(np.uint32))
+    print(f"sample test {'on' if sample == '1' else 'off'}: escalated {st['escalated_rows']}, "
+          f"exact rows {st['fallback_rows']}, tile kernel {st['kernel_ms']:.0f} ms")
+    if sample == "1":
+        assert st["escalated_rows"] == n
