@@ -65,9 +65,10 @@ def cpu_baseline(x_host, k, dim, knn_rows=512, sub=100_000, seed=42):
     to the whole job by each stage's complexity:
 
     - kNN, the faster of two legs: textbook NN-Descent (the reference's default backend is
-      NN-Descent) on a `sub`-point subsample, scaled by n / sub (linear: kinder to the CPU
-      than NN-Descent's growth); exact search of `knn_rows` query rows against all n rows,
-      scaled by n / knn_rows;
+      NN-Descent, built with graph degree 30, nearest_neighbour_cpu.rs:278-289) with lists
+      of 30 of which the first k are kept, on a `sub`-point subsample, scaled by n / sub
+      (linear: kinder to the CPU than NN-Descent's growth); exact search of `knn_rows`
+      query rows against all n rows, scaled by n / knn_rows;
     - fuzzy set, label-propagation init and embedding on the subsample (given its NN-Descent
       graph), scaled by n / sub;
     - clustering with the kd-tree Boruvka MST (the reference's algorithm, O(n log n) per
@@ -85,7 +86,9 @@ def cpu_baseline(x_host, k, dim, knn_rows=512, sub=100_000, seed=42):
     S = len(si)
     xs = np.ascontiguousarray(x_host[si])
     t0 = time.perf_counter()
-    idx, dist, iters = O.knn_nndescent(xs, k, metric=1, seed=seed)
+    idx, dist, iters = O.knn_nndescent(xs, max(30, k), metric=1, seed=seed)
+    idx = np.ascontiguousarray(idx[:, :k])
+    dist = np.ascontiguousarray(dist[:, :k])
     t_nnd = (time.perf_counter() - t0) * n / S
     check = np.sort(rng.choice(S, size=min(256, S), replace=False))
     ei, _ = O.knn_rows(xs, check, k, metric=1)
@@ -107,7 +110,7 @@ def cpu_baseline(x_host, k, dim, knn_rows=512, sub=100_000, seed=42):
         "unit": "points/s",
         "cores": int(O.lib().orc_num_threads()),
         "kind": "port",
-        "sample": (f"kNN: NN-Descent on a {S}-point subsample ({iters} iterations, recall "
+        "sample": (f"kNN: NN-Descent (graph degree 30) on a {S}-point subsample ({iters} iterations, recall "
                    f"{recall:.3f} vs exact on 256 rows) scaled by n/{S} = {t_nnd:.0f} s est., "
                    f"exact search of {knn_rows} rows x {n} scaled by n/{knn_rows} = "
                    f"{t_exh:.0f} s est., the faster taken; fuzzy {tm[1] * lin:.1f} s + init "
