@@ -2552,8 +2552,12 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   if (skip_first)  // the escalation of every row over all columns
     st.flops = 2.0 * (double)ceil_div(n, BM) * BM * (double)npad * (double)dpad * (split ? 3.0 : 1.0);
   st.symmetric = sym ? 1 : 0;
-  last_knn_stats() = st;
-  if (stats) *stats = st;
+  // a stage-level search (one that asks for its statistics) becomes the thread's last kNN;
+  // internal searches, such as the warm start's subsample, do not replace it
+  if (stats) {
+    *stats = st;
+    last_knn_stats() = st;
+  }
 }
 
 KnnStats& last_knn_stats() {

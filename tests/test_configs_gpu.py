@@ -144,7 +144,9 @@ def test_config_4M_x768_hierarchical_approx_500():
     n, d = 4_000_000, 768
     xd, ld = datagen.unit_mixture_device(n, d, seed=5, hierarchical=(100, 10, 0.3, 0.1))
     leaves = ld.cpu().numpy()
-    nn = H.NearestNeighbourParams(dist_metric="cosine", warm_start=True)
+    # EVOC_HIP_KNN_PRECISION=fp8 runs the FP8 first pass instead (auto by default)
+    prec = os.environ.get("EVOC_HIP_KNN_PRECISION", "auto")
+    nn = H.NearestNeighbourParams(dist_metric="cosine", knn_precision=prec, warm_start=True)
     p = H.EvocParams(embedding_dim=8, approx_n_clusters=500)
     t0 = time.perf_counter()
     r = H.evoc_device(xd.data_ptr(), n, d, params=p, nn_params=nn, seed=42, verbose=1)
