@@ -893,6 +893,8 @@ struct RbCols {
   int64_t qcap;         // entries per wavefront queue
   int32_t* qcount;      // entries written per wavefront
   int64_t group;        // workgroups resident at once (one per CU): walks start in step
+  int block0;           // symmetric sweep launched in slices: first block of this launch
+  int nblocks;          // and the whole sweep's block count (0: this launch's grid)
 };
 
 template <int KC>
@@ -951,7 +953,8 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   // queues transposed keys for come after its own, so most of them have finished by then
   // and replaced their rows' thresholds with the tighter largest forward key.
   // (tile and row indices fit 32 bits: n < 2^31; 64-bit products only for addresses)
-  const int bid = SYM ? (int)gridDim.x - 1 - (int)blockIdx.x : (int)blockIdx.x;
+  const int gblk = (int)blockIdx.x + cols.block0;  // block of the whole sweep
+  const int bid = SYM ? (cols.nblocks > 0 ? cols.nblocks : (int)gridDim.x) - 1 - gblk : gblk;
   const int row0 = bid * RB_ROWS;
   const int ntile = (int)cols.ntile;
   int tile0 = 0, nt = ntile, tshift = 0;
@@ -966,7 +969,7 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     tile0 = I * (RB_ROWS / RA_BN);
     nt = nblk * (RB_ROWS / RA_BN);
     const int g = cols.group > 0 ? (int)cols.group : 1;
-    tshift = std::min((int)blockIdx.x % g, nblk - 1) * (RB_ROWS / RA_BN);
+    tshift = std::min(gblk % g, nblk - 1) * (RB_ROWS / RA_BN);
   }
   // absolute column tile of the ct-th tile of this walk
   auto tile_at = [&](int ct) {
@@ -979,7 +982,7 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   const int total = nt * NKC;
   int qn = 0;  // this wavefront's reverse candidates in its global queue
   int lq = 0;  // and staged in LDS
-  const int64_t qbase = SYM ? ((int64_t)blockIdx.x * RB_WAVES + wave) * cols.qcap : 0;
+  const int64_t qbase = SYM ? ((int64_t)gblk * RB_WAVES + wave) * cols.qcap : 0;
   int32_t* const qjw = SYM ? cols.qj + qbase : nullptr;
   int32_t* const qrw = SYM ? cols.qr + qbase : nullptr;
   float* const qkw = SYM ? cols.qk + qbase : nullptr;
@@ -1333,7 +1336,7 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   const int64_t grow = row0 + wave * 32 + (lane & 31);
   if (SYM) {
     if (lq > 0) flush();
-    if (lane == 0) cols.qcount[(int64_t)blockIdx.x * RB_WAVES + wave] = qn;
+    if (lane == 0) cols.qcount[(int64_t)gblk * RB_WAVES + wave] = qn;
     // Publish the row's largest forward key as its threshold when it is smaller. Any
     // transposed key at or above it is beaten by all 32 forward candidates; workgroups
     // still to come read the smaller value (or, from a stale cache line, the old one:
@@ -2285,11 +2288,11 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       EVOC_RA_LAUNCH(1, 256);
     } else {  // d in (256, 512]: eight 32-row wavefronts, two per SIMD
       RbCols cols{Xh.p, colscale.p, nullptr, n, npad / RA_BN, 1, nullptr, nullptr, nullptr, 0,
-                  nullptr};
+                  nullptr, 0, 0, 0};
       if (sym) {
         // threshold pass against a sample of rows, then the symmetric sweep
         RbCols sc{Xs.p, ss.p, nullptr, sym_S, sym_Spad / RA_BN, 0, nullptr, nullptr, nullptr, 0,
-                  nullptr};
+                  nullptr, 0, 0, 0};
         if (metric == 1)
           knn_topc_rb_kernel<RB_NKC, RB_KC, true, false, RB_PASS_C><<<grid, 512, RbRing<RB_KC>::SMEM, s>>>(
               Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, sc);
@@ -2306,12 +2309,23 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
         cols.qcap = sym_qcap;
         cols.qcount = qcount.p;
         cols.group = cu_count();
-        if (metric == 1)
-          knn_topc_rb_kernel<RB_NKC, RB_KC, true, true><<<grid, 512, RbRing<RB_KC>::SMEM + RB_QBYTES, s>>>(
-              Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
-        else
-          knn_topc_rb_kernel<RB_NKC, RB_KC, false, true><<<grid, 512, RbRing<RB_KC>::SMEM + RB_QBYTES, s>>>(
-              Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
+        // Launched in slices of one workgroup per CU, so that each slice's walks start
+        // together and stream their shared column tiles in step through L2 (in one launch
+        // a workgroup starts whenever a CU frees up). EVOC_HIP_KNN_SLICED=0: one launch.
+        const char* slenv = std::getenv("EVOC_HIP_KNN_SLICED");
+        const bool sliced = !(slenv && slenv[0] == '0');
+        const int gsz = sliced ? (int)std::max<int64_t>(1, cols.group) : (int)grid;
+        cols.nblocks = (int)grid;
+        for (int b0 = 0; b0 < (int)grid; b0 += gsz) {
+          cols.block0 = b0;
+          const unsigned g1 = (unsigned)std::min<int>(gsz, (int)grid - b0);
+          if (metric == 1)
+            knn_topc_rb_kernel<RB_NKC, RB_KC, true, true><<<g1, 512, RbRing<RB_KC>::SMEM + RB_QBYTES, s>>>(
+                Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
+          else
+            knn_topc_rb_kernel<RB_NKC, RB_KC, false, true><<<g1, 512, RbRing<RB_KC>::SMEM + RB_QBYTES, s>>>(
+                Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
+        }
       } else {
         if (metric == 1)
           knn_topc_rb_kernel<RB_NKC, RB_KC, true, false><<<grid, 512, RbRing<RB_KC>::SMEM, s>>>(
@@ -2375,7 +2389,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     if (qmax > sym_qcap) {  // a queue overflowed: sweep every pair instead
       sym = false;
       RbCols cols{Xh.p, colscale.p, nullptr, n, npad / RA_BN, 1, nullptr, nullptr, nullptr, 0,
-                  nullptr};
+                  nullptr, 0, 0, 0};
       const unsigned grid = (unsigned)(npad / RA_ROWS);
       if (metric == 1)
         knn_topc_rb_kernel<RB_NKC, RB_KC, true, false><<<grid, 512, RbRing<RB_KC>::SMEM, s>>>(
