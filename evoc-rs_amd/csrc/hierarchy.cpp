@@ -1,10 +1,11 @@
-// Cluster hierarchy on the host: single linkage, condensed tree, leaves, labels,
-// membership strengths, min-cluster-size barcodes, layer selection and the
-// approx_n_clusters search. Each step is O(n) (or O(n log n)), on arrays that are
-// already in host memory.
+// The host side of the cluster hierarchy: single linkage, and the cluster-level steps
+// that read only the splits of a condensation (a few per thousand points): leaves,
+// barcodes, persistence, peaks, the selection of diverse layers and the per-cluster label
+// tables the device's label kernel reads. The condensation itself and the per-point labels
+// and strengths run on the device (tree.hip, clusters.cpp); the readers of full condensed
+// rows here serve min_cluster_size < 2, where every point is a cluster of its own.
 //
 //  linkage      src/clustering/linkage.rs:323-367   (union-find relabel, no parent walk)
-//  condense     src/clustering/condensed_tree.rs:127-248 (explicit-stack right-first DFS)
 //  leaves       src/clustering/condensed_tree.rs:264-294 (flag arrays)
 //  labels       src/clustering/condensed_tree.rs:311-369 (nearest selected ancestor, the
 //               representative the reference's union-find ends with)
@@ -12,9 +13,10 @@
 //  barcodes     src/clustering/persistence.rs:42-118 (fp64)
 //  persistence  src/clustering/persistence.rs:135-173 (fp64, barcode order per bin)
 //  layers       src/clustering/persistence.rs:188-463
-//  search       src/lib.rs:377-447
+//  search       src/lib.rs:377-447 (leaf counts without condensing)
 #include <omp.h>
 
+#include <cstring>
 #include <numeric>
 #include <unordered_map>
 
@@ -68,71 +70,93 @@ Linkage linkage_from_sorted(const int32_t* u, const int32_t* v, const float* w, 
   return L;
 }
 
-CTree condense(const Linkage& L, int64_t mcs) {
+// The clusters of a device condensation as condensed-tree rows, two per split in the
+// walk's order: every row with more than one point, which is what barcodes() and the
+// layer selection read.
+CTree split_rows(const DevCondensed& c) {
   CTree t;
-  const int64_t n = L.n, nr = (int64_t)L.left.size();
-  t.n = n;
-  if (nr == 0) return t;
-  const int64_t root = 2 * nr;
-  std::vector<int64_t> relabel(root + 1, 0);
-  relabel[root] = n;
-  int64_t next = n + 1;
-  t.parent.reserve(n + n / 8 + 64);
-  t.child.reserve(n + n / 8 + 64);
-  t.lambda.reserve(n + n / 8 + 64);
-  t.size.reserve(n + n / 8 + 64);
-  auto emit = [&](int64_t p, int64_t c, float lam, int64_t s) {
-    t.parent.push_back(p);
-    t.child.push_back(c);
-    t.lambda.push_back(lam);
-    t.size.push_back(s);
-  };
-  std::vector<int64_t> st, est;
-  auto eliminate = [&](int64_t x, int64_t pl, float lam) {
-    est.assign(1, x);
-    while (!est.empty()) {
-      const int64_t y = est.back();
-      est.pop_back();
-      if (y < n) {
-        emit(pl, y, lam, 1);
-      } else {
-        est.push_back(L.left[y - n]);
-        est.push_back(L.right[y - n]);
-      }
-    }
-  };
-  st.push_back(root);
-  while (!st.empty()) {
-    const int64_t node = st.back();
-    st.pop_back();
-    if (node < n) continue;
-    const int64_t pl = relabel[node];
-    const int64_t l = L.left[node - n], r = L.right[node - n];
-    const float d = L.dist[node - n];
-    const float lam = d > 0.0f ? 1.0f / d : INFINITY;
-    const int64_t ls = l >= n ? L.size[l - n] : 1, rs = r >= n ? L.size[r - n] : 1;
-    const bool lb = ls >= mcs, rb = rs >= mcs;
-    if (lb && rb) {
-      relabel[l] = next;
-      emit(pl, next++, lam, ls);
-      relabel[r] = next;
-      emit(pl, next++, lam, rs);
-      st.push_back(l);
-      st.push_back(r);
-    } else if (lb) {
-      relabel[l] = pl;
-      eliminate(r, pl, lam);
-      st.push_back(l);
-    } else if (rb) {
-      relabel[r] = pl;
-      eliminate(l, pl, lam);
-      st.push_back(r);
-    } else {
-      eliminate(l, pl, lam);
-      eliminate(r, pl, lam);
-    }
+  t.n = c.n;
+  const int64_t S = c.S;
+  t.parent.resize(2 * S);
+  t.child.resize(2 * S);
+  t.lambda.resize(2 * S);
+  t.size.resize(2 * S);
+  for (int64_t s = 0; s < S; ++s) {
+    t.parent[2 * s] = t.parent[2 * s + 1] = c.split_parent[s];
+    t.child[2 * s] = c.n + 1 + 2 * s;
+    t.child[2 * s + 1] = c.n + 2 + 2 * s;
+    t.lambda[2 * s] = t.lambda[2 * s + 1] = c.split_lam[s];
+    t.size[2 * s] = c.split_ls[s];
+    t.size[2 * s + 1] = c.split_rs[s];
   }
   return t;
+}
+
+// extract_leaves for min_cluster_size >= 2: every cluster n .. n + 2S parents at least one
+// row (the node it labels is processed and emits rows), and only splits emit rows of more
+// than one point, so the leaves are the clusters that parent no split, in id order.
+std::vector<int64_t> split_leaves(const DevCondensed& c) {
+  std::vector<int64_t> out;
+  if (c.n < 2) return out;  // no linkage rows, no condensed tree
+  std::vector<char> isp(2 * c.S + 1, 0);
+  for (int64_t s = 0; s < c.S; ++s) isp[c.split_parent[s] - c.n] = 1;
+  for (int64_t q = 0; q <= 2 * c.S; ++q)
+    if (!isp[q]) out.push_back(c.n + q);
+  return out;
+}
+
+// Per layer and cluster: the label of the points that fall out of the cluster (the rank of
+// its nearest selected ancestor-or-self among the layer's sorted selection, -1 when that is
+// the root or there is none) and the cluster's own rank (-1 if not selected). Parents have
+// smaller ids than their children, so one pass in id order resolves the ancestors.
+void layer_tables(const DevCondensed& c, const std::vector<std::vector<int64_t>>& sel,
+                  std::vector<int32_t>& clab, std::vector<int32_t>& dsel,
+                  std::vector<int64_t>& nsel) {
+  const int64_t n = c.n, C = 2 * c.S + 1, nl = (int64_t)sel.size();
+  clab.assign(nl * C, -1);
+  dsel.assign(nl * C, -1);
+  nsel.assign(nl, 0);
+  std::vector<int64_t> near(C);
+  for (int64_t l = 0; l < nl; ++l) {
+    std::vector<int64_t> sorted(sel[l]);
+    std::sort(sorted.begin(), sorted.end());
+    nsel[l] = (int64_t)sorted.size();
+    int32_t* rk = dsel.data() + l * C;
+    for (size_t q = 0; q < sorted.size(); ++q)
+      if (sorted[q] >= n && sorted[q] < n + C) rk[sorted[q] - n] = (int32_t)q;
+    for (int64_t q = 0; q < C; ++q) {
+      if (rk[q] >= 0) near[q] = q;
+      else if (q > 0) near[q] = near[c.split_parent[(q - 1) / 2] - n];
+      else near[q] = -1;
+    }
+    int32_t* cl = clab.data() + l * C;
+    for (int64_t q = 0; q < C; ++q) cl[q] = near[q] > 0 ? rk[near[q]] : -1;
+  }
+}
+
+// leaves(condense(L, m)).size() for m >= 2 is 1 + the number of linkage rows whose smaller
+// child holds at least m points: those rows are exactly the splits, and each split adds two
+// clusters of which its parent stops being a leaf. So the approx_n_clusters binary search
+// (src/lib.rs:377-447) needs no condensation per probe.
+void search_bounds(const Linkage& L, int64_t target_k, int64_t& lo, int64_t& hi) {
+  const int64_t n = L.n, nr = (int64_t)L.left.size();
+  lo = 2;
+  hi = n / 2;
+  if (hi - lo <= 1) return;
+  std::vector<int64_t> ge(hi + 2, 0);
+  for (int64_t i = 0; i < nr; ++i) {
+    const int64_t ls = L.left[i] >= n ? L.size[L.left[i] - n] : 1;
+    const int64_t rs = L.right[i] >= n ? L.size[L.right[i] - n] : 1;
+    ++ge[std::min(std::min(ls, rs), hi + 1)];
+  }
+  for (int64_t m = hi; m >= 0; --m) ge[m] += ge[m + 1];
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) / 2;
+    if (mid == lo || mid == hi) break;
+    const int64_t mk = 1 + ge[mid];
+    if (mk < target_k) hi = mid;
+    else lo = mid;
+  }
 }
 
 std::vector<int64_t> leaves(const CTree& t) {
@@ -311,96 +335,86 @@ int64_t n_clusters(const std::vector<int64_t>& l) {
 
 }  // namespace
 
-Layers cluster_layers_from_linkage(const Linkage& L, int64_t base_mcs, double min_sim,
-                                   int64_t max_layers) {
-  const int64_t n = L.n;
-  Layers out;
-  const CTree t = condense(L, base_mcs);
-  // the selected clusters of every layer first (the leaves, then the diverse peaks of
-  // the total-persistence curve), then the layers' labels, independently in parallel
-  std::vector<std::vector<int64_t>> sel(1, leaves(t));
-  std::vector<double> pers(1, 0.0);
+LayerSelection select_layers(const CTree& t, std::vector<int64_t> base, int64_t base_mcs,
+                             double min_sim, int64_t max_layers) {
+  const int64_t n = t.n;
+  LayerSelection out;
+  // the base layer (the leaves, persistence 0), then the diverse peaks of the
+  // total-persistence curve, each layer its active clusters
+  out.sel.push_back(std::move(base));
+  out.pers.push_back(0.0);
   const Bar b = barcodes(t, base_mcs);
-  if (!b.birth.empty()) {
-    std::vector<double> sizes, curve;
-    total_persistence(b, sizes, curve);
-    const std::vector<int64_t> pk = find_peaks(curve);
-    std::vector<std::pair<int64_t, double>> idx;
-    for (auto p : pk) idx.push_back({p, curve[p]});
-    std::stable_sort(idx.begin(), idx.end(), [](const std::pair<int64_t, double>& a,
-                                                const std::pair<int64_t, double>& c) {
-      return c.second < a.second;
-    });
-    const int64_t cap = max_layers > 0 ? max_layers - 1 : 0;
-    std::vector<double> chosen;
-    for (auto& pp : idx) {
-      if ((int64_t)chosen.size() >= cap) break;
-      const double bs = sizes[pp.first];
-      bool diverse = true;
-      for (double c : chosen)
-        if (!(jaccard(b, bs, c) <= min_sim)) {
-          diverse = false;
-          break;
-        }
-      if (!diverse) continue;
-      chosen.push_back(bs);
-      std::vector<int64_t> act;
-      for (size_t i = 0; i < b.birth.size(); ++i)
-        if (b.birth[i] <= bs && b.death[i] > bs) act.push_back((int64_t)i + n);
-      sel.push_back(std::move(act));
-      pers.push_back(curve[pp.first]);
-    }
+  if (b.birth.empty()) return out;
+  std::vector<double> sizes, curve;
+  total_persistence(b, sizes, curve);
+  const std::vector<int64_t> pk = find_peaks(curve);
+  std::vector<std::pair<int64_t, double>> idx;
+  for (auto p : pk) idx.push_back({p, curve[p]});
+  std::stable_sort(idx.begin(), idx.end(), [](const std::pair<int64_t, double>& a,
+                                              const std::pair<int64_t, double>& c) {
+    return c.second < a.second;
+  });
+  const int64_t cap = max_layers > 0 ? max_layers - 1 : 0;
+  std::vector<double> chosen;
+  for (auto& pp : idx) {
+    if ((int64_t)chosen.size() >= cap) break;
+    const double bs = sizes[pp.first];
+    bool diverse = true;
+    for (double c : chosen)
+      if (!(jaccard(b, bs, c) <= min_sim)) {
+        diverse = false;
+        break;
+      }
+    if (!diverse) continue;
+    chosen.push_back(bs);
+    std::vector<int64_t> act;
+    for (size_t i = 0; i < b.birth.size(); ++i)
+      if (b.birth[i] <= bs && b.death[i] > bs) act.push_back((int64_t)i + n);
+    out.sel.push_back(std::move(act));
+    out.pers.push_back(curve[pp.first]);
   }
-  const int64_t nl = (int64_t)sel.size();
-  std::vector<std::vector<int64_t>> labs(nl, std::vector<int64_t>(n));
-  std::vector<std::vector<float>> strs(nl, std::vector<float>(n));
-  std::vector<int64_t> nc(nl);
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int64_t l = 0; l < nl; ++l) {
-    labels_and_strengths(t, sel[l], labs[l].data(), strs[l].data());
-    nc[l] = n_clusters(labs[l]);
-  }
-  std::vector<int64_t> order(labs.size());
+  return out;
+}
+
+// Layers in descending order of their cluster counts (a stable sort), labels widened to
+// int64. lab32/str hold nl x n values; ncl[l] is layer l's number of clusters.
+Layers order_layers(int64_t n, const std::vector<int32_t>& lab32, const std::vector<float>& str,
+                    const std::vector<int64_t>& ncl, const std::vector<double>& pers) {
+  const int64_t nl = (int64_t)ncl.size();
+  std::vector<int64_t> order(nl);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t c) { return nc[c] < nc[a]; });
-  for (auto o : order) {
-    out.labels.push_back(std::move(labs[o]));
-    out.strengths.push_back(std::move(strs[o]));
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t a, int64_t c) { return ncl[c] < ncl[a]; });
+  Layers out;
+  out.labels.assign(nl, std::vector<int64_t>(n));
+  out.strengths.assign(nl, std::vector<float>(n));
+  for (int64_t q = 0; q < nl; ++q) {
+    const int64_t o = order[q];
+    const int32_t* src = lab32.data() + o * n;
+    int64_t* dst = out.labels[q].data();
+#pragma omp parallel for schedule(static) if (n > (1 << 16))
+    for (int64_t i = 0; i < n; ++i) dst[i] = src[i];
+    std::memcpy(out.strengths[q].data(), str.data() + o * n, n * sizeof(float));
     out.persistence.push_back(pers[o]);
   }
   return out;
 }
 
-void search_n_clusters_from_linkage(const Linkage& L, int64_t target_k, int64_t* labels,
-                                    float* strengths) {
-  const int64_t n = L.n;
-  int64_t lo = 2, hi = n / 2;
-  while (hi - lo > 1) {
-    const int64_t mid = (lo + hi) / 2;
-    if (mid == lo || mid == hi) break;
-    const int64_t mk = (int64_t)leaves(condense(L, mid)).size();
-    if (mk < target_k) hi = mid;
-    else lo = mid;
+// Labels and strengths of every selected layer from the full rows on the host (used where
+// points are clusters of their own, min_cluster_size < 2, so no point falls out).
+Layers layers_from_rows(const CTree& t, const LayerSelection& s) {
+  const int64_t n = t.n, nl = (int64_t)s.sel.size();
+  std::vector<int32_t> lab32(nl * n);
+  std::vector<float> str(nl * n);
+  std::vector<int64_t> nc(nl);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t l = 0; l < nl; ++l) {
+    std::vector<int64_t> lab(n);
+    labels_and_strengths(t, s.sel[l], lab.data(), str.data() + l * n);
+    for (int64_t i = 0; i < n; ++i) lab32[l * n + i] = (int32_t)lab[i];
+    nc[l] = n_clusters(lab);
   }
-  const CTree tl = condense(L, lo), th = condense(L, hi);
-  const std::vector<int64_t> vl = leaves(tl), vh = leaves(th);
-  std::vector<int64_t> ll(n), lh(n);
-  std::vector<float> sl(n), sh(n);
-  labels_and_strengths(tl, vl, ll.data(), sl.data());
-  labels_and_strengths(th, vh, lh.data(), sh.data());
-  const int64_t dl = std::llabs((int64_t)vl.size() - target_k);
-  const int64_t dh = std::llabs((int64_t)vh.size() - target_k);
-  bool use_lo;
-  if (dl < dh) use_lo = true;
-  else if (dh < dl) use_lo = false;
-  else {
-    int64_t al = 0, ah = 0;
-    for (auto x : ll) al += x >= 0;
-    for (auto x : lh) ah += x >= 0;
-    use_lo = al >= ah;
-  }
-  std::copy((use_lo ? ll : lh).begin(), (use_lo ? ll : lh).end(), labels);
-  std::copy((use_lo ? sl : sh).begin(), (use_lo ? sl : sh).end(), strengths);
+  return order_layers(n, lab32, str, nc, s.pers);
 }
 
 int best_layer_index(const std::vector<double>& p) {

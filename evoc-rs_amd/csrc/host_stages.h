@@ -35,20 +35,42 @@ struct CTree {
   std::vector<float> lambda;
   int64_t n = 0;
 };
-CTree condense(const Linkage& L, int64_t mcs);
 std::vector<int64_t> leaves(const CTree& t);
 void labels_and_strengths(const CTree& t, const std::vector<int64_t>& clusters, int64_t* labels,
                           float* strengths);
+
+// cluster-level views of a device condensation (min_cluster_size >= 2)
+CTree split_rows(const DevCondensed& c);
+std::vector<int64_t> split_leaves(const DevCondensed& c);
+void layer_tables(const DevCondensed& c, const std::vector<std::vector<int64_t>>& sel,
+                  std::vector<int32_t>& clab, std::vector<int32_t>& dsel,
+                  std::vector<int64_t>& nsel);
+// the approx_n_clusters binary search's final bracket [lo, hi] of min_cluster_size
+void search_bounds(const Linkage& L, int64_t target_k, int64_t& lo, int64_t& hi);
 
 struct Layers {
   std::vector<std::vector<int64_t>> labels;
   std::vector<std::vector<float>> strengths;
   std::vector<double> persistence;
 };
-Layers cluster_layers_from_linkage(const Linkage& L, int64_t base_mcs, double min_sim,
-                                   int64_t max_layers);
-void search_n_clusters_from_linkage(const Linkage& L, int64_t target_k, int64_t* labels,
-                                    float* strengths);
+// the selected clusters of every layer and its persistence, from rows holding at least
+// every cluster row (size > 1) in the condensed tree's order, given the base layer's leaves
+struct LayerSelection {
+  std::vector<std::vector<int64_t>> sel;
+  std::vector<double> pers;
+};
+LayerSelection select_layers(const CTree& t, std::vector<int64_t> base, int64_t base_mcs,
+                             double min_sim, int64_t max_layers);
+Layers order_layers(int64_t n, const std::vector<int32_t>& lab32, const std::vector<float>& str,
+                    const std::vector<int64_t>& ncl, const std::vector<double>& pers);
+Layers layers_from_rows(const CTree& t, const LayerSelection& s);
+
+// the tree stages on the device (clusters.cpp): condensation, layers, the n-cluster search
+CTree condense_device(const Linkage& L, int64_t mcs, hipStream_t s, DevPool& pool);
+Layers cluster_layers_device(const Linkage& L, int64_t base_mcs, double min_sim,
+                             int64_t max_layers, hipStream_t s, DevPool& pool);
+void search_n_clusters_device(const Linkage& L, int64_t target_k, int64_t* labels,
+                              float* strengths, hipStream_t s, DevPool& pool);
 int best_layer_index(const std::vector<double>& persistence);
 // adjusted Rand index of two labellings of n points (-1 counts as a label)
 double adjusted_rand_index(const int64_t* a, const int64_t* b, int64_t n);

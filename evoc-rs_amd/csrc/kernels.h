@@ -195,6 +195,50 @@ void core_sq_device(const float* d_emb, int64_t n, int dim, int min_samples, flo
 void mst_device(const float* d_emb, int64_t n, int dim, int min_samples, int32_t* h_u,
                 int32_t* h_v, float* h_w, hipStream_t s, DevPool& pool, int* rounds = nullptr);
 
+// ------------------------------------------------------------------ condensed tree
+// The single-linkage tree (rows as src/clustering/linkage.rs:301-310: row i is node n + i,
+// the root is node 2n - 2) in device memory, with every node's parent, its position in the
+// right-first pre-order walk of condense_tree and the number of points before it there.
+struct DevTree {
+  int64_t n = 0, nr = 0;
+  Buf<int32_t> rows;  // left, right, size: nr each
+  Buf<float> dist;
+  Buf<int32_t> par, pre, lf;  // 2n - 1 nodes each
+  const int32_t* left() const { return rows.p; }
+  const int32_t* right() const { return rows.p + nr; }
+  const int32_t* size() const { return rows.p + 2 * nr; }
+};
+void tree_upload(const int64_t* left, const int64_t* right, const float* dist,
+                 const int64_t* size, int64_t n, DevTree& t, hipStream_t s, DevPool& pool);
+// rows of the condensed tree in the reference's order (src/clustering/condensed_tree.rs:19-37)
+struct CondensedRows {
+  std::vector<int64_t> parent, child, size;
+  std::vector<float> lambda;
+};
+// One condensation. For min_cluster_size >= 2 every point falls out of exactly one
+// cluster: pcl/plam hold, per point, that cluster and lambda (device). Split s (in the
+// walk's order) opens clusters n + 1 + 2s (left child, split_ls points) and n + 2 + 2s
+// (right child) under cluster split_parent[s] at split_lam[s] (host). S splits, E rows.
+struct DevCondensed {
+  int64_t n = 0, mcs = 0, S = 0, E = 0;
+  Buf<int32_t> pcl;
+  Buf<float> plam;
+  std::vector<int32_t> split_parent, split_ls, split_rs;
+  std::vector<float> split_lam;
+};
+// full: also every row, in the reference's order, copied to the host
+void tree_condense(const DevTree& t, int64_t mcs, DevCondensed& out, CondensedRows* full,
+                   hipStream_t s, DevPool& pool);
+// Labels and membership strengths of nl layers of one condensation (min_cluster_size >= 2).
+// Per layer l and cluster c (index c - n, C = 2S + 1 of them): clab[l*C + c] the label of
+// a point that falls out of c (-1: none), dsel[l*C + c] the rank of c among the layer's
+// nsel[l] selected clusters (-1: not selected). Out: labels/strengths nl x n, and per layer
+// stats[2l] = the largest label, stats[2l + 1] = the number of labelled points.
+void tree_layer_labels(const DevCondensed& c, int nl, const std::vector<int32_t>& clab,
+                       const std::vector<int32_t>& dsel, const std::vector<int64_t>& nsel,
+                       std::vector<int32_t>& labels, std::vector<float>& strengths,
+                       std::vector<int>& stats, hipStream_t s, DevPool& pool);
+
 // ------------------------------------------------------------------ sort
 // stable LSD radix sort of (key, value) pairs on the low key_bits bits of key
 void radix_sort_pairs(uint64_t* d_key, uint32_t* d_val, int64_t n, int key_bits, hipStream_t s,

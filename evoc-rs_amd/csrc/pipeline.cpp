@@ -208,12 +208,12 @@ Layers cluster_device(const float* d_emb, int64_t n, int dim, const evoc_params_
   if (p.approx_n_clusters > 0) {
     out.labels.assign(1, std::vector<int64_t>(n));
     out.strengths.assign(1, std::vector<float>(n));
-    search_n_clusters_from_linkage(L, p.approx_n_clusters, out.labels[0].data(),
-                                   out.strengths[0].data());
+    search_n_clusters_device(L, p.approx_n_clusters, out.labels[0].data(),
+                             out.strengths[0].data(), s, pool);
     out.persistence.assign(1, 0.0);
   } else {
-    out = cluster_layers_from_linkage(L, p.base_min_cluster_size, p.min_similarity_threshold,
-                                      p.max_layers);
+    out = cluster_layers_device(L, p.base_min_cluster_size, p.min_similarity_threshold,
+                                p.max_layers, s, pool);
   }
   lap("layers");
   if (layer_ms) *layer_ms = ms_since(t0);
@@ -590,6 +590,46 @@ int evoc_hip_mst(const float* emb, int64_t n, int32_t dim, int32_t min_samples, 
     Buf<float> de;
     to_dev(de, emb, n * dim, c.pool, c.stream);
     mst_device(de.p, n, dim, min_samples, u, v, w, c.stream, c.pool);
+  });
+}
+
+int evoc_hip_condense_tree(const int64_t* left, const int64_t* right, const float* distance,
+                           const int64_t* size, int64_t n, int64_t min_cluster_size,
+                           int64_t* parent, int64_t* child, float* lambda, int64_t* child_size,
+                           int64_t* n_rows) {
+  return guarded([&] {
+    EVOC_CHECK(n_rows && parent && child && lambda && child_size, "condense: NULL buffer");
+    EVOC_CHECK(n >= 1 && n <= (int64_t(1) << 30), "condense: n must be in [1, 2^30]");
+    EVOC_CHECK(n == 1 || (left && right && distance && size), "condense: NULL buffer");
+    EVOC_CHECK(min_cluster_size >= 0, "condense: min_cluster_size must be >= 0");
+    // the rows must form one binary tree over the n points: children of row i are
+    // earlier nodes, each node is a child once, and sizes add up
+    const int64_t nr = n - 1;
+    std::vector<char> used(2 * n, 0);
+    for (int64_t i = 0; i < nr; ++i) {
+      const int64_t l = left[i], r = right[i];
+      EVOC_CHECK(l >= 0 && r >= 0 && l < n + i && r < n + i && l != r && !used[l] && !used[r],
+                 "condense: rows are not a binary tree in linkage order");
+      used[l] = used[r] = 1;
+      const int64_t ls = l < n ? 1 : size[l - n], rs = r < n ? 1 : size[r - n];
+      EVOC_CHECK(size[i] == ls + rs, "condense: a row's size is not the sum of its children's");
+      EVOC_CHECK(std::isfinite(distance[i]) && distance[i] >= 0.0f,
+                 "condense: distances must be finite and >= 0");
+    }
+    Ctx& c = context(0);
+    Linkage L;
+    L.n = n;
+    L.left.assign(left, left + nr);
+    L.right.assign(right, right + nr);
+    L.size.assign(size, size + nr);
+    L.dist.assign(distance, distance + nr);
+    const CTree t = condense_device(L, min_cluster_size, c.stream, c.pool);
+    const size_t m = t.parent.size();
+    std::memcpy(parent, t.parent.data(), m * sizeof(int64_t));
+    std::memcpy(child, t.child.data(), m * sizeof(int64_t));
+    std::memcpy(lambda, t.lambda.data(), m * sizeof(float));
+    std::memcpy(child_size, t.size.data(), m * sizeof(int64_t));
+    *n_rows = (int64_t)m;
   });
 }
 

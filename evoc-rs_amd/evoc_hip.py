@@ -116,6 +116,8 @@ _SIGS = {
                                            C.c_int64, C.c_uint64, f32p]),
     "evoc_hip_core_sq": (C.c_int, [f32p, C.c_int64, C.c_int32, C.c_int32, f32p]),
     "evoc_hip_mst": (C.c_int, [f32p, C.c_int64, C.c_int32, C.c_int32, i32p, i32p, f32p]),
+    "evoc_hip_condense_tree": (C.c_int, [i64p, i64p, f32p, i64p, C.c_int64, C.c_int64, i64p,
+                                         i64p, f32p, i64p, C.POINTER(C.c_int64)]),
     "evoc_hip_cluster_layers": (C.c_int, [f32p, C.c_int64, C.c_int32, C.c_int32, C.c_int64,
                                           C.c_double, C.c_int64, i64p, f32p, f64p,
                                           C.POINTER(C.c_int64)]),
@@ -440,6 +442,31 @@ def mst(emb, min_samples):
     _check(lib().evoc_hip_mst(emb, n, dim, min_samples, u, v, w))
     ne = max(n - 1, 0)
     return u[:ne], v[:ne], w[:ne]
+
+
+def condense_tree(left, right, distance, size, n, min_cluster_size):
+    """condense_tree (src/clustering/condensed_tree.rs:127-248) of n - 1 linkage rows on the
+    device: (parent, child, lambda, child_size) in the reference's row order."""
+    nr = max(n - 1, 0)
+    left = np.ascontiguousarray(left, np.int64)
+    right = np.ascontiguousarray(right, np.int64)
+    distance = np.ascontiguousarray(distance, np.float32)
+    size = np.ascontiguousarray(size, np.int64)
+    if not (left.shape == right.shape == distance.shape == size.shape == (nr,)):
+        raise ValueError(f"condense_tree: need {nr} rows in every array")
+    if nr == 0:  # keep valid pointers for empty inputs
+        left, right, size = (np.zeros(1, np.int64) for _ in range(3))
+        distance = np.zeros(1, np.float32)
+    cap = max(2 * n, 1)
+    P = np.empty(cap, np.int64)
+    Ch = np.empty(cap, np.int64)
+    lam = np.empty(cap, np.float32)
+    CS = np.empty(cap, np.int64)
+    m = C.c_int64(0)
+    _check(lib().evoc_hip_condense_tree(left, right, distance, size, n, min_cluster_size, P, Ch,
+                                        lam, CS, C.byref(m)))
+    k = m.value
+    return P[:k], Ch[:k], lam[:k], CS[:k]
 
 
 def cluster_layers(emb, min_samples=5, base_min_cluster_size=5, min_similarity_threshold=0.2,

@@ -189,6 +189,15 @@ int evoc_hip_core_sq(const float* emb, int64_t n, int32_t dim, int32_t min_sampl
 int evoc_hip_mst(const float* emb, int64_t n, int32_t dim, int32_t min_samples, int32_t* u,
                  int32_t* v, float* w);
 
+/* condense_tree (src/clustering/condensed_tree.rs:127-248) of n - 1 single-linkage rows
+ * (src/clustering/linkage.rs:301-310: row i is node n + i, left/right child ids, distance,
+ * size), on the device. The rows come back in the reference's order into parent, child,
+ * lambda and child_size, which need 2n entries; *n_rows receives their number. */
+int evoc_hip_condense_tree(const int64_t* left, const int64_t* right, const float* distance,
+                           const int64_t* size, int64_t n, int64_t min_cluster_size,
+                           int64_t* parent, int64_t* child, float* lambda, int64_t* child_size,
+                           int64_t* n_rows);
+
 /* build_cluster_layers (src/clustering/persistence.rs:387-463). labels/strengths
  * need max_layers * n entries, persistence max_layers; *n_layers receives the count. */
 int evoc_hip_cluster_layers(const float* emb, int64_t n, int32_t dim, int32_t min_samples,

@@ -231,6 +231,20 @@ extern "C" {
         w: *mut c_float,
     ) -> c_int;
 
+    pub fn evoc_hip_condense_tree(
+        left: *const i64,
+        right: *const i64,
+        distance: *const c_float,
+        size: *const i64,
+        n: i64,
+        min_cluster_size: i64,
+        parent: *mut i64,
+        child: *mut i64,
+        lambda: *mut c_float,
+        child_size: *mut i64,
+        n_rows: *mut i64,
+    ) -> c_int;
+
     pub fn evoc_hip_cluster_layers(
         emb: *const c_float,
         n: i64,

@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 import evoc_hip as H
 
@@ -56,6 +57,26 @@ def test_invalid_arguments_are_status_codes():
     p = H.EvocParams()._c()
     st = L.evoc_hip_run(x.reshape(-1), 1, 3, 3, 1, None, None, 0, C.byref(p), None, 1, 0, C.byref(out))
     assert st == 1 and not out
+
+
+def test_condense_tree_rejects_rows_that_are_not_a_linkage():
+    # rows for n = 4: node 4 = (0, 1), node 5 = (2, 3), node 6 = (4, 5)
+    L = np.array([1, 3, 5], np.int64)
+    R = np.array([0, 2, 4], np.int64)
+    D = np.array([0.5, 0.7, 1.0], np.float32)
+    S = np.array([2, 2, 4], np.int64)
+    bad = [
+        (np.array([1, 1, 5]), R, D, S, "binary tree"),   # point 1 has two parents
+        (np.array([1, 6, 5]), R, D, S, "binary tree"),   # child is not an earlier node
+        (L, R, D, np.array([2, 2, 5]), "size"),          # sizes do not add up
+        (L, R, np.array([0.5, np.nan, 1.0]), S, "finite"),
+        (L, R, np.array([0.5, -0.1, 1.0]), S, "finite"),
+    ]
+    for l, r, d, s, msg in bad:
+        with pytest.raises(H.EvocError, match=msg):
+            H.condense_tree(l, r, d, s, 4, 2)
+    with pytest.raises(ValueError):
+        H.condense_tree(L[:2], R, D, S, 4, 2)
 
 
 def test_device_count_without_gpu_is_not_an_error():
