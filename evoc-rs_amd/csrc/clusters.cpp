@@ -6,6 +6,8 @@
 // the nearest selected ancestor of every cluster) run on the host, on the split records
 // the condensation returns. The approx_n_clusters search (src/lib.rs:377-447) counts
 // leaves per probe without condensing (search_bounds) and condenses twice at the end.
+#include <chrono>
+#include <cstdio>
 #include <cstring>
 
 #include "host_stages.h"
@@ -46,15 +48,12 @@ LeafLayer leaf_layer(const DevTree& t, int64_t m, hipStream_t s, DevPool& pool) 
     tree_condense(t, m, c, nullptr, s, pool);
     std::vector<std::vector<int64_t>> sel(1, split_leaves(c));
     out.nleaves = (int64_t)sel[0].size();
-    std::vector<int32_t> clab, dsel, lab32;
-    std::vector<int64_t> nsel;
-    std::vector<float> str;
-    std::vector<int> stats;
+    std::vector<int32_t> clab, dsel;
+    std::vector<int64_t> nsel, order, ncl, assigned;
     layer_tables(c, sel, clab, dsel, nsel);
-    tree_layer_labels(c, 1, clab, dsel, nsel, lab32, str, stats, s, pool);
-    for (int64_t i = 0; i < n; ++i) out.labels[i] = lab32[i];
-    out.strengths = std::move(str);
-    out.assigned = stats[1];
+    tree_layer_labels(c, 1, clab, dsel, nsel, out.labels.data(), out.strengths.data(), order,
+                      ncl, assigned, s, pool);
+    out.assigned = assigned[0];
   } else {
     CondensedRows rows;
     DevCondensed c;
@@ -80,17 +79,25 @@ CTree condense_device(const Linkage& L, int64_t mcs, hipStream_t s, DevPool& poo
 }
 
 Layers cluster_layers_device(const Linkage& L, int64_t base_mcs, double min_sim,
-                             int64_t max_layers, hipStream_t s, DevPool& pool) {
+                             int64_t max_layers, hipStream_t s, DevPool& pool, int verbose) {
   const int64_t n = L.n;
+  auto tl = std::chrono::steady_clock::now();
+  auto lap = [&](const char* what) {
+    if (verbose < 2) return;
+    const auto t = std::chrono::steady_clock::now();
+    std::printf("      %s: %.1f ms\n", what,
+                std::chrono::duration<double, std::milli>(t - tl).count());
+    tl = t;
+  };
   if (n < 2) {  // an empty condensed tree: one layer, nothing labelled
-    Layers out;
-    out.labels.assign(1, std::vector<int64_t>(n, -1));
-    out.strengths.assign(1, std::vector<float>(n, 0.0f));
-    out.persistence.assign(1, 0.0);
+    Layers out(n, 1);
+    std::fill(out.labels, out.labels + n, -1);
+    std::fill(out.strengths, out.strengths + n, 0.0f);
     return out;
   }
   DevTree t;
   upload(L, t, s, pool);
+  lap("tree upload and walk order");
   if (base_mcs < 2) {
     CondensedRows rows;
     DevCondensed c;
@@ -101,18 +108,20 @@ Layers cluster_layers_device(const Linkage& L, int64_t base_mcs, double min_sim,
   }
   DevCondensed c;
   tree_condense(t, base_mcs, c, nullptr, s, pool);
+  lap("condense");
   const LayerSelection sel =
       select_layers(split_rows(c), split_leaves(c), base_mcs, min_sim, max_layers);
+  lap("layer selection");
   const int nl = (int)sel.sel.size();
-  std::vector<int32_t> clab, dsel, lab32;
-  std::vector<int64_t> nsel;
-  std::vector<float> str;
-  std::vector<int> stats;
+  std::vector<int32_t> clab, dsel;
+  std::vector<int64_t> nsel, order, ncl, assigned;
   layer_tables(c, sel.sel, clab, dsel, nsel);
-  tree_layer_labels(c, nl, clab, dsel, nsel, lab32, str, stats, s, pool);
-  std::vector<int64_t> ncl(nl);
-  for (int l = 0; l < nl; ++l) ncl[l] = (int64_t)stats[2 * l] + 1;
-  return order_layers(n, lab32, str, ncl, sel.pers);
+  Layers out(n, nl);
+  tree_layer_labels(c, nl, clab, dsel, nsel, out.labels, out.strengths, order, ncl, assigned, s,
+                    pool);
+  for (int q = 0; q < nl; ++q) out.persistence[q] = sel.pers[order[q]];
+  lap("labels and strengths");
+  return out;
 }
 
 void search_n_clusters_device(const Linkage& L, int64_t target_k, int64_t* labels,

@@ -326,12 +326,6 @@ double jaccard(const Bar& b, double x, double y) {
   return uni == 0 ? 0.0 : (double)inter / (double)uni;
 }
 
-int64_t n_clusters(const std::vector<int64_t>& l) {
-  int64_t m = -1;
-  for (auto x : l)
-    if (x >= 0) m = std::max(m, x);
-  return m + 1;
-}
 
 }  // namespace
 
@@ -376,26 +370,60 @@ LayerSelection select_layers(const CTree& t, std::vector<int64_t> base, int64_t 
   return out;
 }
 
-// Layers in descending order of their cluster counts (a stable sort), labels widened to
-// int64. lab32/str hold nl x n values; ncl[l] is layer l's number of clusters.
-Layers order_layers(int64_t n, const std::vector<int32_t>& lab32, const std::vector<float>& str,
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+  constexpr size_t CH = size_t(1) << 21;
+  const int64_t nch = (int64_t)((bytes + CH - 1) / CH);
+#pragma omp parallel for schedule(static) if (nch > 1)
+  for (int64_t c = 0; c < nch; ++c) {
+    const size_t a = (size_t)c * CH, e = std::min(bytes, a + CH);
+    std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, e - a);
+  }
+}
+
+Layers::Layers(int64_t n_, int nl) : n(n_) {
+  const size_t m = (size_t)std::max(nl, 1) * (size_t)std::max<int64_t>(n_, 1);
+  labels = static_cast<int64_t*>(std::malloc(m * sizeof(int64_t)));
+  strengths = static_cast<float*>(std::malloc(m * sizeof(float)));
+  if (!labels || !strengths) {
+    std::free(labels);
+    std::free(strengths);
+    throw std::bad_alloc();
+  }
+  persistence.assign(nl, 0.0);
+}
+Layers::~Layers() {
+  std::free(labels);
+  std::free(strengths);
+}
+Layers::Layers(Layers&& o) noexcept
+    : n(o.n), labels(o.labels), strengths(o.strengths), persistence(std::move(o.persistence)) {
+  o.labels = nullptr;
+  o.strengths = nullptr;
+}
+Layers& Layers::operator=(Layers&& o) noexcept {
+  if (this != &o) {
+    std::free(labels);
+    std::free(strengths);
+    n = o.n;
+    labels = o.labels;
+    strengths = o.strengths;
+    persistence = std::move(o.persistence);
+    o.labels = nullptr;
+    o.strengths = nullptr;
+  }
+  return *this;
+}
+
+Layers order_layers(int64_t n, const int64_t* lab, const float* str,
                     const std::vector<int64_t>& ncl, const std::vector<double>& pers) {
-  const int64_t nl = (int64_t)ncl.size();
-  std::vector<int64_t> order(nl);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int64_t a, int64_t c) { return ncl[c] < ncl[a]; });
-  Layers out;
-  out.labels.assign(nl, std::vector<int64_t>(n));
-  out.strengths.assign(nl, std::vector<float>(n));
-  for (int64_t q = 0; q < nl; ++q) {
+  const int nl = (int)ncl.size();
+  const std::vector<int64_t> order = order_by_clusters(ncl);
+  Layers out(n, nl);
+  for (int q = 0; q < nl; ++q) {
     const int64_t o = order[q];
-    const int32_t* src = lab32.data() + o * n;
-    int64_t* dst = out.labels[q].data();
-#pragma omp parallel for schedule(static) if (n > (1 << 16))
-    for (int64_t i = 0; i < n; ++i) dst[i] = src[i];
-    std::memcpy(out.strengths[q].data(), str.data() + o * n, n * sizeof(float));
-    out.persistence.push_back(pers[o]);
+    std::memcpy(out.labels + (size_t)q * n, lab + (size_t)o * n, n * sizeof(int64_t));
+    std::memcpy(out.strengths + (size_t)q * n, str + (size_t)o * n, n * sizeof(float));
+    out.persistence[q] = pers[o];
   }
   return out;
 }
@@ -404,17 +432,17 @@ Layers order_layers(int64_t n, const std::vector<int32_t>& lab32, const std::vec
 // points are clusters of their own, min_cluster_size < 2, so no point falls out).
 Layers layers_from_rows(const CTree& t, const LayerSelection& s) {
   const int64_t n = t.n, nl = (int64_t)s.sel.size();
-  std::vector<int32_t> lab32(nl * n);
+  std::vector<int64_t> lab(nl * n);
   std::vector<float> str(nl * n);
   std::vector<int64_t> nc(nl);
 #pragma omp parallel for schedule(dynamic, 1)
   for (int64_t l = 0; l < nl; ++l) {
-    std::vector<int64_t> lab(n);
-    labels_and_strengths(t, s.sel[l], lab.data(), str.data() + l * n);
-    for (int64_t i = 0; i < n; ++i) lab32[l * n + i] = (int32_t)lab[i];
-    nc[l] = n_clusters(lab);
+    labels_and_strengths(t, s.sel[l], lab.data() + l * n, str.data() + l * n);
+    int64_t m = -1;
+    for (int64_t i = 0; i < n; ++i) m = std::max(m, lab[l * n + i]);
+    nc[l] = m + 1;
   }
-  return order_layers(n, lab32, str, nc, s.pers);
+  return order_layers(n, lab.data(), str.data(), nc, s.pers);
 }
 
 int best_layer_index(const std::vector<double>& p) {

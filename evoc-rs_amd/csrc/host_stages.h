@@ -48,10 +48,23 @@ void layer_tables(const DevCondensed& c, const std::vector<std::vector<int64_t>>
 // the approx_n_clusters binary search's final bracket [lo, hi] of min_cluster_size
 void search_bounds(const Linkage& L, int64_t target_k, int64_t& lo, int64_t& hi);
 
+// The layers of a clustering in host memory: labels and strengths nl x n (malloc'd, so a
+// result can take them over), in descending order of cluster count, and each layer's
+// persistence.
 struct Layers {
-  std::vector<std::vector<int64_t>> labels;
-  std::vector<std::vector<float>> strengths;
+  int64_t n = 0;
+  int64_t* labels = nullptr;
+  float* strengths = nullptr;
   std::vector<double> persistence;
+  Layers() = default;
+  Layers(int64_t n_, int nl);
+  ~Layers();
+  Layers(Layers&& o) noexcept;
+  Layers& operator=(Layers&& o) noexcept;
+  Layers(const Layers&) = delete;
+  Layers& operator=(const Layers&) = delete;
+  int nl() const { return (int)persistence.size(); }
+  const int64_t* layer(int l) const { return labels + (size_t)l * n; }
 };
 // the selected clusters of every layer and its persistence, from rows holding at least
 // every cluster row (size > 1) in the condensed tree's order, given the base layer's leaves
@@ -61,14 +74,15 @@ struct LayerSelection {
 };
 LayerSelection select_layers(const CTree& t, std::vector<int64_t> base, int64_t base_mcs,
                              double min_sim, int64_t max_layers);
-Layers order_layers(int64_t n, const std::vector<int32_t>& lab32, const std::vector<float>& str,
+// layers given in selection order (nl x n) put in descending order of cluster count
+Layers order_layers(int64_t n, const int64_t* lab, const float* str,
                     const std::vector<int64_t>& ncl, const std::vector<double>& pers);
 Layers layers_from_rows(const CTree& t, const LayerSelection& s);
 
 // the tree stages on the device (clusters.cpp): condensation, layers, the n-cluster search
 CTree condense_device(const Linkage& L, int64_t mcs, hipStream_t s, DevPool& pool);
 Layers cluster_layers_device(const Linkage& L, int64_t base_mcs, double min_sim,
-                             int64_t max_layers, hipStream_t s, DevPool& pool);
+                             int64_t max_layers, hipStream_t s, DevPool& pool, int verbose = 0);
 void search_n_clusters_device(const Linkage& L, int64_t target_k, int64_t* labels,
                               float* strengths, hipStream_t s, DevPool& pool);
 int best_layer_index(const std::vector<double>& persistence);

@@ -42,6 +42,19 @@ class DevPool {
     std::lock_guard<std::mutex> g(mu_);
     release_locked();
   }
+  // One pinned host staging buffer of at least `bytes` (grown on demand, contents not
+  // kept), for device-to-host copies at full link speed; one user at a time.
+  void* pinned(size_t bytes) {
+    std::lock_guard<std::mutex> g(mu_);
+    if (bytes > pinned_bytes_) {
+      if (pinned_) (void)hipHostFree(pinned_);
+      pinned_ = nullptr;
+      pinned_bytes_ = 0;
+      EVOC_HIP(hipHostMalloc(&pinned_, bytes, hipHostMallocDefault));
+      pinned_bytes_ = bytes;
+    }
+    return pinned_;
+  }
   ~DevPool() { release(); }
 
  private:
@@ -58,9 +71,14 @@ class DevPool {
   void release_locked() {
     for (auto& kv : free_) (void)hipFree(kv.second);
     free_.clear();
+    if (pinned_) (void)hipHostFree(pinned_);
+    pinned_ = nullptr;
+    pinned_bytes_ = 0;
   }
   std::mutex mu_;
   std::multimap<size_t, void*> free_;
+  void* pinned_ = nullptr;
+  size_t pinned_bytes_ = 0;
 };
 
 template <typename T>
@@ -232,12 +250,25 @@ void tree_condense(const DevTree& t, int64_t mcs, DevCondensed& out, CondensedRo
 // Labels and membership strengths of nl layers of one condensation (min_cluster_size >= 2).
 // Per layer l and cluster c (index c - n, C = 2S + 1 of them): clab[l*C + c] the label of
 // a point that falls out of c (-1: none), dsel[l*C + c] the rank of c among the layer's
-// nsel[l] selected clusters (-1: not selected). Out: labels/strengths nl x n, and per layer
-// stats[2l] = the largest label, stats[2l + 1] = the number of labelled points.
+// nsel[l] selected clusters (-1: not selected). Per layer: ncl[l] clusters (largest label
+// + 1) and assigned[l] labelled points. The layers are copied to the host in descending
+// order of ncl (a stable sort, order[q] = the layer at position q): labels (int64) and
+// strengths of order[q] to labels + q*n and strengths + q*n.
 void tree_layer_labels(const DevCondensed& c, int nl, const std::vector<int32_t>& clab,
                        const std::vector<int32_t>& dsel, const std::vector<int64_t>& nsel,
-                       std::vector<int32_t>& labels, std::vector<float>& strengths,
-                       std::vector<int>& stats, hipStream_t s, DevPool& pool);
+                       int64_t* labels, float* strengths, std::vector<int64_t>& order,
+                       std::vector<int64_t>& ncl, std::vector<int64_t>& assigned, hipStream_t s,
+                       DevPool& pool);
+// memcpy by several host threads (large copies into freshly allocated host memory)
+void parallel_copy(void* dst, const void* src, size_t bytes);
+// positions of layers sorted by descending cluster count, ties in layer order
+inline std::vector<int64_t> order_by_clusters(const std::vector<int64_t>& ncl) {
+  std::vector<int64_t> order(ncl.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int64_t)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t a, int64_t b) { return ncl[b] < ncl[a]; });
+  return order;
+}
 
 // ------------------------------------------------------------------ sort
 // stable LSD radix sort of (key, value) pairs on the low key_bits bits of key

@@ -134,7 +134,7 @@ T* host_alloc(size_t count) {
   return static_cast<T*>(p);
 }
 
-evoc_result_t* alloc_result(int64_t n, int k, int dim, int nl, HostOut& take) {
+evoc_result_t* alloc_result(int64_t n, int k, int dim, int nl, HostOut& take, Layers& lay) {
   auto* r = static_cast<evoc_result_t*>(std::calloc(1, sizeof(evoc_result_t)));
   if (!r) throw std::bad_alloc();
   r->n = n;
@@ -150,14 +150,16 @@ evoc_result_t* alloc_result(int64_t n, int k, int dim, int nl, HostOut& take) {
   std::swap(r->nn_indices, take.idx);
   std::swap(r->nn_distances, take.dist);
   std::swap(r->embedding, take.emb);
+  // the layers' labels and strengths likewise (malloc'd nl x n arrays)
+  std::swap(r->cluster_layers, lay.labels);
+  std::swap(r->membership_strengths, lay.strengths);
   try {
-    r->cluster_layers = static_cast<int64_t*>(al((size_t)nl * n * sizeof(int64_t)));
-    r->membership_strengths = static_cast<float*>(al((size_t)nl * n * sizeof(float)));
     r->persistence_scores = static_cast<double*>(al((size_t)nl * sizeof(double)));
   } catch (...) {
     free_result(r);
     throw;
   }
+  for (int l = 0; l < nl; ++l) r->persistence_scores[l] = lay.persistence[l];
   return r;
 }
 
@@ -206,14 +208,11 @@ Layers cluster_device(const float* d_emb, int64_t n, int dim, const evoc_params_
   if (mst_ms) *mst_ms = ms_since(t0);
   t0 = clk::now();
   if (p.approx_n_clusters > 0) {
-    out.labels.assign(1, std::vector<int64_t>(n));
-    out.strengths.assign(1, std::vector<float>(n));
-    search_n_clusters_device(L, p.approx_n_clusters, out.labels[0].data(),
-                             out.strengths[0].data(), s, pool);
-    out.persistence.assign(1, 0.0);
+    out = Layers(n, 1);
+    search_n_clusters_device(L, p.approx_n_clusters, out.labels, out.strengths, s, pool);
   } else {
     out = cluster_layers_device(L, p.base_min_cluster_size, p.min_similarity_threshold,
-                                p.max_layers, s, pool);
+                                p.max_layers, s, pool, verbose);
   }
   lap("layers");
   if (layer_ms) *layer_ms = ms_since(t0);
@@ -312,9 +311,8 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
       double mst_plain = 0.0, mst_warm = 0.0;
       Layers lp = cluster_device(d_plain, S, dim, p, s, pool, nullptr, nullptr, 0, &mst_plain);
       Layers lw = cluster_device(dw.p, S, dim, p, s, pool, nullptr, nullptr, 0, &mst_warm);
-      const auto& bp = lp.labels[best_layer_index(lp.persistence)];
-      const auto& bw = lw.labels[best_layer_index(lw.persistence)];
-      w.judge_ari = adjusted_rand_index(bp.data(), bw.data(), S);
+      w.judge_ari = adjusted_rand_index(lp.layer(best_layer_index(lp.persistence)),
+                                        lw.layer(best_layer_index(lw.persistence)), S);
       w.judge_mst_ratio = mst_plain > 0.0 ? mst_warm / mst_plain : 1.0;
       return w.judge_ari >= 0.98 && std::fabs(w.judge_mst_ratio - 1.0) <= 0.01;
     };
@@ -394,15 +392,10 @@ evoc_result_t* run_impl(const float* dX, const float* hX, int64_t n, int64_t d, 
   // 5./6. clustering
   if (verbose) std::printf("Running density-based clustering...\n");
   Layers lay = cluster_device(demb.p, n, dim, p, s, pool, &stage[4], &stage[5], verbose);
-  const int nl = (int)lay.labels.size();
+  const int nl = lay.nl();
   knn_back.finish();
   emb_back.finish();
-  evoc_result_t* r = alloc_result(n, k, dim, nl, ho);
-  for (int l = 0; l < nl; ++l) {
-    std::memcpy(r->cluster_layers + (size_t)l * n, lay.labels[l].data(), n * sizeof(int64_t));
-    std::memcpy(r->membership_strengths + (size_t)l * n, lay.strengths[l].data(), n * sizeof(float));
-    r->persistence_scores[l] = lay.persistence[l];
-  }
+  evoc_result_t* r = alloc_result(n, k, dim, nl, ho, lay);
   r->best_layer = best_layer_index(lay.persistence);
   r->knn_fallback_rows = fallback;
   r->warm_start_used = warm_used ? 1 : 0;
@@ -651,12 +644,10 @@ int evoc_hip_cluster_layers(const float* emb, int64_t n, int32_t dim, int32_t mi
     p.min_similarity_threshold = min_sim;
     p.max_layers = (int32_t)max_layers;
     Layers lay = cluster_device(de.p, n, dim, p, c.stream, c.pool, nullptr, nullptr);
-    const int64_t nl = std::min<int64_t>((int64_t)lay.labels.size(), std::max<int64_t>(max_layers, 1));
-    for (int64_t l = 0; l < nl; ++l) {
-      std::memcpy(labels + l * n, lay.labels[l].data(), n * sizeof(int64_t));
-      std::memcpy(strengths + l * n, lay.strengths[l].data(), n * sizeof(float));
-      persistence[l] = lay.persistence[l];
-    }
+    const int64_t nl = std::min<int64_t>(lay.nl(), std::max<int64_t>(max_layers, 1));
+    std::memcpy(labels, lay.labels, (size_t)nl * n * sizeof(int64_t));
+    std::memcpy(strengths, lay.strengths, (size_t)nl * n * sizeof(float));
+    for (int64_t l = 0; l < nl; ++l) persistence[l] = lay.persistence[l];
     *n_layers = nl;
   });
 }
@@ -676,8 +667,8 @@ int evoc_hip_search_n_clusters(const float* emb, int64_t n, int32_t dim, int32_t
     p.min_samples = min_samples;
     p.approx_n_clusters = (int32_t)std::max<int64_t>(target_k, 1);
     Layers lay = cluster_device(de.p, n, dim, p, c.stream, c.pool, nullptr, nullptr);
-    std::memcpy(labels, lay.labels[0].data(), n * sizeof(int64_t));
-    std::memcpy(strengths, lay.strengths[0].data(), n * sizeof(float));
+    std::memcpy(labels, lay.labels, n * sizeof(int64_t));
+    std::memcpy(strengths, lay.strengths, n * sizeof(float));
   });
 }
 

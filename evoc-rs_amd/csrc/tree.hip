@@ -288,7 +288,7 @@ __global__ __launch_bounds__(TT) void tree_death_kernel(const int32_t* __restric
 __global__ __launch_bounds__(TT) void tree_labels_kernel(
     const int32_t* __restrict__ pcl, const float* __restrict__ plam, int64_t n, int nl,
     int64_t C, const int32_t* __restrict__ clab, const int64_t* __restrict__ doff,
-    const unsigned* __restrict__ death, int32_t* __restrict__ labels,
+    const unsigned* __restrict__ death, int64_t* __restrict__ labels,
     float* __restrict__ strengths, int* __restrict__ stats) {
   const int64_t p = (int64_t)blockIdx.x * TT + threadIdx.x;
   const bool have = p < n;
@@ -437,16 +437,17 @@ void tree_condense(const DevTree& t, int64_t mcs, DevCondensed& out, CondensedRo
 
 void tree_layer_labels(const DevCondensed& c, int nl, const std::vector<int32_t>& clab,
                        const std::vector<int32_t>& dsel, const std::vector<int64_t>& nsel,
-                       std::vector<int32_t>& labels, std::vector<float>& strengths,
-                       std::vector<int>& stats, hipStream_t s, DevPool& pool) {
+                       int64_t* labels, float* strengths, std::vector<int64_t>& order,
+                       std::vector<int64_t>& ncl, std::vector<int64_t>& assigned, hipStream_t s,
+                       DevPool& pool) {
   const int64_t n = c.n, C = 2 * c.S + 1;
   EVOC_CHECK(c.mcs >= 2, "tree_layer_labels: needs min_cluster_size >= 2");
   EVOC_CHECK((int64_t)clab.size() == nl * C && (int64_t)dsel.size() == nl * C &&
                  (int64_t)nsel.size() == nl,
              "tree_layer_labels: table sizes");
-  labels.resize((size_t)nl * n);
-  strengths.resize((size_t)nl * n);
-  stats.assign(2 * nl, 0);
+  order.clear();
+  ncl.assign(nl, 0);
+  assigned.assign(nl, 0);
   if (nl == 0 || n == 0) return;
   std::vector<int64_t> doff(nl);
   int64_t nd = 0;
@@ -454,8 +455,8 @@ void tree_layer_labels(const DevCondensed& c, int nl, const std::vector<int32_t>
     doff[l] = nd;
     nd += nsel[l];
   }
-  Buf<int32_t> dclab(pool, nl * C), ddsel(pool, nl * C), dlab(pool, (size_t)nl * n);
-  Buf<int64_t> ddoff(pool, nl);
+  Buf<int32_t> dclab(pool, nl * C), ddsel(pool, nl * C);
+  Buf<int64_t> ddoff(pool, nl), dlab(pool, (size_t)nl * n);
   Buf<unsigned> death(pool, std::max<int64_t>(nd, 1));
   Buf<float> dstr(pool, (size_t)nl * n);
   Buf<int> dstats(pool, 2 * nl);
@@ -468,12 +469,27 @@ void tree_layer_labels(const DevCondensed& c, int nl, const std::vector<int32_t>
   tree_labels_kernel<<<grid_of(n), TT, 0, s>>>(c.pcl.p, c.plam.p, n, nl, C, dclab.p, ddoff.p,
                                                death.p, dlab.p, dstr.p, dstats.p);
   EVOC_HIP(hipGetLastError());
-  EVOC_HIP(hipMemcpyAsync(labels.data(), dlab.p, (size_t)nl * n * sizeof(int32_t),
-                          hipMemcpyDeviceToHost, s));
-  EVOC_HIP(hipMemcpyAsync(strengths.data(), dstr.p, (size_t)nl * n * sizeof(float),
-                          hipMemcpyDeviceToHost, s));
+  std::vector<int> stats(2 * nl);
   EVOC_HIP(hipMemcpyAsync(stats.data(), dstats.p, 2 * nl * sizeof(int), hipMemcpyDeviceToHost, s));
   EVOC_HIP(hipStreamSynchronize(s));
+  for (int l = 0; l < nl; ++l) {
+    ncl[l] = (int64_t)stats[2 * l] + 1;
+    assigned[l] = stats[2 * l + 1];
+  }
+  order = order_by_clusters(ncl);
+  // through the pool's pinned staging buffer: the copy runs at link speed, and the
+  // destination's first touch is spread over host threads
+  const size_t lb = (size_t)n * sizeof(int64_t), sb = (size_t)n * sizeof(float);
+  char* stage = static_cast<char*>(pool.pinned((size_t)nl * (lb + sb)));
+  for (int q = 0; q < nl; ++q) {
+    const int64_t o = order[q];
+    EVOC_HIP(hipMemcpyAsync(stage + q * lb, dlab.p + (size_t)o * n, lb, hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipMemcpyAsync(stage + nl * lb + q * sb, dstr.p + (size_t)o * n, sb,
+                            hipMemcpyDeviceToHost, s));
+  }
+  EVOC_HIP(hipStreamSynchronize(s));
+  parallel_copy(labels, stage, nl * lb);
+  parallel_copy(strengths, stage + nl * lb, nl * sb);
 }
 
 }  // namespace evoc

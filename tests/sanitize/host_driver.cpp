@@ -91,7 +91,7 @@ static evoc::DevCondensed as_condensed(const evoc::CTree& t, int64_t m, std::vec
 static void label_loop(const evoc::DevCondensed& c, const std::vector<int32_t>& pcl,
                        const std::vector<float>& plam, int nl, const std::vector<int32_t>& clab,
                        const std::vector<int32_t>& dsel, const std::vector<int64_t>& nsel,
-                       std::vector<int32_t>& lab, std::vector<float>& str,
+                       std::vector<int64_t>& lab, std::vector<float>& str,
                        std::vector<int64_t>& ncl, std::vector<int64_t>& assigned) {
   const int64_t n = c.n, C = 2 * c.S + 1;
   lab.assign(nl * n, -1);
@@ -145,32 +145,33 @@ static void run_case(int64_t n, int dim, int centres, uint32_t seed) {
     REQUIRE(evoc::split_leaves(c) == evoc::leaves(full));
     const evoc::LayerSelection sel =
         evoc::select_layers(evoc::split_rows(c), evoc::split_leaves(c), mcs, 0.2, max_layers);
-    std::vector<int32_t> clab, dsel, lab;
-    std::vector<int64_t> nsel, ncl, assigned;
+    std::vector<int32_t> clab, dsel;
+    std::vector<int64_t> lab, nsel, ncl, assigned;
     std::vector<float> str;
     evoc::layer_tables(c, sel.sel, clab, dsel, nsel);
     label_loop(c, pcl, plam, (int)sel.sel.size(), clab, dsel, nsel, lab, str, ncl, assigned);
-    lay = evoc::order_layers(n, lab, str, ncl, sel.pers);
+    lay = evoc::order_layers(n, lab.data(), str.data(), ncl, sel.pers);
     // the full-row readers (min_cluster_size < 2 path) agree on the same rows
     const evoc::Layers lr = evoc::layers_from_rows(full, evoc::select_layers(
                                                              full, evoc::leaves(full), mcs, 0.2,
                                                              max_layers));
-    REQUIRE(lr.labels == lay.labels && lr.strengths == lay.strengths &&
-            lr.persistence == lay.persistence);
+    REQUIRE(lr.nl() == lay.nl() && lr.persistence == lay.persistence);
+    REQUIRE(std::memcmp(lr.labels, lay.labels, lay.nl() * n * 8) == 0);
+    REQUIRE(std::memcmp(lr.strengths, lay.strengths, lay.nl() * n * 4) == 0);
   } else {
-    lay.labels.assign(1, std::vector<int64_t>(n, -1));
-    lay.strengths.assign(1, std::vector<float>(n, 0.0f));
-    lay.persistence.assign(1, 0.0);
+    lay = evoc::Layers(n, 1);
+    std::fill(lay.labels, lay.labels + n, -1);
+    std::fill(lay.strengths, lay.strengths + n, 0.0f);
   }
-  REQUIRE((int64_t)lay.labels.size() == onl);
+  REQUIRE((int64_t)lay.nl() == onl);
   for (int64_t l = 0; l < onl; ++l) {
-    REQUIRE(std::memcmp(lay.labels[l].data(), &ol[l * n], n * 8) == 0);
-    REQUIRE(std::memcmp(lay.strengths[l].data(), &os[l * n], n * 4) == 0);
+    REQUIRE(std::memcmp(lay.layer(l), &ol[l * n], n * 8) == 0);
+    REQUIRE(std::memcmp(lay.strengths + l * n, &os[l * n], n * 4) == 0);
     REQUIRE(std::memcmp(&lay.persistence[l], &op[l], 8) == 0);
   }
   const int best = evoc::best_layer_index(lay.persistence);
   REQUIRE(best >= 0 && best < (int)onl);
-  REQUIRE(evoc::adjusted_rand_index(lay.labels[0].data(), lay.labels[0].data(), n) == 1.0);
+  REQUIRE(evoc::adjusted_rand_index(lay.layer(0), lay.layer(0), n) == 1.0);
 
   // the leaf-count identity behind search_bounds, at every probe size
   for (int64_t m = 2; m <= std::max<int64_t>(n / 2, 2) && n >= 2; m += 1 + m / 8) {
