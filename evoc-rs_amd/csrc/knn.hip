@@ -1535,8 +1535,12 @@ __global__ __launch_bounds__(256) void rb_merge_kernel(
 }
 
 // ------------------------------------------------------------------ 3. re-rank
-constexpr int MAXT = 16;  // d <= 1024 handled with the query row held in registers
+// The re-rank and exact kernels hold a query row in registers, T floats per lane: T = 16
+// covers d <= 1024, T = 64 covers d <= 4096 (KNN_MAXD). Lane l sums dimensions l, l + 64,
+// ... in order whatever T is, so both forms give the same canonical distance.
+constexpr int KNN_MAXD = 4096;
 
+template <int MAXT>
 __global__ __launch_bounds__(256) void knn_rerank_kernel(
     const float* __restrict__ X, int64_t n, int d, int64_t ld, const float* __restrict__ nrm,
     const float* __restrict__ sqn, const float* __restrict__ colscale,
@@ -1715,7 +1719,7 @@ __global__ __launch_bounds__(256) void knn_rerank_kernel(
 // the row rather than 64. Every valid candidate (S per lane) gets its exact distance, the
 // best k by (distance, index) are written, and the certificate is the one of
 // knn_rerank_kernel against that bound; a failing row is listed for the exact scan.
-template <int S>
+template <int S, int MAXT>
 __global__ __launch_bounds__(256) void knn_rerank_all_kernel(
     const float* __restrict__ X, int64_t n, int d, int64_t ld, const float* __restrict__ nrm,
     const float* __restrict__ sqn, const float* __restrict__ colscale,
@@ -1897,8 +1901,7 @@ __device__ void exact_block_merge(float* sk, int* si, int k) {
 // column at a time: lanes hold the column's dimensions (the canonical strided partial
 // sums and halving tree of the rerank), and lane f keeps the sorted list of failing
 // row f. Each wavefront writes its lists; the merge kernel combines them.
-constexpr int EX_ROWS = 8;
-
+template <int MAXT, int EX_ROWS>
 __global__ __launch_bounds__(256) void knn_exact_kernel(
     const float* __restrict__ X, int64_t n, int d, int64_t ld, const float* __restrict__ nrm,
     int k, int metric, const int32_t* __restrict__ fail_rows, int nrows, int parts, int64_t span,
@@ -2060,7 +2063,8 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   const bool fp8 = precision == KNN_FP8;
   EVOC_CHECK(!fp8 || metric == 1, "knn: the fp8 precision supports the cosine metric only");
   EVOC_CHECK(k >= 1 && k <= KMAX, "knn: k must be in [1, 32]");
-  EVOC_CHECK(d <= MAXT * 64, "knn: d must be <= 1024");
+  EVOC_CHECK(d <= KNN_MAXD, "knn: d must be <= 4096");
+  const bool wide = d > 1024;  // the re-rank and exact kernels' register-resident rows
   static bool attr_set = false;
   if (!attr_set) {
     // the streaming kernel: at most 134 KB of ring, key tile and thresholds (beside its
@@ -2103,8 +2107,6 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<RB_NKC, RB_KC, false, true>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)(RbRing<RB_KC>::SMEM + RB_QBYTES)));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_exact_kernel,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
     attr_set = true;
   }
   // Two tile kernels: the resident-query kernel for bf16-exact inputs (d <= 512, k <= 16,
@@ -2251,7 +2253,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
             sq.p, S, 0, npad / BN);
       EVOC_HIP(hipGetLastError());
       // (the sampled rows' results land in the output and are rewritten by the search)
-      knn_rerank_kernel<<<(unsigned)ceil_div(S, 4), 256, 0, s>>>(
+      (wide ? knn_rerank_kernel<64> : knn_rerank_kernel<16>)<<<(unsigned)ceil_div(S, 4), 256, 0, s>>>(
           dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, scidx.p, sckey.p, 32, 2, nullptr, k,
           metric, gamma, d_idx, d_dist, sfc.p, sfr.p, sq.p, S);
       EVOC_HIP(hipGetLastError());
@@ -2442,7 +2444,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     nfail = (int)n;
   } else {
     // the FP8 pass's keys and error term are in its own units (row scales s / ||x||)
-    knn_rerank_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(
+    (wide ? knn_rerank_kernel<64> : knn_rerank_kernel<16>)<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(
         dX, n, d, ld, nrm.p, sqn.p, fp8 ? scale8.p : colscale.p, flags.p, sym ? midx.p : cidx.p,
         sym ? mkey.p : ckey.p, C, (ra && dpad < 512) ? 1 : 2, sym ? bnd.p : nullptr, k, metric,
         fp8 ? gamma8 : gamma, d_idx, d_dist, fcount.p, frows.p, nullptr, 0);
@@ -2506,7 +2508,8 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     EVOC_HIP(hipGetLastError());
     if (skip_first) EVOC_HIP(hipEventRecord(e1, s));
     EVOC_HIP(hipMemsetAsync(fcount.p, 0, sizeof(int), s));
-    knn_rerank_all_kernel<KNN_ESC_SHARDS><<<(unsigned)ceil_div(nq, 4), 256, 0, s>>>(
+    (wide ? knn_rerank_all_kernel<KNN_ESC_SHARDS, 64>
+          : knn_rerank_all_kernel<KNN_ESC_SHARDS, 16>)<<<(unsigned)ceil_div(nq, 4), 256, 0, s>>>(
         dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, ecidx.p, eckey.p, k, metric, gamma,
         d_idx, d_dist, fcount.p, frows2.p, qmap.p, nq);
     EVOC_HIP(hipGetLastError());
@@ -2522,21 +2525,22 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     }
   }
   if (nfail > 0) {
-    // the columns are split over `parts` workgroups of four wavefronts, EX_ROWS failing
-    // rows per launch; the per-wavefront lists of each row are then merged
+    // the columns are split over `parts` workgroups of four wavefronts, ex_rows failing
+    // rows per launch (8 held in registers, 2 for wide rows); the per-wavefront lists of
+    // each row are then merged
     const int parts = 1024;
     const int64_t span = ceil_div(n, (int64_t)parts);
-    const int nb = std::min(nfail, EX_ROWS);
+    const int ex_rows = wide ? 2 : 8;
+    const int nb = std::min(nfail, ex_rows);
     Buf<float> pk(pool, (size_t)nb * parts * 4 * k);
     Buf<int32_t> pi(pool, (size_t)nb * parts * 4 * k);
-    for (int f0 = 0; f0 < nfail; f0 += EX_ROWS) {
-      const int rows = std::min(EX_ROWS, nfail - f0);
-      knn_exact_kernel<<<(unsigned)parts, 256, 0, s>>>(dX, n, d, ld, nrm.p, k, metric,
-                                                        frows.p + f0, rows, parts, span, pk.p,
-                                                        pi.p);
+    for (int f0 = 0; f0 < nfail; f0 += ex_rows) {
+      const int rows = std::min(ex_rows, nfail - f0);
+      (wide ? knn_exact_kernel<64, 2> : knn_exact_kernel<16, 8>)<<<(unsigned)parts, 256, 0, s>>>(
+          dX, n, d, ld, nrm.p, k, metric, frows.p + f0, rows, parts, span, pk.p, pi.p);
       knn_exact_merge_kernel<<<(unsigned)rows, 256, 256 * KMAX * 8, s>>>(
           k, frows.p + f0, parts * 4, pk.p, pi.p, d_idx, d_dist);
-      if (verbose && f0 + rows < nfail && ((f0 / EX_ROWS) + 1) % 4096 == 0) {
+      if (verbose && f0 + rows < nfail && ((f0 / ex_rows) + 1) % 4096 == 0) {
         EVOC_HIP(hipStreamSynchronize(s));
         std::printf("  kNN exact search: %d of %d rows done\n", f0 + rows, nfail);
         std::fflush(stdout);

@@ -212,3 +212,30 @@ def test_knn_sampled_certificate_rate_skips_first_pass(sample, monkeypatch):
           f"exact rows {st['fallback_rows']}, tile kernel {st['kernel_ms']:.0f} ms")
     if sample == "1":
         assert st["escalated_rows"] == n
+
+
+@pytest.mark.parametrize("n,d,metric,escalate", [(2000, 1536, "cosine", False),
+                                                 (1200, 3072, "euclidean", False),
+                                                 (1500, 1536, "cosine", True)])
+def test_knn_wide_rows(n, d, metric, escalate, monkeypatch):
+    """Rows wider than 1024 (1536: OpenAI-size sentence embeddings, 3072) take the re-rank
+    and exact kernels that hold 64 values per lane; f32 input (the split first pass), a
+    tight leaf to force escalation, and 30 duplicates for the exact scan."""
+    if escalate:
+        monkeypatch.setenv("EVOC_HIP_KNN_ESCALATE", "1")
+        rng = np.random.default_rng(d)
+        x = (rng.normal(size=d) + 0.02 * rng.normal(size=(n, d))).astype(np.float32)
+    else:
+        x, _ = datagen.unit_mixture(n, d, 25, 0.15, seed=d)
+        x = x.astype(np.float32)
+        x[50:80] = x[3]  # 29 neighbours at distance zero: more than a first pass certifies
+    st = _check(x, 15, metric)
+    print(n, d, metric, st)
+    if not escalate:
+        assert st["fallback_rows"] >= 30
+
+
+def test_knn_rejects_rows_wider_than_4096():
+    x = np.zeros((10, 4097), np.float32)
+    with pytest.raises(H.EvocError, match="4096"):
+        H.knn(x, 3)
