@@ -192,12 +192,12 @@ struct LpTop {
 void lp_top_partition(const int64_t* d_ptr, const int32_t* d_col, const float* d_val, int64_t n,
                       int64_t approx, int n_iter, uint64_t seed, LpTop& P, hipStream_t s,
                       DevPool& pool);
-// R^T_norm G R of the input level, returned as a host CSR (np x np) and, with keep, also
-// left in device memory
+// R^T_norm G R of a level, returned as a host CSR (np x np) and, with keep, also left in
+// device memory (then host_copy = false skips the host CSR of a non-empty graph)
 void lp_top_coarsen(const int64_t* d_ptr, const int32_t* d_col, const float* d_val, int64_t n,
                     int64_t nnz, const LpTop& P, std::vector<int64_t>& cptr,
                     std::vector<int32_t>& ccol, std::vector<float>& cval, hipStream_t s,
-                    DevPool& pool, Graph* keep = nullptr);
+                    DevPool& pool, Graph* keep = nullptr, bool host_copy = true);
 // the embedding slots of a coarse graph in device memory (see coarse_embedding)
 void coarse_slots_device(const Graph& g, Graph& out, hipStream_t s, DevPool& pool);
 // expansion of the input level, centred and scaled, into d_out (n x dim)

@@ -588,8 +588,9 @@ std::vector<float> level(const Sp& g, int dim, const Rows& data, int64_t d, int 
   return res;
 }
 
-// The input level with its graph in device memory: partition, coarse graph, partition
-// means and expansion on the GPU; the coarse levels recurse through level() on the host.
+// One level with its graph in device memory: partition, coarse graph, partition means,
+// coarse embedding and expansion on the GPU, recursing on the device through every coarse
+// level of at least 64 nodes; below that the base case runs through level().
 void top_level(const int64_t* d_ptr, const int32_t* d_col, const float* d_val, int64_t n,
                int64_t nnz, int dim, const Rows& data, int64_t d, int epochs, int64_t approx,
                uint64_t seed, float* d_out, hipStream_t s, DevPool& pool, int verbose) {
@@ -620,24 +621,47 @@ void top_level(const int64_t* d_ptr, const int32_t* d_col, const float* d_val, i
   Sp coarse;
   coarse.rows = coarse.cols = np;
   Graph dcoarse;  // the coarse graph stays in device memory for its expansion and embedding
+  // a coarse level of 64 or more nodes recurses on the device and needs no host copy
   lp_top_coarsen(d_ptr, d_col, d_val, n, nnz, P, coarse.ptr, coarse.col, coarse.val, s, pool,
-                 &dcoarse);
+                 &dcoarse, np < 64);
   lap("coarsen graph");
+  const Graph* dv = dcoarse.n == np ? &dcoarse : nullptr;  // not kept for an empty graph
+  // the coarse level recurses on the device down to the base case (fewer than 64 nodes,
+  // handled by level()); its graph and data stay in device memory
+  const bool recurse = dv != nullptr && np >= 64;
   std::vector<float> cdata;
+  Buf<float> dm;
   if (data.present()) {
-    cdata.resize(np * d);
-    Buf<float> dm(pool, np * d);
+    dm.alloc(pool, np * d);
     partition_means_dev(data.dev, data.ld, d, P.mptr.p, P.mem.p, P.coef.p, np, dm.p, s);
-    EVOC_HIP(hipMemcpyAsync(cdata.data(), dm.p, np * d * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (!recurse) {
+      cdata.resize(np * d);
+      EVOC_HIP(hipMemcpyAsync(cdata.data(), dm.p, np * d * sizeof(float), hipMemcpyDeviceToHost,
+                              s));
+    }
     EVOC_HIP(hipStreamSynchronize(s));
   }
   lap("coarsen data");
   const int ep = std::min(epochs, 255);
-  Rows cr;
-  if (data.present()) cr.host = &cdata;
-  const Graph* dv = dcoarse.n == np ? &dcoarse : nullptr;  // not kept for an empty graph
-  std::vector<float> layout = level(coarse, dim, cr, d, ep, approx / 4, seed + 3000, s, pool,
-                                    verbose, dv);
+  std::vector<float> layout;
+  if (recurse) {
+    Rows cr;
+    if (data.present()) {
+      cr.dev = dm.p;
+      cr.ld = d;
+    }
+    Buf<float> dl(pool, np * dim);
+    top_level(dv->indptr.p, dv->indices.p, dv->weights.p, np, dv->nnz, dim, cr, d, ep,
+              approx / 4, seed + 3000, dl.p, s, pool, verbose);
+    layout.resize(np * dim);
+    EVOC_HIP(hipMemcpyAsync(layout.data(), dl.p, np * dim * sizeof(float), hipMemcpyDeviceToHost,
+                            s));
+    EVOC_HIP(hipStreamSynchronize(s));
+  } else {
+    Rows cr;
+    if (data.present()) cr.host = &cdata;
+    layout = level(coarse, dim, cr, d, ep, approx / 4, seed + 3000, s, pool, verbose, dv);
+  }
   tick = tclk::now();
   if (dv)
     coarse_embedding_dev(*dv, dim, ep, (float)(0.001 * (double)epochs), layout, seed + 4000, s,

@@ -176,6 +176,188 @@ __global__ __launch_bounds__(64 * LP_WAVES) void lp_step_kernel(
   }
 }
 
+// ---- long rows (coarse graphs): one wavefront per node and an LDS sort of its slots
+// Slots are keyed (label or partition << 16 | slot offset), so that after an ascending
+// sort each label's slots are contiguous and in slot order: the per-label sums are the
+// sequential adds of the CSR formulation, and the first slot of a run is the label's first
+// appearance. Labels (seed indices) and partitions are below 16384, and rows up to
+// LP_LONG_MAX slots fit the sort.
+constexpr int LP_LONG_MAX = 16384;
+
+// ascending bitonic sort of cnt (a power of two) keys with their values, one wavefront
+// (the workgroup is one wavefront, so a barrier is only an ordering point)
+__device__ void wave_sort(uint32_t* key, uint32_t* val, int cnt) {
+  const int lane = threadIdx.x;
+  for (int k = 2; k <= cnt; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = lane; t < cnt / 2; t += 64) {
+        const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+        const int l = i + j;
+        const uint32_t a = key[i], b = key[l];
+        if ((a > b) == ((i & k) == 0)) {
+          key[i] = b;
+          key[l] = a;
+          const uint32_t va = val[i];
+          val[i] = val[l];
+          val[l] = va;
+        }
+      }
+      __syncthreads();
+    }
+}
+
+// the slots of row [s0, s1) with a key (16-bit id >= 0) into LDS, in slot order; returns
+// their number, the rest of the power-of-two padding filled with the largest key
+template <typename F>
+__device__ int gather_keys(int64_t s0, int64_t s1, uint32_t* key, uint32_t* val, F&& slot) {
+  const int lane = threadIdx.x;
+  int m = 0;
+  for (int64_t pb = s0; pb < s1; pb += 64) {
+    const int64_t p = pb + lane;
+    int32_t id = -1;
+    float v = 0.0f;
+    if (p < s1) slot(p, id, v);
+    const unsigned long long b = __ballot(id >= 0);
+    if (id >= 0) {
+      const int pos = m + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+      key[pos] = ((uint32_t)id << 16) | (uint32_t)(p - s0);
+      val[pos] = __float_as_uint(v);
+    }
+    m += __popcll(b);
+  }
+  int cnt = 1;
+  while (cnt < m) cnt <<= 1;
+  for (int e = m + lane; e < cnt; e += 64) {
+    key[e] = 0xffffffffu;
+    val[e] = 0u;
+  }
+  __syncthreads();
+  wave_sort(key, val, cnt);
+  return m;
+}
+
+// lp_step_kernel for rows of more than 64 slots: the label of largest summed weight (ties
+// to the later first appearance) when that sum reaches the running best's start of 1.0.
+__global__ __launch_bounds__(64) void lp_step_long_kernel(
+    const int64_t* __restrict__ indptr, const int32_t* __restrict__ col,
+    const float* __restrict__ val, int64_t n, const int64_t* __restrict__ lab_in,
+    int64_t* __restrict__ lab_out) {
+  extern __shared__ __attribute__((aligned(16))) uint32_t lsm[];
+  const int lane = threadIdx.x;
+  const int64_t i = blockIdx.x;
+  const int64_t li = lab_in[i];
+  if (li >= 0) {
+    if (lane == 0) lab_out[i] = li;
+    return;
+  }
+  const int64_t s0 = indptr[i], s1 = indptr[i + 1];
+  const int cap = (int)(s1 - s0) <= 1 ? 1 : 1 << (32 - __clz((int)(s1 - s0) - 1));
+  uint32_t* key = lsm;
+  uint32_t* vv = lsm + cap;
+  const int m = gather_keys(s0, s1, key, vv, [&](int64_t p, int32_t& id, float& v) {
+    const int64_t l = lab_in[col[p]];
+    if (l >= 0) {
+      id = (int32_t)l;
+      v = val[p];
+    }
+  });
+  float bw = -INFINITY;
+  int bfirst = -1, blab = -1;
+  for (int e = lane; e < m; e += 64) {
+    const uint32_t ke = key[e];
+    if (e > 0 && (key[e - 1] >> 16) == (ke >> 16)) continue;  // not the label's first slot
+    float w = __uint_as_float(vv[e]);
+    for (int f = e + 1; f < m && (key[f] >> 16) == (ke >> 16); ++f) w = w + __uint_as_float(vv[f]);
+    const int first = (int)(ke & 0xffffu);
+    if (w > bw || (w == bw && first > bfirst)) {
+      bw = w;
+      bfirst = first;
+      blab = (int)(ke >> 16);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const float ow = __shfl_xor(bw, o, 64);
+    const int of = __shfl_xor(bfirst, o, 64), ol = __shfl_xor(blab, o, 64);
+    if (ow > bw || (ow == bw && of > bfirst)) {
+      bw = ow;
+      bfirst = of;
+      blab = ol;
+    }
+  }
+  if (lane == 0) lab_out[i] = bw >= 1.0f ? (int64_t)blab : -1;
+}
+
+// exp_sums_kernel + exp_node_kernel for rows of more than 64 slots: the per-partition
+// sums in slot order and the distinct partitions in ascending order come from the sort.
+__global__ __launch_bounds__(64) void exp_long_kernel(
+    const int64_t* __restrict__ indptr, int64_t n, int dim, const int32_t* __restrict__ part,
+    const float* __restrict__ sa, const float* __restrict__ layout,
+    const int32_t* __restrict__ ta, const float* __restrict__ tv, int32_t* __restrict__ ra,
+    float* __restrict__ rv, float* __restrict__ res) {
+  extern __shared__ __attribute__((aligned(16))) uint32_t lsm[];
+  const int lane = threadIdx.x;
+  const int64_t i = blockIdx.x;
+  const int64_t s0 = indptr[i], s1 = indptr[i + 1];
+  const int cap = (int)(s1 - s0) <= 1 ? 1 : 1 << (32 - __clz((int)(s1 - s0) - 1));
+  uint32_t* key = lsm;
+  uint32_t* vv = lsm + cap;
+  const int m = gather_keys(s0, s1, key, vv, [&](int64_t p, int32_t& id, float& v) {
+    id = ta[p];
+    v = tv[p];
+  });
+  int ndist = 0;
+  for (int eb = 0; eb < m; eb += 64) {
+    const int e = eb + lane;
+    const bool head = e < m && (e == 0 || (key[e - 1] >> 16) != (key[e] >> 16));
+    const unsigned long long b = __ballot(head);
+    if (head) {
+      float acc = __uint_as_float(vv[e]);
+      for (int f = e + 1; f < m && (key[f] >> 16) == (key[e] >> 16); ++f)
+        acc = acc + __uint_as_float(vv[f]);
+      const int r = ndist + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+      ra[s0 + r] = (int32_t)(key[e] >> 16);
+      rv[s0 + r] = acc;
+    }
+    ndist += __popcll(b);
+  }
+  __threadfence_block();  // ra/rv written by other lanes of this wavefront
+  __syncthreads();
+  float l1 = 0.0f;
+  for (int r = 0; r < ndist; ++r) l1 = l1 + fabsf(rv[s0 + r]);
+  const float inv = l1 > 0.0f ? 1.0f / l1 : 1.0f;
+  if (lane < dim) {
+    float ex = 0.0f;
+    for (int r = 0; r < ndist; ++r) {
+      const float v0 = rv[s0 + r];
+      const float v = l1 > 0.0f ? v0 * inv : v0;
+      const float t = v * layout[(int64_t)ra[s0 + r] * dim + lane];
+      ex = ex + t;
+    }
+    const int32_t pi = part[i];
+    const float sv = sa[pi];
+    const float dv = sv > 0.0f ? sv * (1.0f / fabsf(sv)) : sv;
+    const float t = dv * layout[(int64_t)pi * dim + lane];
+    const float dr = 0.0f + t;
+    res[i * dim + lane] = (ex + dr) * 0.5f;
+  }
+}
+
+__global__ void row_length_max_kernel(const int64_t* __restrict__ indptr, int64_t n,
+                                      unsigned long long* __restrict__ mx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = i < n ? (unsigned long long)(indptr[i + 1] - indptr[i]) : 0ull;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const unsigned long long w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  if ((threadIdx.x & 63) == 0 && v) atomicMax(mx, v);
+}
+__global__ void zero_u64_kernel(unsigned long long* p) { *p = 0ull; }
+
 // Expansion, pass 1 per node: for every slot the partition a of the neighbour and the
 // term sym(i, j) * sa[a] when G(j, i) exists (a = -1 otherwise).
 __global__ __launch_bounds__(256) void exp_terms_kernel(
@@ -377,6 +559,25 @@ __global__ void exp_center_kernel(float* __restrict__ res, int64_t n, int dim,
 
 }  // namespace
 
+// the longest row of a device CSR
+static int64_t max_row_length(const int64_t* d_ptr, int64_t n, hipStream_t s, DevPool& pool) {
+  if (n <= 0) return 0;
+  Buf<unsigned long long> mx(pool, 1);
+  zero_u64_kernel<<<1, 1, 0, s>>>(mx.p);
+  row_length_max_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(d_ptr, n, mx.p);
+  EVOC_HIP(hipGetLastError());
+  unsigned long long h = 0;
+  EVOC_HIP(hipMemcpyAsync(&h, mx.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  EVOC_HIP(hipStreamSynchronize(s));
+  return (int64_t)h;
+}
+// the LDS of the long-row kernels for rows of up to maxdeg slots (keys and values)
+static size_t long_row_lds(int64_t maxdeg) {
+  int64_t cap = 1;
+  while (cap < maxdeg) cap <<= 1;
+  return (size_t)cap * 8;
+}
+
 void expansion_device(const int64_t* d_indptr, const int32_t* d_col, const float* d_val,
                       int64_t n, int64_t nnz, int dim, const int32_t* d_part, const float* d_sa,
                       int64_t np, const float* h_layout, float* d_out, hipStream_t s,
@@ -389,9 +590,20 @@ void expansion_device(const int64_t* d_indptr, const int32_t* d_col, const float
   const unsigned g = (unsigned)ceil_div(n, LP_WAVES);
   exp_terms_kernel<<<g, 64 * LP_WAVES, 0, s>>>(d_indptr, d_col, d_val, n, d_part, d_sa, ta.p,
                                                tv.p);
-  exp_sums_kernel<<<g, 64 * LP_WAVES, 0, s>>>(d_indptr, n, ta.p, tv.p, tf.p, tacc.p);
-  exp_node_kernel<<<g, 64 * LP_WAVES, 0, s>>>(d_indptr, n, dim, d_part, d_sa, layout.p, ta.p, tf.p,
-                                              tacc.p, ra.p, rv.p, res.p);
+  // rows of more than 64 slots on graphs of at most 16384 nodes (the coarse levels): a
+  // wavefront per node sorts its slots
+  const int64_t maxdeg = max_row_length(d_indptr, n, s, pool);
+  if (maxdeg > 64 && n <= LP_LONG_MAX && np <= LP_LONG_MAX) {
+    const size_t lds = long_row_lds(maxdeg);
+    EVOC_HIP(hipFuncSetAttribute((const void*)exp_long_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    exp_long_kernel<<<(unsigned)n, 64, lds, s>>>(d_indptr, n, dim, d_part, d_sa, layout.p, ta.p,
+                                                tv.p, ra.p, rv.p, res.p);
+  } else {
+    exp_sums_kernel<<<g, 64 * LP_WAVES, 0, s>>>(d_indptr, n, ta.p, tv.p, tf.p, tacc.p);
+    exp_node_kernel<<<g, 64 * LP_WAVES, 0, s>>>(d_indptr, n, dim, d_part, d_sa, layout.p, ta.p,
+                                                tf.p, tacc.p, ra.p, rv.p, res.p);
+  }
   {
     const size_t lds = 2 * (size_t)dim * cs_pitch(dim) * sizeof(float);
     EVOC_HIP(hipFuncSetAttribute((const void*)exp_colsum_kernel,
@@ -751,8 +963,20 @@ void lp_top_partition(const int64_t* d_ptr, const int32_t* d_col, const float* d
   // Jacobi votes (labelled nodes keep their label, so every seed label survives and
   // maxl is the largest seed label)
   const unsigned gw = (unsigned)ceil_div(n, 64 * LP_WAVES);
+  // rows of more than 64 slots on graphs of at most 16384 nodes (the coarse levels; a row
+  // is then at most 16384 slots and seed labels are below approx <= 16384): a wavefront
+  // per node sorts its labelled slots
+  const int64_t maxdeg = max_row_length(d_ptr, n, s, pool);
+  const bool long_rows = maxdeg > 64 && n <= LP_LONG_MAX && approx <= LP_LONG_MAX;
+  const size_t lds = long_rows ? long_row_lds(maxdeg) : 0;
+  if (long_rows)
+    EVOC_HIP(hipFuncSetAttribute((const void*)lp_step_long_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   for (int it = 0; it < n_iter; ++it) {
-    lp_step_kernel<<<gw, 64 * LP_WAVES, 0, s>>>(d_ptr, d_col, d_val, n, la.p, lb.p);
+    if (long_rows)
+      lp_step_long_kernel<<<(unsigned)n, 64, lds, s>>>(d_ptr, d_col, d_val, n, la.p, lb.p);
+    else
+      lp_step_kernel<<<gw, 64 * LP_WAVES, 0, s>>>(d_ptr, d_col, d_val, n, la.p, lb.p);
     std::swap(la.p, lb.p);
   }
   // outliers
@@ -800,7 +1024,7 @@ void lp_top_partition(const int64_t* d_ptr, const int32_t* d_col, const float* d
 void lp_top_coarsen(const int64_t* d_ptr, const int32_t* d_col, const float* d_val, int64_t n,
                     int64_t nnz, const LpTop& P, std::vector<int64_t>& cptr,
                     std::vector<int32_t>& ccol, std::vector<float>& cval, hipStream_t s,
-                    DevPool& pool, Graph* keep) {
+                    DevPool& pool, Graph* keep, bool host_copy) {
   const int64_t np = P.np;
   cptr.assign(np + 1, 0);
   ccol.clear();
@@ -841,11 +1065,14 @@ void lp_top_coarsen(const int64_t* d_ptr, const int32_t* d_col, const float* d_v
       nent, nrun, np, start2.p, key2.p, accs.p, dcol.p, dval.p, cnt.p);
   exclusive_scan_i64(cnt.p, dptr.p, np + 1, s, pool);
   EVOC_HIP(hipGetLastError());
-  ccol.resize(nent);
-  cval.resize(nent);
-  EVOC_HIP(hipMemcpyAsync(cptr.data(), dptr.p, (np + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  EVOC_HIP(hipMemcpyAsync(ccol.data(), dcol.p, nent * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  EVOC_HIP(hipMemcpyAsync(cval.data(), dval.p, nent * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (!keep || host_copy) {
+    ccol.resize(nent);
+    cval.resize(nent);
+    EVOC_HIP(hipMemcpyAsync(cptr.data(), dptr.p, (np + 1) * sizeof(int64_t),
+                            hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipMemcpyAsync(ccol.data(), dcol.p, nent * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    EVOC_HIP(hipMemcpyAsync(cval.data(), dval.p, nent * sizeof(float), hipMemcpyDeviceToHost, s));
+  }
   EVOC_HIP(hipStreamSynchronize(s));
   if (keep) {
     keep->n = np;
