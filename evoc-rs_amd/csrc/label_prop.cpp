@@ -1,8 +1,9 @@
 // Label-propagation initial layout (label_propagation_init, src/graph/label_prop.rs:537-564)
-// for the plain form of evoc_hip. The input level (label votes, outliers, partition,
-// coarse graph, partition means, expansion) runs on the GPU; the coarse levels, a few
-// thousand nodes and fewer, run in host C++ with OpenMP, and every coarse-level node
-// embedding runs on the GPU through embed_device. Accumulation orders follow the
+// for the plain form of evoc_hip. Every level of 64 or more nodes (label votes, outliers,
+// partition, coarse graph, partition means, coarse-graph embedding, expansion) runs on the
+// GPU, recursing level by level on device-resident graphs and data (top_level); the base
+// case below 64 nodes (PCA or a random sphere) and a level whose coarse graph has no edges
+// run here in host C++ with OpenMP (level). Accumulation orders follow the
 // reference's sparse algebra (src/utils/sparse.rs) so the layout is reproducible:
 //   * coarsening  R^T_norm G R   (per-partition dense accumulator, columns ascending),
 //   * expansion   ((G o G^T) R_norm)_l1 L  and  (R_norm)_l1 L, averaged, centred, x0.1.
