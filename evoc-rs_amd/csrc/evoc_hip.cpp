@@ -134,7 +134,7 @@ int evoc_hip_knn(const float* x, int64_t n, int64_t d, int32_t metric, int32_t k
     EVOC_CHECK(x && idx_out && dist_out, "knn: NULL buffer");
     EVOC_CHECK(n >= 1 && d >= 1, "knn: n and d must be >= 1");
     EVOC_CHECK(metric == 0 || metric == 1, "knn: unknown metric");
-    EVOC_CHECK(k >= 1 && k <= 32, "knn: k must be in [1, 32]");
+    EVOC_CHECK(k >= 1 && k <= 64, "knn: k must be in [1, 64]");
     check_finite_host(x, n * d, "input");
     Ctx& c = context(0);
     Buf<float> dx(c.pool, n * d);

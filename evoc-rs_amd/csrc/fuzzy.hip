@@ -14,12 +14,13 @@
 namespace evoc {
 namespace {
 
-constexpr int KMAXF = 32;
+constexpr int KMAXF = 64;  // the row's distances in registers: 32 or 64 of them
 
 // canonical f32 exponential: exp evaluated in f64 and rounded once to f32 (correctly
 // rounded except in vanishingly rare double-rounding cases), identical on host and device
 __device__ __forceinline__ float cexp(float x) { return (float)exp((double)x); }
 
+template <int KMAXF>
 __global__ __launch_bounds__(256) void sigma_kernel(const float* __restrict__ dist,
                                                     const int32_t* __restrict__ idx, int64_t n,
                                                     int k, double target, float* __restrict__ sigma,
@@ -271,7 +272,7 @@ __global__ void set_last(int64_t* indptr, int64_t n, const int64_t* cnt) {
 void fuzzy_device(const int32_t* d_idx, const float* d_dist, int64_t n, int k, float effective_k,
                   bool symmetrise, float* d_sigma, float* d_rho, Graph& out, hipStream_t s,
                   DevPool& pool) {
-  EVOC_CHECK(k >= 1 && k <= KMAXF, "fuzzy: k must be in [1, 32]");
+  EVOC_CHECK(k >= 1 && k <= KMAXF, "fuzzy: k must be in [1, 64]");
   EVOC_CHECK(n >= 1, "fuzzy: empty graph");
   // effective_n_neighbors.to_usize().unwrap_or(k) (fuzzy_graph.rs:284-286)
   const size_t kt = (effective_k > -1.0f && std::isfinite(effective_k)) ? (size_t)effective_k : (size_t)k;
@@ -281,7 +282,8 @@ void fuzzy_device(const int32_t* d_idx, const float* d_dist, int64_t n, int k, f
   if (!d_sigma) { sig.alloc(pool, n); d_sigma = sig.p; }
   if (!d_rho) { rh.alloc(pool, n); d_rho = rh.p; }
   const unsigned g256 = (unsigned)ceil_div(n, 256);
-  sigma_kernel<<<g256, 256, 0, s>>>(d_dist, d_idx, n, k, target, d_sigma, d_rho, w.p);
+  (k <= 32 ? sigma_kernel<32> : sigma_kernel<64>)<<<g256, 256, 0, s>>>(d_dist, d_idx, n, k, target,
+                                                                       d_sigma, d_rho, w.p);
   Buf<int64_t> outdeg(pool, n), rowcap(pool, n), off(pool, n), cnt(pool, n);
   Buf<unsigned> indeg(pool, n), cursor(pool, n);
   EVOC_HIP(hipMemsetAsync(indeg.p, 0, n * sizeof(unsigned), s));

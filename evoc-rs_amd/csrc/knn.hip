@@ -1864,9 +1864,11 @@ __global__ __launch_bounds__(256) void knn_rerank_all_kernel(
 }
 
 // ------------------------------------------------------------------ 4. exact rows
-constexpr int KMAX = 32;
+// list capacities of the exact scan: 32 for k <= 32, 64 up to KMAX
+constexpr int KMAX = 64;
 
 // block-wide merge of 256 per-thread sorted lists (k entries each, in LDS) into list 0
+template <int KMAX>
 __device__ void exact_block_merge(float* sk, int* si, int k) {
   const int tid = threadIdx.x;
   for (int stride = 1; stride < 256; stride <<= 1) {
@@ -1901,7 +1903,7 @@ __device__ void exact_block_merge(float* sk, int* si, int k) {
 // column at a time: lanes hold the column's dimensions (the canonical strided partial
 // sums and halving tree of the rerank), and lane f keeps the sorted list of failing
 // row f. Each wavefront writes its lists; the merge kernel combines them.
-template <int MAXT, int EX_ROWS>
+template <int MAXT, int EX_ROWS, int KMAX>
 __global__ __launch_bounds__(256) void knn_exact_kernel(
     const float* __restrict__ X, int64_t n, int d, int64_t ld, const float* __restrict__ nrm,
     int k, int metric, const int32_t* __restrict__ fail_rows, int nrows, int parts, int64_t span,
@@ -1973,6 +1975,7 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(
     }
   }
 }
+template <int KMAX>
 __global__ __launch_bounds__(256) void knn_exact_merge_kernel(
     int k, const int32_t* __restrict__ fail_rows, int parts, const float* __restrict__ part_key,
     const int32_t* __restrict__ part_idx, int32_t* __restrict__ out_idx,
@@ -2003,7 +2006,7 @@ __global__ __launch_bounds__(256) void knn_exact_merge_kernel(
     si[tid * KMAX + q] = li[q];
   }
   __syncthreads();
-  exact_block_merge(sk, si, k);
+  exact_block_merge<KMAX>(sk, si, k);
   for (int q = tid; q < k; q += 256) {
     out_idx[row * k + q] = si[q];
     out_dist[row * k + q] = sk[q];
@@ -2062,7 +2065,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   EVOC_CHECK(precision >= KNN_AUTO && precision <= KNN_FP8, "knn: unknown precision");
   const bool fp8 = precision == KNN_FP8;
   EVOC_CHECK(!fp8 || metric == 1, "knn: the fp8 precision supports the cosine metric only");
-  EVOC_CHECK(k >= 1 && k <= KMAX, "knn: k must be in [1, 32]");
+  EVOC_CHECK(k >= 1 && k <= KMAX, "knn: k must be in [1, 64]");
   EVOC_CHECK(d <= KNN_MAXD, "knn: d must be <= 4096");
   const bool wide = d > 1024;  // the re-rank and exact kernels' register-resident rows
   static bool attr_set = false;
@@ -2107,6 +2110,8 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rb_kernel<RB_NKC, RB_KC, false, true>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)(RbRing<RB_KC>::SMEM + RB_QBYTES)));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_exact_merge_kernel<64>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 256 * 64 * 8));
     attr_set = true;
   }
   // Two tile kernels: the resident-query kernel for bf16-exact inputs (d <= 512, k <= 16,
@@ -2536,10 +2541,17 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     Buf<int32_t> pi(pool, (size_t)nb * parts * 4 * k);
     for (int f0 = 0; f0 < nfail; f0 += ex_rows) {
       const int rows = std::min(ex_rows, nfail - f0);
-      (wide ? knn_exact_kernel<64, 2> : knn_exact_kernel<16, 8>)<<<(unsigned)parts, 256, 0, s>>>(
-          dX, n, d, ld, nrm.p, k, metric, frows.p + f0, rows, parts, span, pk.p, pi.p);
-      knn_exact_merge_kernel<<<(unsigned)rows, 256, 256 * KMAX * 8, s>>>(
-          k, frows.p + f0, parts * 4, pk.p, pi.p, d_idx, d_dist);
+      const bool k64 = k > 32;
+      auto exact = wide ? (k64 ? knn_exact_kernel<64, 2, 64> : knn_exact_kernel<64, 2, 32>)
+                        : (k64 ? knn_exact_kernel<16, 8, 64> : knn_exact_kernel<16, 8, 32>);
+      exact<<<(unsigned)parts, 256, 0, s>>>(dX, n, d, ld, nrm.p, k, metric, frows.p + f0, rows,
+                                            parts, span, pk.p, pi.p);
+      if (k64)
+        knn_exact_merge_kernel<64><<<(unsigned)rows, 256, 256 * 64 * 8, s>>>(
+            k, frows.p + f0, parts * 4, pk.p, pi.p, d_idx, d_dist);
+      else
+        knn_exact_merge_kernel<32><<<(unsigned)rows, 256, 256 * 32 * 8, s>>>(
+            k, frows.p + f0, parts * 4, pk.p, pi.p, d_idx, d_dist);
       if (verbose && f0 + rows < nfail && ((f0 / ex_rows) + 1) % 4096 == 0) {
         EVOC_HIP(hipStreamSynchronize(s));
         std::printf("  kNN exact search: %d of %d rows done\n", f0 + rows, nfail);

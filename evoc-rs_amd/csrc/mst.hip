@@ -23,7 +23,8 @@ namespace {
 
 constexpr int TS = 64;  // points per tile (one wavefront of queries)
 constexpr int MAXD = 16;
-constexpr int KCORE = 32;
+constexpr int KCORE = 32;     // core-distance list of the common kernel
+constexpr int KCORE_MAX = 64;  // min_samples above KCORE take a 64-entry list
 
 // kd ordering: positions are cut into 64-point tiles, and tile t is leaf L + t of the
 // implicit tree. Level by level, every node's points are ordered along the widest
@@ -719,7 +720,7 @@ void kd_sort(const float* d_emb, int64_t n, int dim, Sorted& S, hipStream_t s, D
 bool core_sorted(const Sorted& S, int64_t n, int dim, int min_samples, float* d_core_sorted,
                  hipStream_t s, int32_t* knn_pos = nullptr, uint8_t* knn_tie = nullptr) {
   const int k = (int)std::min<int64_t>(min_samples, n - 1);
-  EVOC_CHECK(k <= KCORE, "min_samples must be <= 32");
+  EVOC_CHECK(k <= KCORE_MAX, "min_samples must be <= 64");
   if (k <= 0) {
     EVOC_HIP(hipMemsetAsync(d_core_sorted, 0, n * sizeof(float), s));
     return false;
@@ -736,12 +737,15 @@ bool core_sorted(const Sorted& S, int64_t n, int dim, int min_samples, float* d_
   } else if (dim == 8 && k <= 8) {
     core_kernel<8, 8, false><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p,
                                               k, d_core_sorted, nullptr, nullptr);
-  } else if (dim == 8) {
+  } else if (dim == 8 && k <= KCORE) {
     core_kernel<8, KCORE, false><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p,
                                                   S.nhi.p, k, d_core_sorted, nullptr, nullptr);
   } else if (k <= 8) {
     core_kernel<0, 8, false><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p, S.nhi.p,
                                               k, d_core_sorted, nullptr, nullptr);
+  } else if (k > KCORE) {
+    core_kernel<0, KCORE_MAX, false><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p,
+                                                      S.nhi.p, k, d_core_sorted, nullptr, nullptr);
   } else {
     core_kernel<0, KCORE, false><<<g, 64, 0, s>>>(S.es.p, n, dim, S.ntiles, S.L, S.nlo.p,
                                                   S.nhi.p, k, d_core_sorted, nullptr, nullptr);

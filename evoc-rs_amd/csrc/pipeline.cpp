@@ -81,10 +81,10 @@ int auto_dim(const evoc_params_t& p) {
 }
 
 void validate_params(const evoc_params_t& p) {
-  EVOC_CHECK(p.n_neighbours >= 1 && p.n_neighbours <= 32, "n_neighbours must be in [1, 32]");
+  EVOC_CHECK(p.n_neighbours >= 1 && p.n_neighbours <= 64, "n_neighbours must be in [1, 64]");
   EVOC_CHECK(p.n_epochs >= 0, "n_epochs must be >= 0");
   EVOC_CHECK(p.embedding_dim >= 0 && p.embedding_dim <= 16, "embedding_dim must be in [0, 16]");
-  EVOC_CHECK(p.min_samples >= 0 && p.min_samples <= 32, "min_samples must be in [0, 32]");
+  EVOC_CHECK(p.min_samples >= 0 && p.min_samples <= 64, "min_samples must be in [0, 64]");
   EVOC_CHECK(p.base_min_cluster_size >= 0, "base_min_cluster_size must be >= 0");
   EVOC_CHECK(p.approx_n_clusters >= 0, "approx_n_clusters must be >= 0");
   EVOC_CHECK(p.max_layers >= 0, "max_layers must be >= 0");
@@ -429,7 +429,7 @@ int evoc_hip_run(const float* data, int64_t n, int64_t d, int64_t row_stride, in
     EVOC_CHECK(data && params, "data or params is NULL");
     EVOC_CHECK(n >= 2 && d >= 1, "need at least 2 points and 1 feature");
     EVOC_CHECK(!pre_idx == !pre_dist, "pre_idx and pre_dist must both be given");
-    EVOC_CHECK(!pre_idx || (pre_k >= 1 && pre_k <= 32), "pre_k must be in [1, 32]");
+    EVOC_CHECK(!pre_idx || (pre_k >= 1 && pre_k <= 64), "pre_k must be in [1, 64]");
     evoc_hip_nn_params_t q;
     if (nn) q = *nn;
     else evoc_hip_default_nn_params(&q);
@@ -479,7 +479,7 @@ int evoc_hip_fuzzy(const int32_t* knn_idx, const float* knn_dist, int64_t n, int
                    int64_t* indptr, int32_t* indices, float* weights, int64_t* nnz) {
   return guarded([&] {
     EVOC_CHECK(knn_idx && knn_dist && indptr && indices && weights && nnz, "fuzzy: NULL buffer");
-    EVOC_CHECK(n >= 1 && k >= 1 && k <= 32, "fuzzy: need n >= 1 and k in [1, 32]");
+    EVOC_CHECK(n >= 1 && k >= 1 && k <= 64, "fuzzy: need n >= 1 and k in [1, 64]");
     check_finite_host(knn_dist, n * k, "knn_dist");
     Ctx& c = context(0);
     Buf<int32_t> di;
@@ -561,7 +561,7 @@ int evoc_hip_core_sq(const float* emb, int64_t n, int32_t dim, int32_t min_sampl
   return guarded([&] {
     EVOC_CHECK(emb && core_sq, "core_sq: NULL buffer");
     EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "core_sq: dim must be in [1, 16]");
-    EVOC_CHECK(min_samples >= 0 && min_samples <= 32, "min_samples must be in [0, 32]");
+    EVOC_CHECK(min_samples >= 0 && min_samples <= 64, "min_samples must be in [0, 64]");
     check_finite_host(emb, n * dim, "embedding");
     Ctx& c = context(0);
     Buf<float> de, dc(c.pool, n);
@@ -577,7 +577,7 @@ int evoc_hip_mst(const float* emb, int64_t n, int32_t dim, int32_t min_samples, 
   return guarded([&] {
     EVOC_CHECK(emb && u && v && w, "mst: NULL buffer");
     EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "mst: dim must be in [1, 16]");
-    EVOC_CHECK(min_samples >= 0 && min_samples <= 32, "min_samples must be in [0, 32]");
+    EVOC_CHECK(min_samples >= 0 && min_samples <= 64, "min_samples must be in [0, 64]");
     check_finite_host(emb, n * dim, "embedding");
     Ctx& c = context(0);
     Buf<float> de;
@@ -632,7 +632,7 @@ int evoc_hip_cluster_layers(const float* emb, int64_t n, int32_t dim, int32_t mi
   return guarded([&] {
     EVOC_CHECK(emb && labels && strengths && persistence && n_layers, "layers: NULL buffer");
     EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "layers: dim must be in [1, 16]");
-    EVOC_CHECK(min_samples >= 0 && min_samples <= 32, "min_samples must be in [0, 32]");
+    EVOC_CHECK(min_samples >= 0 && min_samples <= 64, "min_samples must be in [0, 64]");
     check_finite_host(emb, n * dim, "embedding");
     Ctx& c = context(0);
     Buf<float> de;
@@ -657,7 +657,7 @@ int evoc_hip_search_n_clusters(const float* emb, int64_t n, int32_t dim, int32_t
   return guarded([&] {
     EVOC_CHECK(emb && labels && strengths, "search: NULL buffer");
     EVOC_CHECK(n >= 1 && dim >= 1 && dim <= 16, "search: dim must be in [1, 16]");
-    EVOC_CHECK(min_samples >= 0 && min_samples <= 32, "min_samples must be in [0, 32]");
+    EVOC_CHECK(min_samples >= 0 && min_samples <= 64, "min_samples must be in [0, 64]");
     check_finite_host(emb, n * dim, "embedding");
     Ctx& c = context(0);
     Buf<float> de;

@@ -277,8 +277,8 @@ def _check_precomputed(precomputed_knn, n):
         raise ValueError(f"precomputed kNN indices must have shape ({n}, k), got {pi.shape}")
     if pd.shape != pi.shape:
         raise ValueError(f"precomputed kNN distances have shape {pd.shape}, indices {pi.shape}")
-    if pi.shape[1] > 32:
-        raise ValueError("precomputed kNN: at most 32 neighbours per row")
+    if pi.shape[1] > 64:
+        raise ValueError("precomputed kNN: at most 64 neighbours per row")
     if not np.issubdtype(pi.dtype, np.integer):
         raise ValueError("precomputed kNN indices must be integers")
     if pi.size and (pi.min() < -1 or pi.max() >= n):

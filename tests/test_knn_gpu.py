@@ -239,3 +239,23 @@ def test_knn_rejects_rows_wider_than_4096():
     x = np.zeros((10, 4097), np.float32)
     with pytest.raises(H.EvocError, match="4096"):
         H.knn(x, 3)
+
+
+@pytest.mark.parametrize("n,d,k,metric", [(3000, 64, 40, "euclidean"), (2500, 384, 64, "cosine"),
+                                          (1800, 1536, 48, "cosine")])
+def test_knn_more_than_32_neighbours(n, d, k, metric):
+    """33 to 64 neighbours: the first pass keeps 64 candidates, rows it cannot certify go to
+    the sharded escalation (256 candidates) or the exact scan with 64-entry lists; 70
+    duplicates make some rows need the exact scan."""
+    x, _ = datagen.unit_mixture(n, d, 20, 0.2, seed=k)
+    x = x.astype(np.float32)
+    x[10:80] = x[1]
+    st = _check(x, k, metric)
+    print(n, d, k, metric, st)
+    assert st["fallback_rows"] > 0  # the exact scan with 64-entry lists ran
+
+
+def test_knn_rejects_more_than_64_neighbours():
+    x = np.random.default_rng(0).normal(size=(200, 8)).astype(np.float32)
+    with pytest.raises(H.EvocError, match="64"):
+        H.knn(x, 65)

@@ -363,3 +363,24 @@ def test_non_finite_input_rejected_on_device_paths():
             H.evoc_device(dev.ptr.value, 600, 16, seed=1)
     r = H.evoc(x, seed=1)   # the library is still usable afterwards
     assert r.cluster_layers.shape[1] == 600
+
+
+def test_more_than_32_neighbours_and_min_samples_end_to_end():
+    """n_neighbours and min_samples between 33 and 64 (the wider register lists of the
+    fuzzy set and the core distances): fuzzy set against the oracle, core distances bit for
+    bit, and the whole pipeline on a shared kNN graph identical to the oracle's."""
+    x, gt = datagen.gaussian_mixture(2500, 32, 10, 4.0, 1.0, seed=77)
+    idx, dist = O.knn_exhaustive(x, 48)
+    gp, gi, gw, gs, gr = H.fuzzy_graph(idx, dist, 48.0)
+    op, oi, ow, os_, orh = O.fuzzy_graph(idx, dist, 48.0)
+    assert np.array_equal(gp, op) and np.array_equal(gi, oi)
+    np.testing.assert_allclose(gw, ow, rtol=1e-5, atol=0)
+    e, _ = _embedding()
+    for ms in (33, 40, 64):
+        assert np.array_equal(H.core_sq(e, ms).view(np.uint32), O.core_sq(e, ms).view(np.uint32))
+    p = H.EvocParams(n_neighbours=48, min_samples=40)
+    r = H.evoc(x, params=p, precomputed_knn=(idx, dist), seed=3)
+    o = O.evoc(x, O.params(n_neighbours=48, min_samples=40), seed=3, pre_idx=idx, pre_dist=dist)
+    assert np.array_equal(r.embedding.view(np.uint32), o["embedding"].view(np.uint32))
+    assert np.array_equal(r.cluster_layers, o["labels"])
+    assert np.array_equal(r.persistence_scores, o["persistence"])
