@@ -174,6 +174,45 @@ void free_result(evoc_result_t* r) {
   std::free(r);
 }
 
+// A host matrix with any element strides (faer's column-major MatRef included) into a
+// row-major device copy, through the pool's pinned staging buffer: host threads pack one
+// 32 MB chunk of rows into one half while the other half's copy crosses the link, so the
+// packing overlaps the transfer and the transfer runs at pinned-memory speed.
+void upload_rows(const float* data, int64_t n, int64_t d, int64_t rs, int64_t cs, float* dst,
+                 Ctx& c) {
+  constexpr int64_t CHUNK = int64_t(32) << 20;  // bytes per half
+  const int64_t per = std::max<int64_t>(1, CHUNK / (d * (int64_t)sizeof(float)));
+  float* stage = static_cast<float*>(c.pool.pinned((size_t)(2 * per * d) * sizeof(float)));
+  hipEvent_t ev[2];
+  EVOC_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  EVOC_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  bool pending[2] = {false, false};
+  hipError_t err = hipSuccess;
+  int h = 0;
+  for (int64_t r0 = 0; r0 < n && err == hipSuccess; r0 += per, h ^= 1) {
+    const int64_t rows = std::min(per, n - r0);
+    if (pending[h]) err = hipEventSynchronize(ev[h]);
+    if (err != hipSuccess) break;
+    float* buf = stage + h * per * d;
+    if (rs == d && cs == 1) {
+      parallel_copy(buf, data + r0 * d, (size_t)(rows * d) * sizeof(float));
+    } else {
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < rows; ++i)
+        for (int64_t j = 0; j < d; ++j) buf[i * d + j] = data[(r0 + i) * rs + j * cs];
+    }
+    err = hipMemcpyAsync(dst + r0 * d, buf, (size_t)(rows * d) * sizeof(float),
+                         hipMemcpyHostToDevice, c.stream);
+    if (err == hipSuccess) err = hipEventRecord(ev[h], c.stream);
+    pending[h] = true;
+  }
+  const hipError_t e2 = hipStreamSynchronize(c.stream);
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  EVOC_HIP(err);
+  EVOC_HIP(e2);
+}
+
 // The clustering stage on a device embedding: MST -> linkage -> layers or search.
 Layers cluster_device(const float* d_emb, int64_t n, int dim, const evoc_params_t& p,
                       hipStream_t s, DevPool& pool, double* mst_ms, double* layer_ms,
@@ -435,21 +474,10 @@ int evoc_hip_run(const float* data, int64_t n, int64_t d, int64_t row_stride, in
     else evoc_hip_default_nn_params(&q);
     Ctx& c = context(q.device_id);
     const auto t0 = std::chrono::steady_clock::now();
-    // pack the (possibly strided, e.g. column-major) matrix row-major
-    std::vector<float> packed;
-    const float* xr = data;
-    if (!(row_stride == d && col_stride == 1)) {
-      packed.resize(n * d);
-#pragma omp parallel for schedule(static)
-      for (int64_t i = 0; i < n; ++i)
-        for (int64_t j = 0; j < d; ++j) packed[i * d + j] = data[i * row_stride + j * col_stride];
-      xr = packed.data();
-    }
     Buf<float> dx(c.pool, n * d);
-    EVOC_HIP(hipMemcpyAsync(dx.p, xr, n * d * sizeof(float), hipMemcpyHostToDevice, c.stream));
-    EVOC_HIP(hipStreamSynchronize(c.stream));
+    upload_rows(data, n, d, row_stride, col_stride, dx.p, c);
     const double up = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    evoc_result_t* r = run_impl(dx.p, xr, n, d, d, pre_idx, pre_dist, pre_k, *params, q, seed,
+    evoc_result_t* r = run_impl(dx.p, nullptr, n, d, d, pre_idx, pre_dist, pre_k, *params, q, seed,
                                 verbose, c);
     r->stage_ms[7] = up;
     *out = r;
