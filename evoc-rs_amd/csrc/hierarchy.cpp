@@ -24,9 +24,8 @@
 
 namespace evoc {
 
-Linkage linkage_from_sorted(const int32_t* u, const int32_t* v, const float* w, int64_t ne,
-                            int64_t n) {
-  Linkage L;
+void linkage_from_sorted(const int32_t* u, const int32_t* v, const float* w, int64_t ne,
+                         int64_t n, Linkage& L) {
   L.n = n;
   L.left.resize(ne);
   L.right.resize(ne);
@@ -38,7 +37,10 @@ Linkage linkage_from_sorted(const int32_t* u, const int32_t* v, const float* w, 
     int32_t par, rank;
     int64_t lab, size;
   };
-  std::vector<Vtx> V(n);
+  // kept by the calling thread between calls: a fresh allocation of this size is paid in
+  // page faults on every call
+  static thread_local std::vector<Vtx> V;
+  V.resize(n);
   for (int64_t x = 0; x < n; ++x) V[x] = Vtx{(int32_t)x, 0, x, 1};
   auto find = [&](int64_t x) {
     while (V[x].par != x) {
@@ -67,6 +69,12 @@ Linkage linkage_from_sorted(const int32_t* u, const int32_t* v, const float* w, 
     V[ru].lab = n + i;
     V[ru].size = ns;
   }
+}
+
+Linkage linkage_from_sorted(const int32_t* u, const int32_t* v, const float* w, int64_t ne,
+                            int64_t n) {
+  Linkage L;
+  linkage_from_sorted(u, v, w, ne, n, L);
   return L;
 }
 

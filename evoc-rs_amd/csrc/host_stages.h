@@ -28,6 +28,9 @@ struct Linkage {
 // edges must already be in (w, u, v) order
 Linkage linkage_from_sorted(const int32_t* u, const int32_t* v, const float* w, int64_t ne,
                             int64_t n);
+// the same into L, whose vectors keep their capacity from call to call
+void linkage_from_sorted(const int32_t* u, const int32_t* v, const float* w, int64_t ne,
+                         int64_t n, Linkage& L);
 
 // condensed tree rows (src/clustering/condensed_tree.rs:19-37)
 struct CTree {

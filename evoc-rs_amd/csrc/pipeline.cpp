@@ -226,8 +226,14 @@ Layers cluster_device(const float* d_emb, int64_t n, int dim, const evoc_params_
       tl = clk::now();
     }
   };
-  std::vector<int32_t> u(std::max<int64_t>(n - 1, 1)), v(std::max<int64_t>(n - 1, 1));
-  std::vector<float> w(std::max<int64_t>(n - 1, 1));
+  // host arrays of the edges and the linkage, kept by the calling thread between calls (a
+  // fresh allocation of these sizes is paid in page faults on every call)
+  static thread_local std::vector<int32_t> u, v;
+  static thread_local std::vector<float> w;
+  static thread_local Linkage L;
+  u.resize(std::max<int64_t>(n - 1, 1));
+  v.resize(std::max<int64_t>(n - 1, 1));
+  w.resize(std::max<int64_t>(n - 1, 1));
   mst_device(d_emb, n, dim, p.min_samples, u.data(), v.data(), w.data(), s, pool);
   lap("mst");
   int64_t ne = std::max<int64_t>(n - 1, 0);
@@ -242,7 +248,7 @@ Layers cluster_device(const float* d_emb, int64_t n, int dim, const evoc_params_
     for (int64_t e = 0; e < ne; ++e) t += (double)w[e];
     *mst_total = t;
   }
-  const Linkage L = linkage_from_sorted(u.data(), v.data(), w.data(), ne, n);
+  linkage_from_sorted(u.data(), v.data(), w.data(), ne, n, L);
   lap("linkage");
   if (mst_ms) *mst_ms = ms_since(t0);
   t0 = clk::now();

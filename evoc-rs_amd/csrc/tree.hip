@@ -323,7 +323,8 @@ void tree_upload(const int64_t* left, const int64_t* right, const float* dist,
   t.nr = n - 1;
   const int64_t nr = t.nr, N = 2 * n - 1;
   if (nr == 0) return;
-  std::vector<int32_t> h(3 * nr);
+  static thread_local std::vector<int32_t> h;  // kept between calls: no page faults
+  h.resize(3 * nr);
   for (int64_t i = 0; i < nr; ++i) {
     h[i] = (int32_t)left[i];
     h[nr + i] = (int32_t)right[i];
