@@ -1370,6 +1370,271 @@ template __global__ void knn_topc_rb_kernel<RB_NKC, RB_KC, false, true>(const ui
                                                                  int, const float*, int32_t*,
                                                                  float*, const RbCols);
 
+// ------------------------------------------------------------------ 2c. split, resident
+// Inputs that are not bf16-exact, d <= 512, k <= 16: the products hi*hi + hi*lo + lo*hi of
+// the split x = hi + lo (bf16 each) in f32, as the streaming kernel forms them, with the
+// query rows' hi and lo A-fragments both resident (256 registers at d = 512). That leaves
+// one wavefront of 32 rows per SIMD (four per 128-row workgroup), but each pair of B
+// fragments (hi, lo) read from LDS feeds three MFMAs: three times the matrix work of the
+// bf16 kernel per epilogue, and two thirds of its LDS bytes per MFMA. Column tiles of 32
+// stream through a ring of four stages of 256 dimensions (hi and lo, 32 KB); the tile
+// epilogue (per-value filters, buckets, sorted lists of 16 per row and column half) is the
+// bf16 kernel's plain-sweep epilogue.
+constexpr int RS_WAVES = 4;
+constexpr int RS_ROWS = RS_WAVES * 32;
+constexpr int RS_KC = 256;
+constexpr int RS_LA = 4;  // B fragment pairs read ahead of their MFMAs
+struct RsRing {
+  static constexpr int STB_H = RA_BN * RS_KC * 2;            // 16 KB: the hi half of a stage
+  static constexpr int STB_B = 2 * STB_H;                     // hi and lo
+  static constexpr int STB = STB_B + RS_WAVES * 256;          // + column scales per wave
+  static constexpr int NST = 4;
+  static constexpr int PIECES = RS_KC / 8;
+  static constexpr int LOADS = STB_B / (64 * RS_WAVES * 16);  // 16-byte loads per thread
+  static constexpr size_t SMEM = (size_t)NST * STB;
+};
+static_assert(RsRing::LOADS == 8, "four hi and four lo loads per thread and stage");
+constexpr int RS_EPI_BYTES = RS_WAVES * RB_WREG + RS_WAVES * 64 * 4;
+
+template <int NKC, bool COS>
+__global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
+    const uint16_t* __restrict__ Xh, const uint16_t* __restrict__ Xl, int64_t n, int64_t npad,
+    int dpad, const float* __restrict__ colscale, int32_t* __restrict__ cand_idx,
+    float* __restrict__ cand_key) {
+  using R = RsRing;
+  constexpr int C = 16;
+  constexpr int NST = R::NST;
+  constexpr int LOADS = R::LOADS;
+  constexpr int KS = RS_KC / 16;  // MFMA k-steps per stage
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  __shared__ __attribute__((aligned(16))) char epi[RS_EPI_BYTES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  char* ring = smem;
+  float* sc = reinterpret_cast<float*>(epi + wave * RB_WREG);
+  int2* bucket = reinterpret_cast<int2*>(sc);  // [64 owners][RB_BKT]
+  float* thr = reinterpret_cast<float*>(epi + RS_WAVES * RB_WREG) + wave * 64;  // [owner]
+  const int row0 = (int)blockIdx.x * RS_ROWS;
+  const int nt = (int)(npad / RA_BN);
+  const int total = nt * NKC;
+
+  bf16x8 afh[NKC * KS], afl[NKC * KS];
+  {
+    const int64_t ro = (int64_t)(row0 + wave * 32 + (lane & 31)) * dpad + (lane >> 5) * 8;
+#pragma unroll
+    for (int f = 0; f < NKC * KS; ++f) {
+      afh[f] = *reinterpret_cast<const bf16x8*>(Xh + ro + f * 16);
+      afl[f] = *reinterpret_cast<const bf16x8*>(Xl + ro + f * 16);
+    }
+  }
+  wait_vm<0>();
+  float lk[C];
+  int li[C];
+#pragma unroll
+  for (int q = 0; q < C; ++q) {
+    lk[q] = FINF;
+    li[q] = -1;
+  }
+  thr[lane] = FINF;
+  int bcnt = 0;
+
+  // buffer loads straight into LDS: loads 0..3 of a thread bring hi pieces, 4..7 the lo
+  // pieces at the same offsets; the even and odd lanes both bring their column's scale
+  int voff[LOADS / 2];
+#pragma unroll
+  for (int h = 0; h < LOADS / 2; ++h) {
+    const int s_ = (h * RS_WAVES + wave) * 64 + lane;
+    const int c = s_ / R::PIECES, qs = s_ % R::PIECES;
+    voff[h] = (c * dpad + (qs ^ (c & 15)) * 8) * 2;
+  }
+  const int soff = (lane >> 1) * 4;
+  const int tile_bytes = RA_BN * dpad * 2;
+  int gi = 0, tcur = 0, kci = 0, sloti = 0;
+  auto issue_next = [&]() {
+    char* st = ring + sloti * R::STB;
+    const int64_t tb = (int64_t)tcur * (RA_BN * dpad);
+#pragma unroll
+    for (int h = 0; h < LOADS / 2; ++h) {
+      buffer_load_lds16(Xh + tb, tile_bytes, st + (h * RS_WAVES + wave) * 1024, voff[h],
+                        kci * RS_KC * 2);
+      buffer_load_lds16(Xl + tb, tile_bytes, st + R::STB_H + (h * RS_WAVES + wave) * 1024,
+                        voff[h], kci * RS_KC * 2);
+    }
+    buffer_load_lds4(colscale + (int64_t)tcur * RA_BN, 4 * RA_BN, st + R::STB_B + wave * 256, soff);
+    // past the last stage the tile stays put: the tail reloads the last stage into a slot
+    // no later chunk reads, so every chunk issues the same loads and the waits are static
+    if (gi + 1 < total) {
+      if (++kci == NKC) {
+        kci = 0;
+        ++tcur;
+      }
+    }
+    ++gi;
+    sloti = sloti + 1 == NST ? 0 : sloti + 1;
+  };
+
+  const int lcol = lane & 31, half = lcol >> 4, grp = lane >> 4;
+  const int lrow = (lane >> 5) * 4;
+  const int bcol = lcol * R::PIECES * 16, bm = (lane >> 5) ^ (lcol & 15);
+  for (int g = 0; g < NST - 1; ++g) issue_next();
+  wait_vm<(LOADS + 1) * (NST - 2)>();  // this wavefront's loads of stage 0 have landed
+  ring_barrier();
+  int slotc = 0;
+  for (int ct = 0; ct < nt; ++ct) {
+    f32x16 acc = (f32x16){};
+    float2 csv;
+#pragma unroll
+    for (int kc = 0; kc < NKC; ++kc) {
+      const char* st = ring + slotc * R::STB + bcol;
+      bf16x8 bh[RS_LA], bl[RS_LA];
+#pragma unroll
+      for (int u = 0; u < RS_LA; ++u) {
+        bh[u] = *reinterpret_cast<const bf16x8*>(st + ((2 * u) ^ bm) * 16);
+        bl[u] = *reinterpret_cast<const bf16x8*>(st + R::STB_H + ((2 * u) ^ bm) * 16);
+      }
+#pragma unroll
+      for (int kk = 0; kk < KS; ++kk) {
+        const int f = kc * KS + kk;
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afh[f], bh[kk % RS_LA], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afh[f], bl[kk % RS_LA], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afl[f], bh[kk % RS_LA], acc, 0, 0, 0);
+        if (kk == 0) issue_next();  // stage g + NST - 1, into the slot of stage g - 1
+        if (kk + RS_LA < KS) {
+          bh[kk % RS_LA] = *reinterpret_cast<const bf16x8*>(st + ((2 * (kk + RS_LA)) ^ bm) * 16);
+          bl[kk % RS_LA] =
+              *reinterpret_cast<const bf16x8*>(st + R::STB_H + ((2 * (kk + RS_LA)) ^ bm) * 16);
+        }
+      }
+      if (kc == NKC - 1) {
+        const unsigned ca = lds_addr(ring + slotc * R::STB + R::STB_B + wave * 256 + lcol * 8);
+        asm volatile("ds_read_b64 %0, %1\ns_waitcnt lgkmcnt(0)" : "=v"(csv) : "v"(ca) : "memory");
+      }
+      wait_vm<(LOADS + 1) * (NST - 2)>();  // own loads of stage g + 1 landed
+      ring_barrier();
+      slotc = slotc + 1 == NST ? 0 : slotc + 1;
+    }
+
+    // ---------------- wavefront-private epilogue of column tile ct (as the bf16 kernel's)
+    const int cta = ct;
+    const bool diag = (cta * RA_BN < row0 + RS_ROWS) && (cta * RA_BN + RA_BN > row0);
+    const bool tail = (int64_t)(cta + 1) * RA_BN > n;
+    const float cs = csv.x;
+    f32x4 tq[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) tq[j] = reinterpret_cast<const f32x4*>(thr + 32 * half + lrow)[2 * j];
+    int dv = (int)(cta * RA_BN - row0) - wave * 32 + lcol - lrow;
+    int rem = (int)std::min<int64_t>(n - (int64_t)cta * RA_BN, RA_BN) - lcol;
+    int sb = lrow * RA_SCS + lcol;
+    asm volatile("" : "+v"(dv), "+v"(rem), "+v"(sb));
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const float s_ = acc[e];
+      acc[e] = COS ? -(s_ * cs) : (cs - 2.0f * s_);
+    }
+    if (__builtin_expect(diag || tail, 0)) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e)
+        if (rem <= 0 || dv == (e >> 2) * 8 + (e & 3)) acc[e] = FINF;
+    }
+    {
+      unsigned long long anyp = 0;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) anyp |= __ballot(acc[e] < tq[e >> 2][e & 3]);
+      if (anyp == 0) continue;  // nothing in this tile enters a list
+    }
+    const int cbase = (int)(cta * RA_BN);
+    bool touched = false, overflow = false;
+    int inc = 0;
+    int ob = (int)lds_addr(bucket) + ((grp & 1) * 32 + (grp >> 1) * 4) * RB_BKT * 8;
+    asm volatile("" : "+v"(ob));
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float key = acc[j * 4 + i];
+        const bool pass = key < tq[j][i];
+        const unsigned long long m = __ballot(pass);
+        if (m) {
+          touched = true;
+          const int r0 = j * 8 + i;
+          const int o0 = r0, o1 = r0 + 32, o2 = r0 + 4, o3 = r0 + 36;
+          const int c0 = __builtin_amdgcn_readlane(bcnt, o0);
+          const int c1 = __builtin_amdgcn_readlane(bcnt, o1);
+          const int c2 = __builtin_amdgcn_readlane(bcnt, o2);
+          const int c3 = __builtin_amdgcn_readlane(bcnt, o3);
+          const int n0 = __popcll(m & 0xffffull), n1 = __popcll(m & 0xffff0000ull);
+          const int n2 = __popcll(m & 0xffff00000000ull), n3 = __popcll(m & 0xffff000000000000ull);
+          overflow |= (c0 + n0 > RB_BKT) | (c1 + n1 > RB_BKT) | (c2 + n2 > RB_BKT) |
+                      (c3 + n3 > RB_BKT);
+          const unsigned long long gm = m & (0xffffull << (16 * grp));
+          const int below = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(gm >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((unsigned)gm, 0u));
+          const int base = grp == 0 ? c0 : (grp == 1 ? c1 : (grp == 2 ? c2 : c3));
+          const int pos = base + below;
+          if (pass && pos < RB_BKT)
+            lds_st_b64((unsigned)(ob + r0 * RB_BKT * 8 + pos * 8),
+                       make_int2(__float_as_int(key), cbase + lcol));
+          inc += (lane == o0 ? n0 : 0) + (lane == o1 ? n1 : 0) + (lane == o2 ? n2 : 0) +
+                 (lane == o3 ? n3 : 0);
+        }
+      }
+    if (touched) {
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_wave_barrier();
+      bool drain = overflow;
+      if (!overflow) {
+        bcnt += inc;
+        drain = __ballot(bcnt >= RB_DRAIN) != 0;
+      }
+      if (drain) {
+        const unsigned ba = lds_addr(bucket + lane * RB_BKT);
+        int2 en = lds_ld_b64(ba);
+#pragma unroll 1
+        for (int e = 0; e < bcnt; ++e) {
+          const int2 cur = en;
+          en = lds_ld_b64(ba + (unsigned)(e + 1) * 8u);
+          cand_insert<C>(lk, li, __int_as_float(cur.x), cur.y);
+        }
+        bcnt = 0;
+        if (overflow) {
+          asm volatile("" ::: "memory");
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int e = 0; e < 16; ++e) sc[sb + ((e >> 2) * 8 + (e & 3)) * RA_SCS] = acc[e];
+          asm volatile("" ::: "memory");
+          __builtin_amdgcn_wave_barrier();
+          const float* srow = sc + (lane & 31) * RA_SCS + 16 * (lane >> 5);
+#pragma unroll 1
+          for (int q = 0; q < 16; ++q) cand_insert<C>(lk, li, srow[q], cbase + 16 * (lane >> 5) + q);
+        }
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+        lds_st_b32(lds_addr(thr + lane), __float_as_int(lk[C - 1]));
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+  }
+  if (__ballot(bcnt > 0) != 0) {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+    const unsigned ba = lds_addr(bucket + lane * RB_BKT);
+#pragma unroll 1
+    for (int e = 0; e < bcnt; ++e) {
+      const int2 cur = lds_ld_b64(ba + (unsigned)e * 8u);
+      cand_insert<C>(lk, li, __int_as_float(cur.x), cur.y);
+    }
+  }
+  wait_vm<0>();
+  const int64_t grow = row0 + wave * 32 + (lane & 31);
+  if (grow < n) {
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      cand_key[(grow * 2 + (lane >> 5)) * C + q] = lk[q];
+      cand_idx[(grow * 2 + (lane >> 5)) * C + q] = li[q];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ 2d. symmetric sweep
 constexpr int64_t KNN_SYM_MIN_N = 65536;  // automatic use of the symmetric sweep from this n
 // The sample of the threshold pass: rows q * n / S, zero padded to a multiple of 32.
@@ -2112,6 +2377,14 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
                                  (int)(RbRing<RB_KC>::SMEM + RB_QBYTES)));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_exact_merge_kernel<64>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 256 * 64 * 8));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rs_kernel<1, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RsRing::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rs_kernel<1, false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RsRing::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rs_kernel<2, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RsRing::SMEM));
+    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rs_kernel<2, false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RsRing::SMEM));
     attr_set = true;
   }
   // Two tile kernels: the resident-query kernel for bf16-exact inputs (d <= 512, k <= 16,
@@ -2127,9 +2400,13 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   int64_t npad = 0;
   int dpad = 0;
   bool ra = false, split = false;
-  auto prep = [&](bool use_ra, bool want_lo) {
+  // the split resident-query kernel (inputs that are not bf16-exact, 128 < d <= 512, k <= 16)
+  // takes rows padded to 256 or 512 dimensions
+  const bool rs_possible = !fp8 && k <= 16 && d > 128 && d <= 512;
+  auto prep = [&](bool use_ra, bool want_lo, bool rs_layout = false) {
     npad = use_ra ? ceil_div(n, RA_ROWS) * RA_ROWS : ceil_div(n, BM) * BM;
-    dpad = use_ra ? (d <= 128 ? 128 : d <= 256 ? 256 : 512) : (int)(ceil_div(d, KC) * KC);
+    dpad = use_ra ? (d <= 128 ? 128 : d <= 256 ? 256 : 512)
+                  : rs_layout ? (d <= 256 ? 256 : 512) : (int)(ceil_div(d, KC) * KC);
     Xh.alloc(pool, npad * dpad);
     if (want_lo) Xl.alloc(pool, npad * dpad);
     nrm.alloc(pool, npad);
@@ -2152,14 +2429,14 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       nrm.reset();
       sqn.reset();
       colscale.reset();
-      prep(false, true);
+      prep(false, true, rs_possible);
       split = true;
     }
   } else {
     // (the FP8 first pass is followed, for rows it cannot certify, by a bf16 search in
     // the AUTO form: one product when the input is bf16-representable, else the split)
     const bool want_lo = precision != KNN_BF16;
-    prep(false, want_lo);
+    prep(false, want_lo, rs_possible && want_lo);
     split = want_lo && !((precision == KNN_AUTO || fp8) && hflags[0] == 0);
   }
   // FP8 copy, its key scales and the worst row's quantisation error
@@ -2343,6 +2620,23 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       }
     }
 #undef EVOC_RA_LAUNCH
+  } else if (split && rs_possible && (dpad == 256 || dpad == 512)) {
+    const unsigned grid = (unsigned)(npad / RS_ROWS);
+    if (dpad == 512) {
+      if (metric == 1)
+        knn_topc_rs_kernel<2, true><<<grid, 64 * RS_WAVES, RsRing::SMEM, s>>>(
+            Xh.p, Xl.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);
+      else
+        knn_topc_rs_kernel<2, false><<<grid, 64 * RS_WAVES, RsRing::SMEM, s>>>(
+            Xh.p, Xl.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);
+    } else {
+      if (metric == 1)
+        knn_topc_rs_kernel<1, true><<<grid, 64 * RS_WAVES, RsRing::SMEM, s>>>(
+            Xh.p, Xl.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);
+      else
+        knn_topc_rs_kernel<1, false><<<grid, 64 * RS_WAVES, RsRing::SMEM, s>>>(
+            Xh.p, Xl.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);
+    }
   } else if (fp8) {
     const size_t smem = (size_t)4 * 2 * BM * 4 * 16 + BM * SCS * sizeof(float) +
                         2 * BM * sizeof(float) + 4 * sizeof(int);

@@ -259,3 +259,18 @@ def test_knn_rejects_more_than_64_neighbours():
     x = np.random.default_rng(0).normal(size=(200, 8)).astype(np.float32)
     with pytest.raises(H.EvocError, match="64"):
         H.knn(x, 65)
+
+
+@pytest.mark.parametrize("n,d,metric", [(2999, 200, "euclidean"), (1025, 300, "cosine"),
+                                        (4097, 512, "euclidean"), (129, 129, "cosine")])
+def test_knn_split_resident_query(n, d, metric):
+    """f32 inputs that are not bf16-exact, 128 < d <= 512, k <= 16: the split kernel with
+    hi and lo A-fragments resident (rows padded to 256 or 512 dimensions), ragged n, zero
+    rows and duplicates."""
+    x, _ = datagen.unit_mixture(n, d, 20, 0.2, seed=n + d)
+    x = x.astype(np.float32) * np.float32(1.7)
+    if metric == "cosine":
+        x[5] = 0.0
+    x[20:24] = x[10]
+    st = _check(x, 15, metric)
+    print(n, d, metric, st)
