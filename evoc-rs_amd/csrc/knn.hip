@@ -1370,42 +1370,49 @@ template __global__ void knn_topc_rb_kernel<RB_NKC, RB_KC, false, true>(const ui
                                                                  int, const float*, int32_t*,
                                                                  float*, const RbCols);
 
-// ------------------------------------------------------------------ 2c. split, resident
-// Inputs that are not bf16-exact, d <= 512, k <= 16: the products hi*hi + hi*lo + lo*hi of
-// the split x = hi + lo (bf16 each) in f32, as the streaming kernel forms them, with the
-// query rows' hi and lo A-fragments both resident (256 registers at d = 512). That leaves
-// one wavefront of 32 rows per SIMD (four per 128-row workgroup), but each pair of B
-// fragments (hi, lo) read from LDS feeds three MFMAs: three times the matrix work of the
-// bf16 kernel per epilogue, and two thirds of its LDS bytes per MFMA. Column tiles of 32
-// stream through a ring of four stages of 256 dimensions (hi and lo, 32 KB); the tile
-// epilogue (per-value filters, buckets, sorted lists of 16 per row and column half) is the
-// bf16 kernel's plain-sweep epilogue.
+// ------------------------------------------------------------------ 2c. resident, 1/SIMD
+// One wavefront of 32 query rows per SIMD (four per 128-row workgroup), with the rows'
+// A-fragments for all of d resident in the unified register file (up to 256 registers of
+// the 512 a single wavefront may hold): split inputs (hi*hi + hi*lo + lo*hi of x = hi + lo,
+// bf16 each, as the streaming kernel forms them; d <= 512, hi and lo both resident) and
+// bf16 rows with 512 < d <= 1024. Each B fragment read from LDS feeds three MFMAs (split)
+// or the tile holds 48-64 k-steps (d > 512): the matrix work per epilogue key is 1.5-3
+// times that of the two-wavefront-per-SIMD kernel, which lets a single wavefront keep the
+// matrix core busy through its own epilogue. Column tiles of 32 stream through a ring of
+// four stages of 256 dimensions; the tile epilogue (per-value filters, buckets, sorted
+// lists of C per row and column half) is the bf16 kernel's plain-sweep epilogue. Query
+// rows are the column rows themselves (qmap null: self-join, the diagonal masked) or
+// gathered rows with their global indices in qmap (ascending; the re-search of rows whose
+// certificate failed), over the column tiles [tc0, tc1).
 constexpr int RS_WAVES = 4;
 constexpr int RS_ROWS = RS_WAVES * 32;
 constexpr int RS_KC = 256;
-constexpr int RS_LA = 4;  // B fragment pairs read ahead of their MFMAs
+constexpr int RS_LA = 4;  // B fragments (pairs, split) read ahead of their MFMAs
+template <bool SPL>
 struct RsRing {
-  static constexpr int STB_H = RA_BN * RS_KC * 2;            // 16 KB: the hi half of a stage
-  static constexpr int STB_B = 2 * STB_H;                     // hi and lo
+  static constexpr int STB_H = RA_BN * RS_KC * 2;            // 16 KB: hi pieces of a stage
+  static constexpr int STB_B = SPL ? 2 * STB_H : STB_H;       // hi and lo
   static constexpr int STB = STB_B + RS_WAVES * 256;          // + column scales per wave
   static constexpr int NST = 4;
   static constexpr int PIECES = RS_KC / 8;
-  static constexpr int LOADS = STB_B / (64 * RS_WAVES * 16);  // 16-byte loads per thread
+  static constexpr int HLOADS = STB_H / (64 * RS_WAVES * 16);  // 16-byte loads per thread, hi
   static constexpr size_t SMEM = (size_t)NST * STB;
 };
-static_assert(RsRing::LOADS == 8, "four hi and four lo loads per thread and stage");
+static_assert(RsRing<true>::HLOADS == 4, "four hi (and four lo) loads per thread and stage");
 constexpr int RS_EPI_BYTES = RS_WAVES * RB_WREG + RS_WAVES * 64 * 4;
 
-template <int NKC, bool COS>
+template <int NKC, bool COS, bool SPL, int C>
 __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
-    const uint16_t* __restrict__ Xh, const uint16_t* __restrict__ Xl, int64_t n, int64_t npad,
-    int dpad, const float* __restrict__ colscale, int32_t* __restrict__ cand_idx,
+    const uint16_t* __restrict__ Xh, const uint16_t* __restrict__ Xl,
+    const uint16_t* __restrict__ Qh, const uint16_t* __restrict__ Ql,
+    const int32_t* __restrict__ qmap, int64_t nq, int64_t n, int dpad,
+    const float* __restrict__ colscale, int tc0, int tc1, int32_t* __restrict__ cand_idx,
     float* __restrict__ cand_key) {
-  using R = RsRing;
-  constexpr int C = 16;
+  using R = RsRing<SPL>;
   constexpr int NST = R::NST;
-  constexpr int LOADS = R::LOADS;
-  constexpr int KS = RS_KC / 16;  // MFMA k-steps per stage
+  constexpr int HL = R::HLOADS;
+  constexpr int LOADS = (SPL ? 2 : 1) * HL;  // vector-memory operations of a stage: + 1 scale
+  constexpr int KS = RS_KC / 16;             // MFMA k-steps per stage
   extern __shared__ __attribute__((aligned(16))) char smem[];
   __shared__ __attribute__((aligned(16))) char epi[RS_EPI_BYTES];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1413,17 +1420,34 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
   float* sc = reinterpret_cast<float*>(epi + wave * RB_WREG);
   int2* bucket = reinterpret_cast<int2*>(sc);  // [64 owners][RB_BKT]
   float* thr = reinterpret_cast<float*>(epi + RS_WAVES * RB_WREG) + wave * 64;  // [owner]
-  const int row0 = (int)blockIdx.x * RS_ROWS;
-  const int nt = (int)(npad / RA_BN);
+  const int row0 = (int)blockIdx.x * RS_ROWS;  // first query (list) row of the workgroup
+  const int nt = tc1 - tc0;
   const int total = nt * NKC;
 
-  bf16x8 afh[NKC * KS], afl[NKC * KS];
+  bf16x8 afh[NKC * KS];
+  bf16x8 afl[SPL ? NKC * KS : 1];
   {
     const int64_t ro = (int64_t)(row0 + wave * 32 + (lane & 31)) * dpad + (lane >> 5) * 8;
 #pragma unroll
     for (int f = 0; f < NKC * KS; ++f) {
-      afh[f] = *reinterpret_cast<const bf16x8*>(Xh + ro + f * 16);
-      afl[f] = *reinterpret_cast<const bf16x8*>(Xl + ro + f * 16);
+      afh[f] = *reinterpret_cast<const bf16x8*>(Qh + ro + f * 16);
+      if (SPL) afl[SPL ? f : 0] = *reinterpret_cast<const bf16x8*>(Ql + ro + f * 16);
+    }
+  }
+  // gathered rows: the global indices of the 16 rows whose values this lane holds (the
+  // diagonal mask) and the wavefront's range of them
+  int qg[16];
+  int qlo = 0, qhi = -1;
+  if (qmap) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int64_t r = row0 + wave * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+      qg[e] = r < nq ? qmap[r] : -1;
+    }
+    const int64_t rf = row0 + wave * 32, rl = std::min<int64_t>(rf + 31, nq - 1);
+    if (rf < nq) {
+      qlo = qmap[rf];
+      qhi = qmap[rl];
     }
   }
   wait_vm<0>();
@@ -1437,27 +1461,29 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
   thr[lane] = FINF;
   int bcnt = 0;
 
-  // buffer loads straight into LDS: loads 0..3 of a thread bring hi pieces, 4..7 the lo
-  // pieces at the same offsets; the even and odd lanes both bring their column's scale
-  int voff[LOADS / 2];
+  // buffer loads straight into LDS: loads 0..HL-1 of a thread bring hi pieces, HL..2HL-1
+  // (split) the lo pieces at the same offsets; even and odd lanes both bring their
+  // column's scale
+  int voff[HL];
 #pragma unroll
-  for (int h = 0; h < LOADS / 2; ++h) {
+  for (int h = 0; h < HL; ++h) {
     const int s_ = (h * RS_WAVES + wave) * 64 + lane;
     const int c = s_ / R::PIECES, qs = s_ % R::PIECES;
     voff[h] = (c * dpad + (qs ^ (c & 15)) * 8) * 2;
   }
   const int soff = (lane >> 1) * 4;
   const int tile_bytes = RA_BN * dpad * 2;
-  int gi = 0, tcur = 0, kci = 0, sloti = 0;
+  int gi = 0, tcur = tc0, kci = 0, sloti = 0;
   auto issue_next = [&]() {
     char* st = ring + sloti * R::STB;
     const int64_t tb = (int64_t)tcur * (RA_BN * dpad);
 #pragma unroll
-    for (int h = 0; h < LOADS / 2; ++h) {
+    for (int h = 0; h < HL; ++h) {
       buffer_load_lds16(Xh + tb, tile_bytes, st + (h * RS_WAVES + wave) * 1024, voff[h],
                         kci * RS_KC * 2);
-      buffer_load_lds16(Xl + tb, tile_bytes, st + R::STB_H + (h * RS_WAVES + wave) * 1024,
-                        voff[h], kci * RS_KC * 2);
+      if (SPL)
+        buffer_load_lds16(Xl + tb, tile_bytes, st + R::STB_H + (h * RS_WAVES + wave) * 1024,
+                          voff[h], kci * RS_KC * 2);
     }
     buffer_load_lds4(colscale + (int64_t)tcur * RA_BN, 4 * RA_BN, st + R::STB_B + wave * 256, soff);
     // past the last stage the tile stays put: the tail reloads the last stage into a slot
@@ -1475,9 +1501,12 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
   const int lcol = lane & 31, half = lcol >> 4, grp = lane >> 4;
   const int lrow = (lane >> 5) * 4;
   const int bcol = lcol * R::PIECES * 16, bm = (lane >> 5) ^ (lcol & 15);
-  for (int g = 0; g < NST - 1; ++g) issue_next();
-  wait_vm<(LOADS + 1) * (NST - 2)>();  // this wavefront's loads of stage 0 have landed
-  ring_barrier();
+  // an empty column range (a shard with no tile) issues no load: its lists stay empty
+  if (nt > 0) {
+    for (int g = 0; g < NST - 1; ++g) issue_next();
+    wait_vm<(LOADS + 1) * (NST - 2)>();  // this wavefront's loads of stage 0 have landed
+    ring_barrier();
+  }
   int slotc = 0;
   for (int ct = 0; ct < nt; ++ct) {
     f32x16 acc = (f32x16){};
@@ -1485,23 +1514,26 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
 #pragma unroll
     for (int kc = 0; kc < NKC; ++kc) {
       const char* st = ring + slotc * R::STB + bcol;
-      bf16x8 bh[RS_LA], bl[RS_LA];
+      bf16x8 bh[RS_LA], bl[SPL ? RS_LA : 1];
 #pragma unroll
       for (int u = 0; u < RS_LA; ++u) {
         bh[u] = *reinterpret_cast<const bf16x8*>(st + ((2 * u) ^ bm) * 16);
-        bl[u] = *reinterpret_cast<const bf16x8*>(st + R::STB_H + ((2 * u) ^ bm) * 16);
+        if (SPL) bl[SPL ? u : 0] = *reinterpret_cast<const bf16x8*>(st + R::STB_H + ((2 * u) ^ bm) * 16);
       }
 #pragma unroll
       for (int kk = 0; kk < KS; ++kk) {
         const int f = kc * KS + kk;
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afh[f], bh[kk % RS_LA], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afh[f], bl[kk % RS_LA], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afl[f], bh[kk % RS_LA], acc, 0, 0, 0);
+        if (SPL) {
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afh[f], bl[SPL ? kk % RS_LA : 0], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afl[SPL ? f : 0], bh[kk % RS_LA], acc, 0, 0, 0);
+        }
         if (kk == 0) issue_next();  // stage g + NST - 1, into the slot of stage g - 1
         if (kk + RS_LA < KS) {
           bh[kk % RS_LA] = *reinterpret_cast<const bf16x8*>(st + ((2 * (kk + RS_LA)) ^ bm) * 16);
-          bl[kk % RS_LA] =
-              *reinterpret_cast<const bf16x8*>(st + R::STB_H + ((2 * (kk + RS_LA)) ^ bm) * 16);
+          if (SPL)
+            bl[SPL ? kk % RS_LA : 0] =
+                *reinterpret_cast<const bf16x8*>(st + R::STB_H + ((2 * (kk + RS_LA)) ^ bm) * 16);
         }
       }
       if (kc == NKC - 1) {
@@ -1513,27 +1545,36 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
       slotc = slotc + 1 == NST ? 0 : slotc + 1;
     }
 
-    // ---------------- wavefront-private epilogue of column tile ct (as the bf16 kernel's)
-    const int cta = ct;
-    const bool diag = (cta * RA_BN < row0 + RS_ROWS) && (cta * RA_BN + RA_BN > row0);
+    // ---------------- wavefront-private epilogue of column tile tc0 + ct
+    const int cta = tc0 + ct;
+    const int cbase = cta * RA_BN;
     const bool tail = (int64_t)(cta + 1) * RA_BN > n;
     const float cs = csv.x;
     f32x4 tq[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) tq[j] = reinterpret_cast<const f32x4*>(thr + 32 * half + lrow)[2 * j];
-    int dv = (int)(cta * RA_BN - row0) - wave * 32 + lcol - lrow;
-    int rem = (int)std::min<int64_t>(n - (int64_t)cta * RA_BN, RA_BN) - lcol;
+    int rem = (int)std::min<int64_t>(n - (int64_t)cbase, RA_BN) - lcol;
     int sb = lrow * RA_SCS + lcol;
-    asm volatile("" : "+v"(dv), "+v"(rem), "+v"(sb));
+    asm volatile("" : "+v"(rem), "+v"(sb));
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const float s_ = acc[e];
       acc[e] = COS ? -(s_ * cs) : (cs - 2.0f * s_);
     }
-    if (__builtin_expect(diag || tail, 0)) {
+    if (!qmap) {
+      const bool diag = (cbase < row0 + RS_ROWS) && (cbase + RA_BN > row0);
+      int dv = cbase - row0 - wave * 32 + lcol - lrow;
+      asm volatile("" : "+v"(dv));
+      if (__builtin_expect(diag || tail, 0)) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+          if (rem <= 0 || dv == (e >> 2) * 8 + (e & 3)) acc[e] = FINF;
+      }
+    } else if (__builtin_expect(tail || (cbase + RA_BN > qlo && cbase <= qhi), 0)) {
+      const int cg = cbase + lcol;
 #pragma unroll
       for (int e = 0; e < 16; ++e)
-        if (rem <= 0 || dv == (e >> 2) * 8 + (e & 3)) acc[e] = FINF;
+        if (rem <= 0 || cg == qg[e]) acc[e] = FINF;
     }
     {
       unsigned long long anyp = 0;
@@ -1541,7 +1582,6 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
       for (int e = 0; e < 16; ++e) anyp |= __ballot(acc[e] < tq[e >> 2][e & 3]);
       if (anyp == 0) continue;  // nothing in this tile enters a list
     }
-    const int cbase = (int)(cta * RA_BN);
     bool touched = false, overflow = false;
     int inc = 0;
     int ob = (int)lds_addr(bucket) + ((grp & 1) * 32 + (grp >> 1) * 4) * RB_BKT * 8;
@@ -1625,12 +1665,12 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
     }
   }
   wait_vm<0>();
-  const int64_t grow = row0 + wave * 32 + (lane & 31);
-  if (grow < n) {
+  const int64_t lr = row0 + wave * 32 + (lane & 31);  // list row
+  if (lr < nq) {
 #pragma unroll
     for (int q = 0; q < C; ++q) {
-      cand_key[(grow * 2 + (lane >> 5)) * C + q] = lk[q];
-      cand_idx[(grow * 2 + (lane >> 5)) * C + q] = li[q];
+      cand_key[(lr * 2 + (lane >> 5)) * C + q] = lk[q];
+      cand_idx[(lr * 2 + (lane >> 5)) * C + q] = li[q];
     }
   }
 }
@@ -2323,6 +2363,54 @@ __global__ void knn_iota_kernel(int64_t n, int32_t* __restrict__ v) {
   if (i < n) v[i] = (int32_t)i;
 }
 
+// The resident one-wavefront-per-SIMD kernel's forms: split rows of 256 or 512 dimensions,
+// bf16 rows of 256 to 1024 (16 candidates per column half only above 512: below, the
+// two-wavefront kernel takes the first pass), cosine or squared-Euclidean keys.
+#define EVOC_RS_FORMS(X)                                                                      \
+  X(1, true, 16) X(2, true, 16) X(1, true, 32) X(2, true, 32) X(3, false, 16)              \
+      X(4, false, 16) X(1, false, 32) X(2, false, 32) X(3, false, 32) X(4, false, 32)
+static void rs_set_attributes() {
+#define EVOC_RS_ATTR(NKC_, SPL_, C_)                                                          \
+  EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rs_kernel<NKC_, true, SPL_, C_>,         \
+                               hipFuncAttributeMaxDynamicSharedMemorySize,                    \
+                               (int)RsRing<SPL_>::SMEM));                                     \
+  EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rs_kernel<NKC_, false, SPL_, C_>,        \
+                               hipFuncAttributeMaxDynamicSharedMemorySize,                    \
+                               (int)RsRing<SPL_>::SMEM));
+  EVOC_RS_FORMS(EVOC_RS_ATTR)
+#undef EVOC_RS_ATTR
+}
+static bool rs_supported(bool spl, int C, int dpad) {
+  if (dpad % RS_KC != 0) return false;
+  const int nkc = dpad / RS_KC;
+#define EVOC_RS_HAS(NKC_, SPL_, C_) if (nkc == NKC_ && spl == SPL_ && C == C_) return true;
+  EVOC_RS_FORMS(EVOC_RS_HAS)
+#undef EVOC_RS_HAS
+  return false;
+}
+// the query rows Qh/Ql (nqpad of them, a multiple of RS_ROWS) against column tiles
+// [tc0, tc1) of 32 of the column rows Xh/Xl
+static void rs_launch(bool spl, int C, int dpad, bool cos, const uint16_t* Xh, const uint16_t* Xl,
+                      const uint16_t* Qh, const uint16_t* Ql, const int32_t* qmap, int64_t nq,
+                      int64_t nqpad, int64_t n, const float* colscale, int tc0, int tc1,
+                      int32_t* cidx, float* ckey, hipStream_t s) {
+  const int nkc = dpad / RS_KC;
+  const unsigned grid = (unsigned)(nqpad / RS_ROWS);
+#define EVOC_RS_GO(NKC_, SPL_, C_)                                                            \
+  if (nkc == NKC_ && spl == SPL_ && C == C_) {                                               \
+    if (cos)                                                                                 \
+      knn_topc_rs_kernel<NKC_, true, SPL_, C_><<<grid, 64 * RS_WAVES, RsRing<SPL_>::SMEM, s>>>( \
+          Xh, Xl, Qh, Ql, qmap, nq, n, dpad, colscale, tc0, tc1, cidx, ckey);               \
+    else                                                                                     \
+      knn_topc_rs_kernel<NKC_, false, SPL_, C_><<<grid, 64 * RS_WAVES, RsRing<SPL_>::SMEM, s>>>( \
+          Xh, Xl, Qh, Ql, qmap, nq, n, dpad, colscale, tc0, tc1, cidx, ckey);               \
+    return;                                                                                  \
+  }
+  EVOC_RS_FORMS(EVOC_RS_GO)
+#undef EVOC_RS_GO
+  EVOC_CHECK(false, "knn: no resident-query kernel for this shape");
+}
+
 void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k, int precision,
                 int32_t* d_idx, float* d_dist, KnnStats* stats, hipStream_t s, DevPool& pool,
                 int verbose, KnnCapture* capture) {
@@ -2377,14 +2465,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
                                  (int)(RbRing<RB_KC>::SMEM + RB_QBYTES)));
     EVOC_HIP(hipFuncSetAttribute((const void*)knn_exact_merge_kernel<64>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 256 * 64 * 8));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rs_kernel<1, true>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RsRing::SMEM));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rs_kernel<1, false>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RsRing::SMEM));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rs_kernel<2, true>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RsRing::SMEM));
-    EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rs_kernel<2, false>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)RsRing::SMEM));
+    rs_set_attributes();
     attr_set = true;
   }
   // Two tile kernels: the resident-query kernel for bf16-exact inputs (d <= 512, k <= 16,
@@ -2400,13 +2481,14 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   int64_t npad = 0;
   int dpad = 0;
   bool ra = false, split = false;
-  // the split resident-query kernel (inputs that are not bf16-exact, 128 < d <= 512, k <= 16)
-  // takes rows padded to 256 or 512 dimensions
-  const bool rs_possible = !fp8 && k <= 16 && d > 128 && d <= 512;
-  auto prep = [&](bool use_ra, bool want_lo, bool rs_layout = false) {
+  // the resident one-wavefront-per-SIMD kernel takes rows padded to a multiple of 256
+  // dimensions (128 < d <= 1024): the first pass of split rows (d <= 512) and of bf16 rows
+  // above 512, and the escalation's re-search
+  const bool rs_layout = d > 128 && d <= 1024;
+  auto prep = [&](bool use_ra, bool want_lo) {
     npad = use_ra ? ceil_div(n, RA_ROWS) * RA_ROWS : ceil_div(n, BM) * BM;
     dpad = use_ra ? (d <= 128 ? 128 : d <= 256 ? 256 : 512)
-                  : rs_layout ? (d <= 256 ? 256 : 512) : (int)(ceil_div(d, KC) * KC);
+                  : rs_layout ? (int)(ceil_div(d, RS_KC) * RS_KC) : (int)(ceil_div(d, KC) * KC);
     Xh.alloc(pool, npad * dpad);
     if (want_lo) Xl.alloc(pool, npad * dpad);
     nrm.alloc(pool, npad);
@@ -2429,14 +2511,14 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       nrm.reset();
       sqn.reset();
       colscale.reset();
-      prep(false, true, rs_possible);
+      prep(false, true);
       split = true;
     }
   } else {
     // (the FP8 first pass is followed, for rows it cannot certify, by a bf16 search in
     // the AUTO form: one product when the input is bf16-representable, else the split)
     const bool want_lo = precision != KNN_BF16;
-    prep(false, want_lo, rs_possible && want_lo);
+    prep(false, want_lo);
     split = want_lo && !((precision == KNN_AUTO || fp8) && hflags[0] == 0);
   }
   // FP8 copy, its key scales and the worst row's quantisation error
@@ -2620,23 +2702,9 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       }
     }
 #undef EVOC_RA_LAUNCH
-  } else if (split && rs_possible && (dpad == 256 || dpad == 512)) {
-    const unsigned grid = (unsigned)(npad / RS_ROWS);
-    if (dpad == 512) {
-      if (metric == 1)
-        knn_topc_rs_kernel<2, true><<<grid, 64 * RS_WAVES, RsRing::SMEM, s>>>(
-            Xh.p, Xl.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);
-      else
-        knn_topc_rs_kernel<2, false><<<grid, 64 * RS_WAVES, RsRing::SMEM, s>>>(
-            Xh.p, Xl.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);
-    } else {
-      if (metric == 1)
-        knn_topc_rs_kernel<1, true><<<grid, 64 * RS_WAVES, RsRing::SMEM, s>>>(
-            Xh.p, Xl.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);
-      else
-        knn_topc_rs_kernel<1, false><<<grid, 64 * RS_WAVES, RsRing::SMEM, s>>>(
-            Xh.p, Xl.p, n, npad, dpad, colscale.p, cidx.p, ckey.p);
-    }
+  } else if (!fp8 && k <= 16 && rs_supported(split, 16, dpad)) {
+    rs_launch(split, 16, dpad, metric == 1, Xh.p, Xl.p, Xh.p, Xl.p, nullptr, n, npad, n,
+              colscale.p, 0, (int)(npad / RA_BN), cidx.p, ckey.p, s);
   } else if (fp8) {
     const size_t smem = (size_t)4 * 2 * BM * 4 * 16 + BM * SCS * sizeof(float) +
                         2 * BM * sizeof(float) + 4 * sizeof(int);
@@ -2791,11 +2859,16 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     const unsigned grid = (unsigned)(nqpad / BM);
     const int64_t ntile = npad / BN;
     if (skip_first) EVOC_HIP(hipEventRecord(e0, s));  // then the escalation is the tile pass
+    const bool rs_esc = rs_supported(split, 32, dpad);
     for (int sh = 0; sh < KNN_ESC_SHARDS; ++sh) {
       const int64_t ct0 = ntile * sh / KNN_ESC_SHARDS, ct1 = ntile * (sh + 1) / KNN_ESC_SHARDS;
       int32_t* oi = ecidx.p + (int64_t)sh * nq * 64;
       float* ok = eckey.p + (int64_t)sh * nq * 64;
-      if (split)
+      if (rs_esc)
+        rs_launch(split, 32, dpad, metric == 1, Xh.p, split ? Xl.p : nullptr, Qh.p,
+                  split ? Ql.p : nullptr, qmap.p, nq, nqpad, n, colscale.p,
+                  (int)(ct0 * (BN / RA_BN)), (int)(ct1 * (BN / RA_BN)), oi, ok, s);
+      else if (split)
         knn_topc_kernel<true, 32><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
                                                           metric, oi, ok, Qh.p, Ql.p, qmap.p, nq,
                                                           ct0, ct1);
