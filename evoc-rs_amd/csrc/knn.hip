@@ -1399,20 +1399,34 @@ struct RsRing {
   static constexpr size_t SMEM = (size_t)NST * STB;
 };
 static_assert(RsRing<true>::HLOADS == 4, "four hi (and four lo) loads per thread and stage");
-constexpr int RS_EPI_BYTES = RS_WAVES * RB_WREG + RS_WAVES * 64 * 4;
+// buckets and key-tile dumps, thresholds, and (symmetric sweep) the rows' key scales
+constexpr int RS_EPI_BYTES = RS_WAVES * RB_WREG + RS_WAVES * 64 * 4 + RS_WAVES * 32 * 4;
+// symmetric sweep: LDS queue entries (column, row, dot) per wavefront, after the ring
+constexpr int RS_QL = 160;
+constexpr size_t RS_QBYTES = (size_t)RS_WAVES * RS_QL * 12;
+static_assert(RsRing<true>::SMEM + RS_QBYTES + RS_EPI_BYTES <= 160 * 1024,
+              "the split symmetric sweep's LDS fits one CU");
 
-template <int NKC, bool COS, bool SPL, int C>
+// SYM: the symmetric sweep of a self-join (qmap null, column tiles [0, tc1) = all rows):
+// 128-row block I takes the cyclic range I, I+1, ... of half the blocks, as the
+// two-wavefront kernel's sweep does with 256-row blocks (see RbCols: thresholds in kt,
+// transposed keys queued per wavefront, the largest forward key published at the end).
+// self = 0 (qmap null): the columns are other rows than the queries (the threshold pass
+// against a sample); nothing but the column padding is masked.
+template <int NKC, bool COS, bool SPL, int C, bool SYM = false>
 __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
     const uint16_t* __restrict__ Xh, const uint16_t* __restrict__ Xl,
     const uint16_t* __restrict__ Qh, const uint16_t* __restrict__ Ql,
     const int32_t* __restrict__ qmap, int64_t nq, int64_t n, int dpad,
-    const float* __restrict__ colscale, int tc0, int tc1, int32_t* __restrict__ cand_idx,
-    float* __restrict__ cand_key) {
+    const float* __restrict__ colscale, int tc0, int tc1, int self,
+    int32_t* __restrict__ cand_idx, float* __restrict__ cand_key, const RbCols cols) {
   using R = RsRing<SPL>;
   constexpr int NST = R::NST;
   constexpr int HL = R::HLOADS;
   constexpr int LOADS = (SPL ? 2 : 1) * HL;  // vector-memory operations of a stage: + 1 scale
   constexpr int KS = RS_KC / 16;             // MFMA k-steps per stage
+  constexpr int TPB = RS_ROWS / RA_BN;       // column tiles per 128-row block
+  static_assert(!SYM || C == 16, "the symmetric sweep's merge takes half-lists of 16");
   extern __shared__ __attribute__((aligned(16))) char smem[];
   __shared__ __attribute__((aligned(16))) char epi[RS_EPI_BYTES];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1420,9 +1434,62 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
   float* sc = reinterpret_cast<float*>(epi + wave * RB_WREG);
   int2* bucket = reinterpret_cast<int2*>(sc);  // [64 owners][RB_BKT]
   float* thr = reinterpret_cast<float*>(epi + RS_WAVES * RB_WREG) + wave * 64;  // [owner]
-  const int row0 = (int)blockIdx.x * RS_ROWS;  // first query (list) row of the workgroup
-  const int nt = tc1 - tc0;
+  float* rsc = reinterpret_cast<float*>(epi + RS_WAVES * RB_WREG + RS_WAVES * 64 * 4) + wave * 32;
+  // blocks in descending order in the symmetric sweep (the blocks a workgroup queues
+  // transposed keys for come after its own and have mostly tightened their thresholds)
+  const int gblk = (int)blockIdx.x + (SYM ? cols.block0 : 0);
+  const int bid = SYM ? (cols.nblocks > 0 ? cols.nblocks : (int)gridDim.x) - 1 - gblk : gblk;
+  const int row0 = bid * RS_ROWS;  // first query (list) row of the workgroup
+  const int ntile = tc1;
+  int tile0 = tc0, nt = tc1 - tc0, tshift = 0;
+  if (SYM) {
+    // the workgroups resident together (`group` consecutive launch indices) start their
+    // walks at the same block, so that the L2 serves most of their column reads
+    const int m = ntile / TPB;
+    const int nblk = (m & 1) ? (m + 1) / 2 : m / 2 + (bid < m / 2 ? 1 : 0);
+    tile0 = bid * TPB;
+    nt = nblk * TPB;
+    const int g = cols.group > 0 ? (int)cols.group : 1;
+    tshift = std::min(gblk % g, nblk - 1) * TPB;
+  }
+  // absolute column tile of the ct-th tile of this walk
+  auto tile_at = [&](int ct) {
+    if (!SYM) return tc0 + ct;
+    int r = tshift + ct;
+    if (r >= nt) r -= nt;
+    int a = tile0 + r;
+    if (a >= ntile) a -= ntile;
+    return a;
+  };
   const int total = nt * NKC;
+  int qn = 0;  // this wavefront's reverse candidates in its global queue
+  int lq = 0;  // and staged in LDS
+  const int64_t qbase = SYM ? ((int64_t)gblk * RS_WAVES + wave) * cols.qcap : 0;
+  int32_t* const qjw = SYM ? cols.qj + qbase : nullptr;
+  int32_t* const qrw = SYM ? cols.qr + qbase : nullptr;
+  float* const qkw = SYM ? cols.qk + qbase : nullptr;
+  const int qcap = SYM ? (int)std::min<int64_t>(cols.qcap, INT32_MAX / 2) : 0;
+  // LDS queue: [RS_QL] columns, [RS_QL] rows, [RS_QL] dots per wavefront; copied to the
+  // global queue with every load of the ring landed and the stores waited for, so that the
+  // ring's counted waits see ring loads only
+  const unsigned lqa = lds_addr(smem + R::SMEM + (size_t)wave * RS_QL * 12);
+  auto flush = [&]() {
+    wait_vm<0>();
+    for (int e = lane; e < lq; e += 64) {
+      const int cj = lds_ld_b32(lqa + 4u * e);
+      const int cr = lds_ld_b32(lqa + 4u * (RS_QL + e));
+      const int ck = lds_ld_b32(lqa + 4u * (2 * RS_QL + e));
+      const int pos = qn + e;
+      if (pos < qcap) {
+        qjw[pos] = cj;
+        qrw[pos] = cr;
+        qkw[pos] = __int_as_float(ck);
+      }
+    }
+    wait_vm<0>();
+    qn += lq;
+    lq = 0;
+  };
 
   bf16x8 afh[NKC * KS];
   bf16x8 afl[SPL ? NKC * KS : 1];
@@ -1459,11 +1526,29 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
     li[q] = -1;
   }
   thr[lane] = FINF;
+  // the smallest and largest row scale of this wavefront: a bound that rules out the
+  // transposed keys of most tiles before any per-row work
+  float rs_min = 0.0f, rs_max = 0.0f;
+  if (SYM) {
+    const int64_t r = row0 + wave * 32 + (lane & 31);
+    const float v = r < n ? colscale[r] : 0.0f;
+    if (lane < 32) rsc[lane] = v;
+    rs_min = v;
+    rs_max = v;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      rs_min = fminf(rs_min, __shfl_xor(rs_min, o, 64));
+      rs_max = fmaxf(rs_max, __shfl_xor(rs_max, o, 64));
+    }
+    rs_min = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(rs_min)));
+    rs_max = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(rs_max)));
+  }
   int bcnt = 0;
 
   // buffer loads straight into LDS: loads 0..HL-1 of a thread bring hi pieces, HL..2HL-1
   // (split) the lo pieces at the same offsets; even and odd lanes both bring their
-  // column's scale
+  // column's scale, or (symmetric sweep) the even lanes its scale and the odd lanes its
+  // reverse threshold
   int voff[HL];
 #pragma unroll
   for (int h = 0; h < HL; ++h) {
@@ -1471,12 +1556,13 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
     const int c = s_ / R::PIECES, qs = s_ % R::PIECES;
     voff[h] = (c * dpad + (qs ^ (c & 15)) * 8) * 2;
   }
-  const int soff = (lane >> 1) * 4;
+  const int soff = SYM ? lane * 4 : (lane >> 1) * 4;
   const int tile_bytes = RA_BN * dpad * 2;
-  int gi = 0, tcur = tc0, kci = 0, sloti = 0;
+  int gi = 0, tcur = 0, kci = 0, sloti = 0;
+  int tabs = tile_at(0);  // absolute column tile being issued
   auto issue_next = [&]() {
     char* st = ring + sloti * R::STB;
-    const int64_t tb = (int64_t)tcur * (RA_BN * dpad);
+    const int64_t tb = (int64_t)tabs * (RA_BN * dpad);
 #pragma unroll
     for (int h = 0; h < HL; ++h) {
       buffer_load_lds16(Xh + tb, tile_bytes, st + (h * RS_WAVES + wave) * 1024, voff[h],
@@ -1485,13 +1571,19 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
         buffer_load_lds16(Xl + tb, tile_bytes, st + R::STB_H + (h * RS_WAVES + wave) * 1024,
                           voff[h], kci * RS_KC * 2);
     }
-    buffer_load_lds4(colscale + (int64_t)tcur * RA_BN, 4 * RA_BN, st + R::STB_B + wave * 256, soff);
+    if (SYM)
+      buffer_load_lds4(cols.kt + (int64_t)tabs * (2 * RA_BN), 8 * RA_BN, st + R::STB_B + wave * 256,
+                       soff);
+    else
+      buffer_load_lds4(colscale + (int64_t)tabs * RA_BN, 4 * RA_BN, st + R::STB_B + wave * 256,
+                       soff);
     // past the last stage the tile stays put: the tail reloads the last stage into a slot
     // no later chunk reads, so every chunk issues the same loads and the waits are static
     if (gi + 1 < total) {
       if (++kci == NKC) {
         kci = 0;
         ++tcur;
+        tabs = tile_at(tcur);
       }
     }
     ++gi;
@@ -1545,11 +1637,82 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
       slotc = slotc + 1 == NST ? 0 : slotc + 1;
     }
 
-    // ---------------- wavefront-private epilogue of column tile tc0 + ct
-    const int cta = tc0 + ct;
+    // ---------------- wavefront-private epilogue of the walk's ct-th column tile
+    const int cta = tile_at(ct);
     const int cbase = cta * RA_BN;
     const bool tail = (int64_t)(cta + 1) * RA_BN > n;
     const float cs = csv.x;
+    if (SYM && cta / TPB != bid) {
+      // Transposed keys: column cbase + lcol is a row too and this wavefront's rows are its
+      // candidates; a (row, column) pair is queued when its key, formed with the row's scale,
+      // beats the column's threshold csv.y. Keys fall as dots rise (-(s r) for cosine with
+      // r >= 0, r - 2 s for squared Euclidean) and f32 rounding is monotone, so for dots
+      // s <= smax and row scales r in [rs_min, rs_max] every key is at least
+      // -max(smax rs_min, smax rs_max), resp. rs_min - 2 smax: the whole tile is tested
+      // against that bound first, then (cosine) each group of four rows against its own.
+      // The queue keeps the raw dot; the scatter forms the key again, identically.
+      float smax;
+      {
+        const float m0 = fmaxf(fmaxf(acc[0], acc[1]), acc[2]), m1 = fmaxf(fmaxf(acc[3], acc[4]), acc[5]);
+        const float m2 = fmaxf(fmaxf(acc[6], acc[7]), acc[8]), m3 = fmaxf(fmaxf(acc[9], acc[10]), acc[11]);
+        const float m4 = fmaxf(fmaxf(acc[12], acc[13]), acc[14]);
+        smax = fmaxf(fmaxf(m0, m1), fmaxf(fmaxf(m2, m3), fmaxf(m4, acc[15])));
+      }
+      const int cabs = cbase + lcol;
+      float t0c = csv.y;
+      if (!((int64_t)cabs < n)) t0c = -FINF;
+      const float rkmin = COS ? -fmaxf(smax * rs_min, smax * rs_max) : (rs_min - 2.0f * smax);
+      unsigned m16 = 0u;
+      if (__ballot(rkmin < t0c) != 0) {
+        m16 = 0xffffu;
+        if (COS) {
+          m16 = 0;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const float gm = fmaxf(fmaxf(acc[4 * j], acc[4 * j + 1]), fmaxf(acc[4 * j + 2], acc[4 * j + 3]));
+            m16 |= (-fmaxf(gm * rs_min, gm * rs_max) < t0c ? 15u : 0u) << (4 * j);
+          }
+        }
+        if (row0 + RS_ROWS > n) {  // the last block: rows past n are padding
+#pragma unroll
+          for (int e = 0; e < 16; ++e)
+            if (!((int64_t)(row0 + wave * 32 + (e >> 2) * 8 + lrow + (e & 3)) < n)) m16 &= ~(1u << e);
+        }
+      }
+      if (__ballot(m16 != 0) != 0) {
+        const int rbase = row0 + wave * 32 + lrow;
+#pragma unroll 1
+        for (int j = 0; j < 4; ++j) {
+          if (__ballot(((m16 >> (4 * j)) & 15u) != 0) == 0) continue;
+          f32x4 rs;  // the scales of rows lrow + 8 j .. + 3
+          {
+            const unsigned ra_ = lds_addr(rsc + lrow + j * 8);
+            asm volatile("ds_read_b128 %0, %1\ns_waitcnt lgkmcnt(0)" : "=v"(rs) : "v"(ra_) : "memory");
+          }
+          float sj[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) sj[i] = acc[j * 4 + i];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float rk = COS ? -(sj[i] * rs[i]) : (rs[i] - 2.0f * sj[i]);
+            const bool rp = ((m16 >> (4 * j + i)) & 1u) && (rk < t0c);
+            const unsigned long long rm = __ballot(rp);
+            if (rm) {
+              if (lq > RS_QL - 64) flush();
+              const int below = (int)__builtin_amdgcn_mbcnt_hi(
+                  (unsigned)(rm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)rm, 0u));
+              const unsigned q = (unsigned)(lq + below);
+              if (rp) {
+                lds_st_b32(lqa + 4u * q, cabs);
+                lds_st_b32(lqa + 4u * (RS_QL + q), rbase + j * 8 + i);
+                lds_st_b32(lqa + 4u * (2 * RS_QL + q), __float_as_int(sj[i]));
+              }
+              lq += __popcll(rm);
+            }
+          }
+        }
+      }
+    }
     f32x4 tq[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) tq[j] = reinterpret_cast<const f32x4*>(thr + 32 * half + lrow)[2 * j];
@@ -1562,7 +1725,7 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
       acc[e] = COS ? -(s_ * cs) : (cs - 2.0f * s_);
     }
     if (!qmap) {
-      const bool diag = (cbase < row0 + RS_ROWS) && (cbase + RA_BN > row0);
+      const bool diag = self && (cbase < row0 + RS_ROWS) && (cbase + RA_BN > row0);
       int dv = cbase - row0 - wave * 32 + lcol - lrow;
       asm volatile("" : "+v"(dv));
       if (__builtin_expect(diag || tail, 0)) {
@@ -1666,6 +1829,15 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
   }
   wait_vm<0>();
   const int64_t lr = row0 + wave * 32 + (lane & 31);  // list row
+  if (SYM) {
+    if (lq > 0) flush();
+    if (lane == 0) cols.qcount[(int64_t)gblk * RS_WAVES + wave] = qn;
+    // the row's largest forward key becomes its threshold when smaller: a transposed key
+    // at or above it is beaten by all 32 forward candidates (workgroups still to come read
+    // the new value or, from a stale cache line, the old one: either is an upper bound)
+    const float fwdmax = fmaxf(lk[C - 1], __shfl_xor(lk[C - 1], 32, 64));
+    if (lane < 32 && lr < n && fwdmax < cols.kt[2 * lr + 1]) cols.kt[2 * lr + 1] = fwdmax;
+  }
   if (lr < nq) {
 #pragma unroll
     for (int q = 0; q < C; ++q) {
@@ -1697,9 +1869,9 @@ __global__ void rb_sample_kernel(const uint16_t* __restrict__ Xh, const float* _
 // the transposed keys, S the sample size).
 constexpr int RB_THR_RANK = 6;
 static_assert(RB_THR_RANK <= RB_PASS_C, "the pass lists must hold the threshold rank");
+template <int L>  // candidates per half-list of the pass
 __global__ void rb_thr0_kernel(const float* __restrict__ ckey, const float* __restrict__ colscale,
                                int64_t n, int64_t npad, float* __restrict__ kt) {
-  constexpr int L = RB_PASS_C;
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= npad) return;
   kt[2 * r] = colscale[r];
@@ -2369,6 +2541,8 @@ __global__ void knn_iota_kernel(int64_t n, int32_t* __restrict__ v) {
 #define EVOC_RS_FORMS(X)                                                                      \
   X(1, true, 16) X(2, true, 16) X(1, true, 32) X(2, true, 32) X(3, false, 16)              \
       X(4, false, 16) X(1, false, 32) X(2, false, 32) X(3, false, 32) X(4, false, 32)
+// the symmetric sweep's forms: the first pass of split rows and of bf16 rows above 512
+#define EVOC_RS_SYM_FORMS(X) X(1, true) X(2, true) X(3, false) X(4, false)
 static void rs_set_attributes() {
 #define EVOC_RS_ATTR(NKC_, SPL_, C_)                                                          \
   EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rs_kernel<NKC_, true, SPL_, C_>,         \
@@ -2379,6 +2553,15 @@ static void rs_set_attributes() {
                                (int)RsRing<SPL_>::SMEM));
   EVOC_RS_FORMS(EVOC_RS_ATTR)
 #undef EVOC_RS_ATTR
+#define EVOC_RS_SYM_ATTR(NKC_, SPL_)                                                          \
+  EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rs_kernel<NKC_, true, SPL_, 16, true>,   \
+                               hipFuncAttributeMaxDynamicSharedMemorySize,                    \
+                               (int)(RsRing<SPL_>::SMEM + RS_QBYTES)));                       \
+  EVOC_HIP(hipFuncSetAttribute((const void*)knn_topc_rs_kernel<NKC_, false, SPL_, 16, true>,  \
+                               hipFuncAttributeMaxDynamicSharedMemorySize,                    \
+                               (int)(RsRing<SPL_>::SMEM + RS_QBYTES)));
+  EVOC_RS_SYM_FORMS(EVOC_RS_SYM_ATTR)
+#undef EVOC_RS_SYM_ATTR
 }
 static bool rs_supported(bool spl, int C, int dpad) {
   if (dpad % RS_KC != 0) return false;
@@ -2388,27 +2571,60 @@ static bool rs_supported(bool spl, int C, int dpad) {
 #undef EVOC_RS_HAS
   return false;
 }
+static bool rs_sym_supported(bool spl, int dpad) {
+  if (dpad % RS_KC != 0) return false;
+  const int nkc = dpad / RS_KC;
+#define EVOC_RS_SYM_HAS(NKC_, SPL_) if (nkc == NKC_ && spl == SPL_) return true;
+  EVOC_RS_SYM_FORMS(EVOC_RS_SYM_HAS)
+#undef EVOC_RS_SYM_HAS
+  return false;
+}
 // the query rows Qh/Ql (nqpad of them, a multiple of RS_ROWS) against column tiles
-// [tc0, tc1) of 32 of the column rows Xh/Xl
+// [tc0, tc1) of 32 of the column rows Xh/Xl (n valid); self: the columns are the query
+// rows (qmap null), the diagonal is masked
 static void rs_launch(bool spl, int C, int dpad, bool cos, const uint16_t* Xh, const uint16_t* Xl,
                       const uint16_t* Qh, const uint16_t* Ql, const int32_t* qmap, int64_t nq,
-                      int64_t nqpad, int64_t n, const float* colscale, int tc0, int tc1,
+                      int64_t nqpad, int64_t n, const float* colscale, int tc0, int tc1, int self,
                       int32_t* cidx, float* ckey, hipStream_t s) {
   const int nkc = dpad / RS_KC;
   const unsigned grid = (unsigned)(nqpad / RS_ROWS);
+  const RbCols none{};
 #define EVOC_RS_GO(NKC_, SPL_, C_)                                                            \
   if (nkc == NKC_ && spl == SPL_ && C == C_) {                                               \
     if (cos)                                                                                 \
       knn_topc_rs_kernel<NKC_, true, SPL_, C_><<<grid, 64 * RS_WAVES, RsRing<SPL_>::SMEM, s>>>( \
-          Xh, Xl, Qh, Ql, qmap, nq, n, dpad, colscale, tc0, tc1, cidx, ckey);               \
+          Xh, Xl, Qh, Ql, qmap, nq, n, dpad, colscale, tc0, tc1, self, cidx, ckey, none);   \
     else                                                                                     \
       knn_topc_rs_kernel<NKC_, false, SPL_, C_><<<grid, 64 * RS_WAVES, RsRing<SPL_>::SMEM, s>>>( \
-          Xh, Xl, Qh, Ql, qmap, nq, n, dpad, colscale, tc0, tc1, cidx, ckey);               \
+          Xh, Xl, Qh, Ql, qmap, nq, n, dpad, colscale, tc0, tc1, self, cidx, ckey, none);   \
     return;                                                                                  \
   }
   EVOC_RS_FORMS(EVOC_RS_GO)
 #undef EVOC_RS_GO
   EVOC_CHECK(false, "knn: no resident-query kernel for this shape");
+}
+// one slice of the symmetric sweep: workgroups cols.block0 .. + nwg of the npad / RS_ROWS
+// blocks (cols.nblocks), each over its half of the column tiles
+static void rs_sym_launch(bool spl, int dpad, bool cos, const uint16_t* Xh, const uint16_t* Xl,
+                          int64_t n, int64_t npad, const float* colscale, unsigned nwg,
+                          const RbCols& cols, int32_t* cidx, float* ckey, hipStream_t s) {
+  const int nkc = dpad / RS_KC;
+  const int ntile = (int)(npad / RA_BN);
+#define EVOC_RS_SYM_GO(NKC_, SPL_)                                                            \
+  if (nkc == NKC_ && spl == SPL_) {                                                          \
+    if (cos)                                                                                 \
+      knn_topc_rs_kernel<NKC_, true, SPL_, 16, true>                                         \
+          <<<nwg, 64 * RS_WAVES, RsRing<SPL_>::SMEM + RS_QBYTES, s>>>(                       \
+              Xh, Xl, Xh, Xl, nullptr, n, n, dpad, colscale, 0, ntile, 1, cidx, ckey, cols); \
+    else                                                                                     \
+      knn_topc_rs_kernel<NKC_, false, SPL_, 16, true>                                        \
+          <<<nwg, 64 * RS_WAVES, RsRing<SPL_>::SMEM + RS_QBYTES, s>>>(                       \
+              Xh, Xl, Xh, Xl, nullptr, n, n, dpad, colscale, 0, ntile, 1, cidx, ckey, cols); \
+    return;                                                                                  \
+  }
+  EVOC_RS_SYM_FORMS(EVOC_RS_SYM_GO)
+#undef EVOC_RS_SYM_GO
+  EVOC_CHECK(false, "knn: no symmetric resident-query kernel for this shape");
 }
 
 void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k, int precision,
@@ -2552,35 +2768,19 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   // measured up to ~160 u of the sum of |products| (above the 2 d u of sequential f32
   // adds), hence the 512 u allowance
   const float gamma8 = (2.0f * (float)dpad8 + 512.0f) * u + e_in8;
-  // The symmetric sweep (each pair of row blocks once) for the d in (256, 512] kernel at
-  // large n; EVOC_HIP_KNN_SYMMETRIC=1 / 0 forces it on / off.
+  // The symmetric sweep (each pair of row blocks once) at large n, by the two-wavefront
+  // kernel (bf16 rows, d in (256, 512]) or by the resident one (its first-pass forms: split
+  // rows, bf16 rows above 512); EVOC_HIP_KNN_SYMMETRIC=1 / 0 forces it on / off.
+  const bool rs_first = !ra && !fp8 && k <= 16 && rs_supported(split, 16, dpad);
   bool sym = false;
-  if (ra && dpad == 512) {
+  if ((ra && dpad == 512) || (rs_first && rs_sym_supported(split, dpad))) {
     const char* env = std::getenv("EVOC_HIP_KNN_SYMMETRIC");
     sym = env ? (env[0] == '1') : (n >= KNN_SYM_MIN_N);
   }
   int64_t sym_S = 0, sym_Spad = 0, sym_qcap = 0;
-  Buf<uint16_t> Xs;
+  Buf<uint16_t> Xs, Xsl;
   Buf<float> ss, kt, qk;
   Buf<int32_t> qj, qr, qcount;
-  if (sym) {
-    sym_S = std::min<int64_t>(65536, std::max<int64_t>(256, n / 64 / 256 * 256));
-    sym_S = std::min<int64_t>(sym_S, n);
-    sym_Spad = ceil_div(sym_S, RA_BN) * RA_BN;
-    Xs.alloc(pool, sym_Spad * dpad);
-    ss.alloc(pool, sym_Spad);
-    kt.alloc(pool, 2 * npad);
-    rb_sample_kernel<<<(unsigned)ceil_div(sym_Spad * (dpad / 8), 256), 256, 0, s>>>(
-        Xh.p, colscale.p, n, dpad, sym_S, sym_Spad, Xs.p, ss.p);
-    // about 32 rows x n/2 columns x RB_THR_RANK/S reverse keys per wavefront; three times that
-    const int64_t expect = 16 * RB_THR_RANK * ceil_div(n, sym_S);
-    sym_qcap = std::max<int64_t>(16384, ceil_div(3 * expect, 256) * 256);
-    const int64_t nwaves = (npad / RB_ROWS) * RB_WAVES;
-    qj.alloc(pool, nwaves * sym_qcap);
-    qr.alloc(pool, nwaves * sym_qcap);
-    qk.alloc(pool, nwaves * sym_qcap);
-    qcount.alloc(pool, nwaves);
-  }
   // The first pass's certificate rate, estimated on evenly spaced sample rows searched
   // with the streaming kernel (bf16 or split, as the escalation would search them; the
   // FP8 pass certifies fewer rows than that). Where more than half of them fail (neighbour
@@ -2632,6 +2832,30 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       }
     }
   }
+  if (skip_first) sym = false;
+  if (sym) {
+    sym_S = std::min<int64_t>(65536, std::max<int64_t>(256, n / 64 / 256 * 256));
+    sym_S = std::min<int64_t>(sym_S, n);
+    sym_Spad = ceil_div(sym_S, RA_BN) * RA_BN;
+    Xs.alloc(pool, sym_Spad * dpad);
+    ss.alloc(pool, sym_Spad);
+    kt.alloc(pool, 2 * npad);
+    rb_sample_kernel<<<(unsigned)ceil_div(sym_Spad * (dpad / 8), 256), 256, 0, s>>>(
+        Xh.p, colscale.p, n, dpad, sym_S, sym_Spad, Xs.p, ss.p);
+    if (split) {  // the sample's lo parts (its scales written again, identically)
+      Xsl.alloc(pool, sym_Spad * dpad);
+      rb_sample_kernel<<<(unsigned)ceil_div(sym_Spad * (dpad / 8), 256), 256, 0, s>>>(
+          Xl.p, colscale.p, n, dpad, sym_S, sym_Spad, Xsl.p, ss.p);
+    }
+    // about 32 rows x n/2 columns x RB_THR_RANK/S reverse keys per wavefront; three times that
+    const int64_t expect = 16 * RB_THR_RANK * ceil_div(n, sym_S);
+    sym_qcap = std::max<int64_t>(16384, ceil_div(3 * expect, 256) * 256);
+    const int64_t nwaves = npad / 32;  // 32 rows per wavefront in both kernels
+    qj.alloc(pool, nwaves * sym_qcap);
+    qr.alloc(pool, nwaves * sym_qcap);
+    qk.alloc(pool, nwaves * sym_qcap);
+    qcount.alloc(pool, nwaves);
+  }
   hipEvent_t e0, e1;
   EVOC_HIP(hipEventCreate(&e0));
   EVOC_HIP(hipEventCreate(&e1));
@@ -2665,7 +2889,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
         else
           knn_topc_rb_kernel<RB_NKC, RB_KC, false, false, RB_PASS_C><<<grid, 512, RbRing<RB_KC>::SMEM, s>>>(
               Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, sc);
-        rb_thr0_kernel<<<(unsigned)ceil_div(npad, 256), 256, 0, s>>>(ckey.p, colscale.p, n, npad,
+        rb_thr0_kernel<RB_PASS_C><<<(unsigned)ceil_div(npad, 256), 256, 0, s>>>(ckey.p, colscale.p, n, npad,
                                                                     kt.p);
         EVOC_HIP(hipEventRecord(e0, s));
         cols.kt = kt.p;
@@ -2702,9 +2926,38 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       }
     }
 #undef EVOC_RA_LAUNCH
-  } else if (!fp8 && k <= 16 && rs_supported(split, 16, dpad)) {
-    rs_launch(split, 16, dpad, metric == 1, Xh.p, Xl.p, Xh.p, Xl.p, nullptr, n, npad, n,
-              colscale.p, 0, (int)(npad / RA_BN), cidx.p, ckey.p, s);
+  } else if (rs_first) {
+    if (sym) {
+      // threshold pass against the sample (16 per column half: RB_THR_RANK of them are
+      // read), then the symmetric sweep, in slices of one workgroup per CU as above
+      rs_launch(split, 16, dpad, metric == 1, Xs.p, split ? Xsl.p : nullptr, Xh.p, Xl.p, nullptr,
+                n, npad, sym_S, ss.p, 0, (int)(sym_Spad / RA_BN), 0, cidx.p, ckey.p, s);
+      rb_thr0_kernel<16><<<(unsigned)ceil_div(npad, 256), 256, 0, s>>>(ckey.p, colscale.p, n, npad,
+                                                                     kt.p);
+      EVOC_HIP(hipGetLastError());
+      EVOC_HIP(hipEventRecord(e0, s));
+      RbCols cols{};
+      cols.kt = kt.p;
+      cols.qj = qj.p;
+      cols.qr = qr.p;
+      cols.qk = qk.p;
+      cols.qcap = sym_qcap;
+      cols.qcount = qcount.p;
+      cols.group = cu_count();
+      const int nblk = (int)(npad / RS_ROWS);
+      const char* slenv = std::getenv("EVOC_HIP_KNN_SLICED");
+      const bool sliced = !(slenv && slenv[0] == '0');
+      const int gsz = sliced ? (int)std::max<int64_t>(1, cols.group) : nblk;
+      cols.nblocks = nblk;
+      for (int b0 = 0; b0 < nblk; b0 += gsz) {
+        cols.block0 = b0;
+        rs_sym_launch(split, dpad, metric == 1, Xh.p, Xl.p, n, npad, colscale.p,
+                      (unsigned)std::min<int>(gsz, nblk - b0), cols, cidx.p, ckey.p, s);
+      }
+    } else {
+      rs_launch(split, 16, dpad, metric == 1, Xh.p, Xl.p, Xh.p, Xl.p, nullptr, n, npad, n,
+                colscale.p, 0, (int)(npad / RA_BN), 1, cidx.p, ckey.p, s);
+    }
   } else if (fp8) {
     const size_t smem = (size_t)4 * 2 * BM * 4 * 16 + BM * SCS * sizeof(float) +
                         2 * BM * sizeof(float) + 4 * sizeof(int);
@@ -2749,7 +3002,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   Buf<float> mkey, bnd;
   Buf<int32_t> midx;
   if (sym) {
-    const int64_t nwaves = (npad / RB_ROWS) * RB_WAVES;
+    const int64_t nwaves = npad / 32;
     std::vector<int32_t> hq(nwaves);
     EVOC_HIP(hipMemcpyAsync(hq.data(), qcount.p, nwaves * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     EVOC_HIP(hipStreamSynchronize(s));
@@ -2757,15 +3010,20 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     for (auto q : hq) qmax = std::max<int64_t>(qmax, q);
     if (qmax > sym_qcap) {  // a queue overflowed: sweep every pair instead
       sym = false;
-      RbCols cols{Xh.p, colscale.p, nullptr, n, npad / RA_BN, 1, nullptr, nullptr, nullptr, 0,
-                  nullptr, 0, 0, 0};
-      const unsigned grid = (unsigned)(npad / RA_ROWS);
-      if (metric == 1)
-        knn_topc_rb_kernel<RB_NKC, RB_KC, true, false><<<grid, 512, RbRing<RB_KC>::SMEM, s>>>(
-            Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
-      else
-        knn_topc_rb_kernel<RB_NKC, RB_KC, false, false><<<grid, 512, RbRing<RB_KC>::SMEM, s>>>(
-            Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
+      if (ra) {
+        RbCols cols{Xh.p, colscale.p, nullptr, n, npad / RA_BN, 1, nullptr, nullptr, nullptr, 0,
+                    nullptr, 0, 0, 0};
+        const unsigned grid = (unsigned)(npad / RA_ROWS);
+        if (metric == 1)
+          knn_topc_rb_kernel<RB_NKC, RB_KC, true, false><<<grid, 512, RbRing<RB_KC>::SMEM, s>>>(
+              Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
+        else
+          knn_topc_rb_kernel<RB_NKC, RB_KC, false, false><<<grid, 512, RbRing<RB_KC>::SMEM, s>>>(
+              Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
+      } else {
+        rs_launch(split, 16, dpad, metric == 1, Xh.p, Xl.p, Xh.p, Xl.p, nullptr, n, npad, n,
+                  colscale.p, 0, (int)(npad / RA_BN), 1, cidx.p, ckey.p, s);
+      }
       EVOC_HIP(hipGetLastError());
     } else {
       Buf<int32_t> rcnt(pool, n), ridx(pool, n * RB_RCAP), dropped(pool, n);
@@ -2867,7 +3125,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       if (rs_esc)
         rs_launch(split, 32, dpad, metric == 1, Xh.p, split ? Xl.p : nullptr, Qh.p,
                   split ? Ql.p : nullptr, qmap.p, nq, nqpad, n, colscale.p,
-                  (int)(ct0 * (BN / RA_BN)), (int)(ct1 * (BN / RA_BN)), oi, ok, s);
+                  (int)(ct0 * (BN / RA_BN)), (int)(ct1 * (BN / RA_BN)), 0, oi, ok, s);
       else if (split)
         knn_topc_kernel<true, 32><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
                                                           metric, oi, ok, Qh.p, Ql.p, qmap.p, nq,
@@ -2939,11 +3197,11 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   st.candidates = C;
   st.mfma_ms = mfma_ms;
   double pairs = (double)npad * (double)npad;
-  if (sym) {  // block I swept nblk(I) blocks of 256 columns
-    const int64_t m = npad / RB_ROWS;
+  if (sym) {  // block I swept nblk(I) blocks of 256 (two-wavefront) or 128 (resident) columns
+    const int64_t br = ra ? RB_ROWS : RS_ROWS, m = npad / br;
     int64_t blocks = 0;
     for (int64_t I = 0; I < m; ++I) blocks += (m & 1) ? (m + 1) / 2 : m / 2 + (I < m / 2 ? 1 : 0);
-    pairs = (double)blocks * RB_ROWS * RB_ROWS;
+    pairs = (double)blocks * br * br;
   }
   st.flops = fp8 ? 2.0 * pairs * (double)dpad8 : 2.0 * pairs * (double)dpad * (split ? 3.0 : 1.0);
   if (skip_first)  // the escalation of every row over all columns

@@ -274,3 +274,49 @@ def test_knn_split_resident_query(n, d, metric):
     x[20:24] = x[10]
     st = _check(x, 15, metric)
     print(n, d, metric, st)
+
+
+@pytest.mark.parametrize("n,d,metric,bf16", [
+    (3000, 300, "cosine", False),     # split, 24 row blocks of 128: the middle pair once
+    (2700, 200, "euclidean", False),  # split, d padded to 256, 21 blocks: odd count
+    (777, 400, "cosine", False),      # 7 blocks, ragged last block
+    (129, 300, "euclidean", False),   # 2 blocks, one row in the last
+    (100, 256, "cosine", False),      # one block: the diagonal only
+    (2000, 700, "cosine", True),      # bf16 rows above 512: three stages of 256
+    (1500, 1000, "euclidean", True),  # and four
+])
+def test_knn_resident_symmetric_sweep(n, d, metric, bf16, monkeypatch):
+    """The resident kernel's symmetric sweep (each pair of 128-row blocks once, transposed
+    keys queued, merged and certified), for f32 rows that are not bf16-exact (split) and
+    bf16 rows above 512 dimensions, gives the oracle's exact kNN; zero rows and duplicates."""
+    monkeypatch.setenv("EVOC_HIP_KNN_SYMMETRIC", "1")
+    x, _ = datagen.unit_mixture(n, d, 12, 0.2, seed=n + d, bf16=bf16)
+    x = x.astype(np.float32)
+    if not bf16:
+        x *= np.float32(1.7)
+    if metric == "cosine":
+        x[5] = 0.0
+    x[20:24] = x[10]
+    st = _check(x, 15, metric)
+    assert st["symmetric"], st
+    # the matrix work names the kernel: 128-row blocks, each over half the blocks (the
+    # middle pair of an even count once), d padded to 256, three products per pair if split
+    m, dpad = -(-n // 128), -(-d // 256) * 256
+    blocks = sum((m + 1) // 2 if m % 2 else m // 2 + (i < m // 2) for i in range(m))
+    assert st["kernel_flop"] == 2.0 * blocks * 128 * 128 * dpad * (1 if bf16 else 3), st
+
+
+def test_knn_resident_symmetric_equals_plain_at_200k(monkeypatch):
+    """At a size the oracle cannot check quickly: the resident kernel's symmetric and plain
+    sweeps of split rows return identical graphs, the symmetric one at about half the
+    matrix work."""
+    x, _ = datagen.unit_mixture(200_000, 300, 60, 0.2, seed=9)
+    x = x.astype(np.float32)
+    monkeypatch.setenv("EVOC_HIP_KNN_SYMMETRIC", "0")
+    pi, pd, ps = H.knn(x, 15, metric="cosine", return_stats=True)
+    monkeypatch.setenv("EVOC_HIP_KNN_SYMMETRIC", "1")
+    si, sd, ss = H.knn(x, 15, metric="cosine", return_stats=True)
+    assert not ps["symmetric"] and ss["symmetric"]
+    assert ss["kernel_flop"] < 0.51 * ps["kernel_flop"]
+    assert np.array_equal(pi, si)
+    assert np.array_equal(pd.view(np.uint32), sd.view(np.uint32))
