@@ -61,10 +61,11 @@ def to_bf16_values(x):
 
 
 def unit_mixture_device(n, d, n_centres=200, zipf=1.1, sd=0.15, noise_frac=0.05, seed=42,
-                        hierarchical=None):
+                        hierarchical=None, bf16=True):
     """The same kind of mixture as `unit_mixture`, generated on the GPU with torch (seeded;
     torch is only the allocator and RNG here). Values are rounded to bf16-representable
-    float32. Returns (x, labels) as device tensors; noise points are labelled -1.
+    float32 (bf16=False: left as computed in float32). Returns (x, labels) as device
+    tensors; noise points are labelled -1.
 
     hierarchical=(n_parents, children_per_parent, child_spread, leaf_sd) instead draws
     parents ~ N(0, I), children = parent + child_spread * N(0, I), points =
@@ -100,7 +101,8 @@ def unit_mixture_device(n, d, n_centres=200, zipf=1.1, sd=0.15, noise_frac=0.05,
             x[n_cl:] = z / z.norm(dim=1, keepdim=True)
         labels = torch.cat([lab, torch.full((n_noise,), -1, device="cuda", dtype=lab.dtype)])
     perm = torch.randperm(n, device="cuda", generator=g)
-    x = x[perm].to(torch.bfloat16).to(torch.float32).contiguous()
+    x = x[perm]
+    x = (x.to(torch.bfloat16).to(torch.float32) if bf16 else x).contiguous()
     labels = labels[perm]
     torch.cuda.synchronize()
     return x, labels
