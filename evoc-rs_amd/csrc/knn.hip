@@ -1440,6 +1440,16 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
   const int gblk = (int)blockIdx.x + (SYM ? cols.block0 : 0);
   const int bid = SYM ? (cols.nblocks > 0 ? cols.nblocks : (int)gridDim.x) - 1 - gblk : gblk;
   const int row0 = bid * RS_ROWS;  // first query (list) row of the workgroup
+  if (!SYM && gridDim.y > 1) {
+    // column shards: workgroup row y takes the y-th share of [tc0, tc1) and writes its
+    // lists at y * nq * 2C
+    const int all = tc1 - tc0, sh = (int)blockIdx.y, ns = (int)gridDim.y;
+    const int a = tc0 + (int)((int64_t)all * sh / ns), b = tc0 + (int)((int64_t)all * (sh + 1) / ns);
+    tc0 = a;
+    tc1 = b;
+    cand_idx += (int64_t)sh * nq * 2 * C;
+    cand_key += (int64_t)sh * nq * 2 * C;
+  }
   const int ntile = tc1;
   int tile0 = tc0, nt = tc1 - tc0, tshift = 0;
   if (SYM) {
@@ -2009,6 +2019,65 @@ __global__ __launch_bounds__(256) void rb_merge_kernel(
     if (cnt > RB_RCAP) b = -FINF;
     bound[row] = b;
   }
+}
+
+// The certificate-rate sample's column shards merged per row (one wavefront a row): the
+// 32 best of the G shards' lists (two sorted half-lists of 16 each, shard t's at
+// (t * nrows + row) * 32) ranked by (key, index), and a bound at or below every left-out
+// column's key: each half-list bounds the columns of its half and shard by its 16th key
+// (+inf when not full), and a candidate ranked out lies at or above the 32nd kept.
+constexpr int KNN_SAMPLE_SHARDS_MAX = 8;
+__global__ __launch_bounds__(256) void knn_shard_merge_kernel(
+    const float* __restrict__ ckey, const int32_t* __restrict__ cidx, int G, int64_t nrows,
+    float* __restrict__ mkey, int32_t* __restrict__ midx, float* __restrict__ bound) {
+  __shared__ float sk[4][32 * KNN_SAMPLE_SHARDS_MAX];
+  __shared__ int32_t si[4][32 * KNN_SAMPLE_SHARDS_MAX];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + wave;
+  if (row >= nrows) return;
+  const int K = 32 * G;
+  float b = FINF;
+  for (int e = lane; e < K; e += 64) {
+    const int64_t at = ((int64_t)(e >> 5) * nrows + row) * 32 + (e & 31);
+    const int32_t idx = cidx[at];
+    const float key = idx >= 0 ? ckey[at] : FINF;
+    sk[wave][e] = key;
+    si[wave][e] = idx;
+    if ((e & 15) == 15) b = fminf(b, key);  // a half-list's 16th key
+  }
+  __builtin_amdgcn_wave_barrier();
+  __threadfence_block();
+  int nvalid = 0;
+  for (int e = lane; e < K; e += 64) nvalid += si[wave][e] >= 0 ? 1 : 0;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    nvalid += __shfl_xor(nvalid, o, 64);
+    b = fminf(b, __shfl_xor(b, o, 64));
+  }
+  float key31 = FINF;
+  for (int e = lane; e < K; e += 64) {
+    const float key = sk[wave][e];
+    const int32_t idx = si[wave][e];
+    if (idx < 0) continue;
+    int rank = 0;
+    for (int f = 0; f < K; ++f) {
+      const float k2 = sk[wave][f];
+      const int32_t i2 = si[wave][f];
+      rank += (i2 >= 0 && (k2 < key || (k2 == key && i2 < idx))) ? 1 : 0;
+    }
+    if (rank < 32) {
+      mkey[row * 32 + rank] = key;
+      midx[row * 32 + rank] = idx;
+    }
+    if (rank == 31) key31 = key;
+  }
+  for (int q = nvalid + lane; q < 32; q += 64) {
+    mkey[row * 32 + q] = FINF;
+    midx[row * 32 + q] = -1;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) key31 = fminf(key31, __shfl_xor(key31, o, 64));
+  if (lane == 0) bound[row] = nvalid > 32 ? fminf(b, key31) : b;
 }
 
 // ------------------------------------------------------------------ 3. re-rank
@@ -2585,9 +2654,9 @@ static bool rs_sym_supported(bool spl, int dpad) {
 static void rs_launch(bool spl, int C, int dpad, bool cos, const uint16_t* Xh, const uint16_t* Xl,
                       const uint16_t* Qh, const uint16_t* Ql, const int32_t* qmap, int64_t nq,
                       int64_t nqpad, int64_t n, const float* colscale, int tc0, int tc1, int self,
-                      int32_t* cidx, float* ckey, hipStream_t s) {
+                      int32_t* cidx, float* ckey, hipStream_t s, int shards = 1) {
   const int nkc = dpad / RS_KC;
-  const unsigned grid = (unsigned)(nqpad / RS_ROWS);
+  const dim3 grid((unsigned)(nqpad / RS_ROWS), (unsigned)shards);
   const RbCols none{};
 #define EVOC_RS_GO(NKC_, SPL_, C_)                                                            \
   if (nkc == NKC_ && spl == SPL_ && C == C_) {                                               \
@@ -2803,22 +2872,38 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       EVOC_HIP(hipMemsetAsync(sfc.p, 0, sizeof(int), s));
       knn_gather_rows_kernel<<<(unsigned)ceil_div(Spad * (dpad / 8), 256), 256, 0, s>>>(
           Xh.p, split ? Xl.p : nullptr, sq.p, S, Spad, dpad, SQh.p, split ? SQl.p : nullptr);
-      const int nops = split ? 4 : 2;
-      const int nst = split ? 2 : 4;
-      const size_t smem = (size_t)nst * nops * BM * 4 * 16 + BM * SCS * sizeof(float) +
-                          2 * BM * sizeof(float) + 4 * sizeof(int);
-      if (split)
-        knn_topc_kernel<true, 16><<<(unsigned)(Spad / BM), 256, smem, s>>>(
-            Xh.p, Xl.p, n, npad, dpad, colscale.p, metric, scidx.p, sckey.p, SQh.p, SQl.p, sq.p, S,
-            0, npad / BN);
-      else
-        knn_topc_kernel<false, 16><<<(unsigned)(Spad / BM), 256, smem, s>>>(
-            Xh.p, nullptr, n, npad, dpad, colscale.p, metric, scidx.p, sckey.p, SQh.p, nullptr,
-            sq.p, S, 0, npad / BN);
+      Buf<float> sbnd;
+      if (rs_supported(split, 16, dpad)) {
+        // the resident kernel over column shards (the sample's few row blocks alone would
+        // leave most CUs idle), merged into 32 candidates and a bound per row
+        const int G = (int)std::max<int64_t>(
+            1, std::min<int64_t>(KNN_SAMPLE_SHARDS_MAX, cu_count() / (Spad / RS_ROWS)));
+        Buf<int32_t> gidx(pool, (int64_t)G * S * 32);
+        Buf<float> gkey(pool, (int64_t)G * S * 32);
+        sbnd.alloc(pool, S);
+        rs_launch(split, 16, dpad, metric == 1, Xh.p, split ? Xl.p : nullptr, SQh.p,
+                  split ? SQl.p : nullptr, sq.p, S, Spad, n, colscale.p, 0, (int)(npad / RA_BN), 0,
+                  gidx.p, gkey.p, s, G);
+        knn_shard_merge_kernel<<<(unsigned)ceil_div(S, 4), 256, 0, s>>>(gkey.p, gidx.p, G, S,
+                                                                         sckey.p, scidx.p, sbnd.p);
+      } else {
+        const int nops = split ? 4 : 2;
+        const int nst = split ? 2 : 4;
+        const size_t smem = (size_t)nst * nops * BM * 4 * 16 + BM * SCS * sizeof(float) +
+                            2 * BM * sizeof(float) + 4 * sizeof(int);
+        if (split)
+          knn_topc_kernel<true, 16><<<(unsigned)(Spad / BM), 256, smem, s>>>(
+              Xh.p, Xl.p, n, npad, dpad, colscale.p, metric, scidx.p, sckey.p, SQh.p, SQl.p, sq.p,
+              S, 0, npad / BN);
+        else
+          knn_topc_kernel<false, 16><<<(unsigned)(Spad / BM), 256, smem, s>>>(
+              Xh.p, nullptr, n, npad, dpad, colscale.p, metric, scidx.p, sckey.p, SQh.p, nullptr,
+              sq.p, S, 0, npad / BN);
+      }
       EVOC_HIP(hipGetLastError());
       // (the sampled rows' results land in the output and are rewritten by the search)
       (wide ? knn_rerank_kernel<64> : knn_rerank_kernel<16>)<<<(unsigned)ceil_div(S, 4), 256, 0, s>>>(
-          dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, scidx.p, sckey.p, 32, 2, nullptr, k,
+          dX, n, d, ld, nrm.p, sqn.p, colscale.p, flags.p, scidx.p, sckey.p, 32, 2, sbnd.p, k,
           metric, gamma, d_idx, d_dist, sfc.p, sfr.p, sq.p, S);
       EVOC_HIP(hipGetLastError());
       int sfail = 0;
@@ -3117,16 +3202,15 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     const unsigned grid = (unsigned)(nqpad / BM);
     const int64_t ntile = npad / BN;
     if (skip_first) EVOC_HIP(hipEventRecord(e0, s));  // then the escalation is the tile pass
-    const bool rs_esc = rs_supported(split, 32, dpad);
-    for (int sh = 0; sh < KNN_ESC_SHARDS; ++sh) {
+    if (rs_supported(split, 32, dpad)) {  // the shards in one launch (grid rows)
+      rs_launch(split, 32, dpad, metric == 1, Xh.p, split ? Xl.p : nullptr, Qh.p,
+                split ? Ql.p : nullptr, qmap.p, nq, nqpad, n, colscale.p, 0, (int)(npad / RA_BN), 0,
+                ecidx.p, eckey.p, s, KNN_ESC_SHARDS);
+    } else for (int sh = 0; sh < KNN_ESC_SHARDS; ++sh) {
       const int64_t ct0 = ntile * sh / KNN_ESC_SHARDS, ct1 = ntile * (sh + 1) / KNN_ESC_SHARDS;
       int32_t* oi = ecidx.p + (int64_t)sh * nq * 64;
       float* ok = eckey.p + (int64_t)sh * nq * 64;
-      if (rs_esc)
-        rs_launch(split, 32, dpad, metric == 1, Xh.p, split ? Xl.p : nullptr, Qh.p,
-                  split ? Ql.p : nullptr, qmap.p, nq, nqpad, n, colscale.p,
-                  (int)(ct0 * (BN / RA_BN)), (int)(ct1 * (BN / RA_BN)), 0, oi, ok, s);
-      else if (split)
+      if (split)
         knn_topc_kernel<true, 32><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
                                                           metric, oi, ok, Qh.p, Ql.p, qmap.p, nq,
                                                           ct0, ct1);

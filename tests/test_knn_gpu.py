@@ -320,3 +320,18 @@ def test_knn_resident_symmetric_equals_plain_at_200k(monkeypatch):
     assert ss["kernel_flop"] < 0.51 * ps["kernel_flop"]
     assert np.array_equal(pi, si)
     assert np.array_equal(pd.view(np.uint32), sd.view(np.uint32))
+
+
+def test_knn_sampled_certificate_rate_keeps_first_pass_split():
+    """300,000 split f32 rows (d = 300): the certificate-rate sample (the resident kernel over
+    column shards, merged with a bound per row) passes, so the symmetric first pass runs
+    and few rows are escalated; sampled rows equal the exact search."""
+    n, d = 300_000, 300
+    x, _ = datagen.unit_mixture(n, d, 60, 0.2, seed=11)
+    x = x.astype(np.float32)
+    gi, gd, st = H.knn(x, 15, metric="cosine", return_stats=True)
+    assert st["symmetric"] and st["escalated_rows"] < n // 100, st
+    rows = np.sort(np.random.default_rng(11).choice(n, size=512, replace=False))
+    oi, od = O.knn_rows(x, rows, 15, metric=1)
+    assert np.array_equal(gi[rows], oi)
+    assert np.array_equal(gd[rows].view(np.uint32), od.view(np.uint32))
