@@ -895,6 +895,14 @@ struct RbCols {
   int64_t group;        // workgroups resident at once (one per CU): walks start in step
   int block0;           // symmetric sweep launched in slices: first block of this launch
   int nblocks;          // and the whole sweep's block count (0: this launch's grid)
+  // A slice with fewer blocks than CUs (the sweep's last) splits each block's walk into
+  // `parts` consecutive pieces, one workgroup each (launch index = block * parts + part):
+  // part q writes its lists to plist_idx/key + (q * prows + row) * 32 (the slice's rows
+  // are 0 .. prows - 1, the lowest blocks) and its queue to slot block0 + launch index.
+  int parts;
+  int64_t prows;
+  int32_t* plist_idx;
+  float* plist_key;
 };
 
 template <int KC>
@@ -953,7 +961,10 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   // queues transposed keys for come after its own, so most of them have finished by then
   // and replaced their rows' thresholds with the tighter largest forward key.
   // (tile and row indices fit 32 bits: n < 2^31; 64-bit products only for addresses)
-  const int gblk = (int)blockIdx.x + cols.block0;  // block of the whole sweep
+  const int P = SYM && cols.parts > 1 ? cols.parts : 1;  // pieces of each block's walk
+  const int gblk = (int)blockIdx.x / P + cols.block0;  // block of the whole sweep
+  const int part = (int)blockIdx.x % P;
+  const int qslot = (int)blockIdx.x + cols.block0;  // this workgroup's queues
   const int bid = SYM ? (cols.nblocks > 0 ? cols.nblocks : (int)gridDim.x) - 1 - gblk : gblk;
   const int row0 = bid * RB_ROWS;
   const int ntile = (int)cols.ntile;
@@ -979,10 +990,13 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     if (a >= ntile) a -= ntile;
     return a;
   };
-  const int total = nt * NKC;
+  // this workgroup's piece [ct0, ct1) of the walk (all of it unless the walk is split)
+  const int ct0 = P > 1 ? (int)((int64_t)nt * part / P) : 0;
+  const int ct1 = P > 1 ? (int)((int64_t)nt * (part + 1) / P) : nt;
+  const int total = (ct1 - ct0) * NKC;
   int qn = 0;  // this wavefront's reverse candidates in its global queue
   int lq = 0;  // and staged in LDS
-  const int64_t qbase = SYM ? ((int64_t)gblk * RB_WAVES + wave) * cols.qcap : 0;
+  const int64_t qbase = SYM ? ((int64_t)qslot * RB_WAVES + wave) * cols.qcap : 0;
   int32_t* const qjw = SYM ? cols.qj + qbase : nullptr;
   int32_t* const qrw = SYM ? cols.qr + qbase : nullptr;
   float* const qkw = SYM ? cols.qk + qbase : nullptr;
@@ -1058,8 +1072,8 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   // threshold) sits in 8 consecutive bytes; without thresholds both lanes load the scale
   const int soff = SYM ? lane * 4 : (lane >> 1) * 4;
   const int tile_bytes = RA_BN * dpad * 2;
-  int gi = 0, tcur = 0;  // walk index of the tile being issued
-  int tabs = tile_at(0);  // its absolute column tile
+  int gi = 0, tcur = ct0;  // walk index of the tile being issued
+  int tabs = tile_at(ct0);  // its absolute column tile
   int kci = 0, sloti = 0;
   auto issue_next = [&]() {
     char* st = ring + sloti * R::STB;
@@ -1097,7 +1111,7 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   wait_vm<(LOADS + 1) * (NST - 2)>();  // this wavefront's loads of stage 0 have landed
   ring_barrier();
   int slotc = 0;  // ring slot of the stage being read
-  for (int ct = 0; ct < nt; ++ct) {
+  for (int ct = ct0; ct < ct1; ++ct) {
     f32x16 acc = (f32x16){};
     float2 csv;  // the column's key scale and (symmetric sweep) reverse threshold
 #pragma unroll
@@ -1336,7 +1350,7 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   const int64_t grow = row0 + wave * 32 + (lane & 31);
   if (SYM) {
     if (lq > 0) flush();
-    if (lane == 0) cols.qcount[(int64_t)gblk * RB_WAVES + wave] = qn;
+    if (lane == 0) cols.qcount[(int64_t)qslot * RB_WAVES + wave] = qn;
     // Publish the row's largest forward key as its threshold when it is smaller. Any
     // transposed key at or above it is beaten by all 32 forward candidates; workgroups
     // still to come read the smaller value (or, from a stale cache line, the old one:
@@ -1345,10 +1359,14 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     if (lane < 32 && grow < n && fwdmax < cols.kt[2 * grow + 1]) cols.kt[2 * grow + 1] = fwdmax;
   }
   if (grow < n) {
+    // a piece of a split walk writes its partial lists (the same layout, C = 16) to its
+    // part's rows of the piece lists
+    float* const okey = P > 1 ? cols.plist_key + part * cols.prows * 2 * C : cand_key;
+    int32_t* const oidx = P > 1 ? cols.plist_idx + part * cols.prows * 2 * C : cand_idx;
 #pragma unroll
     for (int q = 0; q < C; ++q) {
-      cand_key[(grow * 2 + (lane >> 5)) * C + q] = lk[q];
-      cand_idx[(grow * 2 + (lane >> 5)) * C + q] = li[q];
+      okey[(grow * 2 + (lane >> 5)) * C + q] = lk[q];
+      oidx[(grow * 2 + (lane >> 5)) * C + q] = li[q];
     }
   }
 }
@@ -1437,7 +1455,10 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
   float* rsc = reinterpret_cast<float*>(epi + RS_WAVES * RB_WREG + RS_WAVES * 64 * 4) + wave * 32;
   // blocks in descending order in the symmetric sweep (the blocks a workgroup queues
   // transposed keys for come after its own and have mostly tightened their thresholds)
-  const int gblk = (int)blockIdx.x + (SYM ? cols.block0 : 0);
+  const int P = SYM && cols.parts > 1 ? cols.parts : 1;  // pieces of each block's walk
+  const int gblk = (int)blockIdx.x / P + (SYM ? cols.block0 : 0);
+  const int part = (int)blockIdx.x % P;
+  const int qslot = (int)blockIdx.x + (SYM ? cols.block0 : 0);  // this workgroup's queues
   const int bid = SYM ? (cols.nblocks > 0 ? cols.nblocks : (int)gridDim.x) - 1 - gblk : gblk;
   const int row0 = bid * RS_ROWS;  // first query (list) row of the workgroup
   if (!SYM && gridDim.y > 1) {
@@ -1471,10 +1492,13 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
     if (a >= ntile) a -= ntile;
     return a;
   };
-  const int total = nt * NKC;
+  // this workgroup's piece [ct0, ct1) of the walk (all of it unless the walk is split)
+  const int ct0 = P > 1 ? (int)((int64_t)nt * part / P) : 0;
+  const int ct1 = P > 1 ? (int)((int64_t)nt * (part + 1) / P) : nt;
+  const int total = (ct1 - ct0) * NKC;
   int qn = 0;  // this wavefront's reverse candidates in its global queue
   int lq = 0;  // and staged in LDS
-  const int64_t qbase = SYM ? ((int64_t)gblk * RS_WAVES + wave) * cols.qcap : 0;
+  const int64_t qbase = SYM ? ((int64_t)qslot * RS_WAVES + wave) * cols.qcap : 0;
   int32_t* const qjw = SYM ? cols.qj + qbase : nullptr;
   int32_t* const qrw = SYM ? cols.qr + qbase : nullptr;
   float* const qkw = SYM ? cols.qk + qbase : nullptr;
@@ -1568,8 +1592,8 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
   }
   const int soff = SYM ? lane * 4 : (lane >> 1) * 4;
   const int tile_bytes = RA_BN * dpad * 2;
-  int gi = 0, tcur = 0, kci = 0, sloti = 0;
-  int tabs = tile_at(0);  // absolute column tile being issued
+  int gi = 0, tcur = ct0, kci = 0, sloti = 0;
+  int tabs = tile_at(ct0);  // absolute column tile being issued
   auto issue_next = [&]() {
     char* st = ring + sloti * R::STB;
     const int64_t tb = (int64_t)tabs * (RA_BN * dpad);
@@ -1604,13 +1628,13 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
   const int lrow = (lane >> 5) * 4;
   const int bcol = lcol * R::PIECES * 16, bm = (lane >> 5) ^ (lcol & 15);
   // an empty column range (a shard with no tile) issues no load: its lists stay empty
-  if (nt > 0) {
+  if (ct1 > ct0) {
     for (int g = 0; g < NST - 1; ++g) issue_next();
     wait_vm<(LOADS + 1) * (NST - 2)>();  // this wavefront's loads of stage 0 have landed
     ring_barrier();
   }
   int slotc = 0;
-  for (int ct = 0; ct < nt; ++ct) {
+  for (int ct = ct0; ct < ct1; ++ct) {
     f32x16 acc = (f32x16){};
     float2 csv;
 #pragma unroll
@@ -1841,7 +1865,7 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
   const int64_t lr = row0 + wave * 32 + (lane & 31);  // list row
   if (SYM) {
     if (lq > 0) flush();
-    if (lane == 0) cols.qcount[(int64_t)gblk * RS_WAVES + wave] = qn;
+    if (lane == 0) cols.qcount[(int64_t)qslot * RS_WAVES + wave] = qn;
     // the row's largest forward key becomes its threshold when smaller: a transposed key
     // at or above it is beaten by all 32 forward candidates (workgroups still to come read
     // the new value or, from a stale cache line, the old one: either is an upper bound)
@@ -1849,10 +1873,13 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
     if (lane < 32 && lr < n && fwdmax < cols.kt[2 * lr + 1]) cols.kt[2 * lr + 1] = fwdmax;
   }
   if (lr < nq) {
+    // a piece of a split walk writes its partial lists to its part's rows of the piece lists
+    float* const okey = P > 1 ? cols.plist_key + part * cols.prows * 2 * C : cand_key;
+    int32_t* const oidx = P > 1 ? cols.plist_idx + part * cols.prows * 2 * C : cand_idx;
 #pragma unroll
     for (int q = 0; q < C; ++q) {
-      cand_key[(lr * 2 + (lane >> 5)) * C + q] = lk[q];
-      cand_idx[(lr * 2 + (lane >> 5)) * C + q] = li[q];
+      okey[(lr * 2 + (lane >> 5)) * C + q] = lk[q];
+      oidx[(lr * 2 + (lane >> 5)) * C + q] = li[q];
     }
   }
 }
@@ -1955,13 +1982,15 @@ __global__ __launch_bounds__(256) void rb_scatter_kernel(const int32_t* __restri
 // (key, index); the 32 best are kept, sorted. Every column not kept has a key >= the
 // returned bound: forward non-candidates >= their half's last key, reverse keys never
 // queued >= thr0 (the smallest value the threshold took during the sweep), entries ranked out (here, or dropped at the scatter above the largest
-// forward key) >= the 32nd kept. A full bucket voids the bound.
+// forward key) >= the 32nd kept. A full bucket voids the bound. Rows below fb_rows (a
+// split walk's, their pieces merged beforehand) take the forward bound from fbound.
 constexpr int RB_RCAP = 768;
 __global__ __launch_bounds__(256) void rb_merge_kernel(
     const float* __restrict__ ckey, const int32_t* __restrict__ cidx, const int32_t* __restrict__ rcnt,
     const float* __restrict__ rkey, const int32_t* __restrict__ ridx, const float* __restrict__ kt,
     const int32_t* __restrict__ dropped, int64_t n, float* __restrict__ mkey,
-    int32_t* __restrict__ midx, float* __restrict__ bound) {
+    int32_t* __restrict__ midx, float* __restrict__ bound, const float* __restrict__ fbound,
+    int64_t fb_rows) {
   __shared__ float sk[4][32 + RB_RCAP];
   __shared__ int32_t si[4][32 + RB_RCAP];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2014,22 +2043,25 @@ __global__ __launch_bounds__(256) void rb_merge_kernel(
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) key31 = fminf(key31, __shfl_xor(key31, o, 64));
   if (lane == 0) {
-    float b = fminf(fminf(ckey[(row * 2) * 16 + 15], ckey[(row * 2 + 1) * 16 + 15]), kt[2 * row + 1]);
+    const float fb = row < fb_rows ? fbound[row]
+                                   : fminf(ckey[(row * 2) * 16 + 15], ckey[(row * 2 + 1) * 16 + 15]);
+    float b = fminf(fb, kt[2 * row + 1]);
     if (nvalid > 32 || dropped[row]) b = fminf(b, key31);  // some candidate was ranked out
     if (cnt > RB_RCAP) b = -FINF;
     bound[row] = b;
   }
 }
 
-// The certificate-rate sample's column shards merged per row (one wavefront a row): the
-// 32 best of the G shards' lists (two sorted half-lists of 16 each, shard t's at
-// (t * nrows + row) * 32) ranked by (key, index), and a bound at or below every left-out
+// Column shards (the certificate-rate sample's) or pieces of split walks (the symmetric
+// sweep's last slice) merged per row, one wavefront a row: the 32 best of the G lists
+// (two sorted half-lists of 16 each, list t's at (t * stride + row) * 32) ranked by
+// (key, index), and a bound at or below every left-out
 // column's key: each half-list bounds the columns of its half and shard by its 16th key
 // (+inf when not full), and a candidate ranked out lies at or above the 32nd kept.
 constexpr int KNN_SAMPLE_SHARDS_MAX = 8;
 __global__ __launch_bounds__(256) void knn_shard_merge_kernel(
     const float* __restrict__ ckey, const int32_t* __restrict__ cidx, int G, int64_t nrows,
-    float* __restrict__ mkey, int32_t* __restrict__ midx, float* __restrict__ bound) {
+    int64_t stride, float* __restrict__ mkey, int32_t* __restrict__ midx, float* __restrict__ bound) {
   __shared__ float sk[4][32 * KNN_SAMPLE_SHARDS_MAX];
   __shared__ int32_t si[4][32 * KNN_SAMPLE_SHARDS_MAX];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2038,7 +2070,7 @@ __global__ __launch_bounds__(256) void knn_shard_merge_kernel(
   const int K = 32 * G;
   float b = FINF;
   for (int e = lane; e < K; e += 64) {
-    const int64_t at = ((int64_t)(e >> 5) * nrows + row) * 32 + (e & 31);
+    const int64_t at = ((int64_t)(e >> 5) * stride + row) * 32 + (e & 31);
     const int32_t idx = cidx[at];
     const float key = idx >= 0 ? ckey[at] : FINF;
     sk[wave][e] = key;
@@ -2850,6 +2882,12 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   Buf<uint16_t> Xs, Xsl;
   Buf<float> ss, kt, qk;
   Buf<int32_t> qj, qr, qcount;
+  // the sweep's slices (one workgroup per CU, EVOC_HIP_KNN_SLICED=0: one launch), its
+  // queues (one per wavefront and launched workgroup) and the pieces of the last slice
+  int64_t sym_blocks = 0, sym_gsz = 0, sym_rem = 0, sym_nq = 0, sym_prows = 0;
+  int sym_parts = 1;
+  Buf<int32_t> pidx;
+  Buf<float> pkey, fbnd;
   // The first pass's certificate rate, estimated on evenly spaced sample rows searched
   // with the streaming kernel (bf16 or split, as the escalation would search them; the
   // FP8 pass certifies fewer rows than that). Where more than half of them fail (neighbour
@@ -2884,8 +2922,8 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
         rs_launch(split, 16, dpad, metric == 1, Xh.p, split ? Xl.p : nullptr, SQh.p,
                   split ? SQl.p : nullptr, sq.p, S, Spad, n, colscale.p, 0, (int)(npad / RA_BN), 0,
                   gidx.p, gkey.p, s, G);
-        knn_shard_merge_kernel<<<(unsigned)ceil_div(S, 4), 256, 0, s>>>(gkey.p, gidx.p, G, S,
-                                                                         sckey.p, scidx.p, sbnd.p);
+        knn_shard_merge_kernel<<<(unsigned)ceil_div(S, 4), 256, 0, s>>>(
+            gkey.p, gidx.p, G, S, S, sckey.p, scidx.p, sbnd.p);
       } else {
         const int nops = split ? 4 : 2;
         const int nst = split ? 2 : 4;
@@ -2935,11 +2973,29 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     // about 32 rows x n/2 columns x RB_THR_RANK/S reverse keys per wavefront; three times that
     const int64_t expect = 16 * RB_THR_RANK * ceil_div(n, sym_S);
     sym_qcap = std::max<int64_t>(16384, ceil_div(3 * expect, 256) * 256);
-    const int64_t nwaves = npad / 32;  // 32 rows per wavefront in both kernels
-    qj.alloc(pool, nwaves * sym_qcap);
-    qr.alloc(pool, nwaves * sym_qcap);
-    qk.alloc(pool, nwaves * sym_qcap);
-    qcount.alloc(pool, nwaves);
+    // Slices of one workgroup per CU; when the last holds fewer blocks than CUs, each of
+    // its blocks' walks is split into `parts` pieces (at most 8: the piece merge's width),
+    // one workgroup each, so that the slice fills the CUs it would leave idle.
+    const int64_t brows = ra ? RB_ROWS : RS_ROWS, bwaves = ra ? RB_WAVES : RS_WAVES;
+    sym_blocks = npad / brows;
+    const char* slenv = std::getenv("EVOC_HIP_KNN_SLICED");
+    const bool sliced = !(slenv && slenv[0] == '0');
+    sym_gsz = sliced ? std::max<int64_t>(1, cu_count()) : sym_blocks;
+    sym_rem = sym_blocks % sym_gsz;
+    sym_parts = sym_rem > 0 ? (int)std::max<int64_t>(
+                                  1, std::min<int64_t>(KNN_SAMPLE_SHARDS_MAX, sym_gsz / sym_rem))
+                            : 1;
+    sym_nq = (sym_blocks + sym_rem * (sym_parts - 1)) * bwaves;
+    qj.alloc(pool, sym_nq * sym_qcap);
+    qr.alloc(pool, sym_nq * sym_qcap);
+    qk.alloc(pool, sym_nq * sym_qcap);
+    qcount.alloc(pool, sym_nq);
+    if (sym_parts > 1) {  // the last slice holds blocks 0 .. rem - 1 (descending order)
+      sym_prows = sym_rem * brows;
+      pidx.alloc(pool, sym_parts * sym_prows * 32);
+      pkey.alloc(pool, sym_parts * sym_prows * 32);
+      fbnd.alloc(pool, sym_prows);
+    }
   }
   hipEvent_t e0, e1;
   EVOC_HIP(hipEventCreate(&e0));
@@ -2987,13 +3043,15 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
         // Launched in slices of one workgroup per CU, so that each slice's walks start
         // together and stream their shared column tiles in step through L2 (in one launch
         // a workgroup starts whenever a CU frees up). EVOC_HIP_KNN_SLICED=0: one launch.
-        const char* slenv = std::getenv("EVOC_HIP_KNN_SLICED");
-        const bool sliced = !(slenv && slenv[0] == '0');
-        const int gsz = sliced ? (int)std::max<int64_t>(1, cols.group) : (int)grid;
         cols.nblocks = (int)grid;
-        for (int b0 = 0; b0 < (int)grid; b0 += gsz) {
+        cols.prows = sym_prows;
+        cols.plist_idx = pidx.p;
+        cols.plist_key = pkey.p;
+        for (int b0 = 0; b0 < (int)grid; b0 += (int)sym_gsz) {
           cols.block0 = b0;
-          const unsigned g1 = (unsigned)std::min<int>(gsz, (int)grid - b0);
+          const int nb = (int)std::min<int64_t>(sym_gsz, (int64_t)grid - b0);
+          cols.parts = nb < sym_gsz ? sym_parts : 1;
+          const unsigned g1 = (unsigned)(nb * cols.parts);
           if (metric == 1)
             knn_topc_rb_kernel<RB_NKC, RB_KC, true, true><<<g1, 512, RbRing<RB_KC>::SMEM + RB_QBYTES, s>>>(
                 Xh.p, n, npad, dpad, colscale.p, cidx.p, ckey.p, cols);
@@ -3030,14 +3088,16 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       cols.qcount = qcount.p;
       cols.group = cu_count();
       const int nblk = (int)(npad / RS_ROWS);
-      const char* slenv = std::getenv("EVOC_HIP_KNN_SLICED");
-      const bool sliced = !(slenv && slenv[0] == '0');
-      const int gsz = sliced ? (int)std::max<int64_t>(1, cols.group) : nblk;
       cols.nblocks = nblk;
-      for (int b0 = 0; b0 < nblk; b0 += gsz) {
+      cols.prows = sym_prows;
+      cols.plist_idx = pidx.p;
+      cols.plist_key = pkey.p;
+      for (int b0 = 0; b0 < nblk; b0 += (int)sym_gsz) {
         cols.block0 = b0;
+        const int nb = (int)std::min<int64_t>(sym_gsz, (int64_t)nblk - b0);
+        cols.parts = nb < sym_gsz ? sym_parts : 1;
         rs_sym_launch(split, dpad, metric == 1, Xh.p, Xl.p, n, npad, colscale.p,
-                      (unsigned)std::min<int>(gsz, nblk - b0), cols, cidx.p, ckey.p, s);
+                      (unsigned)(nb * cols.parts), cols, cidx.p, ckey.p, s);
       }
     } else {
       rs_launch(split, 16, dpad, metric == 1, Xh.p, Xl.p, Xh.p, Xl.p, nullptr, n, npad, n,
@@ -3087,7 +3147,7 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
   Buf<float> mkey, bnd;
   Buf<int32_t> midx;
   if (sym) {
-    const int64_t nwaves = npad / 32;
+    const int64_t nwaves = sym_nq;
     std::vector<int32_t> hq(nwaves);
     EVOC_HIP(hipMemcpyAsync(hq.data(), qcount.p, nwaves * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     EVOC_HIP(hipStreamSynchronize(s));
@@ -3114,6 +3174,11 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       Buf<int32_t> rcnt(pool, n), ridx(pool, n * RB_RCAP), dropped(pool, n);
       Buf<float> rkey(pool, n * RB_RCAP), fmax(pool, n);
       EVOC_HIP(hipMemsetAsync(rcnt.p, 0, n * sizeof(int32_t), s));
+      // the last slice's split walks: their pieces' lists merged into the forward lists
+      const int64_t fb_rows = sym_parts > 1 ? std::min<int64_t>(n, sym_prows) : 0;
+      if (fb_rows > 0)
+        knn_shard_merge_kernel<<<(unsigned)ceil_div(fb_rows, 4), 256, 0, s>>>(
+            pkey.p, pidx.p, sym_parts, fb_rows, sym_prows, ckey.p, cidx.p, fbnd.p);
       rb_fwdmax_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(ckey.p, cidx.p, n, fmax.p,
                                                                  dropped.p);
       if (metric == 1)
@@ -3129,7 +3194,8 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       bnd.alloc(pool, n);
       rb_merge_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(ckey.p, cidx.p, rcnt.p, rkey.p,
                                                               ridx.p, kt.p, dropped.p, n,
-                                                              mkey.p, midx.p, bnd.p);
+                                                              mkey.p, midx.p, bnd.p, fbnd.p,
+                                                              fb_rows);
       EVOC_HIP(hipGetLastError());
     }
   }
