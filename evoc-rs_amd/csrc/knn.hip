@@ -1097,6 +1097,11 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
   };
 
   const int lcol = lane & 31, half = lcol >> 4, grp = lane >> 4;
+  // the (row, half) list this lane owns is counted in slot ((lane & 31) >> 3) * 4 + (lane & 3)
+  // of the tile epilogue, in lane group (((lane & 31) >> 2) & 1) * 2 + (lane >> 5) of that
+  // slot's ballot
+  const int myslot = (lcol >> 3) * 4 + (lane & 3);
+  const int gsh = 16 * (((lcol >> 2) & 1) * 2 + (lane >> 5));
   const int lrow = (lane >> 5) * 4;
   // B piece q = 2*kk + (lane >> 5) of column lcol sits at slot lcol*PIECES + (q ^ (lcol &
   // 15)) = lcol*PIECES + (2*kk ^ bm)
@@ -1259,7 +1264,12 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
     }
     const int cbase = (int)(cta * RA_BN);
     bool touched = false, overflow = false;
-    int inc = 0;
+    // the ballot of the slot this lane's list is counted in; the slot id is made opaque per
+    // tile, so that the compiler keeps no per-slot lane masks across the walk (they would
+    // spill scalar registers)
+    unsigned mlo = 0u, mhi = 0u;
+    int lsl = myslot;
+    asm volatile("" : "+v"(lsl));
     // owner of this lane's (row, half) group in slot (j, i): row j*8+i (+4 for lanes >= 32),
     // half (lane >> 4) & 1; its bucket is ob + (j*8+i) * RB_BKT * 8 bytes
     int ob = (int)lds_addr(bucket) + ((grp & 1) * 32 + (grp >> 1) * 4) * RB_BKT * 8;
@@ -1293,8 +1303,10 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
           if (pass && pos < RB_BKT)
             lds_st_b64((unsigned)(ob + r0 * RB_BKT * 8 + pos * 8),
                        make_int2(__float_as_int(key), cbase + lcol));
-          inc += (lane == o0 ? n0 : 0) + (lane == o1 ? n1 : 0) + (lane == o2 ? n2 : 0) +
-                 (lane == o3 ? n3 : 0);
+          if (lsl == j * 4 + i) {
+            mlo = (unsigned)m;
+            mhi = (unsigned)(m >> 32);
+          }
         }
       }
     if (touched) {
@@ -1302,7 +1314,8 @@ __global__ __launch_bounds__(512, 1) void knn_topc_rb_kernel(
       __builtin_amdgcn_wave_barrier();
       bool drain = overflow;
       if (!overflow) {
-        bcnt += inc;
+        const unsigned long long mine = ((unsigned long long)mhi << 32) | mlo;
+        bcnt += (int)__popcll((mine >> gsh) & 0xffffull);
         drain = __ballot(bcnt >= RB_DRAIN) != 0;
       }
       if (drain) {
@@ -1625,6 +1638,11 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
   };
 
   const int lcol = lane & 31, half = lcol >> 4, grp = lane >> 4;
+  // the (row, half) list this lane owns is counted in slot ((lane & 31) >> 3) * 4 + (lane & 3)
+  // of the tile epilogue, in lane group (((lane & 31) >> 2) & 1) * 2 + (lane >> 5) of that
+  // slot's ballot
+  const int myslot = (lcol >> 3) * 4 + (lane & 3);
+  const int gsh = 16 * (((lcol >> 2) & 1) * 2 + (lane >> 5));
   const int lrow = (lane >> 5) * 4;
   const int bcol = lcol * R::PIECES * 16, bm = (lane >> 5) ^ (lcol & 15);
   // an empty column range (a shard with no tile) issues no load: its lists stay empty
@@ -1780,7 +1798,12 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
       if (anyp == 0) continue;  // nothing in this tile enters a list
     }
     bool touched = false, overflow = false;
-    int inc = 0;
+    // the ballot of the slot this lane's list is counted in; the slot id is made opaque per
+    // tile, so that the compiler keeps no per-slot lane masks across the walk (they would
+    // spill scalar registers)
+    unsigned mlo = 0u, mhi = 0u;
+    int lsl = myslot;
+    asm volatile("" : "+v"(lsl));
     int ob = (int)lds_addr(bucket) + ((grp & 1) * 32 + (grp >> 1) * 4) * RB_BKT * 8;
     asm volatile("" : "+v"(ob));
 #pragma unroll
@@ -1810,8 +1833,10 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
           if (pass && pos < RB_BKT)
             lds_st_b64((unsigned)(ob + r0 * RB_BKT * 8 + pos * 8),
                        make_int2(__float_as_int(key), cbase + lcol));
-          inc += (lane == o0 ? n0 : 0) + (lane == o1 ? n1 : 0) + (lane == o2 ? n2 : 0) +
-                 (lane == o3 ? n3 : 0);
+          if (lsl == j * 4 + i) {
+            mlo = (unsigned)m;
+            mhi = (unsigned)(m >> 32);
+          }
         }
       }
     if (touched) {
@@ -1819,7 +1844,8 @@ __global__ __launch_bounds__(64 * RS_WAVES, 1) void knn_topc_rs_kernel(
       __builtin_amdgcn_wave_barrier();
       bool drain = overflow;
       if (!overflow) {
-        bcnt += inc;
+        const unsigned long long mine = ((unsigned long long)mhi << 32) | mlo;
+        bcnt += (int)__popcll((mine >> gsh) & 0xffffull);
         drain = __ballot(bcnt >= RB_DRAIN) != 0;
       }
       if (drain) {
