@@ -3298,18 +3298,20 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
       rs_launch(split, 32, dpad, metric == 1, Xh.p, split ? Xl.p : nullptr, Qh.p,
                 split ? Ql.p : nullptr, qmap.p, nq, nqpad, n, colscale.p, 0, (int)(npad / RA_BN), 0,
                 ecidx.p, eckey.p, s, KNN_ESC_SHARDS);
-    } else for (int sh = 0; sh < KNN_ESC_SHARDS; ++sh) {
-      const int64_t ct0 = ntile * sh / KNN_ESC_SHARDS, ct1 = ntile * (sh + 1) / KNN_ESC_SHARDS;
-      int32_t* oi = ecidx.p + (int64_t)sh * nq * 64;
-      float* ok = eckey.p + (int64_t)sh * nq * 64;
-      if (split)
-        knn_topc_kernel<true, 32><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
-                                                          metric, oi, ok, Qh.p, Ql.p, qmap.p, nq,
-                                                          ct0, ct1);
-      else
-        knn_topc_kernel<false, 32><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad,
-                                                           colscale.p, metric, oi, ok, Qh.p,
-                                                           nullptr, qmap.p, nq, ct0, ct1);
+    } else {  // the streaming kernel, one launch per shard
+      for (int sh = 0; sh < KNN_ESC_SHARDS; ++sh) {
+        const int64_t ct0 = ntile * sh / KNN_ESC_SHARDS, ct1 = ntile * (sh + 1) / KNN_ESC_SHARDS;
+        int32_t* oi = ecidx.p + (int64_t)sh * nq * 64;
+        float* ok = eckey.p + (int64_t)sh * nq * 64;
+        if (split)
+          knn_topc_kernel<true, 32><<<grid, 256, smem, s>>>(Xh.p, Xl.p, n, npad, dpad, colscale.p,
+                                                            metric, oi, ok, Qh.p, Ql.p, qmap.p,
+                                                            nq, ct0, ct1);
+        else
+          knn_topc_kernel<false, 32><<<grid, 256, smem, s>>>(Xh.p, nullptr, n, npad, dpad,
+                                                             colscale.p, metric, oi, ok, Qh.p,
+                                                             nullptr, qmap.p, nq, ct0, ct1);
+      }
     }
     EVOC_HIP(hipGetLastError());
     if (skip_first) EVOC_HIP(hipEventRecord(e1, s));
