@@ -740,6 +740,10 @@ __global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
     const bool lo_half = lane < 32;
     bool touched = false, overflow = false;
     int inc = 0;  // entries this tile adds to this lane's row
+    // the lane id made opaque per tile, so that the compiler keeps no per-slot lane masks
+    // across the walk (they would spill scalar registers)
+    int lsl = lane;
+    asm volatile("" : "+v"(lsl));
 #pragma unroll
     for (int tm = 0; tm < 2; ++tm)
 #pragma unroll
@@ -764,8 +768,8 @@ __global__ __launch_bounds__(256, 1) void knn_topc_ra_kernel(
             if (pass && pos < RA_BKT)
               lds_st_b64(lds_addr(bucket + r * RA_BKT + pos),
                          make_int2(__float_as_int(key), cbase + lcol));  // absolute column
-            inc += (lane == r_lo) ? n_lo : 0;
-            inc += (lane == r_hi) ? n_hi : 0;
+            inc += (lsl == r_lo) ? n_lo : 0;
+            inc += (lsl == r_hi) ? n_hi : 0;
           }
         }
       }
