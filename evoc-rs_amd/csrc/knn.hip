@@ -3298,10 +3298,19 @@ void knn_device(const float* dX, int64_t n, int d, int64_t ld, int metric, int k
     const unsigned grid = (unsigned)(nqpad / BM);
     const int64_t ntile = npad / BN;
     if (skip_first) EVOC_HIP(hipEventRecord(e0, s));  // then the escalation is the tile pass
-    if (rs_supported(split, 32, dpad)) {  // the shards in one launch (grid rows)
+    if (rs_supported(split, 32, dpad) && nqpad / RS_ROWS < cu_count()) {
+      // few rows: the shards in one launch (grid rows), so that together they fill the CUs
       rs_launch(split, 32, dpad, metric == 1, Xh.p, split ? Xl.p : nullptr, Qh.p,
                 split ? Ql.p : nullptr, qmap.p, nq, nqpad, n, colscale.p, 0, (int)(npad / RA_BN), 0,
                 ecidx.p, eckey.p, s, KNN_ESC_SHARDS);
+    } else if (rs_supported(split, 32, dpad)) {  // one launch per shard: each streams its columns
+      for (int sh = 0; sh < KNN_ESC_SHARDS; ++sh) {
+        const int64_t ct0 = ntile * sh / KNN_ESC_SHARDS, ct1 = ntile * (sh + 1) / KNN_ESC_SHARDS;
+        rs_launch(split, 32, dpad, metric == 1, Xh.p, split ? Xl.p : nullptr, Qh.p,
+                  split ? Ql.p : nullptr, qmap.p, nq, nqpad, n, colscale.p,
+                  (int)(ct0 * (BN / RA_BN)), (int)(ct1 * (BN / RA_BN)), 0,
+                  ecidx.p + (int64_t)sh * nq * 64, eckey.p + (int64_t)sh * nq * 64, s);
+      }
     } else {  // the streaming kernel, one launch per shard
       for (int sh = 0; sh < KNN_ESC_SHARDS; ++sh) {
         const int64_t ct0 = ntile * sh / KNN_ESC_SHARDS, ct1 = ntile * (sh + 1) / KNN_ESC_SHARDS;
